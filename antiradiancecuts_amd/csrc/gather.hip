@@ -1,0 +1,734 @@
+// AVPL radiance / antiradiance gather on gfx950 — replaces CudaGather
+// (cudaGather.h:25-70, cudaGather.cu:156-1051).
+//
+// Kernels
+//   k_pack        AVPL SoA (AvplsGpuParam) -> 80-byte records: position,
+//                 unit normal, raw and unit incident direction, radiance with
+//                 the AVPL-side Lambertian BRDF folded in, antiradiance with
+//                 the disk normalisation 1/(pi r^2) folded in.
+//   k_exact       exact gather (kernel_radiance_simple :269-314 and
+//                 kernel_antiradiance :446-557 fused): one pixel per lane,
+//                 AVPL records read with wave-uniform (scalar) loads, runs of
+//                 64 pairs summed in FP32 and added to the pixel total with
+//                 Kahan compensation; optional split over AVPL ranges.
+//   k_exact_f64   the same sums in FP64 with the reference's formulas (the
+//                 reference image for error metrics).
+//   k_finalize    sums the splits (Kahan), writes radiance, antiradiance and
+//                 result = radiance - antiradiance (kernel_combine :782-803).
+//   k_cut_sa      light-cut radiance with the solid-angle criterion
+//                 (kernel_bvh :316-444): per-pixel DFS, stack in LDS.
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "arc_internal.hpp"
+#include "gather_math.hpp"
+
+using arc::DevBuf;
+using arc::f3;
+using namespace arc::gm;
+
+struct arc_gather {
+    arc_ctx* ctx = nullptr;
+    int width = 0, height = 0;
+    DevBuf mats;
+    int num_mats = 0;
+    bool general = false;  // some material has a specular lobe
+    const float4* gpos = nullptr;
+    const float4* gnrm = nullptr;
+    const float4* galb = nullptr;
+    float4* out_res = nullptr;
+    float4* out_rad = nullptr;
+    float4* out_anti = nullptr;
+    DevBuf int_rad, int_anti, int_res;
+    DevBuf rec, part, counters;
+    DevBuf stage;  // AVPL SoA staged from host memory
+    arc_bvh* bvh = nullptr;
+    hipEvent_t ev[6] = {};
+};
+
+namespace {
+
+constexpr int kThreads = 256;
+constexpr int kRecF4 = 5;  // float4 per packed AVPL record
+constexpr int kRun = 64;   // pairs summed plainly before a compensated add
+
+enum Counter { C_VISITED = 0, C_EVALUATED = 1, C_ANTI_VISITED = 2, C_ANTI_EVALUATED = 3, C_ERRORS = 4, C_NUM = 8 };
+
+struct Kahan {
+    float s = 0.f, c = 0.f;
+    __device__ inline void add(float x) {
+        const float y = x - c;
+        const float t = s + y;
+        c = (t - s) - y;
+        s = t;
+    }
+};
+
+__device__ inline f3 f3of(float4 v) { return f3{v.x, v.y, v.z}; }
+
+// ------------------------------------------------------------- pack ----
+__global__ __launch_bounds__(kThreads) void k_pack(arc::AvplsDev a, const arc_material* __restrict__ mats,
+                                                   int general, int model, float inv_pi_r2,
+                                                   float4* __restrict__ rec) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= a.n) return;
+    const f3 p = a.pos[i], q = a.nrm[i], w = a.dir[i], L = a.rad[i];
+    const f3 A = a.anti ? a.anti[i] : f3{0.f, 0.f, 0.f};
+    const int mi = a.mat[i];
+    const float nq = 1.0f / sqrtf(q.x * q.x + q.y * q.y + q.z * q.z);
+    const float w2 = w.x * w.x + w.y * w.y + w.z * w.z;
+    const float nw = w2 > 0.f ? 1.0f / sqrtf(w2) : 0.f;
+    float lr = L.x, lg = L.y, lb = L.z;
+    if (!general && w2 != 0.f) {  // Lambertian AVPL: f_r = diffuse / pi, constant per AVPL
+        const arc_material& m = mats[mat_slot(mi)];
+        lr *= kOneOverPi * m.diffuse.x;
+        lg *= kOneOverPi * m.diffuse.y;
+        lb *= kOneOverPi * m.diffuse.z;
+    }
+    const float as = model == ARC_ANTIRAD_DISK ? inv_pi_r2 : 1.0f;
+    float4* r = rec + (size_t)kRecF4 * i;
+    r[0] = make_float4(p.x, p.y, p.z, w.x * nw);
+    r[1] = make_float4(q.x * nq, q.y * nq, q.z * nq, w.y * nw);
+    r[2] = make_float4(w.x, w.y, w.z, w.z * nw);
+    r[3] = make_float4(lr, lg, lb, (float)mi);
+    r[4] = make_float4(A.x * as, A.y * as, A.z * as, w2 == 0.f ? 1.f : 0.f);
+}
+
+// ----------------------------------------------------------- pixels ----
+struct PixelF {
+    float px, py, pz;  // position
+    float nx, ny, nz;  // unit normal
+    float br, bg, bb;  // diffuse / pi (Lambertian pixel BRDF)
+    int mat;
+};
+
+__device__ inline PixelF load_pixel(const float4* gpos, const float4* gnrm, const float4* galb,
+                                    const arc_material* mats, int pix) {
+    PixelF P;
+    const float4 p = gpos[pix], n = gnrm[pix];
+    P.px = p.x; P.py = p.y; P.pz = p.z;
+    const float s = rsq(n.x * n.x + n.y * n.y + n.z * n.z);
+    P.nx = n.x * s; P.ny = n.y * s; P.nz = n.z * s;
+    P.mat = (int)n.w;
+    float dr, dg, db;
+    if (galb) {
+        const float4 a = galb[pix];
+        dr = a.x; dg = a.y; db = a.z;
+    } else {
+        const arc_material& m = mats[mat_slot(P.mat)];
+        dr = m.diffuse.x; dg = m.diffuse.y; db = m.diffuse.z;
+    }
+    P.br = kOneOverPi * dr; P.bg = kOneOverPi * dg; P.bb = kOneOverPi * db;
+    return P;
+}
+
+// ------------------------------------------------------------ exact ----
+// part layout: [split][6][npix] = radiance rgb, antiradiance rgb.
+template <int MODEL, bool GENERAL, bool DO_RAD, bool DO_ANTI>
+__global__ __launch_bounds__(kThreads) void k_exact(const float4* __restrict__ rec, int n_avpl, int chunk,
+                                                    const float4* __restrict__ gpos,
+                                                    const float4* __restrict__ gnrm,
+                                                    const float4* __restrict__ galb,
+                                                    const arc_material* __restrict__ mats, int npix,
+                                                    float camx, float camy, float camz, float radius,
+                                                    float cos_half, float* __restrict__ part) {
+    const int pix = blockIdx.x * kThreads + threadIdx.x;
+    const bool live = pix < npix;
+    const PixelF P = load_pixel(gpos, gnrm, galb, mats, live ? pix : 0);
+    const float r2 = radius * radius;
+    const int a0 = blockIdx.y * chunk;
+    const int a1 = min(n_avpl, a0 + chunk);
+    Kahan kr[3], ka[3];
+    for (int base = a0; base < a1; base += kRun) {
+        const int end = min(a1, base + kRun);
+        float sr0 = 0.f, sr1 = 0.f, sr2 = 0.f, sa0 = 0.f, sa1 = 0.f, sa2 = 0.f;
+#pragma unroll 2
+        for (int i = base; i < end; ++i) {
+            const float4* r = rec + (size_t)kRecF4 * i;
+            const float4 r0 = r[0], r1 = r[1], r2v = r[2], r3 = r[3], r4 = r[4];
+            // d = a - p, from pixel to AVPL
+            const float dx = r0.x - P.px, dy = r0.y - P.py, dz = r0.z - P.pz;
+            const float dd = dx * dx + dy * dy + dz * dz;
+            const float inv = rsq(dd);
+            const float cn = P.nx * dx + P.ny * dy + P.nz * dz;        // |d| cos at the pixel
+            const float ca = -(r1.x * dx + r1.y * dy + r1.z * dz);     // |d| cos at the AVPL
+            if (GENERAL) {
+                using V3 = V<float>;
+                const V3 pos = mk(P.px, P.py, P.pz), nrm = mk(P.nx, P.ny, P.nz);
+                const V3 apos = mk(r0.x, r0.y, r0.z), anrm = mk(r1.x, r1.y, r1.z);
+                const V3 aw = mk(r2v.x, r2v.y, r2v.z);
+                const V3 cam = mk(camx, camy, camz);
+                Mat<float> pm = material<float>(mats, P.mat);
+                if (galb) pm.diffuse = mk(P.br / kOneOverPi, P.bg / kOneOverPi, P.bb / kOneOverPi);
+                const Mat<float> am = material<float>(mats, (int)r3.w);
+                if (DO_RAD) {
+                    const V3 v = radiance(pos, nrm, pm, apos, anrm, aw, mk(r3.x, r3.y, r3.z), am, cam);
+                    sr0 += v.x; sr1 += v.y; sr2 += v.z;
+                }
+                if (DO_ANTI) {
+                    // record holds A / (pi r^2) for the disk model; undo to reuse the formula
+                    V3 v;
+                    if (MODEL == ARC_ANTIRAD_DISK)
+                        v = antirad_disk(pos, nrm, pm, apos, anrm, aw, mk(r4.x, r4.y, r4.z) * (kPi * radius * radius),
+                                         cam, radius);
+                    else
+                        v = antirad_cone(pos, nrm, pm, apos, aw, mk(r4.x, r4.y, r4.z), cam, cos_half);
+                    sa0 += v.x; sa1 += v.y; sa2 += v.z;
+                }
+            } else {
+                if (DO_RAD) {
+                    const float c1 = fminf(fmaxf(cn * inv, 0.f), 1.f);
+                    const float c2 = fminf(fmaxf(ca * inv, 0.f), 1.f);
+                    const float g = c1 * c2 * (inv * inv);
+                    sr0 += r3.x * g; sr1 += r3.y * g; sr2 += r3.z * g;
+                }
+                if (DO_ANTI) {
+                    float m;
+                    if (MODEL == ARC_ANTIRAD_DISK) {
+                        // line a + t w hits the tangent plane at t = -cn / (w . n) > 0, within the
+                        // disk, on the far side of the AVPL's surface, pixel facing against w.
+                        // With h = d + t w_unit, |h| < r  <=>  |h denom|^2 < r^2 denom^2, and
+                        // h denom = d denom - cn w_unit: no division needed.
+                        const float denom = r2v.x * P.nx + r2v.y * P.ny + r2v.z * P.nz;
+                        const float hx = dx * denom - cn * r0.w;
+                        const float hy = dy * denom - cn * r1.w;
+                        const float hz = dz * denom - cn * r2v.w;
+                        const float h2 = hx * hx + hy * hy + hz * hz;
+                        const float lim = r2 * (denom * denom);
+                        m = (denom < 0.f && cn > 0.f && h2 < lim && ca < 0.f) ? 1.f : 0.f;
+                    } else {
+                        // cone of half-angle acos(cos_half) about w, cosine at the pixel / dist^2
+                        const float c = -(dx * r0.w + dy * r1.w + dz * r2v.w) * inv;
+                        const float cp = cn * inv;
+                        m = (c > kConeEps && cp > kConeEps && c > cos_half) ? cp * (inv * inv) : 0.f;
+                    }
+                    sa0 += r4.x * m; sa1 += r4.y * m; sa2 += r4.z * m;
+                }
+            }
+        }
+        if (DO_RAD) { kr[0].add(sr0); kr[1].add(sr1); kr[2].add(sr2); }
+        if (DO_ANTI) { ka[0].add(sa0); ka[1].add(sa1); ka[2].add(sa2); }
+    }
+    if (!live) return;
+    float* o = part + (size_t)blockIdx.y * 6 * npix + pix;
+    if (!GENERAL) {  // the Lambertian pixel BRDF is constant per pixel
+        if (DO_RAD) {
+            o[0] = kr[0].s * P.br; o[(size_t)npix] = kr[1].s * P.bg; o[2 * (size_t)npix] = kr[2].s * P.bb;
+        }
+        if (DO_ANTI) {
+            o[3 * (size_t)npix] = ka[0].s * P.br; o[4 * (size_t)npix] = ka[1].s * P.bg;
+            o[5 * (size_t)npix] = ka[2].s * P.bb;
+        }
+    } else {
+        if (DO_RAD) { o[0] = kr[0].s; o[(size_t)npix] = kr[1].s; o[2 * (size_t)npix] = kr[2].s; }
+        if (DO_ANTI) {
+            o[3 * (size_t)npix] = ka[0].s; o[4 * (size_t)npix] = ka[1].s; o[5 * (size_t)npix] = ka[2].s;
+        }
+    }
+}
+
+// Sums the splits and writes the float4 targets (kernel_combine semantics:
+// result = radiance - antiradiance including w, i.e. 1 - 1 = 0).
+__global__ __launch_bounds__(kThreads) void k_finalize(const float* __restrict__ part, int splits, int npix,
+                                                       int have_rad, int have_anti,
+                                                       const float4* __restrict__ rad_in,
+                                                       float4* __restrict__ out_rad,
+                                                       float4* __restrict__ out_anti,
+                                                       float4* __restrict__ out_res) {
+    const int pix = blockIdx.x * kThreads + threadIdx.x;
+    if (pix >= npix) return;
+    Kahan k[6];
+    for (int s = 0; s < splits; ++s)
+        for (int c = 0; c < 6; ++c)
+            if ((c < 3 && have_rad) || (c >= 3 && have_anti))
+                k[c].add(part[((size_t)s * 6 + c) * npix + pix]);
+    float4 rad = have_rad ? make_float4(k[0].s, k[1].s, k[2].s, 1.f) : rad_in[pix];
+    const float4 anti = make_float4(k[3].s, k[4].s, k[5].s, 1.f);
+    if (have_rad && out_rad) out_rad[pix] = rad;
+    if (have_anti && out_anti) out_anti[pix] = anti;
+    if (out_res)
+        out_res[pix] = make_float4(rad.x - anti.x, rad.y - anti.y, rad.z - anti.z, rad.w - anti.w);
+}
+
+// --------------------------------------------------------- exact f64 ----
+template <int MODEL>
+__global__ __launch_bounds__(kThreads) void k_exact_f64(arc::AvplsDev a, const float4* __restrict__ gpos,
+                                                        const float4* __restrict__ gnrm,
+                                                        const float4* __restrict__ galb,
+                                                        const arc_material* __restrict__ mats, int npix,
+                                                        float camx, float camy, float camz, float radius,
+                                                        double cos_half, int chunk,
+                                                        double* __restrict__ part) {
+    using V3 = V<double>;
+    const int pix = blockIdx.x * kThreads + threadIdx.x;
+    const bool live = pix < npix;
+    const int q = live ? pix : 0;
+    const float4 p4 = gpos[q], n4 = gnrm[q];
+    const V3 pos = mk<double>(p4.x, p4.y, p4.z), nrm = mk<double>(n4.x, n4.y, n4.z);
+    Mat<double> pm = material<double>(mats, (int)n4.w);
+    if (galb) pm.diffuse = mk<double>(galb[q].x, galb[q].y, galb[q].z);
+    const V3 cam = mk<double>(camx, camy, camz);
+    const int a0 = blockIdx.y * chunk, a1 = min(a.n, a0 + chunk);
+    V3 R = mk<double>(0, 0, 0), A = mk<double>(0, 0, 0);
+    for (int i = a0; i < a1; ++i) {
+        const V3 apos = ld3<double>(a.pos[i]), anrm = ld3<double>(a.nrm[i]), aw = ld3<double>(a.dir[i]);
+        const Mat<double> am = material<double>(mats, a.mat[i]);
+        R = R + radiance(pos, nrm, pm, apos, anrm, aw, ld3<double>(a.rad[i]), am, cam);
+        if (a.anti) {
+            const V3 Av = ld3<double>(a.anti[i]);
+            if (MODEL == ARC_ANTIRAD_DISK)
+                A = A + antirad_disk(pos, nrm, pm, apos, anrm, aw, Av, cam, (double)radius);
+            else
+                A = A + antirad_cone(pos, nrm, pm, apos, aw, Av, cam, cos_half);
+        }
+    }
+    if (!live) return;
+    double* o = part + (size_t)blockIdx.y * 6 * npix + pix;
+    o[0] = R.x; o[(size_t)npix] = R.y; o[2 * (size_t)npix] = R.z;
+    o[3 * (size_t)npix] = A.x; o[4 * (size_t)npix] = A.y; o[5 * (size_t)npix] = A.z;
+}
+
+__global__ __launch_bounds__(kThreads) void k_finalize_f64(const double* __restrict__ part, int splits, int npix,
+                                                           double* __restrict__ rad3, double* __restrict__ anti3) {
+    const int pix = blockIdx.x * kThreads + threadIdx.x;
+    if (pix >= npix) return;
+    double s[6] = {0, 0, 0, 0, 0, 0};
+    for (int k = 0; k < splits; ++k)
+        for (int c = 0; c < 6; ++c) s[c] += part[((size_t)k * 6 + c) * npix + pix];
+    for (int c = 0; c < 3; ++c) {
+        if (rad3) rad3[3 * (size_t)pix + c] = s[c];
+        if (anti3) anti3[3 * (size_t)pix + c] = s[3 + c];
+    }
+}
+
+// ------------------------------------------------------ cut (solid) ----
+// Decision arithmetic of kernel_bvh (cudaGather.cu:405-414) in FP32 with IEEE
+// division and square root and no fused multiply-add, so every traversal
+// decision equals the oracle's REF32 decision.
+__device__ __attribute__((noinline)) bool cluster_visible(f3 n, f3 p, f3 bmin, f3 bmax) {
+#pragma clang fp contract(off)
+    const float ax = bmax.x - p.x, ay = bmax.y - p.y, az = bmax.z - p.z;
+    const float sa = 1.0f / sqrtf(ax * ax + ay * ay + az * az);
+    const float c1 = n.x * (ax * sa) + n.y * (ay * sa) + n.z * (az * sa);
+    const float bx = bmin.x - p.x, by = bmin.y - p.y, bz = bmin.z - p.z;
+    const float sb = 1.0f / sqrtf(bx * bx + by * by + bz * bz);
+    const float c2 = n.x * (bx * sb) + n.y * (by * sb) + n.z * (bz * sb);
+    return (c1 > 0.f) || (c2 > 0.f);
+}
+
+__device__ __attribute__((noinline)) bool solid_angle_below(f3 p, f3 rep, f3 bmin, f3 bmax, float thresh) {
+#pragma clang fp contract(off)
+    const float ex = bmax.x - bmin.x, ey = bmax.y - bmin.y, ez = bmax.z - bmin.z;
+    const float radius = 0.5f * sqrtf(ex * ex + ey * ey + ez * ez);
+    const float dx = rep.x - p.x, dy = rep.y - p.y, dz = rep.z - p.z;
+    const float dist = sqrtf(dx * dx + dy * dy + dz * dz);
+    const float omega = (2.0f * kPi) * (1.0f - dist / sqrtf(radius * radius + dist * dist));
+    return omega < thresh;
+}
+
+struct CutTree {
+    const arc_bvh_node* nodes;
+    const int* mat;
+    const f3* pos;
+    const f3* nrm;
+    const f3* rad;
+    const f3* dir;
+    int n;  // leaves
+};
+
+__global__ __launch_bounds__(kThreads) void k_cut_sa(CutTree t, const float4* __restrict__ gpos,
+                                                     const float4* __restrict__ gnrm,
+                                                     const float4* __restrict__ galb,
+                                                     const arc_material* __restrict__ mats, int W, int H,
+                                                     float camx, float camy, float camz, float thresh,
+                                                     int stack_size, int sentinel,
+                                                     float4* __restrict__ out_rad,
+                                                     unsigned long long* __restrict__ counters) {
+    extern __shared__ int stk[];  // [stack_size][kThreads]
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const int x = blockIdx.x * 16 + (wave & 1) * 8 + (lane & 7);
+    const int y = blockIdx.y * 16 + (wave >> 1) * 8 + (lane >> 3);
+    const bool live = x < W && y < H;
+    unsigned long long nvis = 0, neval = 0, nerr = 0;
+    if (live) {
+        using V3 = V<float>;
+        const int pix = y * W + x;
+        const float4 p4 = gpos[pix], n4 = gnrm[pix];
+        const f3 p = f3of(p4), nraw = f3of(n4);
+        const V3 pos = mk(p4.x, p4.y, p4.z), nrm = mk(n4.x, n4.y, n4.z);
+        Mat<float> pm = material<float>(mats, (int)n4.w);
+        if (galb) pm.diffuse = mk(galb[pix].x, galb[pix].y, galb[pix].z);
+        const V3 cam = mk(camx, camy, camz);
+        auto eval = [&](int idx) {
+            const Mat<float> am = material<float>(mats, t.mat[idx]);
+            return radiance(pos, nrm, pm, ld3<float>(t.pos[idx]), ld3<float>(t.nrm[idx]), ld3<float>(t.dir[idx]),
+                            ld3<float>(t.rad[idx]), am, cam);
+        };
+        V3 R = mk(0.f, 0.f, 0.f);
+        int sp = 0;
+        stk[sp * kThreads + tid] = -1;
+        sp = 1;
+        bool error = false;
+        while (sp > 0) {
+            const int node = stk[(--sp) * kThreads + tid];
+            if (node >= 0) {
+                R = R + eval(node);
+                neval++;
+                continue;
+            }
+            const int k = -(node + 1);
+            if (k >= t.n - 1) {
+                error = true;
+                break;
+            }
+            nvis++;
+            const arc_bvh_node b = t.nodes[k];
+            const f3 bmin = f3{b.bbMin.x, b.bbMin.y, b.bbMin.z}, bmax = f3{b.bbMax.x, b.bbMax.y, b.bbMax.z};
+            if (!cluster_visible(nraw, p, bmin, bmax)) continue;
+            const int idx = t.n + k;
+            if (solid_angle_below(p, t.pos[idx], bmin, bmax, thresh)) {
+                R = R + eval(idx);
+                neval++;
+            } else {
+                stk[sp * kThreads + tid] = b.left;
+                stk[(sp + 1) * kThreads + tid] = b.right;
+                sp += 2;
+                if (sp >= stack_size) {
+                    error = true;
+                    break;
+                }
+            }
+        }
+        float4 o = make_float4(R.x, R.y, R.z, 1.f);
+        if (error) {
+            nerr = 1;
+            if (sentinel) o = make_float4(100000.f, 0.f, 100000.f, 1.f);
+        }
+        out_rad[pix] = o;
+    }
+    if (counters) {
+        // one atomic per wave
+        for (int off = 32; off > 0; off >>= 1) {
+            nvis += __shfl_down(nvis, off);
+            neval += __shfl_down(neval, off);
+            nerr += __shfl_down(nerr, off);
+        }
+        if (lane == 0) {
+            atomicAdd(&counters[C_VISITED], nvis);
+            atomicAdd(&counters[C_EVALUATED], neval);
+            atomicAdd(&counters[C_ERRORS], nerr);
+        }
+    }
+}
+
+// ------------------------------------------------------------- host ----
+inline int blocks(long long n) { return (int)((n + kThreads - 1) / kThreads); }
+
+int pick_splits(int npix, int n_avpl) {
+    const int pblocks = blocks(npix);
+    int s = (2048 + pblocks - 1) / pblocks;
+    s = std::max(1, std::min(s, (n_avpl + 255) / 256));
+    return s;
+}
+
+template <int MODEL, bool GENERAL, bool DO_RAD, bool DO_ANTI>
+void launch_exact(arc_gather* g, int n, int chunk, int splits, float3 cam, float radius, float cos_half) {
+    const int npix = g->width * g->height;
+    hipLaunchKernelGGL((k_exact<MODEL, GENERAL, DO_RAD, DO_ANTI>), dim3(blocks(npix), splits), dim3(kThreads), 0,
+                       g->ctx->stream, g->rec.as<float4>(), n, chunk, g->gpos, g->gnrm, g->galb,
+                       g->mats.as<arc_material>(), npix, cam.x, cam.y, cam.z, radius, cos_half,
+                       g->part.as<float>());
+}
+
+template <bool DO_RAD, bool DO_ANTI>
+void dispatch_exact(arc_gather* g, int model, int n, int chunk, int splits, float3 cam, float radius,
+                    float cos_half) {
+    if (model == ARC_ANTIRAD_DISK) {
+        if (g->general) launch_exact<ARC_ANTIRAD_DISK, true, DO_RAD, DO_ANTI>(g, n, chunk, splits, cam, radius, cos_half);
+        else launch_exact<ARC_ANTIRAD_DISK, false, DO_RAD, DO_ANTI>(g, n, chunk, splits, cam, radius, cos_half);
+    } else {
+        if (g->general) launch_exact<ARC_ANTIRAD_CONE, true, DO_RAD, DO_ANTI>(g, n, chunk, splits, cam, radius, cos_half);
+        else launch_exact<ARC_ANTIRAD_CONE, false, DO_RAD, DO_ANTI>(g, n, chunk, splits, cam, radius, cos_half);
+    }
+}
+
+// AVPL pointers on the device, staging host input when asked (AvplsGpu,
+// data_gpu.cu:5-52).
+int stage_avpls(arc_gather* g, const arc_avpls* in, int flags, arc::AvplsDev& out) {
+    arc_ctx* ctx = g->ctx;
+    const int n = in->numAvpls;
+    out.n = n;
+    if (!(flags & ARC_INPUT_HOST)) {
+        out.pos = reinterpret_cast<const f3*>(in->position);
+        out.nrm = reinterpret_cast<const f3*>(in->normal);
+        out.dir = reinterpret_cast<const f3*>(in->incDirection);
+        out.rad = reinterpret_cast<const f3*>(in->incRadiance);
+        out.anti = reinterpret_cast<const f3*>(in->antiradiance);
+        out.mat = in->materialIndex;
+        out.bounce = in->bounce;
+        return ARC_OK;
+    }
+    const size_t v3 = sizeof(f3) * (size_t)n, vi = sizeof(int) * (size_t)n;
+    ARC_TRY(arc::ensure(ctx, g->stage, 5 * v3 + 2 * vi));
+    char* base = g->stage.as<char>();
+    const void* src[7] = {in->position, in->normal, in->incDirection, in->incRadiance, in->antiradiance,
+                          in->materialIndex, in->bounce};
+    const size_t sz[7] = {v3, v3, v3, v3, v3, vi, vi};
+    char* dst[7];
+    size_t off = 0;
+    for (int k = 0; k < 7; ++k) {
+        dst[k] = base + off;
+        off += sz[k];
+        if (src[k]) ARC_HIP(ctx, hipMemcpyAsync(dst[k], src[k], sz[k], hipMemcpyHostToDevice, ctx->stream));
+    }
+    out.pos = reinterpret_cast<const f3*>(dst[0]);
+    out.nrm = reinterpret_cast<const f3*>(dst[1]);
+    out.dir = reinterpret_cast<const f3*>(dst[2]);
+    out.rad = reinterpret_cast<const f3*>(dst[3]);
+    out.anti = in->antiradiance ? reinterpret_cast<const f3*>(dst[4]) : nullptr;
+    out.mat = reinterpret_cast<const int*>(dst[5]);
+    out.bounce = in->bounce ? reinterpret_cast<const int*>(dst[6]) : nullptr;
+    return ARC_OK;
+}
+
+int validate_params(arc_gather* g, const arc_avpls* avpls, const arc_gather_params* p) {
+    if (!g || !avpls || !p) return ARC_E_INVALID;
+    if (!g->gpos || !g->gnrm) return arc::fail(g->ctx, ARC_E_STATE, "no G-buffer bound");
+    if (avpls->numAvpls < 0) return arc::fail(g->ctx, ARC_E_INVALID, "numAvpls < 0");
+    if (avpls->numAvpls > 0 && (!avpls->position || !avpls->normal || !avpls->incDirection ||
+                                !avpls->incRadiance || !avpls->materialIndex))
+        return arc::fail(g->ctx, ARC_E_INVALID, "AVPL arrays missing");
+    if (p->antiradianceModel != ARC_ANTIRAD_DISK && p->antiradianceModel != ARC_ANTIRAD_CONE)
+        return arc::fail(g->ctx, ARC_E_INVALID, "antiradianceModel");
+    if (p->stackSize < 2 || p->stackSize > 64) return arc::fail(g->ctx, ARC_E_INVALID, "stackSize out of [2, 64]");
+    return ARC_OK;
+}
+
+float4* target(DevBuf& internal, float4* user, arc_gather* g, int& status) {
+    if (user) return user;
+    status = arc::ensure(g->ctx, internal, sizeof(float4) * (size_t)g->width * g->height);
+    return internal.as<float4>();
+}
+
+}  // namespace
+
+extern "C" {
+
+void arc_gather_params_default(arc_gather_params* p) {
+    if (!p) return;
+    std::memset(p, 0, sizeof(*p));
+    p->useLightCuts = 1;                        // CConfigManager.cpp:15
+    p->cutMode = ARC_CUT_SOLID_ANGLE;
+    p->clusterRefinementThreshold = (float)(M_PI / 10.0);  // CConfigManager.cpp:37
+    p->errorBound = 0.01f;
+    p->antiradianceModel = ARC_ANTIRAD_DISK;
+    p->antiradianceCut = 0;
+    p->photonRadiusScale = 2.0f;                // CConfigManager.cpp:26
+    p->coneHalfAngle = (float)(M_PI / 18.0);
+    p->writeSentinel = 1;
+    p->stackSize = 32;                          // STACK_SIZE, cudaGather.cu:19
+    p->seed = 5489;
+}
+
+int arc_gather_create(arc_ctx* ctx, int width, int height, const arc_material* materials, int num_materials,
+                      arc_gather** out) {
+    if (!ctx || !out || width <= 0 || height <= 0 || !materials || num_materials <= 0 || num_materials > 256)
+        return ARC_E_INVALID;
+    arc_gather* g = new arc_gather();
+    g->ctx = ctx;
+    g->width = width;
+    g->height = height;
+    g->num_mats = num_materials;
+    for (int i = 0; i < num_materials; ++i) {
+        const arc_float3& s = materials[i].specular;
+        if (std::sqrt(s.x * s.x + s.y * s.y + s.z * s.z) > 0.f) g->general = true;
+    }
+    // zero-padded to kMaxMaterials entries: kernels mask indices into the table
+    std::vector<arc_material> table(kMaxMaterials);
+    std::memset(table.data(), 0, sizeof(arc_material) * table.size());
+    std::memcpy(table.data(), materials, sizeof(arc_material) * (size_t)num_materials);
+    int st = arc::ensure(ctx, g->mats, sizeof(arc_material) * table.size());
+    if (st == ARC_OK) {
+        hipError_t e = hipMemcpy(g->mats.p, table.data(), sizeof(arc_material) * table.size(),
+                                 hipMemcpyHostToDevice);
+        if (e != hipSuccess) st = arc::hip_fail(ctx, e, "hipMemcpy(materials)", __FILE__, __LINE__);
+    }
+    if (st == ARC_OK) st = arc::ensure(ctx, g->counters, sizeof(unsigned long long) * C_NUM);
+    for (int i = 0; st == ARC_OK && i < 6; ++i) {
+        hipError_t e = hipEventCreate(&g->ev[i]);
+        if (e != hipSuccess) st = arc::hip_fail(ctx, e, "hipEventCreate", __FILE__, __LINE__);
+    }
+    if (st == ARC_OK) st = arc_bvh_create(ctx, 0, &g->bvh);
+    if (st != ARC_OK) {
+        arc_gather_destroy(g);
+        return st;
+    }
+    *out = g;
+    return ARC_OK;
+}
+
+void arc_gather_destroy(arc_gather* g) {
+    if (!g) return;
+    for (int i = 0; i < 6; ++i)
+        if (g->ev[i]) (void)hipEventDestroy(g->ev[i]);
+    arc::release(g->mats);
+    arc::release(g->int_rad);
+    arc::release(g->int_anti);
+    arc::release(g->int_res);
+    arc::release(g->rec);
+    arc::release(g->part);
+    arc::release(g->counters);
+    arc::release(g->stage);
+    arc_bvh_destroy(g->bvh);
+    delete g;
+}
+
+int arc_gather_set_gbuffer(arc_gather* g, const float* pos4, const float* nrm4, const float* albedo4) {
+    if (!g || !pos4 || !nrm4) return ARC_E_INVALID;
+    g->gpos = reinterpret_cast<const float4*>(pos4);
+    g->gnrm = reinterpret_cast<const float4*>(nrm4);
+    g->galb = reinterpret_cast<const float4*>(albedo4);
+    return ARC_OK;
+}
+
+int arc_gather_set_outputs(arc_gather* g, float* result4, float* radiance4, float* antiradiance4) {
+    if (!g) return ARC_E_INVALID;
+    g->out_res = reinterpret_cast<float4*>(result4);
+    g->out_rad = reinterpret_cast<float4*>(radiance4);
+    g->out_anti = reinterpret_cast<float4*>(antiradiance4);
+    return ARC_OK;
+}
+
+arc_bvh* arc_gather_get_bvh(arc_gather* g) { return (g && g->bvh && g->bvh->n > 0) ? g->bvh : nullptr; }
+
+int arc_gather_run(arc_gather* g, const arc_avpls* avpls, arc_float3 camPos, float sceneExtent,
+                   const arc_gather_params* p, int flags, arc_gather_stats* stats) {
+    ARC_TRY(validate_params(g, avpls, p));
+    arc_ctx* ctx = g->ctx;
+    const bool profile = (flags & ARC_RUN_PROFILE) != 0;
+    const bool count = (flags & ARC_RUN_COUNTERS) != 0;
+    const int npix = g->width * g->height;
+    const int n = avpls->numAvpls;
+    const float radius = p->photonRadiusScale * sceneExtent / 100.f;  // cudaGather.cu:949
+    const float cos_half = std::cos(p->coneHalfAngle);
+    const float3 cam = make_float3(camPos.x, camPos.y, camPos.z);
+    int st = ARC_OK;
+    float4* rad = target(g->int_rad, g->out_rad, g, st);
+    ARC_TRY(st);
+    float4* res = g->out_res;
+    float4* anti = g->out_anti;
+    if (stats) std::memset(stats, 0, sizeof(*stats));
+    if (count) ARC_HIP(ctx, hipMemsetAsync(g->counters.p, 0, sizeof(unsigned long long) * C_NUM, ctx->stream));
+    if (profile) ARC_HIP(ctx, hipEventRecord(g->ev[0], ctx->stream));
+
+    arc::AvplsDev a{};
+    ARC_TRY(stage_avpls(g, avpls, flags, a));
+    if (n > 0) {
+        ARC_TRY(arc::ensure(ctx, g->rec, sizeof(float4) * kRecF4 * (size_t)n));
+        hipLaunchKernelGGL(k_pack, dim3(blocks(n)), dim3(kThreads), 0, ctx->stream, a, g->mats.as<arc_material>(),
+                           g->general ? 1 : 0, p->antiradianceModel, 1.0f / (kPi * radius * radius),
+                           g->rec.as<float4>());
+        ARC_LAUNCHED(ctx);
+    }
+    if (profile) ARC_HIP(ctx, hipEventRecord(g->ev[1], ctx->stream));
+
+    const int splits = pick_splits(npix, std::max(n, 1));
+    const int chunk = (std::max(n, 1) + splits - 1) / splits;
+    ARC_TRY(arc::ensure(ctx, g->part, sizeof(float) * 6 * (size_t)splits * npix));
+    unsigned long long* counters = count ? g->counters.as<unsigned long long>() : nullptr;
+
+    bool cut = p->useLightCuts && n > 1;
+    if (p->useLightCuts && p->cutMode != ARC_CUT_SOLID_ANGLE)
+        return arc::fail(ctx, ARC_E_UNSUPPORTED, "cutMode not available in this build");
+    if (cut) {
+        // AvplBvh(avpls, false) per frame (cudaGather.cu:858)
+        ARC_TRY(arc::bvh_build_avpls(g->bvh, a, nullptr, p->seed, (flags & ARC_TRUST_INPUT) == 0));
+        if (profile) ARC_HIP(ctx, hipEventRecord(g->ev[2], ctx->stream));
+        arc_bvh* b = g->bvh;
+        CutTree t{b->nodes.as<arc_bvh_node>(), b->nd_mat.as<int>(), b->nd_pos.as<f3>(), b->nd_nrm.as<f3>(),
+                  b->nd_rad.as<f3>(), b->nd_dir.as<f3>(), b->n};
+        dim3 grid((g->width + 15) / 16, (g->height + 15) / 16);
+        const size_t lds = sizeof(int) * (size_t)p->stackSize * kThreads;
+        hipLaunchKernelGGL(k_cut_sa, grid, dim3(kThreads), lds, ctx->stream, t, g->gpos, g->gnrm, g->galb,
+                           g->mats.as<arc_material>(), g->width, g->height, cam.x, cam.y, cam.z,
+                           p->clusterRefinementThreshold, p->stackSize, p->writeSentinel, rad, counters);
+        ARC_LAUNCHED(ctx);
+        if (profile) ARC_HIP(ctx, hipEventRecord(g->ev[3], ctx->stream));
+        if (n > 0) dispatch_exact<false, true>(g, p->antiradianceModel, n, chunk, splits, cam, radius, cos_half);
+        ARC_LAUNCHED(ctx);
+        if (profile) ARC_HIP(ctx, hipEventRecord(g->ev[4], ctx->stream));
+        hipLaunchKernelGGL(k_finalize, dim3(blocks(npix)), dim3(kThreads), 0, ctx->stream, g->part.as<float>(),
+                           n > 0 ? splits : 0, npix, 0, 1, rad, nullptr, anti, res);
+        ARC_LAUNCHED(ctx);
+    } else {
+        if (profile) ARC_HIP(ctx, hipEventRecord(g->ev[2], ctx->stream));
+        if (n > 0) dispatch_exact<true, true>(g, p->antiradianceModel, n, chunk, splits, cam, radius, cos_half);
+        ARC_LAUNCHED(ctx);
+        if (profile) {
+            ARC_HIP(ctx, hipEventRecord(g->ev[3], ctx->stream));
+            ARC_HIP(ctx, hipEventRecord(g->ev[4], ctx->stream));
+        }
+        hipLaunchKernelGGL(k_finalize, dim3(blocks(npix)), dim3(kThreads), 0, ctx->stream, g->part.as<float>(),
+                           n > 0 ? splits : 0, npix, 1, 1, rad, rad, anti, res);
+        ARC_LAUNCHED(ctx);
+    }
+    if (profile) ARC_HIP(ctx, hipEventRecord(g->ev[5], ctx->stream));
+    if (stats) {
+        stats->numLeafs = cut ? g->bvh->n : 0;
+        if (count) {
+            unsigned long long c[C_NUM];
+            ARC_HIP(ctx, hipMemcpyAsync(c, g->counters.p, sizeof(c), hipMemcpyDeviceToHost, ctx->stream));
+            ARC_HIP(ctx, hipStreamSynchronize(ctx->stream));
+            stats->visitedNodes = (long long)c[C_VISITED];
+            stats->evaluatedNodes = (long long)c[C_EVALUATED];
+            stats->antiVisitedNodes = (long long)c[C_ANTI_VISITED];
+            stats->antiEvaluatedNodes = (long long)c[C_ANTI_EVALUATED];
+            stats->errorPixels = (long long)c[C_ERRORS];
+        }
+        if (profile) {
+            ARC_HIP(ctx, hipEventSynchronize(g->ev[5]));
+            float t[5];
+            for (int i = 0; i < 5; ++i) ARC_HIP(ctx, hipEventElapsedTime(&t[i], g->ev[i], g->ev[i + 1]));
+            stats->msUpload = t[0];
+            stats->msBvh = t[1];
+            stats->msRadiance = t[2];
+            stats->msAntiradiance = t[3];
+            stats->msCombine = t[4];
+            ARC_HIP(ctx, hipEventElapsedTime(&stats->msTotal, g->ev[0], g->ev[5]));
+        }
+    }
+    return ARC_OK;
+}
+
+int arc_gather_exact_f64(arc_gather* g, const arc_avpls* avpls, arc_float3 camPos, float sceneExtent,
+                         const arc_gather_params* p, int flags, double* radiance3, double* antiradiance3) {
+    ARC_TRY(validate_params(g, avpls, p));
+    arc_ctx* ctx = g->ctx;
+    const int npix = g->width * g->height;
+    const int n = avpls->numAvpls;
+    const float radius = p->photonRadiusScale * sceneExtent / 100.f;
+    const double cos_half = std::cos((double)p->coneHalfAngle);
+    arc::AvplsDev a{};
+    ARC_TRY(stage_avpls(g, avpls, flags, a));
+    const int splits = pick_splits(npix, std::max(n, 1));
+    const int chunk = (std::max(n, 1) + splits - 1) / splits;
+    ARC_TRY(arc::ensure(ctx, g->part, sizeof(double) * 6 * (size_t)splits * npix));
+    if (n > 0) {
+        if (p->antiradianceModel == ARC_ANTIRAD_DISK)
+            hipLaunchKernelGGL(k_exact_f64<ARC_ANTIRAD_DISK>, dim3(blocks(npix), splits), dim3(kThreads), 0,
+                               ctx->stream, a, g->gpos, g->gnrm, g->galb, g->mats.as<arc_material>(), npix,
+                               camPos.x, camPos.y, camPos.z, radius, cos_half, chunk, g->part.as<double>());
+        else
+            hipLaunchKernelGGL(k_exact_f64<ARC_ANTIRAD_CONE>, dim3(blocks(npix), splits), dim3(kThreads), 0,
+                               ctx->stream, a, g->gpos, g->gnrm, g->galb, g->mats.as<arc_material>(), npix,
+                               camPos.x, camPos.y, camPos.z, radius, cos_half, chunk, g->part.as<double>());
+        ARC_LAUNCHED(ctx);
+    }
+    hipLaunchKernelGGL(k_finalize_f64, dim3(blocks(npix)), dim3(kThreads), 0, ctx->stream, g->part.as<double>(),
+                       n > 0 ? splits : 0, npix, radiance3, antiradiance3);
+    ARC_LAUNCHED(ctx);
+    return ARC_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,434 @@
+// Karras LBVH on gfx950 — replaces Bvh::create and its AvplBvh / SimpleBvh
+// specialisations (bvh.cu:325-454, 673-808, 859-907) and morton.h.
+//
+// Pipeline (one stream, no host round trip):
+//   k_bounds_*   per-axis min / max of positions and normals, init FLT_MAX /
+//                FLT_MIN as the reference's thrust::reduce (bvh.cu:346-361)
+//   k_encode     (v - min) / (max - min) with IEEE division, 10-bit
+//                quantisation, 30-bit interleave (morton.h:34-50, 82-92) or
+//                the 60-bit 6D code truncated to 32 bits (morton.h:94-107,
+//                bvh.h:17); value = input index
+//   sort_pairs   stable radix sort, so equal codes keep input order
+//   k_permute    sorted leaf positions / normals and leaf node data
+//   k_karras     one thread per internal node (bvh.cu:188-278)
+//   k_refit      one thread per leaf, bottom-up AABB merge with an
+//                agent-scope acquire/release counter per node; the AvplBvh
+//                variant also forms cluster representatives (bvh.cu:726-744)
+//
+// This file is compiled with -ffp-contract=off: the refit's float results
+// (cluster radiance sums, representative choice) must match the oracle bit
+// for bit, so no multiply-add may be fused.
+#include <cfloat>
+
+#include "arc_internal.hpp"
+
+namespace {
+
+using arc::f3;
+
+constexpr int kBlock = 256;
+constexpr int kReduceBlocks = 512;
+
+__device__ inline f3 ldf3(const f3* p, size_t i) { return p[i]; }
+
+// ------------------------------------------------------------ bounds ----
+// partials: [block][12] = pos min xyz, pos max xyz, nrm min xyz, nrm max xyz
+__global__ __launch_bounds__(kBlock) void k_bounds_partial(const f3* __restrict__ pos,
+                                                           const f3* __restrict__ nrm, int n,
+                                                           float* __restrict__ partials) {
+    float v[12];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        v[k] = FLT_MAX;
+        v[3 + k] = FLT_MIN;  // the reference's max-init (Q1), a tiny positive value
+        v[6 + k] = FLT_MAX;
+        v[9 + k] = FLT_MIN;
+    }
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const f3 p = pos[i];
+        v[0] = fminf(v[0], p.x); v[1] = fminf(v[1], p.y); v[2] = fminf(v[2], p.z);
+        v[3] = fmaxf(v[3], p.x); v[4] = fmaxf(v[4], p.y); v[5] = fmaxf(v[5], p.z);
+        if (nrm) {
+            const f3 q = nrm[i];
+            v[6] = fminf(v[6], q.x); v[7] = fminf(v[7], q.y); v[8] = fminf(v[8], q.z);
+            v[9] = fmaxf(v[9], q.x); v[10] = fmaxf(v[10], q.y); v[11] = fmaxf(v[11], q.z);
+        }
+    }
+    __shared__ float s[12][kBlock];
+#pragma unroll
+    for (int k = 0; k < 12; ++k) s[k][threadIdx.x] = v[k];
+    __syncthreads();
+    for (int off = kBlock / 2; off > 0; off >>= 1) {
+        if ((int)threadIdx.x < off) {
+#pragma unroll
+            for (int k = 0; k < 12; ++k) {
+                const float a = s[k][threadIdx.x], b = s[k][threadIdx.x + off];
+                s[k][threadIdx.x] = ((k / 3) & 1) ? fmaxf(a, b) : fminf(a, b);
+            }
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x < 12) partials[blockIdx.x * 12 + threadIdx.x] = s[threadIdx.x][0];
+}
+
+__global__ __launch_bounds__(64) void k_bounds_final(const float* __restrict__ partials, int nparts,
+                                                     float* __restrict__ bounds) {
+    const int k = threadIdx.x;
+    if (k >= 12) return;
+    const bool is_max = (k / 3) & 1;
+    float r = is_max ? FLT_MIN : FLT_MAX;
+    for (int b = 0; b < nparts; ++b) {
+        const float x = partials[b * 12 + k];
+        r = is_max ? fmaxf(r, x) : fminf(r, x);
+    }
+    bounds[k] = r;
+}
+
+// ------------------------------------------------------------ Morton ----
+__device__ inline uint32_t expand2(uint32_t x) {  // morton.h:34-42
+    x &= 0x000003ffu;
+    x = (x ^ (x << 16)) & 0xff0000ffu;
+    x = (x ^ (x << 8)) & 0x0300f00fu;
+    x = (x ^ (x << 4)) & 0xc30c30c3u;
+    x = (x ^ (x << 2)) & 0x49249249u;
+    return x;
+}
+
+__device__ inline uint32_t quant10(float c) {
+    // fminf(fmaxf(c * 1024, 0), 1023), truncating conversion; NaN -> 0
+    return (uint32_t)fminf(fmaxf(c * 1024.0f, 0.0f), 1023.0f);
+}
+
+// 6-way interleave of 10-bit inputs, bit i of input k at 6 i + k
+// (morton.h:67-80); only the low 32 bits survive (MC_SIZE = uint32_t).
+__device__ inline uint32_t interleave6_lo32(uint32_t a, uint32_t b, uint32_t c, uint32_t d,
+                                            uint32_t e, uint32_t f) {
+    uint64_t r = 0;
+    const uint32_t in[6] = {a, b, c, d, e, f};
+#pragma unroll
+    for (int i = 0; i < 10; ++i)
+#pragma unroll
+        for (int k = 0; k < 6; ++k) r |= (uint64_t)((in[k] >> i) & 1u) << (6 * i + k);
+    return (uint32_t)r;
+}
+
+__global__ __launch_bounds__(kBlock) void k_encode(const f3* __restrict__ pos, const f3* __restrict__ nrm,
+                                                   int n, const float* __restrict__ bounds,
+                                                   int consider_normals, uint32_t* __restrict__ keys,
+                                                   int* __restrict__ vals) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const f3 p = pos[i];
+    // Normalize functor (bvh.cu:73-81): component-wise IEEE division
+    const float x = (p.x - bounds[0]) / (bounds[3] - bounds[0]);
+    const float y = (p.y - bounds[1]) / (bounds[4] - bounds[1]);
+    const float z = (p.z - bounds[2]) / (bounds[5] - bounds[2]);
+    uint32_t code;
+    if (consider_normals) {
+        const f3 q = nrm[i];
+        const float u = (q.x - bounds[6]) / (bounds[9] - bounds[6]);
+        const float v = (q.y - bounds[7]) / (bounds[10] - bounds[7]);
+        const float w = (q.z - bounds[8]) / (bounds[11] - bounds[8]);
+        code = interleave6_lo32(quant10(x), quant10(y), quant10(z), quant10(u), quant10(v), quant10(w));
+    } else {
+        code = (expand2(quant10(z)) << 2) + (expand2(quant10(y)) << 1) + expand2(quant10(x));
+    }
+    keys[i] = code;
+    vals[i] = i;
+}
+
+// Flags AVPLs with bounce <= 0 (they would misalign leaves and node data).
+__global__ __launch_bounds__(kBlock) void k_check_bounce(const int* __restrict__ bounce, int n,
+                                                         int* __restrict__ flag) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n && bounce[i] <= 0) atomicOr(flag, 1);
+}
+
+// --------------------------------------------------------- permute ----
+__global__ __launch_bounds__(kBlock) void k_permute_points(const int* __restrict__ perm,
+                                                           const f3* __restrict__ pos,
+                                                           const f3* __restrict__ nrm, int n,
+                                                           f3* __restrict__ spos, f3* __restrict__ snrm) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    const int o = perm[s];
+    spos[s] = pos[o];
+    if (nrm) snrm[s] = nrm[o];
+}
+
+struct NodeData {  // AvplBvhNodeDataParam over 2n-1 slots
+    int* size;
+    int* mat;
+    const float* rnd;
+    f3* pos;
+    f3* nrm;
+    f3* rad;
+    f3* dir;
+    f3* anti;  // antiradiance sums (extension; may be null)
+};
+
+__global__ __launch_bounds__(kBlock) void k_permute_avpls(const int* __restrict__ perm,
+                                                          arc::AvplsDev a, f3* __restrict__ spos,
+                                                          f3* __restrict__ snrm, NodeData nd) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= a.n) return;
+    const int o = perm[s];
+    const f3 p = a.pos[o], q = a.nrm[o];
+    spos[s] = p;
+    snrm[s] = q;
+    nd.size[s] = 1;
+    nd.mat[s] = a.mat[o];
+    nd.pos[s] = p;
+    nd.nrm[s] = q;
+    nd.rad[s] = a.rad[o];
+    nd.dir[s] = a.dir[o];
+    if (nd.anti) nd.anti[s] = a.anti ? a.anti[o] : f3{0.f, 0.f, 0.f};
+}
+
+__global__ __launch_bounds__(kBlock) void k_random(uint64_t seed, int count, float* __restrict__ out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < count) out[i] = arc::hash_uniform(seed, (uint64_t)i);
+}
+
+// ---------------------------------------------------------- Karras ----
+// delta (bvh.cu:124-135): -1 outside [0, num_nodes]; on equal codes the
+// common prefix continues into the slot indices.
+__device__ inline int delta(int i, int j, const uint32_t* __restrict__ m, int num_nodes) {
+    if (j < 0 || j > num_nodes) return -1;
+    const uint32_t x = m[i] ^ m[j];
+    return x ? __clz(x) : 32 + __clz((uint32_t)(i ^ j));
+}
+
+__global__ __launch_bounds__(kBlock) void k_karras(const uint32_t* __restrict__ m, int num_nodes,
+                                                   arc_bvh_node* __restrict__ nodes,
+                                                   int* __restrict__ parents, int* __restrict__ ranges) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= num_nodes) return;
+    const int root_slot = num_nodes + 1;
+    // direction of the range: sign(0) = +1 (bvh.cu:170-173, 203-204)
+    const int d = (delta(i, i + 1, m, num_nodes) - delta(i, i - 1, m, num_nodes)) < 0 ? -1 : 1;
+    const int dmin = delta(i, i - d, m, num_nodes);
+    int lmax = 128;  // grows by 4x (bvh.cu:208-211)
+    while (delta(i, i + lmax * d, m, num_nodes) > dmin) lmax *= 4;
+    int l = 0;
+    for (int t = lmax / 2; t > 0; t /= 2)
+        if (delta(i, i + (l + t) * d, m, num_nodes) > dmin) l += t;
+    const int j = i + l * d;
+    const int dnode = delta(i, j, m, num_nodes);
+    int s = 0;
+    for (int k = 2;; k *= 2) {  // t = ceil(l / k) down to 1 (bvh.cu:226-238)
+        const int t = (l + k - 1) / k;
+        if (delta(i, i + (s + t) * d, m, num_nodes) > dnode) s += t;
+        if (t <= 1) break;
+    }
+    const int split = i + s * d + min(d, 0);
+    const int lo = min(i, j), hi = max(i, j);
+    const int left = (lo == split) ? split : -(split + 1);
+    const int right = (hi == split + 1) ? (split + 1) : -(split + 2);
+    parents[left >= 0 ? left : root_slot - (left + 1)] = i;
+    parents[right >= 0 ? right : root_slot - (right + 1)] = i;
+    arc_bvh_node nd;
+    nd.left = left;
+    nd.right = right;
+    nd.bbMin = arc_float3{0.f, 0.f, 0.f};
+    nd.bbMax = arc_float3{0.f, 0.f, 0.f};
+    nd.visited = 0;
+    nodes[i] = nd;
+    ranges[2 * i] = lo;
+    ranges[2 * i + 1] = hi;
+    if (i == 0) parents[root_slot] = -1;
+}
+
+// ----------------------------------------------------------- refit ----
+__device__ inline float len3(f3 v) {
+    // sqrtf(dot(v, v)), dot evaluated left to right without fusion
+    return sqrtf(v.x * v.x + v.y * v.y + v.z * v.z);
+}
+
+// cluster() (bvh.cu:726-744)
+__device__ inline void cluster(int target, int left, int right, int n, const NodeData& nd) {
+    const int tid = n + target;
+    const int lid = left < 0 ? n - (left + 1) : left;
+    const int rid = right < 0 ? n - (right + 1) : right;
+    const f3 rl = nd.rad[lid], rr = nd.rad[rid];
+    const float i1 = len3(rl), i2 = len3(rr);
+    int repr = lid;
+    if (nd.rnd[tid] > (i1 / (i1 + i2))) repr = rid;  // NaN ratio keeps the left child
+    nd.size[tid] = nd.size[lid] + nd.size[rid];
+    nd.mat[tid] = nd.mat[repr];
+    nd.rad[tid] = f3{rl.x + rr.x, rl.y + rr.y, rl.z + rr.z};
+    nd.dir[tid] = nd.dir[repr];
+    nd.pos[tid] = nd.pos[repr];
+    nd.nrm[tid] = nd.nrm[repr];
+    if (nd.anti) {
+        const f3 al = nd.anti[lid], ar = nd.anti[rid];
+        nd.anti[tid] = f3{al.x + ar.x, al.y + ar.y, al.z + ar.z};
+    }
+}
+
+// kernel_innerNodes (bvh.cu:280-316).  The reference reads the sibling's box
+// without ordering (quirk Q3); here the arrival counter is an agent-scope
+// acq_rel RMW: the first arrival's writes are released before its increment
+// and acquired by the second arrival before it reads them.
+template <bool kCluster>
+__global__ __launch_bounds__(kBlock) void k_refit(arc_bvh_node* __restrict__ nodes,
+                                                  const int* __restrict__ parents,
+                                                  const f3* __restrict__ spos, int n, NodeData nd) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int parent = parents[i];
+    while (parent >= 0) {
+        const int old = __hip_atomic_fetch_add(&nodes[parent].visited, 1, __ATOMIC_ACQ_REL,
+                                               __HIP_MEMORY_SCOPE_AGENT);
+        if (old == 0) return;
+        const int l = nodes[parent].left, r = nodes[parent].right;
+        f3 lmin, lmax, rmin, rmax;
+        if (l < 0) {
+            const arc_bvh_node& c = nodes[-(l + 1)];
+            lmin = f3{c.bbMin.x, c.bbMin.y, c.bbMin.z};
+            lmax = f3{c.bbMax.x, c.bbMax.y, c.bbMax.z};
+        } else {
+            lmin = lmax = spos[l];
+        }
+        if (r < 0) {
+            const arc_bvh_node& c = nodes[-(r + 1)];
+            rmin = f3{c.bbMin.x, c.bbMin.y, c.bbMin.z};
+            rmax = f3{c.bbMax.x, c.bbMax.y, c.bbMax.z};
+        } else {
+            rmin = rmax = spos[r];
+        }
+        nodes[parent].bbMin = arc_float3{fminf(lmin.x, rmin.x), fminf(lmin.y, rmin.y), fminf(lmin.z, rmin.z)};
+        nodes[parent].bbMax = arc_float3{fmaxf(lmax.x, rmax.x), fmaxf(lmax.y, rmax.y), fmaxf(lmax.z, rmax.z)};
+        if (kCluster) cluster(parent, l, r, n, nd);
+        parent = parents[n + parent];
+    }
+}
+
+inline int blocks_for(int n) { return (n + kBlock - 1) / kBlock; }
+
+int ensure_tree(arc_bvh* b, int n) {
+    arc_ctx* ctx = b->ctx;
+    ARC_TRY(arc::ensure(ctx, b->codes, sizeof(uint32_t) * (size_t)n));
+    ARC_TRY(arc::ensure(ctx, b->perm, sizeof(int) * (size_t)n));
+    ARC_TRY(arc::ensure(ctx, b->nodes, sizeof(arc_bvh_node) * (size_t)(n - 1)));
+    ARC_TRY(arc::ensure(ctx, b->parents, sizeof(int) * (size_t)(2 * n - 1)));
+    ARC_TRY(arc::ensure(ctx, b->ranges, sizeof(int) * (size_t)(2 * (n - 1))));
+    ARC_TRY(arc::ensure(ctx, b->positions, sizeof(f3) * (size_t)n));
+    ARC_TRY(arc::ensure(ctx, b->normals, sizeof(f3) * (size_t)n));
+    ARC_TRY(arc::ensure(ctx, b->bounds, sizeof(float) * 16));
+    ARC_TRY(arc::ensure(ctx, b->partials, sizeof(float) * 12 * kReduceBlocks));
+    return ARC_OK;
+}
+
+// Bounds + encode + sort (Bvh::create up to sort(), bvh.cu:346-374).
+int encode_and_sort(arc_bvh* b, const f3* pos, const f3* nrm, int n) {
+    arc_ctx* ctx = b->ctx;
+    const int nparts = std::min(kReduceBlocks, std::max(1, blocks_for(n)));
+    float* bounds = b->bounds.as<float>();
+    hipLaunchKernelGGL(k_bounds_partial, dim3(nparts), dim3(kBlock), 0, ctx->stream, pos, nrm, n,
+                       b->partials.as<float>());
+    hipLaunchKernelGGL(k_bounds_final, dim3(1), dim3(64), 0, ctx->stream, b->partials.as<float>(),
+                       nparts, bounds);
+    hipLaunchKernelGGL(k_encode, dim3(blocks_for(n)), dim3(kBlock), 0, ctx->stream, pos, nrm, n,
+                       bounds, b->consider_normals && nrm ? 1 : 0, b->codes.as<uint32_t>(),
+                       b->perm.as<int>());
+    ARC_LAUNCHED(ctx);
+    const int key_bits = (b->consider_normals && nrm) ? 32 : 30;
+    return arc::sort_pairs(ctx, b->sort, b->codes.as<uint32_t>(), b->perm.as<int>(), n, key_bits);
+}
+
+int build_tree(arc_bvh* b, int n, const NodeData* nd) {
+    arc_ctx* ctx = b->ctx;
+    hipLaunchKernelGGL(k_karras, dim3(blocks_for(n - 1)), dim3(kBlock), 0, ctx->stream,
+                       b->codes.as<uint32_t>(), n - 1, b->nodes.as<arc_bvh_node>(), b->parents.as<int>(),
+                       b->ranges.as<int>());
+    if (nd)
+        hipLaunchKernelGGL(k_refit<true>, dim3(blocks_for(n)), dim3(kBlock), 0, ctx->stream,
+                           b->nodes.as<arc_bvh_node>(), b->parents.as<int>(), b->positions.as<f3>(), n, *nd);
+    else
+        hipLaunchKernelGGL(k_refit<false>, dim3(blocks_for(n)), dim3(kBlock), 0, ctx->stream,
+                           b->nodes.as<arc_bvh_node>(), b->parents.as<int>(), b->positions.as<f3>(), n,
+                           NodeData{});
+    ARC_LAUNCHED(ctx);
+    return ARC_OK;
+}
+
+}  // namespace
+
+namespace arc {
+
+int random_numbers(arc_ctx* ctx, uint64_t seed, int count, float* out) {
+    if (count <= 0) return ARC_OK;
+    hipLaunchKernelGGL(k_random, dim3(blocks_for(count)), dim3(kBlock), 0, ctx->stream, seed, count, out);
+    ARC_LAUNCHED(ctx);
+    return ARC_OK;
+}
+
+int bvh_build_points(arc_bvh* b, const f3* pos, const f3* nrm, int n) {
+    if (n <= 1) {
+        b->n = 0;
+        return fail(b->ctx, ARC_E_TOO_FEW_POINTS, "not enough points for bvh construction");
+    }
+    if (b->consider_normals && !nrm) return fail(b->ctx, ARC_E_INVALID, "normals required");
+    ARC_TRY(ensure_tree(b, n));
+    ARC_TRY(encode_and_sort(b, pos, nrm, n));
+    hipLaunchKernelGGL(k_permute_points, dim3(blocks_for(n)), dim3(kBlock), 0, b->ctx->stream,
+                       b->perm.as<int>(), pos, nrm, n, b->positions.as<f3>(), b->normals.as<f3>());
+    ARC_LAUNCHED(b->ctx);
+    ARC_TRY(build_tree(b, n, nullptr));
+    b->n = n;
+    b->has_node_data = false;
+    return ARC_OK;
+}
+
+int bvh_build_avpls(arc_bvh* b, const AvplsDev& a, const float* rand, uint64_t seed, bool validate) {
+    arc_ctx* ctx = b->ctx;
+    const int n = a.n;
+    if (n <= 1) {
+        b->n = 0;
+        return fail(ctx, ARC_E_TOO_FEW_POINTS, "not enough points for bvh construction");
+    }
+    ARC_TRY(ensure_tree(b, n));
+    const size_t slots = (size_t)(2 * n - 1);
+    ARC_TRY(ensure(ctx, b->nd_size, sizeof(int) * slots));
+    ARC_TRY(ensure(ctx, b->nd_mat, sizeof(int) * slots));
+    ARC_TRY(ensure(ctx, b->nd_rand, sizeof(float) * slots));
+    ARC_TRY(ensure(ctx, b->nd_pos, sizeof(f3) * slots));
+    ARC_TRY(ensure(ctx, b->nd_nrm, sizeof(f3) * slots));
+    ARC_TRY(ensure(ctx, b->nd_rad, sizeof(f3) * slots));
+    ARC_TRY(ensure(ctx, b->nd_dir, sizeof(f3) * slots));
+    ARC_TRY(ensure(ctx, b->nd_anti, sizeof(f3) * slots));
+    if (validate) {
+        int* flag = reinterpret_cast<int*>(b->bounds.as<float>() + 12);
+        ARC_HIP(ctx, hipMemsetAsync(flag, 0, sizeof(int), ctx->stream));
+        hipLaunchKernelGGL(k_check_bounce, dim3(blocks_for(n)), dim3(kBlock), 0, ctx->stream, a.bounce, n, flag);
+        ARC_LAUNCHED(ctx);
+        int host_flag = 0;
+        ARC_HIP(ctx, hipMemcpyAsync(&host_flag, flag, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+        ARC_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        if (host_flag) {
+            b->n = 0;
+            return fail(ctx, ARC_E_INVALID, "AVPL with bounce <= 0: leaves would not match node data");
+        }
+    }
+    // AvplBvh passes considerNormals = false from CudaGather (cudaGather.cu:858);
+    // a Bvh created with normals considered keeps 6D codes here as well.
+    ARC_TRY(encode_and_sort(b, a.pos, a.nrm, n));
+    if (rand) {
+        ARC_HIP(ctx, hipMemcpyAsync(b->nd_rand.p, rand, sizeof(float) * slots, hipMemcpyDeviceToDevice,
+                                    ctx->stream));
+    } else {
+        ARC_TRY(random_numbers(ctx, seed, (int)slots, b->nd_rand.as<float>()));
+    }
+    NodeData nd{b->nd_size.as<int>(), b->nd_mat.as<int>(), b->nd_rand.as<float>(), b->nd_pos.as<f3>(),
+                b->nd_nrm.as<f3>(),   b->nd_rad.as<f3>(), b->nd_dir.as<f3>(),    b->nd_anti.as<f3>()};
+    hipLaunchKernelGGL(k_permute_avpls, dim3(blocks_for(n)), dim3(kBlock), 0, ctx->stream, b->perm.as<int>(),
+                       a, b->positions.as<f3>(), b->normals.as<f3>(), nd);
+    ARC_LAUNCHED(ctx);
+    ARC_TRY(build_tree(b, n, &nd));
+    b->n = n;
+    b->has_node_data = true;
+    return ARC_OK;
+}
+
+}  // namespace arc

@@ -1,0 +1,253 @@
+"""Python host objects over the C ABI (device memory held in torch tensors).
+
+Mirrors the reference's host classes for this path: ``Bvh`` covers
+``SimpleBvh`` / ``AvplBvh`` (bvh.h:53-161) and ``Gather`` covers
+``CudaGather`` (cudaGather.h:25-70).  All compute runs in libarc.so; torch
+only provides device buffers and the stream.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import ffi
+from .ffi import check
+
+NODE_DTYPE = np.dtype([("left", "<i4"), ("right", "<i4"), ("bbMin", "<f4", 3), ("bbMax", "<f4", 3),
+                       ("visited", "<i4")])
+assert NODE_DTYPE.itemsize == 36
+
+
+def _torch():
+    import torch  # local import: CPU-only tests never need it
+
+    return torch
+
+
+class Context:
+    """arc_ctx bound to torch's current stream on ``cuda:device``."""
+
+    def __init__(self, device: int = 0, use_torch_stream: bool = True):
+        torch = _torch()
+        if not torch.cuda.is_available():
+            raise RuntimeError("no GPU visible: the HIP path has no CPU fallback")
+        self.lib = ffi.lib()
+        self.device = device
+        h = C.c_void_p()
+        check(self.lib.arc_ctx_create(device, C.byref(h)), "arc_ctx_create")
+        self.h = h
+        if use_torch_stream:
+            s = torch.cuda.current_stream(device).cuda_stream
+            check(self.lib.arc_ctx_set_stream(self.h, C.c_void_p(s)), "arc_ctx_set_stream", self.h)
+
+    def sync(self) -> None:
+        check(self.lib.arc_ctx_synchronize(self.h), "arc_ctx_synchronize", self.h)
+
+    def close(self) -> None:
+        if self.h:
+            self.lib.arc_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):  # pragma: no cover - best effort
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def to_host(self, ptr: int, dtype, count: int) -> np.ndarray:
+        out = np.empty(count, dtype=dtype)
+        if count:
+            check(self.lib.arc_copy_to_host(self.h, out.ctypes.data_as(C.c_void_p), C.c_void_p(ptr), out.nbytes),
+                  "arc_copy_to_host", self.h)
+        return out
+
+
+def device_array(a: np.ndarray, device: int = 0):
+    torch = _torch()
+    return torch.from_numpy(np.ascontiguousarray(a)).to(f"cuda:{device}")
+
+
+@dataclass
+class DeviceAvpls:
+    """AVPL SoA in device memory (AvplsGpuParam layout)."""
+
+    tensors: dict
+    struct: ffi.arc_avpls
+
+    @classmethod
+    def from_scene(cls, scene, device: int = 0) -> "DeviceAvpls":
+        return cls.from_arrays(scene.avpl_pos, scene.avpl_nrm, scene.avpl_dir, scene.avpl_rad, scene.avpl_anti,
+                               scene.avpl_mat, scene.avpl_bounce, device)
+
+    @classmethod
+    def from_arrays(cls, pos, nrm, dirs, rad, anti, mat, bounce, device: int = 0) -> "DeviceAvpls":
+        t = {k: device_array(v, device) for k, v in
+             dict(pos=pos, nrm=nrm, dir=dirs, rad=rad, anti=anti, mat=mat.astype(np.int32),
+                  bounce=bounce.astype(np.int32)).items()}
+        s = ffi.arc_avpls()
+        s.position, s.normal, s.incDirection = t["pos"].data_ptr(), t["nrm"].data_ptr(), t["dir"].data_ptr()
+        s.incRadiance, s.antiradiance = t["rad"].data_ptr(), t["anti"].data_ptr()
+        s.materialIndex, s.bounce = t["mat"].data_ptr(), t["bounce"].data_ptr()
+        s.numAvpls = int(pos.shape[0])
+        return cls(t, s)
+
+
+class Bvh:
+    """Bvh / SimpleBvh / AvplBvh over the C ABI."""
+
+    def __init__(self, ctx: Context, consider_normals: bool = False):
+        self.ctx = ctx
+        self.lib = ctx.lib
+        h = C.c_void_p()
+        check(self.lib.arc_bvh_create(ctx.h, int(consider_normals), C.byref(h)), "arc_bvh_create", ctx.h)
+        self.h = h
+        self.avpl = False
+
+    def close(self) -> None:
+        if self.h:
+            self.lib.arc_bvh_destroy(self.h)
+            self.h = None
+
+    def build_points(self, pos, nrm=None) -> None:
+        """SimpleBvh(positions, normals) on device tensors (N, 3) float32."""
+        n = int(pos.shape[0])
+        st = self.lib.arc_bvh_build_points(self.h, C.c_void_p(pos.data_ptr()),
+                                           C.c_void_p(nrm.data_ptr()) if nrm is not None else None, n, 0)
+        check(st, "arc_bvh_build_points", self.ctx.h)
+        self.avpl = False
+
+    def build_avpls(self, avpls: DeviceAvpls, rand=None, seed: int = 5489, trust: bool = False) -> None:
+        st = self.lib.arc_bvh_build_avpls(self.h, C.byref(avpls.struct),
+                                          C.c_void_p(rand.data_ptr()) if rand is not None else None, seed,
+                                          ffi.ARC_TRUST_INPUT if trust else 0)
+        check(st, "arc_bvh_build_avpls", self.ctx.h)
+        self.avpl = True
+
+    def to_host(self) -> dict:
+        """Copies every output array to numpy (synchronous)."""
+        p, d, nd = ffi.arc_bvh_param(), ffi.arc_bvh_data(), ffi.arc_avpl_node_data()
+        check(self.lib.arc_bvh_get_param(self.h, C.byref(p)), "arc_bvh_get_param", self.ctx.h)
+        check(self.lib.arc_bvh_get_data(self.h, C.byref(d)), "arc_bvh_get_data", self.ctx.h)
+        check(self.lib.arc_bvh_get_node_data(self.h, C.byref(nd)), "arc_bvh_get_node_data", self.ctx.h)
+        n = p.numLeafs
+        h = self.ctx.to_host
+        out = dict(
+            n=n,
+            codes=h(d.morton, np.uint32, n),
+            perm=h(d.perm, np.int32, n),
+            nodes=h(p.nodes, NODE_DTYPE, n - 1),
+            parents=h(d.parents, np.int32, 2 * n - 1),
+            ranges=h(d.ranges, np.int32, 2 * (n - 1)).reshape(-1, 2),
+            positions=h(p.positions, np.float32, 3 * n).reshape(n, 3),
+            normals=h(p.normals, np.float32, 3 * n).reshape(n, 3),
+        )
+        if nd.size:
+            s = 2 * n - 1
+            out.update(
+                nd_size=h(nd.size, np.int32, s), nd_mat=h(nd.materialIndex, np.int32, s),
+                nd_rand=h(nd.randomNumbers, np.float32, s),
+                nd_pos=h(nd.position, np.float32, 3 * s).reshape(s, 3),
+                nd_nrm=h(nd.normal, np.float32, 3 * s).reshape(s, 3),
+                nd_rad=h(nd.incRadiance, np.float32, 3 * s).reshape(s, 3),
+                nd_dir=h(nd.incDirection, np.float32, 3 * s).reshape(s, 3),
+            )
+        return out
+
+    def check_integrity(self) -> int:
+        ms = C.c_int(0)
+        check(self.lib.arc_bvh_check_integrity(self.h, C.byref(ms)), "arc_bvh_check_integrity", self.ctx.h)
+        return ms.value
+
+
+def default_params(**kw) -> ffi.arc_gather_params:
+    p = ffi.arc_gather_params()
+    ffi.lib().arc_gather_params_default(C.byref(p))
+    for k, v in kw.items():
+        if not hasattr(p, k):
+            raise AttributeError(k)
+        setattr(p, k, v)
+    return p
+
+
+def materials_struct(mats: np.ndarray):
+    arr = (ffi.arc_material * len(mats))()
+    for i, m in enumerate(np.asarray(mats, np.float32)):
+        arr[i].emissive = ffi.arc_float3(*m[0:3])
+        arr[i].diffuse = ffi.arc_float3(*m[3:6])
+        arr[i].specular = ffi.arc_float3(*m[6:9])
+        arr[i].exponent = float(m[9])
+    return arr
+
+
+class Gather:
+    """CudaGather over the C ABI, with the G-buffer and targets as device tensors."""
+
+    def __init__(self, ctx: Context, width: int, height: int, materials: np.ndarray):
+        torch = _torch()
+        self.ctx, self.lib = ctx, ctx.lib
+        self.width, self.height = width, height
+        mats = materials_struct(materials)
+        h = C.c_void_p()
+        check(self.lib.arc_gather_create(ctx.h, width, height, mats, len(materials), C.byref(h)),
+              "arc_gather_create", ctx.h)
+        self.h = h
+        dev = f"cuda:{ctx.device}"
+        npix = width * height
+        self.result = torch.zeros((npix, 4), dtype=torch.float32, device=dev)
+        self.radiance = torch.zeros((npix, 4), dtype=torch.float32, device=dev)
+        self.antiradiance = torch.zeros((npix, 4), dtype=torch.float32, device=dev)
+        check(self.lib.arc_gather_set_outputs(self.h, C.c_void_p(self.result.data_ptr()),
+                                              C.c_void_p(self.radiance.data_ptr()),
+                                              C.c_void_p(self.antiradiance.data_ptr())),
+              "arc_gather_set_outputs", ctx.h)
+        self._gb = None
+
+    def close(self) -> None:
+        if self.h:
+            self.lib.arc_gather_destroy(self.h)
+            self.h = None
+
+    def set_gbuffer(self, pos4, nrm4, albedo4=None) -> None:
+        self._gb = (pos4, nrm4, albedo4)
+        check(self.lib.arc_gather_set_gbuffer(self.h, C.c_void_p(pos4.data_ptr()), C.c_void_p(nrm4.data_ptr()),
+                                              C.c_void_p(albedo4.data_ptr()) if albedo4 is not None else None),
+              "arc_gather_set_gbuffer", self.ctx.h)
+
+    def run(self, avpls: DeviceAvpls, cam, extent: float, params=None, flags: int = 0) -> ffi.arc_gather_stats:
+        params = params or default_params()
+        stats = ffi.arc_gather_stats()
+        st = self.lib.arc_gather_run(self.h, C.byref(avpls.struct), ffi.arc_float3(*map(float, cam)),
+                                     float(extent), C.byref(params), flags, C.byref(stats))
+        check(st, "arc_gather_run", self.ctx.h)
+        return stats
+
+    def exact_f64(self, avpls: DeviceAvpls, cam, extent: float, params=None):
+        torch = _torch()
+        params = params or default_params()
+        dev = f"cuda:{self.ctx.device}"
+        npix = self.width * self.height
+        rad = torch.zeros((npix, 3), dtype=torch.float64, device=dev)
+        anti = torch.zeros((npix, 3), dtype=torch.float64, device=dev)
+        st = self.lib.arc_gather_exact_f64(self.h, C.byref(avpls.struct), ffi.arc_float3(*map(float, cam)),
+                                           float(extent), C.byref(params), 0, C.c_void_p(rad.data_ptr()),
+                                           C.c_void_p(anti.data_ptr()))
+        check(st, "arc_gather_exact_f64", self.ctx.h)
+        return rad, anti
+
+    def bvh_to_host(self) -> dict:
+        h = self.lib.arc_gather_get_bvh(self.h)
+        if not h:
+            raise RuntimeError("no cut frame has run")
+        b = Bvh.__new__(Bvh)
+        b.ctx, b.lib, b.h, b.avpl = self.ctx, self.lib, C.c_void_p(h), True
+        return b.to_host()
+
+
+def photon_radius(params, extent: float) -> float:
+    return float(np.float32(params.photonRadiusScale) * np.float32(extent) / np.float32(100.0))
+
+
+CONE_10DEG = math.pi / 18.0

@@ -1,0 +1,256 @@
+/*
+ * arc.h — C ABI of the MI355X-native AVPL gather and Karras LBVH.
+ *
+ * Drop-in boundary for the reference's CUDA hot path.  Each declaration names
+ * the reference interface it replaces (file:line, flosmn/AntiradianceCuts,
+ * Framework/).  Plain pointers and sizes only: device pointers are HIP device
+ * addresses on the context's device; host pointers are marked as such.
+ *
+ * Errors: every entry point returns ARC_OK (0) or a negative ARC_E_* code; the
+ * reference ignored errors in release builds (CudaResources/cudaUtil.hpp:32-34)
+ * and signalled traversal failures in-band; here both are reported, and the
+ * in-band sentinel pixel is kept as an option (arc_gather_params.writeSentinel).
+ *
+ * Threading: one arc_ctx per host thread.  A context owns a HIP stream (or
+ * borrows one, arc_ctx_set_stream); all work of objects made from it is
+ * ordered on that stream.  Entry points are asynchronous with respect to the
+ * host unless stated otherwise; arc_ctx_synchronize waits.
+ */
+#ifndef ARC_H_
+#define ARC_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---------------------------------------------------------------- status */
+#define ARC_OK 0
+#define ARC_E_INVALID (-1)         /* bad argument                                  */
+#define ARC_E_TOO_FEW_POINTS (-2)  /* n <= 1: "not enough points" (bvh.cu:327-330)  */
+#define ARC_E_HIP (-3)             /* HIP runtime error (see arc_ctx_last_error)    */
+#define ARC_E_NOMEM (-4)           /* device allocation failed                      */
+#define ARC_E_NO_DEVICE (-5)       /* no HIP device                                 */
+#define ARC_E_UNSUPPORTED (-6)     /* combination not implemented                   */
+#define ARC_E_STATE (-7)           /* object not ready (e.g. no G-buffer bound)     */
+
+/* ------------------------------------------------------------ POD types */
+/* CUDA float3 layout (12 bytes). */
+typedef struct arc_float3 { float x, y, z; } arc_float3;
+
+/* MATERIAL (Material.h:6-12), glm::vec3 members: 40 bytes. */
+typedef struct arc_material {
+    arc_float3 emissive, diffuse, specular;
+    float exponent;
+} arc_material;
+
+/* BvhNode (bvh.h:21-28): child >= 0 is a leaf (sorted slot), child < 0 is
+ * internal node -(child + 1). */
+typedef struct arc_bvh_node {
+    int left, right;
+    arc_float3 bbMin, bbMax;
+    int visited;
+} arc_bvh_node;
+
+/* AvplsGpuParam (data_gpu.h:14-24): SoA of numAvpls entries. */
+typedef struct arc_avpls {
+    const arc_float3* position;
+    const arc_float3* normal;
+    const arc_float3* incDirection;
+    const arc_float3* incRadiance;
+    const arc_float3* antiradiance;
+    const int* materialIndex;
+    const int* bounce;
+    int numAvpls;
+} arc_avpls;
+
+/* BvhParam (bvh.h:44-51): device pointers owned by the arc_bvh. */
+typedef struct arc_bvh_param {
+    int numLeafs;
+    int numNodes;
+    arc_bvh_node* nodes;      /* [numNodes]                        */
+    arc_float3* positions;    /* [numLeafs] leaf positions, sorted */
+    arc_float3* normals;      /* [numLeafs] leaf normals, sorted   */
+} arc_bvh_param;
+
+/* BvhData (bvh.h:30-36) plus the outputs the reference keeps transient. */
+typedef struct arc_bvh_data {
+    uint32_t* morton;         /* [numLeafs] sorted codes (BvhData::morton)            */
+    int* parents;             /* [numLeafs + numNodes] leaves then internal nodes     */
+    int numLeafs;
+    int numNodes;
+    int* perm;                /* [numLeafs] input index of each sorted slot           */
+    int* ranges;              /* [2 * numNodes] first, last leaf of each node (bvh.cu:222) */
+} arc_bvh_data;
+
+/* AvplBvhNodeDataParam (bvh.h:104-113): SoA over 2 * numLeafs - 1 slots,
+ * leaves at [0, numLeafs), internal node k at numLeafs + k. */
+typedef struct arc_avpl_node_data {
+    int* size;
+    int* materialIndex;
+    float* randomNumbers;
+    arc_float3* position;
+    arc_float3* normal;
+    arc_float3* incRadiance;
+    arc_float3* incDirection;
+} arc_avpl_node_data;
+
+/* --------------------------------------------------------------- context */
+typedef struct arc_ctx arc_ctx;
+
+/* CudaContext (CudaResources/cudaContext.hpp:15-45) picks the device; here
+ * the caller names it. */
+int arc_ctx_create(int device, arc_ctx** out);
+void arc_ctx_destroy(arc_ctx* ctx);
+/* Order all work on `hip_stream` (a hipStream_t); NULL restores the
+ * context's own stream. */
+int arc_ctx_set_stream(arc_ctx* ctx, void* hip_stream);
+void* arc_ctx_get_stream(arc_ctx* ctx);
+int arc_ctx_synchronize(arc_ctx* ctx);
+const char* arc_ctx_last_error(arc_ctx* ctx);
+const char* arc_status_string(int status);
+/* Library version string, e.g. "arc 0.1.0 gfx950". */
+const char* arc_version(void);
+
+/* Device memory helpers so host code needs no HIP headers (CudaBuffer,
+ * CudaResources/cudaBuffer.hpp:11-49).  Copies are synchronous. */
+int arc_device_alloc(arc_ctx* ctx, size_t bytes, void** out);
+int arc_device_free(arc_ctx* ctx, void* ptr);
+int arc_copy_to_device(arc_ctx* ctx, void* dst, const void* host_src, size_t bytes);
+int arc_copy_to_host(arc_ctx* ctx, void* host_dst, const void* src, size_t bytes);
+
+/* ------------------------------------------------------------------ LBVH */
+typedef struct arc_bvh arc_bvh;
+
+#define ARC_BVH_CONSIDER_NORMALS 1   /* 6D position+normal codes, truncated to 32 bits (bvh.h:17) */
+#define ARC_INPUT_HOST 256           /* input pointers are host memory (copied in)               */
+#define ARC_TRUST_INPUT 512          /* skip the synchronous bounce > 0 check of AVPL inputs      */
+
+/* Bvh(bool considerNormals) (bvh.h:56). */
+int arc_bvh_create(arc_ctx* ctx, int consider_normals, arc_bvh** out);
+void arc_bvh_destroy(arc_bvh* bvh);
+
+/* SimpleBvh(positions, normals, considerNormals) (bvh.cu:859-877) followed by
+ * Bvh::create (bvh.cu:325-454): bounds, normalise, Morton encode, stable key
+ * sort, Karras radix tree, bottom-up AABB refit.  `normals` may be NULL when
+ * the Bvh does not consider normals.  flags: ARC_INPUT_HOST.
+ * Returns ARC_E_TOO_FEW_POINTS when n <= 1. */
+int arc_bvh_build_points(arc_bvh* bvh, const arc_float3* positions, const arc_float3* normals,
+                         int n, int flags);
+
+/* AvplBvh(avpls, considerNormals) (bvh.cu:746-774): leaves are the AVPLs,
+ * node data (AvplBvhNodeData, bvh.cu:673-724) is permuted with the sort and
+ * internal nodes get cluster representatives (bvh.cu:726-744) drawn with
+ * rand[slot] (2n-1 floats in [0,1), device or host per flags).  When rand is
+ * NULL the numbers come from a counter-based generator keyed by `seed`
+ * (arc_bvh_random_numbers states it).  Every AVPL must have bounce > 0:
+ * the reference builds leaves from bounce > 0 but node data from all AVPLs
+ * (bvh.cu:753-759 vs 771); a bounce-0 AVPL returns ARC_E_INVALID. */
+int arc_bvh_build_avpls(arc_bvh* bvh, const arc_avpls* avpls, const float* rand, uint64_t seed,
+                        int flags);
+
+/* The generator used when rand is NULL: writes 2n-1 floats in [0,1) for
+ * `seed` into `out` (device memory). */
+int arc_bvh_random_numbers(arc_ctx* ctx, uint64_t seed, int count, float* out);
+
+/* Bvh::getBvhParam (bvh.h:72), getBvhData (bvh.h:73),
+ * AvplBvh::getAvplBvhNodeDataParam (bvh.h:141).  Pointers are borrowed and
+ * stay valid until the next build or destroy.  Node data is all-NULL for a
+ * points-only build. */
+int arc_bvh_get_param(arc_bvh* bvh, arc_bvh_param* out);
+int arc_bvh_get_data(arc_bvh* bvh, arc_bvh_data* out);
+int arc_bvh_get_node_data(arc_bvh* bvh, arc_avpl_node_data* out);
+
+/* Bvh::getNode (bvh.cu:660-667): copies one node to the host (synchronous). */
+int arc_bvh_get_node(arc_bvh* bvh, int i, arc_bvh_node* out);
+
+/* Bvh::checkTreeIntegrity (bvh.cu:568-639) on the device arrays (copied to
+ * the host, synchronous).  Returns ARC_OK when every node is reached once,
+ * every internal node has two children and parents agree; ARC_E_STATE
+ * otherwise.  *max_stack receives the largest DFS stack depth. */
+int arc_bvh_check_integrity(arc_bvh* bvh, int* max_stack);
+
+/* ---------------------------------------------------------------- gather */
+typedef struct arc_gather arc_gather;
+
+#define ARC_CUT_SOLID_ANGLE 0   /* kernel_bvh: Omega < clusterRefinementThreshold (cudaGather.cu:410-414) */
+#define ARC_CUT_ERROR_BOUND 1   /* Lightcuts-style: refine while bound > errorBound * estimate           */
+
+#define ARC_ANTIRAD_DISK 0      /* getAntiradiance, cudaGather.cu:109-144                 */
+#define ARC_ANTIRAD_CONE 1      /* GetAntiradiance, Shaders/Gather.frag:149-188 x BRDF    */
+
+/* The CONF_VARS fields the hot path reads (CConfigManager.h:6-52), plus the
+ * extension switches.  arc_gather_params_default fills the reference's
+ * defaults (CConfigManager.cpp:14-37). */
+typedef struct arc_gather_params {
+    int useLightCuts;                 /* 1: cut radiance (kernel_bvh), 0: exact (kernel_radiance_simple) */
+    int cutMode;                      /* ARC_CUT_*                                   */
+    float clusterRefinementThreshold; /* solid-angle threshold, default pi/10        */
+    float errorBound;                 /* ARC_CUT_ERROR_BOUND relative bound, 0.01    */
+    int antiradianceModel;            /* ARC_ANTIRAD_*                               */
+    int antiradianceCut;              /* 1: antiradiance over the LBVH (ERROR_BOUND) */
+    float photonRadiusScale;          /* disk radius = scale * sceneExtent / 100     */
+    float coneHalfAngle;              /* radians, CONE model                         */
+    int writeSentinel;                /* 1: (1e5, 0, 1e5, 1) on traversal error      */
+    int stackSize;                    /* traversal stack entries, 32 in kernel_bvh   */
+    uint64_t seed;                    /* cluster-representative generator seed       */
+} arc_gather_params;
+
+void arc_gather_params_default(arc_gather_params* p);
+
+typedef struct arc_gather_stats {
+    /* phase times in ms (ARC_RUN_PROFILE), as the CudaTimer prints of run() */
+    float msUpload, msBvh, msRadiance, msAntiradiance, msCombine, msTotal;
+    /* work counters (ARC_RUN_COUNTERS) */
+    long long visitedNodes;           /* inner nodes tested                        */
+    long long evaluatedNodes;         /* (pixel, node or leaf) radiance evaluations */
+    long long antiVisitedNodes;
+    long long antiEvaluatedNodes;
+    long long errorPixels;            /* traversal stack overflows                 */
+    int numLeafs;                     /* AVPLs in the LBVH of this frame            */
+} arc_gather_stats;
+
+#define ARC_RUN_PROFILE 1      /* time each phase with HIP events (synchronises)  */
+#define ARC_RUN_COUNTERS 2     /* count visited / evaluated nodes                  */
+/* ARC_INPUT_HOST (256): the arc_avpls pointers are host memory, uploaded as
+ * AvplsGpu does (data_gpu.cu:5-52). */
+
+/* CudaGather(camera, pos, normal, out x3, materials, ...) (cudaGather.h:28-35,
+ * cudaGather.cu:805-829).  Materials are host memory, at most 256 entries. */
+int arc_gather_create(arc_ctx* ctx, int width, int height, const arc_material* materials,
+                      int num_materials, arc_gather** out);
+void arc_gather_destroy(arc_gather* g);
+
+/* The G-buffer textures of the constructor: float4 (x, y, z, 1) positions and
+ * float4 (nx, ny, nz, materialIndex) normals, width * height each, row-major;
+ * albedo (float4, may be NULL) overrides the pixel material's diffuse colour. */
+int arc_gather_set_gbuffer(arc_gather* g, const float* pos4, const float* nrm4, const float* albedo4);
+
+/* The output textures of the constructor, float4 each, any may be NULL:
+ * result = radiance - antiradiance (kernel_combine, cudaGather.cu:782-803). */
+int arc_gather_set_outputs(arc_gather* g, float* result4, float* radiance4, float* antiradiance4);
+
+/* CudaGather::run (cudaGather.cu:836-1051): radiance gather (exact, or cut
+ * over an AvplBvh rebuilt from this frame's AVPLs), antiradiance gather with
+ * photonRadius = photonRadiusScale * sceneExtent / 100, combine.
+ * flags: ARC_RUN_PROFILE | ARC_RUN_COUNTERS | ARC_INPUT_HOST.  stats may be
+ * NULL. */
+int arc_gather_run(arc_gather* g, const arc_avpls* avpls, arc_float3 camPos, float sceneExtent,
+                   const arc_gather_params* params, int flags, arc_gather_stats* stats);
+
+/* Exact gather in FP64 (the reference image for error metrics): writes
+ * radiance and antiradiance as 3 doubles per pixel (device memory).
+ * Not in the reference; the reference's exact kernels are FP32. */
+int arc_gather_exact_f64(arc_gather* g, const arc_avpls* avpls, arc_float3 camPos, float sceneExtent,
+                         const arc_gather_params* params, int flags, double* radiance3, double* antiradiance3);
+
+/* The AvplBvh of the last cut frame (borrowed; NULL before the first). */
+arc_bvh* arc_gather_get_bvh(arc_gather* g);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* ARC_H_ */

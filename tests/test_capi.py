@@ -1,0 +1,61 @@
+"""The C ABI library loads and exports every symbol its headers declare.
+
+CPU-only: no compute entry point is called (no GPU here); only the pure host
+functions (version, status strings, parameter defaults) run.
+"""
+import ctypes
+import math
+import re
+from pathlib import Path
+
+import pytest
+
+from antiradiancecuts_amd import ffi
+
+ROOT = Path(__file__).resolve().parents[1]
+DECL = re.compile(r"^\s*(?:const\s+)?[A-Za-z_][\w\s\*]*?\b(arc_\w+)\s*\(", re.M)
+
+
+def declared(header: str):
+    text = (ROOT / "include" / header).read_text()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(DECL.findall(text)))
+
+
+def test_arc_h_symbols_exported():
+    names = declared("arc.h")
+    assert len(names) >= 25
+    lib = ctypes.CDLL(str(ffi.LIB_PATH))
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+    # the Python bindings cover the whole header
+    assert sorted(n for n, _, _ in ffi.ARC_SYMBOLS) == names
+
+
+def test_arc_scene_h_symbols_exported():
+    names = declared("arc_scene.h")
+    lib = ctypes.CDLL(str(ffi.SCENE_LIB_PATH))
+    assert not [n for n in names if not hasattr(lib, n)]
+    assert sorted(n for n, _, _ in ffi.SCENE_SYMBOLS) == names
+
+
+def test_host_only_entry_points():
+    L = ffi.lib()
+    assert L.arc_version().decode().startswith("arc ")
+    assert L.arc_status_string(ffi.ARC_E_TOO_FEW_POINTS).decode() == "not enough points for bvh construction"
+    assert L.arc_status_string(12345).decode() == "unknown status"
+    p = ffi.arc_gather_params()
+    L.arc_gather_params_default(ctypes.byref(p))
+    # CConfigManager.cpp:15,26,37 and STACK_SIZE (cudaGather.cu:19)
+    assert p.useLightCuts == 1
+    assert p.photonRadiusScale == pytest.approx(2.0)
+    assert p.clusterRefinementThreshold == pytest.approx(math.pi / 10, rel=1e-6)
+    assert p.stackSize == 32
+    assert p.antiradianceModel == ffi.ARC_ANTIRAD_DISK
+
+
+def test_struct_layouts_match_header():
+    assert ctypes.sizeof(ffi.arc_bvh_node) == 36
+    assert ctypes.sizeof(ffi.arc_material) == 40
+    assert ctypes.sizeof(ffi.arc_float3) == 12
+    assert ctypes.sizeof(ffi.arc_avpls) == 8 * 7 + 8

@@ -1,0 +1,168 @@
+"""Pins of the oracle's LBVH restatement (bvh.cu:73-454, 673-808).
+
+Codes are recomputed with numpy (IEEE float32 normalisation, fmax/fmin
+quantisation, naive bit spreading), the order with a stable argsort, the
+topology with an independent top-down builder, boxes and cluster data with
+brute force over each node's leaf range.
+"""
+import numpy as np
+import pytest
+
+from antiradiancecuts_amd import scene as S
+from tests import lbvh_checks as K
+
+FLT_MIN = np.float32(np.finfo(np.float32).tiny)
+
+
+def spread3(q):
+    q = q.astype(np.uint64)
+    r = np.zeros_like(q)
+    for i in range(10):
+        r |= ((q >> np.uint64(i)) & np.uint64(1)) << np.uint64(3 * i)
+    return r
+
+
+def expected_codes(pos):
+    pos = pos.astype(np.float32)
+    mn = np.fmin.reduce(np.vstack([pos, np.full((1, 3), np.finfo(np.float32).max, np.float32)]), 0)
+    mx = np.fmax.reduce(np.vstack([pos, np.full((1, 3), FLT_MIN, np.float32)]), 0)
+    with np.errstate(invalid="ignore", divide="ignore"):
+        c = (pos - mn) / (mx - mn)
+        q = np.fmin(np.fmax(c * np.float32(1024.0), np.float32(0.0)), np.float32(1023.0)).astype(np.uint32)
+    code = spread3(q[:, 0]) | (spread3(q[:, 1]) << np.uint64(1)) | (spread3(q[:, 2]) << np.uint64(2))
+    return code.astype(np.uint32)
+
+
+def cases():
+    rng = np.random.default_rng(42)
+    yield "smoke8", S.SMOKE_POINTS
+    yield "two", rng.random((2, 3), dtype=np.float32)
+    yield "three", rng.random((3, 3), dtype=np.float32)
+    yield "uniform1k", rng.random((1000, 3), dtype=np.float32)
+    yield "uniform5k", rng.random((5000, 3), dtype=np.float32)
+    yield "dups", rng.random((20, 3), dtype=np.float32)[rng.integers(0, 20, 3000)]
+    yield "identical", np.tile(np.float32([[0.3, 0.2, 0.9]]), (257, 1))
+    line = np.linspace(0, 1, 777, dtype=np.float32)
+    yield "line", np.stack([line, line * 0.5, line * 0.25], 1)
+    yield "line_rev", np.stack([line, line, line], 1)[::-1].copy()
+    flat = rng.random((900, 3), dtype=np.float32)
+    flat[:, 2] = 0.5  # degenerate axis: 0/0 -> NaN -> quantised to 0
+    yield "flat", flat
+    yield "mixture", S.points(4000, seed=3)
+
+
+@pytest.mark.parametrize("name,pos", list(cases()), ids=[c[0] for c in cases()])
+def test_lbvh_build(orc, name, pos):
+    t = orc.lbvh_build(pos)
+    assert t["status"] == 0
+    want = expected_codes(pos)
+    perm = np.argsort(want, kind="stable")
+    np.testing.assert_array_equal(t["perm"], perm)
+    np.testing.assert_array_equal(t["codes"], want[perm])
+    np.testing.assert_array_equal(t["positions"], pos[perm])
+    K.check_topology(t["codes"], t["nodes"], t["parents"], t["ranges"])
+    K.check_karras_ranges(t["ranges"])
+    K.check_boxes(t["nodes"], t["ranges"], t["positions"])
+    np.testing.assert_array_equal(t["nodes"]["visited"], 2)
+    ms = orc.C.c_int(0) if hasattr(orc, "C") else None
+    import ctypes
+    ms = ctypes.c_int(0)
+    assert orc.lib().orc_check_tree_integrity(orc._p(t["nodes"]), orc._p(t["parents"]), len(pos),
+                                              ctypes.byref(ms)) == 0
+    assert ms.value >= 2
+
+
+def test_lbvh_too_few_points(orc):
+    assert orc.lbvh_build(np.zeros((1, 3), np.float32))["status"] == -1
+
+
+def test_bounds_flt_min_quirk(orc):
+    pts = np.array([[-1.0, 2.0, -5.0], [-3.0, 4.0, -7.0]], np.float32)
+    mn, mx = orc.bounds(pts)
+    np.testing.assert_array_equal(mn, [-3.0, 2.0, -7.0])
+    # max starts at FLT_MIN (a tiny positive number), not -FLT_MAX
+    np.testing.assert_array_equal(mx, [FLT_MIN, 4.0, FLT_MIN])
+
+
+def test_all_negative_axis_codes(orc):
+    # with max stuck at FLT_MIN the negative axis normalises into [0, ~1]
+    rng = np.random.default_rng(5)
+    pos = rng.random((500, 3), dtype=np.float32)
+    pos[:, 1] -= 2.0
+    t = orc.lbvh_build(pos)
+    np.testing.assert_array_equal(t["codes"], np.sort(expected_codes(pos), kind="stable"))
+
+
+def test_6d_codes_truncated(orc):
+    rng = np.random.default_rng(9)
+    pos = rng.random((600, 3), dtype=np.float32)
+    nrm = S.unit_normals(600, seed=9)
+    t = orc.lbvh_build(pos, nrm, consider_normals=True)
+    L = orc.lib()
+    mnp, mxp = orc.bounds(pos)
+    mnn, mxn = orc.bounds(nrm)
+    cp = (pos - mnp) / (mxp - mnp)
+    cn = (nrm - mnn) / (mxn - mnn)
+    want = np.array([L.orc_morton6d(*map(float, a), *map(float, b)) & 0xFFFFFFFF for a, b in zip(cp, cn)],
+                    np.uint32)
+    perm = np.argsort(want, kind="stable")
+    np.testing.assert_array_equal(t["perm"], perm)
+    K.check_topology(t["codes"], t["nodes"], t["parents"], t["ranges"])
+
+
+def test_delta_definition(orc):
+    codes = np.array([0b0001, 0b0001, 0b0011, 0b1000], np.uint32)
+    L = orc.lib()
+    p = orc._p(codes)
+    assert L.orc_delta(0, -1, p, 3) == -1
+    assert L.orc_delta(0, 4, p, 3) == -1
+    assert L.orc_delta(0, 1, p, 3) == 32 + 31   # equal codes: 32 + clz(0 ^ 1)
+    assert L.orc_delta(0, 2, p, 3) == 30        # 0b01 ^ 0b11 = 0b10
+    assert L.orc_delta(2, 3, p, 3) == 28
+    assert L.orc_delta(1, 1, p, 3) == 64
+
+
+def avpl_like(n, seed, zero_rad=False, bounce0=False):
+    sc = S.cornell_256(seed=seed, width=8, height=8, num_avpls=n)
+    if zero_rad:
+        sc.avpl_rad[:] = 0
+    if bounce0:
+        sc.avpl_bounce[n // 2] = 0
+    return sc
+
+
+def test_avpl_bvh_cluster(orc):
+    sc = avpl_like(3000, 11)
+    rand = np.random.default_rng(3).random(2 * 3000 - 1, dtype=np.float32)
+    t = orc.avpl_bvh_build(sc, rand)
+    assert t["status"] == 0
+    t["nd_rand"] = rand
+    # leaves = AVPLs in code order
+    perm = t["perm"]
+    np.testing.assert_array_equal(t["nd_pos"][:3000], sc.avpl_pos[perm])
+    np.testing.assert_array_equal(t["nd_rad"][:3000], sc.avpl_rad[perm])
+    np.testing.assert_array_equal(t["nd_mat"][:3000], sc.avpl_mat[perm])
+    K.check_topology(t["codes"], t["nodes"], t["parents"], t["ranges"])
+    K.check_boxes(t["nodes"], t["ranges"], t["positions"])
+    K.check_cluster(t, sc.avpl_rad[perm])
+    st = K.right_choice_stats(t)
+    # the right child is taken exactly when rand > i1 / (i1 + i2) = 1 - p_right
+    chose_right = st[:, 2] > 0.5
+    np.testing.assert_array_equal(chose_right, st[:, 0] > 1.0 - st[:, 1] + 0 * st[:, 1])
+
+
+def test_avpl_bvh_zero_radiance_keeps_left(orc):
+    sc = avpl_like(200, 12, zero_rad=True)
+    rand = np.random.default_rng(4).random(399, dtype=np.float32)
+    t = orc.avpl_bvh_build(sc, rand)
+    n = 200
+    nodes = t["nodes"]
+    for k in range(n - 1):  # 0/0 ratio is NaN: the comparison fails, left child stays
+        c = nodes["left"][k]
+        c = c if c >= 0 else n + (-(c + 1))
+        np.testing.assert_array_equal(t["nd_pos"][n + k], t["nd_pos"][c])
+
+
+def test_avpl_bvh_rejects_bounce0(orc):
+    sc = avpl_like(100, 13, bounce0=True)
+    assert orc.avpl_bvh_build(sc, np.zeros(199, np.float32))["status"] == -2
