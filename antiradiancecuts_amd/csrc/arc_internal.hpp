@@ -90,6 +90,7 @@ struct arc_bvh {
     arc::DevBuf nd_anti;                        // per-slot antiradiance sums (cut extension)
     arc::DevBuf bounds;                         // 12 floats + validation flag
     arc::DevBuf partials;                       // reduction partials
+    arc::DevBuf ready;                          // refit: nodes spanning blocks, ready to merge
     arc::SortScratch sort;
 };
 

@@ -95,7 +95,13 @@ void arc_ctx_destroy(arc_ctx* ctx) {
 
 int arc_ctx_set_stream(arc_ctx* ctx, void* s) {
     if (!ctx) return ARC_E_INVALID;
-    ctx->stream = s ? static_cast<hipStream_t>(s) : ctx->own_stream;
+    ctx->stream = static_cast<hipStream_t>(s);  // NULL is the device's default (null) stream
+    return ARC_OK;
+}
+
+int arc_ctx_use_own_stream(arc_ctx* ctx) {
+    if (!ctx) return ARC_E_INVALID;
+    ctx->stream = ctx->own_stream;
     return ARC_OK;
 }
 
@@ -151,7 +157,7 @@ void arc_bvh_destroy(arc_bvh* b) {
     arc::DevBuf* bufs[] = {&b->codes,   &b->perm,    &b->nodes,  &b->parents, &b->ranges,   &b->positions,
                            &b->normals, &b->in_pos,  &b->in_nrm, &b->nd_size, &b->nd_mat,   &b->nd_rand,
                            &b->nd_pos,  &b->nd_nrm,  &b->nd_rad, &b->nd_dir,  &b->nd_anti,  &b->bounds,
-                           &b->partials, &b->sort.keys_alt, &b->sort.vals_alt, &b->sort.counts};
+                           &b->partials, &b->ready, &b->sort.keys_alt, &b->sort.vals_alt, &b->sort.counts};
     for (arc::DevBuf* d : bufs) arc::release(*d);
     delete b;
 }

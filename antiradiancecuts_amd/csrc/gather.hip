@@ -44,7 +44,11 @@ struct arc_gather {
     DevBuf rec, part, counters;
     DevBuf stage;  // AVPL SoA staged from host memory
     arc_bvh* bvh = nullptr;
-    hipEvent_t ev[6] = {};
+    // per-frame phase events: frame slot, then 6 marks (start, upload done,
+    // bvh done, radiance done, antiradiance done, end)
+    static constexpr int kRing = 64;
+    hipEvent_t ev[kRing][6] = {};
+    int ring_next = 0, ring_count = 0;
 };
 
 namespace {
@@ -52,6 +56,10 @@ namespace {
 constexpr int kThreads = 256;
 constexpr int kRecF4 = 5;  // float4 per packed AVPL record
 constexpr int kRun = 64;   // pairs summed plainly before a compensated add
+// Relative guard band of the FP32 antiradiance decisions (FP32 rounding of
+// these quantities is below 1e-6 relative); pairs inside it are re-decided
+// in FP64.
+constexpr float kGuard = 1e-5f;
 
 enum Counter { C_VISITED = 0, C_EVALUATED = 1, C_ANTI_VISITED = 2, C_ANTI_EVALUATED = 3, C_ERRORS = 4, C_NUM = 8 };
 
@@ -92,7 +100,7 @@ __global__ __launch_bounds__(kThreads) void k_pack(arc::AvplsDev a, const arc_ma
     r[1] = make_float4(q.x * nq, q.y * nq, q.z * nq, w.y * nw);
     r[2] = make_float4(w.x, w.y, w.z, w.z * nw);
     r[3] = make_float4(lr, lg, lb, (float)mi);
-    r[4] = make_float4(A.x * as, A.y * as, A.z * as, w2 == 0.f ? 1.f : 0.f);
+    r[4] = make_float4(A.x * as, A.y * as, A.z * as, kGuard * sqrtf(w2));  // guard for |w . n|
 }
 
 // ----------------------------------------------------------- pixels ----
@@ -123,20 +131,62 @@ __device__ inline PixelF load_pixel(const float4* gpos, const float4* gnrm, cons
     return P;
 }
 
+// ------------------------------------------------------- decisions ----
+// The antiradiance tests are discontinuous (inside / outside a disk or cone,
+// sign tests of dot products).  A decision is made in FP32 when the FP32
+// value is farther from its threshold than a guard that bounds the FP32
+// rounding error (with a wide margin); pairs inside the guard band are
+// re-decided in FP64 from the raw inputs with the reference's formulas, which
+// is the decision the FP64 oracle makes.  Pairs in the band are rare except
+// for coplanar pixel / AVPL pairs, whose signs the reference leaves to
+// rounding noise.
+
+// getAntiradiance's tests (cudaGather.cu:116-140) in FP64.
+__device__ __attribute__((noinline)) bool disk_accept_f64(float4 p4, float4 n4, f3 a, f3 an, f3 w, float radius) {
+    using V3 = V<double>;
+    const V3 pos = mk<double>(p4.x, p4.y, p4.z), nrm = mk<double>(n4.x, n4.y, n4.z);
+    const V3 apos = ld3<double>(a), anrm = ld3<double>(an), aw = ld3<double>(w);
+    const double denom = dot(aw, nrm);
+    if (denom == 0.0) return false;
+    const double t = dot(pos - apos, nrm) / denom;
+    if (t <= 0.0) return false;
+    const V3 hit = apos + t * normalize(aw);
+    if (length(hit - pos) >= (double)radius) return false;
+    if (dot(normalize(hit - apos), aw) <= 0.0) return false;
+    if (dot(nrm, aw) >= 0.0) return false;
+    if (dot(normalize(pos - apos), anrm) >= 0.0) return false;
+    return true;
+}
+
+// GetAntiradiance's tests (Shaders/Gather.frag:155-166) in FP64.
+__device__ __attribute__((noinline)) bool cone_accept_f64(float4 p4, float4 n4, f3 a, f3 w, double cos_half) {
+    using V3 = V<double>;
+    const V3 pos = mk<double>(p4.x, p4.y, p4.z), nrm = mk<double>(n4.x, n4.y, n4.z);
+    const V3 apos = ld3<double>(a), aw = ld3<double>(w);
+    const V3 to_pos = normalize(pos - apos);
+    const double c = dot(to_pos, normalize(aw));
+    if (c <= (double)kConeEps) return false;
+    if (dot(nrm, -to_pos) <= (double)kConeEps) return false;
+    return c > cos_half;
+}
+
 // ------------------------------------------------------------ exact ----
 // part layout: [split][6][npix] = radiance rgb, antiradiance rgb.
 template <int MODEL, bool GENERAL, bool DO_RAD, bool DO_ANTI>
-__global__ __launch_bounds__(kThreads) void k_exact(const float4* __restrict__ rec, int n_avpl, int chunk,
+__global__ __launch_bounds__(kThreads) void k_exact(const float4* __restrict__ rec, arc::AvplsDev raw, int chunk,
                                                     const float4* __restrict__ gpos,
                                                     const float4* __restrict__ gnrm,
                                                     const float4* __restrict__ galb,
                                                     const arc_material* __restrict__ mats, int npix,
                                                     float camx, float camy, float camz, float radius,
-                                                    float cos_half, float* __restrict__ part) {
+                                                    float cos_half, double cos_half_d, float* __restrict__ part) {
     const int pix = blockIdx.x * kThreads + threadIdx.x;
     const bool live = pix < npix;
-    const PixelF P = load_pixel(gpos, gnrm, galb, mats, live ? pix : 0);
+    const int q = live ? pix : 0;
+    const PixelF P = load_pixel(gpos, gnrm, galb, mats, q);
     const float r2 = radius * radius;
+    const float cone_lo = fmaxf(kConeEps, cos_half) - kGuard;
+    const int n_avpl = raw.n;
     const int a0 = blockIdx.y * chunk;
     const int a1 = min(n_avpl, a0 + chunk);
     Kahan kr[3], ka[3];
@@ -153,6 +203,42 @@ __global__ __launch_bounds__(kThreads) void k_exact(const float4* __restrict__ r
             const float inv = rsq(dd);
             const float cn = P.nx * dx + P.ny * dy + P.nz * dz;        // |d| cos at the pixel
             const float ca = -(r1.x * dx + r1.y * dy + r1.z * dz);     // |d| cos at the AVPL
+            // antiradiance decision and weight
+            float m = 0.f;
+            if (DO_ANTI) {
+                bool in;
+                if (MODEL == ARC_ANTIRAD_DISK) {
+                    // line a + t w hits the tangent plane at t = -cn / (w . n) > 0, within the
+                    // disk, on the far side of the AVPL's surface, pixel facing against w.
+                    // With h = d + t w_unit, |h| < r  <=>  |h denom|^2 < r^2 denom^2, and
+                    // h denom = d denom - cn w_unit: no division needed.
+                    const float denom = r2v.x * P.nx + r2v.y * P.ny + r2v.z * P.nz;
+                    const float hx = dx * denom - cn * r0.w;
+                    const float hy = dy * denom - cn * r1.w;
+                    const float hz = dz * denom - cn * r2v.w;
+                    const float h2 = hx * hx + hy * hy + hz * hz;
+                    const float dn2 = denom * denom;
+                    const float lim = r2 * dn2;
+                    const float g = kGuard * (dd * dn2 + cn * cn);
+                    const float dl = kGuard * (dd * inv);
+                    in = denom < 0.f && cn > 0.f && h2 < lim && ca < 0.f;
+                    const bool amb = h2 < lim + g &&
+                                     (!(fabsf(h2 - lim) > g) || !(fabsf(cn) > dl) || !(fabsf(ca) > dl) ||
+                                      !(fabsf(denom) > r4.w));
+                    if (amb) in = disk_accept_f64(gpos[q], gnrm[q], raw.pos[i], raw.nrm[i], raw.dir[i], radius);
+                    m = in ? 1.f : 0.f;
+                } else {
+                    // cone of half-angle acos(cos_half) about w, cosine at the pixel / dist^2
+                    const float c = -(dx * r0.w + dy * r1.w + dz * r2v.w) * inv;
+                    const float cp = cn * inv;
+                    in = c > kConeEps && cp > kConeEps && c > cos_half;
+                    const bool amb = c > cone_lo && cp > kConeEps - kGuard &&
+                                     (!(fabsf(c - cos_half) > kGuard) || !(fabsf(cp - kConeEps) > kGuard) ||
+                                      !(fabsf(c - kConeEps) > kGuard));
+                    if (amb) in = cone_accept_f64(gpos[q], gnrm[q], raw.pos[i], raw.dir[i], cos_half_d);
+                    m = in ? cp * (inv * inv) : 0.f;
+                }
+            }
             if (GENERAL) {
                 using V3 = V<float>;
                 const V3 pos = mk(P.px, P.py, P.pz), nrm = mk(P.nx, P.ny, P.nz);
@@ -161,50 +247,24 @@ __global__ __launch_bounds__(kThreads) void k_exact(const float4* __restrict__ r
                 const V3 cam = mk(camx, camy, camz);
                 Mat<float> pm = material<float>(mats, P.mat);
                 if (galb) pm.diffuse = mk(P.br / kOneOverPi, P.bg / kOneOverPi, P.bb / kOneOverPi);
-                const Mat<float> am = material<float>(mats, (int)r3.w);
                 if (DO_RAD) {
+                    const Mat<float> am = material<float>(mats, (int)r3.w);
                     const V3 v = radiance(pos, nrm, pm, apos, anrm, aw, mk(r3.x, r3.y, r3.z), am, cam);
                     sr0 += v.x; sr1 += v.y; sr2 += v.z;
                 }
-                if (DO_ANTI) {
-                    // record holds A / (pi r^2) for the disk model; undo to reuse the formula
-                    V3 v;
-                    if (MODEL == ARC_ANTIRAD_DISK)
-                        v = antirad_disk(pos, nrm, pm, apos, anrm, aw, mk(r4.x, r4.y, r4.z) * (kPi * radius * radius),
-                                         cam, radius);
-                    else
-                        v = antirad_cone(pos, nrm, pm, apos, aw, mk(r4.x, r4.y, r4.z), cam, cos_half);
-                    sa0 += v.x; sa1 += v.y; sa2 += v.z;
+                if (DO_ANTI && m != 0.f) {
+                    // record holds A / (pi r^2) (disk) or A (cone); the pixel BRDF varies per pair
+                    const V3 b = f_r3(apos, pos, cam, nrm, pm) * m;
+                    sa0 += r4.x * b.x; sa1 += r4.y * b.y; sa2 += r4.z * b.z;
                 }
             } else {
                 if (DO_RAD) {
                     const float c1 = fminf(fmaxf(cn * inv, 0.f), 1.f);
                     const float c2 = fminf(fmaxf(ca * inv, 0.f), 1.f);
-                    const float g = c1 * c2 * (inv * inv);
-                    sr0 += r3.x * g; sr1 += r3.y * g; sr2 += r3.z * g;
+                    const float gg = c1 * c2 * (inv * inv);
+                    sr0 += r3.x * gg; sr1 += r3.y * gg; sr2 += r3.z * gg;
                 }
-                if (DO_ANTI) {
-                    float m;
-                    if (MODEL == ARC_ANTIRAD_DISK) {
-                        // line a + t w hits the tangent plane at t = -cn / (w . n) > 0, within the
-                        // disk, on the far side of the AVPL's surface, pixel facing against w.
-                        // With h = d + t w_unit, |h| < r  <=>  |h denom|^2 < r^2 denom^2, and
-                        // h denom = d denom - cn w_unit: no division needed.
-                        const float denom = r2v.x * P.nx + r2v.y * P.ny + r2v.z * P.nz;
-                        const float hx = dx * denom - cn * r0.w;
-                        const float hy = dy * denom - cn * r1.w;
-                        const float hz = dz * denom - cn * r2v.w;
-                        const float h2 = hx * hx + hy * hy + hz * hz;
-                        const float lim = r2 * (denom * denom);
-                        m = (denom < 0.f && cn > 0.f && h2 < lim && ca < 0.f) ? 1.f : 0.f;
-                    } else {
-                        // cone of half-angle acos(cos_half) about w, cosine at the pixel / dist^2
-                        const float c = -(dx * r0.w + dy * r1.w + dz * r2v.w) * inv;
-                        const float cp = cn * inv;
-                        m = (c > kConeEps && cp > kConeEps && c > cos_half) ? cp * (inv * inv) : 0.f;
-                    }
-                    sa0 += r4.x * m; sa1 += r4.y * m; sa2 += r4.z * m;
-                }
+                if (DO_ANTI) { sa0 += r4.x * m; sa1 += r4.y * m; sa2 += r4.z * m; }
             }
         }
         if (DO_RAD) { kr[0].add(sr0); kr[1].add(sr1); kr[2].add(sr2); }
@@ -433,23 +493,24 @@ int pick_splits(int npix, int n_avpl) {
 }
 
 template <int MODEL, bool GENERAL, bool DO_RAD, bool DO_ANTI>
-void launch_exact(arc_gather* g, int n, int chunk, int splits, float3 cam, float radius, float cos_half) {
+void launch_exact(arc_gather* g, const arc::AvplsDev& a, int chunk, int splits, float3 cam, float radius,
+                  double cos_half) {
     const int npix = g->width * g->height;
     hipLaunchKernelGGL((k_exact<MODEL, GENERAL, DO_RAD, DO_ANTI>), dim3(blocks(npix), splits), dim3(kThreads), 0,
-                       g->ctx->stream, g->rec.as<float4>(), n, chunk, g->gpos, g->gnrm, g->galb,
-                       g->mats.as<arc_material>(), npix, cam.x, cam.y, cam.z, radius, cos_half,
+                       g->ctx->stream, g->rec.as<float4>(), a, chunk, g->gpos, g->gnrm, g->galb,
+                       g->mats.as<arc_material>(), npix, cam.x, cam.y, cam.z, radius, (float)cos_half, cos_half,
                        g->part.as<float>());
 }
 
 template <bool DO_RAD, bool DO_ANTI>
-void dispatch_exact(arc_gather* g, int model, int n, int chunk, int splits, float3 cam, float radius,
-                    float cos_half) {
+void dispatch_exact(arc_gather* g, int model, const arc::AvplsDev& a, int chunk, int splits, float3 cam,
+                    float radius, double cos_half) {
     if (model == ARC_ANTIRAD_DISK) {
-        if (g->general) launch_exact<ARC_ANTIRAD_DISK, true, DO_RAD, DO_ANTI>(g, n, chunk, splits, cam, radius, cos_half);
-        else launch_exact<ARC_ANTIRAD_DISK, false, DO_RAD, DO_ANTI>(g, n, chunk, splits, cam, radius, cos_half);
+        if (g->general) launch_exact<ARC_ANTIRAD_DISK, true, DO_RAD, DO_ANTI>(g, a, chunk, splits, cam, radius, cos_half);
+        else launch_exact<ARC_ANTIRAD_DISK, false, DO_RAD, DO_ANTI>(g, a, chunk, splits, cam, radius, cos_half);
     } else {
-        if (g->general) launch_exact<ARC_ANTIRAD_CONE, true, DO_RAD, DO_ANTI>(g, n, chunk, splits, cam, radius, cos_half);
-        else launch_exact<ARC_ANTIRAD_CONE, false, DO_RAD, DO_ANTI>(g, n, chunk, splits, cam, radius, cos_half);
+        if (g->general) launch_exact<ARC_ANTIRAD_CONE, true, DO_RAD, DO_ANTI>(g, a, chunk, splits, cam, radius, cos_half);
+        else launch_exact<ARC_ANTIRAD_CONE, false, DO_RAD, DO_ANTI>(g, a, chunk, splits, cam, radius, cos_half);
     }
 }
 
@@ -555,10 +616,11 @@ int arc_gather_create(arc_ctx* ctx, int width, int height, const arc_material* m
         if (e != hipSuccess) st = arc::hip_fail(ctx, e, "hipMemcpy(materials)", __FILE__, __LINE__);
     }
     if (st == ARC_OK) st = arc::ensure(ctx, g->counters, sizeof(unsigned long long) * C_NUM);
-    for (int i = 0; st == ARC_OK && i < 6; ++i) {
-        hipError_t e = hipEventCreate(&g->ev[i]);
-        if (e != hipSuccess) st = arc::hip_fail(ctx, e, "hipEventCreate", __FILE__, __LINE__);
-    }
+    for (int f = 0; st == ARC_OK && f < arc_gather::kRing; ++f)
+        for (int i = 0; st == ARC_OK && i < 6; ++i) {
+            hipError_t e = hipEventCreate(&g->ev[f][i]);
+            if (e != hipSuccess) st = arc::hip_fail(ctx, e, "hipEventCreate", __FILE__, __LINE__);
+        }
     if (st == ARC_OK) st = arc_bvh_create(ctx, 0, &g->bvh);
     if (st != ARC_OK) {
         arc_gather_destroy(g);
@@ -570,8 +632,9 @@ int arc_gather_create(arc_ctx* ctx, int width, int height, const arc_material* m
 
 void arc_gather_destroy(arc_gather* g) {
     if (!g) return;
-    for (int i = 0; i < 6; ++i)
-        if (g->ev[i]) (void)hipEventDestroy(g->ev[i]);
+    for (int f = 0; f < arc_gather::kRing; ++f)
+        for (int i = 0; i < 6; ++i)
+            if (g->ev[f][i]) (void)hipEventDestroy(g->ev[f][i]);
     arc::release(g->mats);
     arc::release(g->int_rad);
     arc::release(g->int_anti);
@@ -606,12 +669,14 @@ int arc_gather_run(arc_gather* g, const arc_avpls* avpls, arc_float3 camPos, flo
                    const arc_gather_params* p, int flags, arc_gather_stats* stats) {
     ARC_TRY(validate_params(g, avpls, p));
     arc_ctx* ctx = g->ctx;
-    const bool profile = (flags & ARC_RUN_PROFILE) != 0;
+    const bool profile = (flags & (ARC_RUN_PROFILE | ARC_RUN_TIMELINE)) != 0;
+    const int slot = g->ring_next;
+    hipEvent_t* ev = g->ev[slot];
     const bool count = (flags & ARC_RUN_COUNTERS) != 0;
     const int npix = g->width * g->height;
     const int n = avpls->numAvpls;
     const float radius = p->photonRadiusScale * sceneExtent / 100.f;  // cudaGather.cu:949
-    const float cos_half = std::cos(p->coneHalfAngle);
+    const double cos_half = std::cos((double)p->coneHalfAngle);
     const float3 cam = make_float3(camPos.x, camPos.y, camPos.z);
     int st = ARC_OK;
     float4* rad = target(g->int_rad, g->out_rad, g, st);
@@ -620,7 +685,7 @@ int arc_gather_run(arc_gather* g, const arc_avpls* avpls, arc_float3 camPos, flo
     float4* anti = g->out_anti;
     if (stats) std::memset(stats, 0, sizeof(*stats));
     if (count) ARC_HIP(ctx, hipMemsetAsync(g->counters.p, 0, sizeof(unsigned long long) * C_NUM, ctx->stream));
-    if (profile) ARC_HIP(ctx, hipEventRecord(g->ev[0], ctx->stream));
+    if (profile) ARC_HIP(ctx, hipEventRecord(ev[0], ctx->stream));
 
     arc::AvplsDev a{};
     ARC_TRY(stage_avpls(g, avpls, flags, a));
@@ -631,7 +696,7 @@ int arc_gather_run(arc_gather* g, const arc_avpls* avpls, arc_float3 camPos, flo
                            g->rec.as<float4>());
         ARC_LAUNCHED(ctx);
     }
-    if (profile) ARC_HIP(ctx, hipEventRecord(g->ev[1], ctx->stream));
+    if (profile) ARC_HIP(ctx, hipEventRecord(ev[1], ctx->stream));
 
     const int splits = pick_splits(npix, std::max(n, 1));
     const int chunk = (std::max(n, 1) + splits - 1) / splits;
@@ -644,7 +709,7 @@ int arc_gather_run(arc_gather* g, const arc_avpls* avpls, arc_float3 camPos, flo
     if (cut) {
         // AvplBvh(avpls, false) per frame (cudaGather.cu:858)
         ARC_TRY(arc::bvh_build_avpls(g->bvh, a, nullptr, p->seed, (flags & ARC_TRUST_INPUT) == 0));
-        if (profile) ARC_HIP(ctx, hipEventRecord(g->ev[2], ctx->stream));
+        if (profile) ARC_HIP(ctx, hipEventRecord(ev[2], ctx->stream));
         arc_bvh* b = g->bvh;
         CutTree t{b->nodes.as<arc_bvh_node>(), b->nd_mat.as<int>(), b->nd_pos.as<f3>(), b->nd_nrm.as<f3>(),
                   b->nd_rad.as<f3>(), b->nd_dir.as<f3>(), b->n};
@@ -654,26 +719,26 @@ int arc_gather_run(arc_gather* g, const arc_avpls* avpls, arc_float3 camPos, flo
                            g->mats.as<arc_material>(), g->width, g->height, cam.x, cam.y, cam.z,
                            p->clusterRefinementThreshold, p->stackSize, p->writeSentinel, rad, counters);
         ARC_LAUNCHED(ctx);
-        if (profile) ARC_HIP(ctx, hipEventRecord(g->ev[3], ctx->stream));
-        if (n > 0) dispatch_exact<false, true>(g, p->antiradianceModel, n, chunk, splits, cam, radius, cos_half);
+        if (profile) ARC_HIP(ctx, hipEventRecord(ev[3], ctx->stream));
+        if (n > 0) dispatch_exact<false, true>(g, p->antiradianceModel, a, chunk, splits, cam, radius, cos_half);
         ARC_LAUNCHED(ctx);
-        if (profile) ARC_HIP(ctx, hipEventRecord(g->ev[4], ctx->stream));
+        if (profile) ARC_HIP(ctx, hipEventRecord(ev[4], ctx->stream));
         hipLaunchKernelGGL(k_finalize, dim3(blocks(npix)), dim3(kThreads), 0, ctx->stream, g->part.as<float>(),
                            n > 0 ? splits : 0, npix, 0, 1, rad, nullptr, anti, res);
         ARC_LAUNCHED(ctx);
     } else {
-        if (profile) ARC_HIP(ctx, hipEventRecord(g->ev[2], ctx->stream));
-        if (n > 0) dispatch_exact<true, true>(g, p->antiradianceModel, n, chunk, splits, cam, radius, cos_half);
+        if (profile) ARC_HIP(ctx, hipEventRecord(ev[2], ctx->stream));
+        if (n > 0) dispatch_exact<true, true>(g, p->antiradianceModel, a, chunk, splits, cam, radius, cos_half);
         ARC_LAUNCHED(ctx);
         if (profile) {
-            ARC_HIP(ctx, hipEventRecord(g->ev[3], ctx->stream));
-            ARC_HIP(ctx, hipEventRecord(g->ev[4], ctx->stream));
+            ARC_HIP(ctx, hipEventRecord(ev[3], ctx->stream));
+            ARC_HIP(ctx, hipEventRecord(ev[4], ctx->stream));
         }
         hipLaunchKernelGGL(k_finalize, dim3(blocks(npix)), dim3(kThreads), 0, ctx->stream, g->part.as<float>(),
                            n > 0 ? splits : 0, npix, 1, 1, rad, rad, anti, res);
         ARC_LAUNCHED(ctx);
     }
-    if (profile) ARC_HIP(ctx, hipEventRecord(g->ev[5], ctx->stream));
+    if (profile) ARC_HIP(ctx, hipEventRecord(ev[5], ctx->stream));
     if (stats) {
         stats->numLeafs = cut ? g->bvh->n : 0;
         if (count) {
@@ -686,19 +751,37 @@ int arc_gather_run(arc_gather* g, const arc_avpls* avpls, arc_float3 camPos, flo
             stats->antiEvaluatedNodes = (long long)c[C_ANTI_EVALUATED];
             stats->errorPixels = (long long)c[C_ERRORS];
         }
-        if (profile) {
-            ARC_HIP(ctx, hipEventSynchronize(g->ev[5]));
+        if (flags & ARC_RUN_PROFILE) {
+            ARC_HIP(ctx, hipEventSynchronize(ev[5]));
             float t[5];
-            for (int i = 0; i < 5; ++i) ARC_HIP(ctx, hipEventElapsedTime(&t[i], g->ev[i], g->ev[i + 1]));
+            for (int i = 0; i < 5; ++i) ARC_HIP(ctx, hipEventElapsedTime(&t[i], ev[i], ev[i + 1]));
             stats->msUpload = t[0];
             stats->msBvh = t[1];
             stats->msRadiance = t[2];
             stats->msAntiradiance = t[3];
             stats->msCombine = t[4];
-            ARC_HIP(ctx, hipEventElapsedTime(&stats->msTotal, g->ev[0], g->ev[5]));
+            ARC_HIP(ctx, hipEventElapsedTime(&stats->msTotal, ev[0], ev[5]));
         }
     }
+    if (profile) {
+        g->ring_next = (slot + 1) % arc_gather::kRing;
+        g->ring_count = std::min(g->ring_count + 1, arc_gather::kRing);
+    }
     return ARC_OK;
+}
+
+int arc_gather_profile_history(arc_gather* g, int frames, float* out) {
+    if (!g || frames < 0 || (frames > 0 && !out)) return ARC_E_INVALID;
+    arc_ctx* ctx = g->ctx;
+    const int n = std::min(frames, g->ring_count);
+    for (int k = 0; k < n; ++k) {
+        const int slot = (g->ring_next - n + k + arc_gather::kRing) % arc_gather::kRing;
+        hipEvent_t* ev = g->ev[slot];
+        ARC_HIP(ctx, hipEventSynchronize(ev[5]));
+        for (int i = 0; i < 5; ++i) ARC_HIP(ctx, hipEventElapsedTime(&out[6 * k + i], ev[i], ev[i + 1]));
+        ARC_HIP(ctx, hipEventElapsedTime(&out[6 * k + 5], ev[0], ev[5]));
+    }
+    return n;
 }
 
 int arc_gather_exact_f64(arc_gather* g, const arc_avpls* avpls, arc_float3 camPos, float sceneExtent,

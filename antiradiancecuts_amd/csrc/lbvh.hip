@@ -11,9 +11,10 @@
 //   sort_pairs   stable radix sort, so equal codes keep input order
 //   k_permute    sorted leaf positions / normals and leaf node data
 //   k_karras     one thread per internal node (bvh.cu:188-278)
-//   k_refit      one thread per leaf, bottom-up AABB merge with an
-//                agent-scope acquire/release counter per node; the AvplBvh
-//                variant also forms cluster representatives (bvh.cu:726-744)
+//   k_refit_*    bottom-up AABB merge, one thread per leaf, block-local
+//                counters in LDS and a one-workgroup pass over the nodes that
+//                span blocks; the AvplBvh variant also forms cluster
+//                representatives (bvh.cu:726-744)
 //
 // This file is compiled with -ffp-contract=off: the refit's float results
 // (cluster radiance sums, representative choice) must match the oracle bit
@@ -71,17 +72,21 @@ __global__ __launch_bounds__(kBlock) void k_bounds_partial(const f3* __restrict_
     if (threadIdx.x < 12) partials[blockIdx.x * 12 + threadIdx.x] = s[threadIdx.x][0];
 }
 
-__global__ __launch_bounds__(64) void k_bounds_final(const float* __restrict__ partials, int nparts,
-                                                     float* __restrict__ bounds) {
-    const int k = threadIdx.x;
-    if (k >= 12) return;
+// One wave per component: lanes stride over the partials, then a shuffle tree.
+__global__ __launch_bounds__(768) void k_bounds_final(const float* __restrict__ partials, int nparts,
+                                                      float* __restrict__ bounds) {
+    const int k = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const bool is_max = (k / 3) & 1;
     float r = is_max ? FLT_MIN : FLT_MAX;
-    for (int b = 0; b < nparts; ++b) {
+    for (int b = lane; b < nparts; b += 64) {
         const float x = partials[b * 12 + k];
         r = is_max ? fmaxf(r, x) : fminf(r, x);
     }
-    bounds[k] = r;
+    for (int off = 32; off > 0; off >>= 1) {
+        const float o = __shfl_xor(r, off);
+        r = is_max ? fmaxf(r, o) : fminf(r, o);
+    }
+    if (lane == 0) bounds[k] = r;
 }
 
 // ------------------------------------------------------------ Morton ----
@@ -201,8 +206,10 @@ __device__ inline int delta(int i, int j, const uint32_t* __restrict__ m, int nu
 
 __global__ __launch_bounds__(kBlock) void k_karras(const uint32_t* __restrict__ m, int num_nodes,
                                                    arc_bvh_node* __restrict__ nodes,
-                                                   int* __restrict__ parents, int* __restrict__ ranges) {
+                                                   int* __restrict__ parents, int* __restrict__ ranges,
+                                                   int* __restrict__ nready) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i == 0) *nready = 0;  // ready-list length of the refit that follows
     if (i >= num_nodes) return;
     const int root_slot = num_nodes + 1;
     // direction of the range: sign(0) = +1 (bvh.cu:170-173, 203-204)
@@ -266,41 +273,97 @@ __device__ inline void cluster(int target, int left, int right, int n, const Nod
     }
 }
 
-// kernel_innerNodes (bvh.cu:280-316).  The reference reads the sibling's box
-// without ordering (quirk Q3); here the arrival counter is an agent-scope
-// acq_rel RMW: the first arrival's writes are released before its increment
-// and acquired by the second arrival before it reads them.
+// kernel_innerNodes (bvh.cu:280-316).  One thread per leaf walks up; the
+// second thread to arrive at a node merges both children.  The reference
+// reads the sibling's box without ordering (quirk Q3); here every arrival
+// counter is an acq_rel RMW, so the first arrival's writes are visible to
+// the second arrival before it reads them.
+//
+// Two kernels, no cross-XCD fence:
+//   k_refit_local: a block owns kRefitLeaves consecutive leaves.  Karras node
+//     k has k at one end of its leaf range, so a node whose range lies inside
+//     the block's leaves has its index there too and both of its subtrees are
+//     merged by this block: its counter lives in LDS (workgroup scope).  At a
+//     node whose range crosses a block boundary the walk stops after a
+//     relaxed increment of the node's `visited`; the thread that makes it 2
+//     appends the node to the ready list.
+//   k_refit_top: one workgroup, after the kernel boundary has published every
+//     local result, climbs from the ready nodes through the remaining
+//     crossing nodes with workgroup-scope counters on `visited`.
+// Every internal node ends with visited == 2, as the reference's counters do.
+constexpr int kRefitLeaves = 1024;
+
 template <bool kCluster>
-__global__ __launch_bounds__(kBlock) void k_refit(arc_bvh_node* __restrict__ nodes,
-                                                  const int* __restrict__ parents,
-                                                  const f3* __restrict__ spos, int n, NodeData nd) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
+__device__ inline void merge_node(int parent, arc_bvh_node* nodes, const f3* spos, int n, const NodeData& nd) {
+    const int l = nodes[parent].left, r = nodes[parent].right;
+    f3 lmin, lmax, rmin, rmax;
+    if (l < 0) {
+        const arc_bvh_node& c = nodes[-(l + 1)];
+        lmin = f3{c.bbMin.x, c.bbMin.y, c.bbMin.z};
+        lmax = f3{c.bbMax.x, c.bbMax.y, c.bbMax.z};
+    } else {
+        lmin = lmax = spos[l];
+    }
+    if (r < 0) {
+        const arc_bvh_node& c = nodes[-(r + 1)];
+        rmin = f3{c.bbMin.x, c.bbMin.y, c.bbMin.z};
+        rmax = f3{c.bbMax.x, c.bbMax.y, c.bbMax.z};
+    } else {
+        rmin = rmax = spos[r];
+    }
+    nodes[parent].bbMin = arc_float3{fminf(lmin.x, rmin.x), fminf(lmin.y, rmin.y), fminf(lmin.z, rmin.z)};
+    nodes[parent].bbMax = arc_float3{fmaxf(lmax.x, rmax.x), fmaxf(lmax.y, rmax.y), fmaxf(lmax.z, rmax.z)};
+    if (kCluster) cluster(parent, l, r, n, nd);
+}
+
+template <bool kCluster>
+__global__ __launch_bounds__(kRefitLeaves) void k_refit_local(arc_bvh_node* __restrict__ nodes,
+                                                              const int* __restrict__ parents,
+                                                              const int* __restrict__ ranges,
+                                                              const f3* __restrict__ spos, int n, NodeData nd,
+                                                              int* __restrict__ ready, int* __restrict__ nready) {
+    __shared__ int arrivals[kRefitLeaves];
+    arrivals[threadIdx.x] = 0;
+    __syncthreads();
+    const int lo = blockIdx.x * kRefitLeaves;
+    const int hi = min(n, lo + kRefitLeaves) - 1;
+    const int i = lo + threadIdx.x;
+    if (i >= n) return;  // no barrier follows
     int parent = parents[i];
     while (parent >= 0) {
-        const int old = __hip_atomic_fetch_add(&nodes[parent].visited, 1, __ATOMIC_ACQ_REL,
-                                               __HIP_MEMORY_SCOPE_AGENT);
-        if (old == 0) return;
-        const int l = nodes[parent].left, r = nodes[parent].right;
-        f3 lmin, lmax, rmin, rmax;
-        if (l < 0) {
-            const arc_bvh_node& c = nodes[-(l + 1)];
-            lmin = f3{c.bbMin.x, c.bbMin.y, c.bbMin.z};
-            lmax = f3{c.bbMax.x, c.bbMax.y, c.bbMax.z};
-        } else {
-            lmin = lmax = spos[l];
+        const bool local = ranges[2 * parent] >= lo && ranges[2 * parent + 1] <= hi;
+        if (!local) {
+            const int old = __hip_atomic_fetch_add(&nodes[parent].visited, 1, __ATOMIC_RELAXED,
+                                                   __HIP_MEMORY_SCOPE_AGENT);
+            if (old == 1) ready[atomicAdd(nready, 1)] = parent;
+            return;
         }
-        if (r < 0) {
-            const arc_bvh_node& c = nodes[-(r + 1)];
-            rmin = f3{c.bbMin.x, c.bbMin.y, c.bbMin.z};
-            rmax = f3{c.bbMax.x, c.bbMax.y, c.bbMax.z};
-        } else {
-            rmin = rmax = spos[r];
-        }
-        nodes[parent].bbMin = arc_float3{fminf(lmin.x, rmin.x), fminf(lmin.y, rmin.y), fminf(lmin.z, rmin.z)};
-        nodes[parent].bbMax = arc_float3{fmaxf(lmax.x, rmax.x), fmaxf(lmax.y, rmax.y), fmaxf(lmax.z, rmax.z)};
-        if (kCluster) cluster(parent, l, r, n, nd);
+        if (__hip_atomic_fetch_add(&arrivals[parent - lo], 1, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP) == 0)
+            return;
+        merge_node<kCluster>(parent, nodes, spos, n, nd);
+        nodes[parent].visited = 2;
         parent = parents[n + parent];
+    }
+}
+
+template <bool kCluster>
+__global__ __launch_bounds__(kRefitLeaves) void k_refit_top(arc_bvh_node* __restrict__ nodes,
+                                                            const int* __restrict__ parents,
+                                                            const f3* __restrict__ spos, int n, NodeData nd,
+                                                            const int* __restrict__ ready,
+                                                            const int* __restrict__ nready) {
+    const int count = *nready;
+    for (int j = threadIdx.x; j < count; j += blockDim.x) {
+        int node = ready[j];
+        while (true) {
+            merge_node<kCluster>(node, nodes, spos, n, nd);
+            const int parent = parents[n + node];
+            if (parent < 0) break;
+            // 0: the sibling subtree is still pending and will climb on; 1: both done
+            if (__hip_atomic_fetch_add(&nodes[parent].visited, 1, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP) != 1)
+                break;
+            node = parent;
+        }
     }
 }
 
@@ -316,6 +379,7 @@ int ensure_tree(arc_bvh* b, int n) {
     ARC_TRY(arc::ensure(ctx, b->positions, sizeof(f3) * (size_t)n));
     ARC_TRY(arc::ensure(ctx, b->normals, sizeof(f3) * (size_t)n));
     ARC_TRY(arc::ensure(ctx, b->bounds, sizeof(float) * 16));
+    ARC_TRY(arc::ensure(ctx, b->ready, sizeof(int) * (size_t)n));
     ARC_TRY(arc::ensure(ctx, b->partials, sizeof(float) * 12 * kReduceBlocks));
     return ARC_OK;
 }
@@ -327,7 +391,7 @@ int encode_and_sort(arc_bvh* b, const f3* pos, const f3* nrm, int n) {
     float* bounds = b->bounds.as<float>();
     hipLaunchKernelGGL(k_bounds_partial, dim3(nparts), dim3(kBlock), 0, ctx->stream, pos, nrm, n,
                        b->partials.as<float>());
-    hipLaunchKernelGGL(k_bounds_final, dim3(1), dim3(64), 0, ctx->stream, b->partials.as<float>(),
+    hipLaunchKernelGGL(k_bounds_final, dim3(1), dim3(768), 0, ctx->stream, b->partials.as<float>(),
                        nparts, bounds);
     hipLaunchKernelGGL(k_encode, dim3(blocks_for(n)), dim3(kBlock), 0, ctx->stream, pos, nrm, n,
                        bounds, b->consider_normals && nrm ? 1 : 0, b->codes.as<uint32_t>(),
@@ -341,14 +405,23 @@ int build_tree(arc_bvh* b, int n, const NodeData* nd) {
     arc_ctx* ctx = b->ctx;
     hipLaunchKernelGGL(k_karras, dim3(blocks_for(n - 1)), dim3(kBlock), 0, ctx->stream,
                        b->codes.as<uint32_t>(), n - 1, b->nodes.as<arc_bvh_node>(), b->parents.as<int>(),
-                       b->ranges.as<int>());
-    if (nd)
-        hipLaunchKernelGGL(k_refit<true>, dim3(blocks_for(n)), dim3(kBlock), 0, ctx->stream,
-                           b->nodes.as<arc_bvh_node>(), b->parents.as<int>(), b->positions.as<f3>(), n, *nd);
-    else
-        hipLaunchKernelGGL(k_refit<false>, dim3(blocks_for(n)), dim3(kBlock), 0, ctx->stream,
-                           b->nodes.as<arc_bvh_node>(), b->parents.as<int>(), b->positions.as<f3>(), n,
-                           NodeData{});
+                       b->ranges.as<int>(), reinterpret_cast<int*>(b->bounds.as<float>() + 13));
+    const int rblocks = (n + kRefitLeaves - 1) / kRefitLeaves;
+    int* nready = reinterpret_cast<int*>(b->bounds.as<float>() + 13);  // zeroed by k_karras
+    int* ready = b->ready.as<int>();
+    const NodeData d = nd ? *nd : NodeData{};
+    auto* nodes = b->nodes.as<arc_bvh_node>();
+    if (nd) {
+        hipLaunchKernelGGL(k_refit_local<true>, dim3(rblocks), dim3(kRefitLeaves), 0, ctx->stream, nodes,
+                           b->parents.as<int>(), b->ranges.as<int>(), b->positions.as<f3>(), n, d, ready, nready);
+        hipLaunchKernelGGL(k_refit_top<true>, dim3(1), dim3(kRefitLeaves), 0, ctx->stream, nodes,
+                           b->parents.as<int>(), b->positions.as<f3>(), n, d, ready, nready);
+    } else {
+        hipLaunchKernelGGL(k_refit_local<false>, dim3(rblocks), dim3(kRefitLeaves), 0, ctx->stream, nodes,
+                           b->parents.as<int>(), b->ranges.as<int>(), b->positions.as<f3>(), n, d, ready, nready);
+        hipLaunchKernelGGL(k_refit_top<false>, dim3(1), dim3(kRefitLeaves), 0, ctx->stream, nodes,
+                           b->parents.as<int>(), b->positions.as<f3>(), n, d, ready, nready);
+    }
     ARC_LAUNCHED(ctx);
     return ARC_OK;
 }

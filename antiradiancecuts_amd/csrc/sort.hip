@@ -5,8 +5,9 @@
 //   hist    — each 256-thread block counts the digits of a 4096-key tile;
 //             lanes with equal digits are found with 8 wave ballots so only
 //             one lane per (wave, digit) touches LDS.
-//   scan    — one block turns the digit-major count table into exclusive
-//             offsets (all keys of smaller digits, then earlier tiles).
+//   scan    — one block per digit turns its row of tile counts into an
+//             exclusive prefix and records the digit total; each scatter
+//             block adds the exclusive scan of the 256 totals itself.
 //   scatter — each block re-reads its tile in index order, 256 keys a round;
 //             rank = keys of the same digit in earlier rounds + earlier waves
 //             + earlier lanes of this wave (ballot match), so equal digits
@@ -19,7 +20,6 @@ constexpr int kThreads = 256;
 constexpr int kItems = 16;
 constexpr int kTile = kThreads * kItems;
 constexpr int kRadix = 256;
-constexpr int kScanThreads = 1024;
 
 __device__ inline uint64_t match_digit(uint64_t active, uint32_t digit) {
     uint64_t m = active;
@@ -53,29 +53,39 @@ __global__ __launch_bounds__(kThreads) void k_hist(const uint32_t* __restrict__ 
     counts[(size_t)tid * num_tiles + blockIdx.x] = h[tid];
 }
 
-// Exclusive scan of `total` counts in place, one block.
-__global__ __launch_bounds__(kScanThreads) void k_scan(uint32_t* __restrict__ counts, int total) {
-    __shared__ uint32_t part[kScanThreads];
+// Block-wide exclusive scan of one value per thread (kThreads threads).
+__device__ inline uint32_t block_exclusive_scan(uint32_t v, uint32_t* sh, uint32_t* total) {
     const int tid = threadIdx.x;
-    const int per = (total + kScanThreads - 1) / kScanThreads;
-    const int lo = min(total, tid * per), hi = min(total, lo + per);
-    uint32_t s = 0;
-    for (int i = lo; i < hi; ++i) s += counts[i];
-    part[tid] = s;
+    sh[tid] = v;
     __syncthreads();
-    // Hillis-Steele inclusive scan over the per-thread sums
-    for (int off = 1; off < kScanThreads; off <<= 1) {
-        uint32_t v = tid >= off ? part[tid - off] : 0u;
+    for (int off = 1; off < kThreads; off <<= 1) {
+        const uint32_t u = tid >= off ? sh[tid - off] : 0u;
         __syncthreads();
-        part[tid] += v;
+        sh[tid] += u;
         __syncthreads();
     }
-    uint32_t run = part[tid] - s;  // exclusive prefix of this thread's chunk
-    for (int i = lo; i < hi; ++i) {
-        const uint32_t c = counts[i];
-        counts[i] = run;
-        run += c;
+    const uint32_t incl = sh[tid];
+    *total = sh[kThreads - 1];
+    __syncthreads();
+    return incl - v;
+}
+
+// Block d scans row d of the digit-major count table (exclusive, in place)
+// and writes the digit's total count.
+__global__ __launch_bounds__(kThreads) void k_scan_digit(uint32_t* __restrict__ counts, int num_tiles,
+                                                        uint32_t* __restrict__ totals) {
+    __shared__ uint32_t sh[kThreads];
+    uint32_t* row = counts + (size_t)blockIdx.x * num_tiles;
+    uint32_t run = 0;
+    for (int base = 0; base < num_tiles; base += kThreads) {
+        const int t = base + (int)threadIdx.x;
+        const uint32_t v = t < num_tiles ? row[t] : 0u;
+        uint32_t chunk_total;
+        const uint32_t ex = block_exclusive_scan(v, sh, &chunk_total);
+        if (t < num_tiles) row[t] = run + ex;
+        run += chunk_total;
     }
+    if (threadIdx.x == 0) totals[blockIdx.x] = run;
 }
 
 __global__ __launch_bounds__(kThreads) void k_scatter(const uint32_t* __restrict__ keys_in,
@@ -83,14 +93,17 @@ __global__ __launch_bounds__(kThreads) void k_scatter(const uint32_t* __restrict
                                                       uint32_t* __restrict__ keys_out,
                                                       int* __restrict__ vals_out, int n, int shift,
                                                       const uint32_t* __restrict__ offsets,
-                                                      int num_tiles) {
+                                                      const uint32_t* __restrict__ totals, int num_tiles) {
     __shared__ uint32_t base_off[kRadix];
     __shared__ uint32_t run[kRadix];
     __shared__ uint32_t wcount[kThreads / 64][kRadix];
+    __shared__ uint32_t sh[kThreads];
     const int tid = threadIdx.x;
     const int wave = tid >> 6;
     const int lane = tid & 63;
-    base_off[tid] = offsets[(size_t)tid * num_tiles + blockIdx.x];
+    uint32_t all;
+    const uint32_t digit_base = block_exclusive_scan(totals[tid], sh, &all);
+    base_off[tid] = digit_base + offsets[(size_t)tid * num_tiles + blockIdx.x];
     run[tid] = 0;
     const int base = blockIdx.x * kTile;
     for (int j = 0; j < kItems; ++j) {
@@ -129,21 +142,22 @@ int sort_pairs(arc_ctx* ctx, SortScratch& s, uint32_t* keys, int* vals, int n, i
     const int num_tiles = (n + kTile - 1) / kTile;
     ARC_TRY(ensure(ctx, s.keys_alt, sizeof(uint32_t) * (size_t)n));
     ARC_TRY(ensure(ctx, s.vals_alt, sizeof(int) * (size_t)n));
-    ARC_TRY(ensure(ctx, s.counts, sizeof(uint32_t) * (size_t)kRadix * num_tiles));
+    ARC_TRY(ensure(ctx, s.counts, sizeof(uint32_t) * ((size_t)kRadix * num_tiles + kRadix)));
     const int passes = (key_bits + 7) / 8;
     uint32_t* kin = keys;
     int* vin = vals;
     uint32_t* kout = s.keys_alt.as<uint32_t>();
     int* vout = s.vals_alt.as<int>();
     uint32_t* counts = s.counts.as<uint32_t>();
+    uint32_t* totals = counts + (size_t)kRadix * num_tiles;
     for (int p = 0; p < passes; ++p) {
         const int shift = 8 * p;
         hipLaunchKernelGGL(k_hist, dim3(num_tiles), dim3(kThreads), 0, ctx->stream, kin, n, shift,
                            counts, num_tiles);
-        hipLaunchKernelGGL(k_scan, dim3(1), dim3(kScanThreads), 0, ctx->stream, counts,
-                           kRadix * num_tiles);
+        hipLaunchKernelGGL(k_scan_digit, dim3(kRadix), dim3(kThreads), 0, ctx->stream, counts, num_tiles,
+                           totals);
         hipLaunchKernelGGL(k_scatter, dim3(num_tiles), dim3(kThreads), 0, ctx->stream, kin, vin,
-                           kout, vout, n, shift, counts, num_tiles);
+                           kout, vout, n, shift, counts, totals, num_tiles);
         ARC_LAUNCHED(ctx);
         std::swap(kin, kout);
         std::swap(vin, vout);

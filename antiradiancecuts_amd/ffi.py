@@ -31,6 +31,7 @@ ARC_ANTIRAD_DISK = 0
 ARC_ANTIRAD_CONE = 1
 ARC_RUN_PROFILE = 1
 ARC_RUN_COUNTERS = 2
+ARC_RUN_TIMELINE = 4
 
 vp = C.c_void_p
 
@@ -101,6 +102,7 @@ ARC_SYMBOLS = [
     ("arc_ctx_create", C.c_int, [C.c_int, P(vp)]),
     ("arc_ctx_destroy", None, [vp]),
     ("arc_ctx_set_stream", C.c_int, [vp, vp]),
+    ("arc_ctx_use_own_stream", C.c_int, [vp]),
     ("arc_ctx_get_stream", vp, [vp]),
     ("arc_ctx_synchronize", C.c_int, [vp]),
     ("arc_ctx_last_error", C.c_char_p, [vp]),
@@ -127,6 +129,7 @@ ARC_SYMBOLS = [
                                  P(arc_gather_stats)]),
     ("arc_gather_exact_f64", C.c_int, [vp, P(arc_avpls), arc_float3, C.c_float, P(arc_gather_params),
                                        C.c_int, vp, vp]),
+    ("arc_gather_profile_history", C.c_int, [vp, C.c_int, vp]),
     ("arc_gather_get_bvh", vp, [vp]),
 ]
 

@@ -1,0 +1,211 @@
+#!/usr/bin/env python3
+"""Frame benchmark of the AVPL gather on one MI355X.
+
+A step is one frame of CudaGather::run (cudaGather.cu:836-1051) on the
+1920x1080 x 1,000,000-AVPL workload: the AvplBvh over the frame's AVPLs is
+rebuilt, radiance is gathered over a light cut of it, antiradiance is
+gathered, and result = radiance - antiradiance is written.  All inputs are in
+HBM before the timed region.
+
+Prints one JSON line: pixel-AVPL pairs/s (W*H*N per frame time), ms/frame,
+the error of the frame against the exact FP64 gather on a pixel sample, the
+roofline of the dominant kernel (timed with HIP events on the library's
+stream inside the timed region) and the CPU baseline (the oracle timed on
+this host's cores on a bounded sample of the same frame).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+METRIC = "pixel-AVPL pairs/s + ms/frame at 1080p×1M AVPLs; rel-L2 vs FP64 oracle"
+PEAK_FP32_TFLOPS = 157.3   # MI355X vector FP32 (published)
+PEAK_HBM_GBS = 8000.0
+# Algorithmic FLOP per unit (FMA = 2): radiance pair 31, disk antiradiance
+# pair 37, cone antiradiance pair 33, visited inner cut node 32, evaluated
+# cut node +31.
+FLOP_RAD_PAIR, FLOP_DISK_PAIR, FLOP_CONE_PAIR = 31, 37, 33
+FLOP_VISIT, FLOP_EVAL = 32, 31
+PHASES = ("upload", "bvh", "radiance", "antiradiance", "combine", "total")
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--width", type=int, default=1920)
+    ap.add_argument("--height", type=int, default=1080)
+    ap.add_argument("--avpls", type=int, default=1_000_000)
+    ap.add_argument("--tris", type=int, default=200_000)
+    ap.add_argument("--error-stride", type=int, default=4,
+                    help="FP64 error on every k-th pixel in x and y (0 = skip)")
+    ap.add_argument("--cpu-sample", type=int, default=1024,
+                    help="CPU baseline on 1/k of the pixels (0 = skip)")
+    return ap.parse_args()
+
+
+def main() -> int:
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
+              "(the gather shards by pixel tiles with no exchange; see the design notes)", file=sys.stderr)
+        return 2
+    import torch
+
+    from antiradiancecuts_amd import ffi
+    from antiradiancecuts_amd import scene as S
+    from antiradiancecuts_amd.runtime import Context, DeviceAvpls, Gather, default_params, device_array
+
+    t0 = time.time()
+    sc = S.clutter_1080p(width=args.width, height=args.height, num_avpls=args.avpls, num_tris=args.tris)
+    t_scene = time.time() - t0
+    ctx = Context(0)
+    g = Gather(ctx, sc.width, sc.height, sc.materials)
+    g.set_gbuffer(device_array(sc.pos4), device_array(sc.nrm4), device_array(sc.albedo4))
+    av = DeviceAvpls.from_scene(sc)
+    params = default_params(useLightCuts=1, antiradianceModel=ffi.ARC_ANTIRAD_DISK)
+    flags = ffi.ARC_RUN_TIMELINE | ffi.ARC_TRUST_INPUT
+
+    for _ in range(args.warmup):
+        g.run(av, sc.cam, sc.extent, params, flags)
+    torch.cuda.synchronize()
+    start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    start.record()
+    for _ in range(args.steps):
+        g.run(av, sc.cam, sc.extent, params, flags)
+    stop.record()
+    torch.cuda.synchronize()
+    ms_per_step = start.elapsed_time(stop) / args.steps
+    hist = np.zeros((args.steps, 6), np.float32)
+    got = g.lib.arc_gather_profile_history(g.h, args.steps, hist.ctypes.data)
+    hist = hist[:max(got, 0)]
+    phase_ms = {k: float(hist[:, i].mean()) for i, k in enumerate(PHASES)} if len(hist) else {}
+    frame_ms = hist[:, 5] if len(hist) else np.array([ms_per_step])
+
+    # untimed: work counters and the error against the exact FP64 gather
+    st = g.run(av, sc.cam, sc.extent, params, ffi.ARC_RUN_COUNTERS | ffi.ARC_TRUST_INPUT)
+    torch.cuda.synchronize()
+    result = g.result[:, :3].double().cpu().numpy()
+    err = None
+    if args.error_stride > 0:
+        k = args.error_stride
+        ys, xs = np.arange(0, sc.height, k), np.arange(0, sc.width, k)
+        pix = (ys[:, None] * sc.width + xs[None, :]).ravel()
+        sub = S.Scene(**{**sc.__dict__, "width": len(xs), "height": len(ys), "pos4": sc.pos4[pix].copy(),
+                         "nrm4": sc.nrm4[pix].copy(), "albedo4": sc.albedo4[pix].copy()})
+        gs = Gather(ctx, sub.width, sub.height, sub.materials)
+        gs.set_gbuffer(device_array(sub.pos4), device_array(sub.nrm4), device_array(sub.albedo4))
+        t1 = time.time()
+        r64, a64 = gs.exact_f64(av, sc.cam, sc.extent, params)
+        torch.cuda.synchronize()
+        t_f64 = time.time() - t1
+        ref = (r64 - a64).cpu().numpy()
+        diff = result[pix] - ref
+        err = {"rel_l2": float(np.linalg.norm(diff) / np.linalg.norm(ref)),
+               "max_abs": float(np.abs(diff).max()),
+               "ref_norm_rms": float(np.sqrt(np.mean(ref ** 2))),
+               "pixels": int(len(pix)), "sample": f"every {k}th pixel in x and y", "f64_seconds": round(t_f64, 2)}
+        gs.close()
+
+    # roofline of the dominant kernel (phase timed on the library's stream)
+    npix, n = sc.npix, sc.num_avpls
+    dom = max(("bvh", "radiance", "antiradiance"), key=lambda k: phase_ms.get(k, 0.0))
+    if dom == "antiradiance":
+        flop = float(npix) * n * FLOP_DISK_PAIR
+        roof = {"kernel": "k_exact (antiradiance, disk)", "bound": "valu"}
+    elif dom == "radiance":
+        flop = float(st.visitedNodes) * FLOP_VISIT + float(st.evaluatedNodes) * (FLOP_EVAL + FLOP_RAD_PAIR)
+        roof = {"kernel": "k_cut_sa (light-cut radiance)", "bound": "valu"}
+    else:
+        flop = None
+        roof = {"kernel": "LBVH rebuild", "bound": "hbm"}
+    t_dom = phase_ms.get(dom, float("nan"))
+    if flop is not None:
+        ach = flop / (t_dom * 1e-3) / 1e12
+        roof.update({"achieved": round(ach, 3), "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+                     "frac": round(ach / PEAK_FP32_TFLOPS, 4), "traffic": None, "ms": round(t_dom, 4)})
+    else:
+        bytes_ = 276.0 * n
+        ach = bytes_ / (t_dom * 1e-3) / 1e9
+        roof.update({"achieved": round(ach, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                     "frac": round(ach / PEAK_HBM_GBS, 4), "traffic": None, "ms": round(t_dom, 4)})
+
+    cpu = cpu_baseline(sc, params, args.cpu_sample) if args.cpu_sample > 0 else None
+
+    value = npix * float(n) / (ms_per_step * 1e-3)
+    line = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "pixel-AVPL pairs/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 4),
+        "ms_per_frame_p50": round(float(np.percentile(frame_ms, 50)), 4),
+        "ms_per_frame_p99": round(float(np.percentile(frame_ms, 99)), 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": f"clutter_{sc.width}x{sc.height}: Cornell walls + {args.tris} random triangles, "
+                        f"{n} AVPLs (bounce 1-4, L log-uniform [1e-3,10], 30% antiradiance); AvplBvh rebuilt "
+                        "per frame; light-cut radiance (solid angle < pi/10); exact disk antiradiance",
+            "width": sc.width, "height": sc.height, "avpls": n, "parallelism": "single GPU",
+            "phase_ms": {k: round(v, 4) for k, v in phase_ms.items()},
+            "cut": {"visited_nodes": int(st.visitedNodes), "evaluated_nodes": int(st.evaluatedNodes),
+                    "error_pixels": int(st.errorPixels)},
+            "scene_build_s": round(t_scene, 2),
+        },
+        "error_vs_f64": err,
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+    g.close()
+    ctx.close()
+    return 0
+
+
+def cpu_baseline(sc, params, k: int):
+    """Oracle (C++ restatement, REF32) on this host: full AvplBvh build, then
+    the cut radiance and the exact antiradiance on 1/k of the pixels,
+    extrapolated to a frame."""
+    from oracle import pyoracle as orc
+    from antiradiancecuts_amd.runtime import photon_radius
+
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    rand = np.random.default_rng(0).random(2 * sc.num_avpls - 1, dtype=np.float32)
+    t0 = time.time()
+    tree = orc.avpl_bvh_build(sc, rand)
+    t_bvh = time.time() - t0
+    pix = np.arange(0, sc.npix, k, dtype=np.int32)
+    ref = orc.SceneRef(sc, photon_radius(params, sc.extent))
+    t0 = time.time()
+    orc.gather_cut_solid_angle(ref, tree, params.clusterRefinementThreshold, 32, pix)
+    t_cut = time.time() - t0
+    t0 = time.time()
+    orc.gather_exact(ref, 32, pix, what=2)
+    t_anti = time.time() - t0
+    frame_s = t_bvh + (t_cut + t_anti) * sc.npix / len(pix)
+    return {"value": sc.npix * float(sc.num_avpls) / frame_s, "unit": "pixel-AVPL pairs/s", "cores": threads,
+            "kind": "port", "ms_per_frame": round(frame_s * 1e3, 1),
+            "sample": f"AvplBvh build of all {sc.num_avpls} AVPLs (1 thread) + cut radiance and exact "
+                      f"antiradiance on every {k}th pixel ({len(pix)} px, {threads} threads), extrapolated "
+                      f"to the {sc.npix}-pixel frame; measured {t_bvh + t_cut + t_anti:.1f} s"}
+
+
+if __name__ == "__main__":
+    sys.exit(main())

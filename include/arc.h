@@ -104,9 +104,11 @@ typedef struct arc_ctx arc_ctx;
  * the caller names it. */
 int arc_ctx_create(int device, arc_ctx** out);
 void arc_ctx_destroy(arc_ctx* ctx);
-/* Order all work on `hip_stream` (a hipStream_t); NULL restores the
- * context's own stream. */
+/* Order all work on `hip_stream` (a hipStream_t); NULL is the device's
+ * default (null) stream, as used by hosts that never create streams. */
 int arc_ctx_set_stream(arc_ctx* ctx, void* hip_stream);
+/* Go back to the stream the context created (the state after create). */
+int arc_ctx_use_own_stream(arc_ctx* ctx);
 void* arc_ctx_get_stream(arc_ctx* ctx);
 int arc_ctx_synchronize(arc_ctx* ctx);
 const char* arc_ctx_last_error(arc_ctx* ctx);
@@ -214,6 +216,7 @@ typedef struct arc_gather_stats {
 
 #define ARC_RUN_PROFILE 1      /* time each phase with HIP events (synchronises)  */
 #define ARC_RUN_COUNTERS 2     /* count visited / evaluated nodes                  */
+#define ARC_RUN_TIMELINE 4     /* record phase events without synchronising       */
 /* ARC_INPUT_HOST (256): the arc_avpls pointers are host memory, uploaded as
  * AvplsGpu does (data_gpu.cu:5-52). */
 
@@ -245,6 +248,13 @@ int arc_gather_run(arc_gather* g, const arc_avpls* avpls, arc_float3 camPos, flo
  * Not in the reference; the reference's exact kernels are FP32. */
 int arc_gather_exact_f64(arc_gather* g, const arc_avpls* avpls, arc_float3 camPos, float sceneExtent,
                          const arc_gather_params* params, int flags, double* radiance3, double* antiradiance3);
+
+/* Phase times (ms) of the last `frames` runs recorded with ARC_RUN_PROFILE or
+ * ARC_RUN_TIMELINE, oldest first, 6 floats per frame: upload, bvh,
+ * radiance, antiradiance, combine, total.  At most 64 frames are kept; waits
+ * for the newest.  Returns the number of frames written or a negative status.
+ * Replaces the CudaTimer prints of run() (cudaGather.cu:845-1049). */
+int arc_gather_profile_history(arc_gather* g, int frames, float* out);
 
 /* The AvplBvh of the last cut frame (borrowed; NULL before the first). */
 arc_bvh* arc_gather_get_bvh(arc_gather* g);

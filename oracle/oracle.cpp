@@ -312,17 +312,28 @@ void pair(const orc_scene* s, const Pixel<T>& px, int i, V<T>& r, V<T>& a) {
 }
 
 template <typename T>
-void gather_exact(const orc_scene* s, const int* pixels, int npix, double* rad3, double* anti3) {
+void gather_exact(const orc_scene* s, const int* pixels, int npix, double* rad3, double* anti3, int what) {
+    const bool do_rad = what & 1, do_anti = what & 2;
 #pragma omp parallel for schedule(dynamic, 4)
     for (int q = 0; q < npix; ++q) {
         const int p = pixels ? pixels[q] : q;
         const Pixel<T> px = pixel_of<T>(s, p);
+        const V<T> cam = mk<T>(s->cam[0], s->cam[1], s->cam[2]);
         V<T> R = mk<T>(0, 0, 0), A = mk<T>(0, 0, 0);
         for (int i = 0; i < s->num_avpls; ++i) {
-            V<T> r, a;
-            pair<T>(s, px, i, r, a);
-            R = R + r;
-            A = A + a;
+            const V<T> apos = ld<T>(s->apos3 + 3 * (size_t)i), anrm = ld<T>(s->anrm3 + 3 * (size_t)i);
+            const V<T> aw = ld<T>(s->adir3 + 3 * (size_t)i);
+            if (do_rad) {
+                const Mat<T> am = material<T>(s, s->amat[i]);
+                R = R + radiance(px.pos, px.nrm, px.mat, apos, anrm, aw, ld<T>(s->arad3 + 3 * (size_t)i), am, cam);
+            }
+            if (do_anti) {
+                const V<T> Av = ld<T>(s->aanti3 + 3 * (size_t)i);
+                if (s->antirad_model == ORC_ANTIRAD_CONE)
+                    A = A + antirad_cone(px.pos, px.nrm, px.mat, apos, aw, Av, cam, (T)s->cone_half_angle);
+                else
+                    A = A + antirad_disk(px.pos, px.nrm, px.mat, apos, anrm, aw, Av, cam, (T)s->photon_radius);
+            }
         }
         rad3[3 * (size_t)q + 0] = R.x;
         rad3[3 * (size_t)q + 1] = R.y;
@@ -612,11 +623,16 @@ int orc_pair(const orc_scene* s, int precision, int pixel, int avpl, double* rad
 
 int orc_gather_exact(const orc_scene* s, int precision, const int* pixels, int npix,
                      double* rad3, double* anti3) {
+    return orc_gather_exact_parts(s, precision, pixels, npix, rad3, anti3, 3);
+}
+
+int orc_gather_exact_parts(const orc_scene* s, int precision, const int* pixels, int npix,
+                           double* rad3, double* anti3, int what) {
     if (!s || npix < 0) return -1;
     if (precision == 64)
-        gather_exact<double>(s, pixels, npix, rad3, anti3);
+        gather_exact<double>(s, pixels, npix, rad3, anti3, what);
     else
-        gather_exact<float>(s, pixels, npix, rad3, anti3);
+        gather_exact<float>(s, pixels, npix, rad3, anti3, what);
     return 0;
 }
 

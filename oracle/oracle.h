@@ -118,6 +118,10 @@ int orc_pair(const orc_scene* s, int precision, int pixel, int avpl, double* rad
  * pair checks plus superposition and linearity in L and A. */
 int orc_gather_exact(const orc_scene* s, int precision, const int* pixels, int npix,
                      double* rad3, double* anti3);
+/* As orc_gather_exact, summing only the radiance (what & 1) and / or the
+ * antiradiance (what & 2); the other output is left zero. */
+int orc_gather_exact_parts(const orc_scene* s, int precision, const int* pixels, int npix,
+                           double* rad3, double* anti3, int what);
 
 /* Light-cut radiance (kernel_bvh, cudaGather.cu:316-444) over an AvplBvh:
  * stack DFS (32 entries, right child first), corner visibility cull,

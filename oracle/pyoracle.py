@@ -65,6 +65,7 @@ def lib() -> C.CDLL:
                                          vp, vp, vp, vp, vp, vp]),
             "orc_pair": (i32, [C.POINTER(orc_scene), i32, i32, i32, vp, vp]),
             "orc_gather_exact": (i32, [C.POINTER(orc_scene), i32, vp, i32, vp, vp]),
+            "orc_gather_exact_parts": (i32, [C.POINTER(orc_scene), i32, vp, i32, vp, vp, i32]),
             "orc_gather_cut_solid_angle": (i32, [C.POINTER(orc_scene), i32, i32, vp, vp, vp, vp, vp, vp, f32,
                                                  vp, i32, vp, vp, C.POINTER(C.c_longlong),
                                                  C.POINTER(C.c_longlong)]),
@@ -173,12 +174,12 @@ class SceneRef:
         self.s = s
 
 
-def gather_exact(ref: SceneRef, precision: int = 64, pixels: np.ndarray | None = None):
+def gather_exact(ref: SceneRef, precision: int = 64, pixels: np.ndarray | None = None, what: int = 3):
     npix = ref.s.width * ref.s.height if pixels is None else len(pixels)
     rad = np.zeros((npix, 3), np.float64)
     anti = np.zeros((npix, 3), np.float64)
     px = None if pixels is None else np.ascontiguousarray(pixels, np.int32)
-    st = lib().orc_gather_exact(C.byref(ref.s), precision, _p(px), npix, _p(rad), _p(anti))
+    st = lib().orc_gather_exact_parts(C.byref(ref.s), precision, _p(px), npix, _p(rad), _p(anti), what)
     assert st == 0
     return rad, anti
 
