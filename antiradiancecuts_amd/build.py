@@ -30,6 +30,7 @@ HIP_SOURCES = {
     "sort.hip": [],
     "lbvh.hip": ["-ffp-contract=off"],
     "gather.hip": [],
+    "gather_anti.hip": [],
 }
 
 HIPCC_FLAGS = [

@@ -21,59 +21,14 @@
 #include <cstring>
 #include <vector>
 
-#include "arc_internal.hpp"
-#include "gather_math.hpp"
+#include "gather_common.hpp"
 
 using arc::DevBuf;
 using arc::f3;
 using namespace arc::gm;
-
-struct arc_gather {
-    arc_ctx* ctx = nullptr;
-    int width = 0, height = 0;
-    DevBuf mats;
-    int num_mats = 0;
-    bool general = false;  // some material has a specular lobe
-    const float4* gpos = nullptr;
-    const float4* gnrm = nullptr;
-    const float4* galb = nullptr;
-    float4* out_res = nullptr;
-    float4* out_rad = nullptr;
-    float4* out_anti = nullptr;
-    DevBuf int_rad, int_anti, int_res;
-    DevBuf rec, part, counters;
-    DevBuf stage;  // AVPL SoA staged from host memory
-    arc_bvh* bvh = nullptr;
-    // per-frame phase events: frame slot, then 6 marks (start, upload done,
-    // bvh done, radiance done, antiradiance done, end)
-    static constexpr int kRing = 64;
-    hipEvent_t ev[kRing][6] = {};
-    int ring_next = 0, ring_count = 0;
-};
+using namespace arc::gd;
 
 namespace {
-
-constexpr int kThreads = 256;
-constexpr int kRecF4 = 5;  // float4 per packed AVPL record
-constexpr int kRun = 64;   // pairs summed plainly before a compensated add
-// Relative guard band of the FP32 antiradiance decisions (FP32 rounding of
-// these quantities is below 1e-6 relative); pairs inside it are re-decided
-// in FP64.
-constexpr float kGuard = 1e-5f;
-
-enum Counter { C_VISITED = 0, C_EVALUATED = 1, C_ANTI_VISITED = 2, C_ANTI_EVALUATED = 3, C_ERRORS = 4, C_NUM = 8 };
-
-struct Kahan {
-    float s = 0.f, c = 0.f;
-    __device__ inline void add(float x) {
-        const float y = x - c;
-        const float t = s + y;
-        c = (t - s) - y;
-        s = t;
-    }
-};
-
-__device__ inline f3 f3of(float4 v) { return f3{v.x, v.y, v.z}; }
 
 // ------------------------------------------------------------- pack ----
 __global__ __launch_bounds__(kThreads) void k_pack(arc::AvplsDev a, const arc_material* __restrict__ mats,
@@ -103,73 +58,6 @@ __global__ __launch_bounds__(kThreads) void k_pack(arc::AvplsDev a, const arc_ma
     r[4] = make_float4(A.x * as, A.y * as, A.z * as, kGuard * sqrtf(w2));  // guard for |w . n|
 }
 
-// ----------------------------------------------------------- pixels ----
-struct PixelF {
-    float px, py, pz;  // position
-    float nx, ny, nz;  // unit normal
-    float br, bg, bb;  // diffuse / pi (Lambertian pixel BRDF)
-    int mat;
-};
-
-__device__ inline PixelF load_pixel(const float4* gpos, const float4* gnrm, const float4* galb,
-                                    const arc_material* mats, int pix) {
-    PixelF P;
-    const float4 p = gpos[pix], n = gnrm[pix];
-    P.px = p.x; P.py = p.y; P.pz = p.z;
-    const float s = rsq(n.x * n.x + n.y * n.y + n.z * n.z);
-    P.nx = n.x * s; P.ny = n.y * s; P.nz = n.z * s;
-    P.mat = (int)n.w;
-    float dr, dg, db;
-    if (galb) {
-        const float4 a = galb[pix];
-        dr = a.x; dg = a.y; db = a.z;
-    } else {
-        const arc_material& m = mats[mat_slot(P.mat)];
-        dr = m.diffuse.x; dg = m.diffuse.y; db = m.diffuse.z;
-    }
-    P.br = kOneOverPi * dr; P.bg = kOneOverPi * dg; P.bb = kOneOverPi * db;
-    return P;
-}
-
-// ------------------------------------------------------- decisions ----
-// The antiradiance tests are discontinuous (inside / outside a disk or cone,
-// sign tests of dot products).  A decision is made in FP32 when the FP32
-// value is farther from its threshold than a guard that bounds the FP32
-// rounding error (with a wide margin); pairs inside the guard band are
-// re-decided in FP64 from the raw inputs with the reference's formulas, which
-// is the decision the FP64 oracle makes.  Pairs in the band are rare except
-// for coplanar pixel / AVPL pairs, whose signs the reference leaves to
-// rounding noise.
-
-// getAntiradiance's tests (cudaGather.cu:116-140) in FP64.
-__device__ __attribute__((noinline)) bool disk_accept_f64(float4 p4, float4 n4, f3 a, f3 an, f3 w, float radius) {
-    using V3 = V<double>;
-    const V3 pos = mk<double>(p4.x, p4.y, p4.z), nrm = mk<double>(n4.x, n4.y, n4.z);
-    const V3 apos = ld3<double>(a), anrm = ld3<double>(an), aw = ld3<double>(w);
-    const double denom = dot(aw, nrm);
-    if (denom == 0.0) return false;
-    const double t = dot(pos - apos, nrm) / denom;
-    if (t <= 0.0) return false;
-    const V3 hit = apos + t * normalize(aw);
-    if (length(hit - pos) >= (double)radius) return false;
-    if (dot(normalize(hit - apos), aw) <= 0.0) return false;
-    if (dot(nrm, aw) >= 0.0) return false;
-    if (dot(normalize(pos - apos), anrm) >= 0.0) return false;
-    return true;
-}
-
-// GetAntiradiance's tests (Shaders/Gather.frag:155-166) in FP64.
-__device__ __attribute__((noinline)) bool cone_accept_f64(float4 p4, float4 n4, f3 a, f3 w, double cos_half) {
-    using V3 = V<double>;
-    const V3 pos = mk<double>(p4.x, p4.y, p4.z), nrm = mk<double>(n4.x, n4.y, n4.z);
-    const V3 apos = ld3<double>(a), aw = ld3<double>(w);
-    const V3 to_pos = normalize(pos - apos);
-    const double c = dot(to_pos, normalize(aw));
-    if (c <= (double)kConeEps) return false;
-    if (dot(nrm, -to_pos) <= (double)kConeEps) return false;
-    return c > cos_half;
-}
-
 // ------------------------------------------------------------ exact ----
 // part layout: [split][6][npix] = radiance rgb, antiradiance rgb.
 template <int MODEL, bool GENERAL, bool DO_RAD, bool DO_ANTI>
@@ -178,14 +66,12 @@ __global__ __launch_bounds__(kThreads) void k_exact(const float4* __restrict__ r
                                                     const float4* __restrict__ gnrm,
                                                     const float4* __restrict__ galb,
                                                     const arc_material* __restrict__ mats, int npix,
-                                                    float camx, float camy, float camz, float radius,
-                                                    float cos_half, double cos_half_d, float* __restrict__ part) {
+                                                    float camx, float camy, float camz, AntiParams ap,
+                                                    float* __restrict__ part) {
     const int pix = blockIdx.x * kThreads + threadIdx.x;
     const bool live = pix < npix;
     const int q = live ? pix : 0;
     const PixelF P = load_pixel(gpos, gnrm, galb, mats, q);
-    const float r2 = radius * radius;
-    const float cone_lo = fmaxf(kConeEps, cos_half) - kGuard;
     const int n_avpl = raw.n;
     const int a0 = blockIdx.y * chunk;
     const int a1 = min(n_avpl, a0 + chunk);
@@ -204,41 +90,9 @@ __global__ __launch_bounds__(kThreads) void k_exact(const float4* __restrict__ r
             const float cn = P.nx * dx + P.ny * dy + P.nz * dz;        // |d| cos at the pixel
             const float ca = -(r1.x * dx + r1.y * dy + r1.z * dz);     // |d| cos at the AVPL
             // antiradiance decision and weight
-            float m = 0.f;
-            if (DO_ANTI) {
-                bool in;
-                if (MODEL == ARC_ANTIRAD_DISK) {
-                    // line a + t w hits the tangent plane at t = -cn / (w . n) > 0, within the
-                    // disk, on the far side of the AVPL's surface, pixel facing against w.
-                    // With h = d + t w_unit, |h| < r  <=>  |h denom|^2 < r^2 denom^2, and
-                    // h denom = d denom - cn w_unit: no division needed.
-                    const float denom = r2v.x * P.nx + r2v.y * P.ny + r2v.z * P.nz;
-                    const float hx = dx * denom - cn * r0.w;
-                    const float hy = dy * denom - cn * r1.w;
-                    const float hz = dz * denom - cn * r2v.w;
-                    const float h2 = hx * hx + hy * hy + hz * hz;
-                    const float dn2 = denom * denom;
-                    const float lim = r2 * dn2;
-                    const float g = kGuard * (dd * dn2 + cn * cn);
-                    const float dl = kGuard * (dd * inv);
-                    in = denom < 0.f && cn > 0.f && h2 < lim && ca < 0.f;
-                    const bool amb = h2 < lim + g &&
-                                     (!(fabsf(h2 - lim) > g) || !(fabsf(cn) > dl) || !(fabsf(ca) > dl) ||
-                                      !(fabsf(denom) > r4.w));
-                    if (amb) in = disk_accept_f64(gpos[q], gnrm[q], raw.pos[i], raw.nrm[i], raw.dir[i], radius);
-                    m = in ? 1.f : 0.f;
-                } else {
-                    // cone of half-angle acos(cos_half) about w, cosine at the pixel / dist^2
-                    const float c = -(dx * r0.w + dy * r1.w + dz * r2v.w) * inv;
-                    const float cp = cn * inv;
-                    in = c > kConeEps && cp > kConeEps && c > cos_half;
-                    const bool amb = c > cone_lo && cp > kConeEps - kGuard &&
-                                     (!(fabsf(c - cos_half) > kGuard) || !(fabsf(cp - kConeEps) > kGuard) ||
-                                      !(fabsf(c - kConeEps) > kGuard));
-                    if (amb) in = cone_accept_f64(gpos[q], gnrm[q], raw.pos[i], raw.dir[i], cos_half_d);
-                    m = in ? cp * (inv * inv) : 0.f;
-                }
-            }
+            const float m =
+                DO_ANTI ? anti_weight<MODEL>(P, r0, r1, r2v, r4.w, dx, dy, dz, dd, inv, cn, ca, ap, gpos, gnrm, q, raw, i)
+                        : 0.f;
             if (GENERAL) {
                 using V3 = V<float>;
                 const V3 pos = mk(P.px, P.py, P.pz), nrm = mk(P.nx, P.ny, P.nz);
@@ -290,8 +144,11 @@ __global__ __launch_bounds__(kThreads) void k_exact(const float4* __restrict__ r
 
 // Sums the splits and writes the float4 targets (kernel_combine semantics:
 // result = radiance - antiradiance including w, i.e. 1 - 1 = 0).
-__global__ __launch_bounds__(kThreads) void k_finalize(const float* __restrict__ part, int splits, int npix,
-                                                       int have_rad, int have_anti,
+// Radiance comes from rad_splits splits of `part`, or from rad_in when
+// rad_splits < 0 (the cut kernel wrote it); antiradiance from anti_splits
+// splits (0: none).
+__global__ __launch_bounds__(kThreads) void k_finalize(const float* __restrict__ part, int rad_splits,
+                                                       int anti_splits, int npix,
                                                        const float4* __restrict__ rad_in,
                                                        float4* __restrict__ out_rad,
                                                        float4* __restrict__ out_anti,
@@ -299,14 +156,14 @@ __global__ __launch_bounds__(kThreads) void k_finalize(const float* __restrict__
     const int pix = blockIdx.x * kThreads + threadIdx.x;
     if (pix >= npix) return;
     Kahan k[6];
-    for (int s = 0; s < splits; ++s)
-        for (int c = 0; c < 6; ++c)
-            if ((c < 3 && have_rad) || (c >= 3 && have_anti))
-                k[c].add(part[((size_t)s * 6 + c) * npix + pix]);
-    float4 rad = have_rad ? make_float4(k[0].s, k[1].s, k[2].s, 1.f) : rad_in[pix];
+    for (int s = 0; s < rad_splits; ++s)
+        for (int c = 0; c < 3; ++c) k[c].add(part[((size_t)s * 6 + c) * npix + pix]);
+    for (int s = 0; s < anti_splits; ++s)
+        for (int c = 3; c < 6; ++c) k[c].add(part[((size_t)s * 6 + c) * npix + pix]);
+    const float4 rad = rad_splits >= 0 ? make_float4(k[0].s, k[1].s, k[2].s, 1.f) : rad_in[pix];
     const float4 anti = make_float4(k[3].s, k[4].s, k[5].s, 1.f);
-    if (have_rad && out_rad) out_rad[pix] = rad;
-    if (have_anti && out_anti) out_anti[pix] = anti;
+    if (rad_splits >= 0 && out_rad) out_rad[pix] = rad;
+    if (out_anti) out_anti[pix] = anti;
     if (out_res)
         out_res[pix] = make_float4(rad.x - anti.x, rad.y - anti.y, rad.z - anti.z, rad.w - anti.w);
 }
@@ -498,7 +355,7 @@ void launch_exact(arc_gather* g, const arc::AvplsDev& a, int chunk, int splits, 
     const int npix = g->width * g->height;
     hipLaunchKernelGGL((k_exact<MODEL, GENERAL, DO_RAD, DO_ANTI>), dim3(blocks(npix), splits), dim3(kThreads), 0,
                        g->ctx->stream, g->rec.as<float4>(), a, chunk, g->gpos, g->gnrm, g->galb,
-                       g->mats.as<arc_material>(), npix, cam.x, cam.y, cam.z, radius, (float)cos_half, cos_half,
+                       g->mats.as<arc_material>(), npix, cam.x, cam.y, cam.z, anti_params(radius, cos_half),
                        g->part.as<float>());
 }
 
@@ -563,6 +420,8 @@ int validate_params(arc_gather* g, const arc_avpls* avpls, const arc_gather_para
     if (p->antiradianceModel != ARC_ANTIRAD_DISK && p->antiradianceModel != ARC_ANTIRAD_CONE)
         return arc::fail(g->ctx, ARC_E_INVALID, "antiradianceModel");
     if (p->stackSize < 2 || p->stackSize > 64) return arc::fail(g->ctx, ARC_E_INVALID, "stackSize out of [2, 64]");
+    if (p->antiradianceMode != ARC_ANTI_EXACT && p->antiradianceMode != ARC_ANTI_CLUSTERED)
+        return arc::fail(g->ctx, ARC_E_INVALID, "antiradianceMode");
     return ARC_OK;
 }
 
@@ -584,7 +443,7 @@ void arc_gather_params_default(arc_gather_params* p) {
     p->clusterRefinementThreshold = (float)(M_PI / 10.0);  // CConfigManager.cpp:37
     p->errorBound = 0.01f;
     p->antiradianceModel = ARC_ANTIRAD_DISK;
-    p->antiradianceCut = 0;
+    p->antiradianceMode = ARC_ANTI_CLUSTERED;   // useClusteredDeferred = 1, CConfigManager.cpp:16
     p->photonRadiusScale = 2.0f;                // CConfigManager.cpp:26
     p->coneHalfAngle = (float)(M_PI / 18.0);
     p->writeSentinel = 1;
@@ -703,9 +562,12 @@ int arc_gather_run(arc_gather* g, const arc_avpls* avpls, arc_float3 camPos, flo
     ARC_TRY(arc::ensure(ctx, g->part, sizeof(float) * 6 * (size_t)splits * npix));
     unsigned long long* counters = count ? g->counters.as<unsigned long long>() : nullptr;
 
-    bool cut = p->useLightCuts && n > 1;
+    const bool cut = p->useLightCuts && n > 1;
+    const bool clustered = p->antiradianceMode == ARC_ANTI_CLUSTERED && n > 0;
     if (p->useLightCuts && p->cutMode != ARC_CUT_SOLID_ANGLE)
         return arc::fail(ctx, ARC_E_UNSUPPORTED, "cutMode not available in this build");
+    // ---- radiance: light cut over this frame's AvplBvh (cudaGather.cu:854-923) or exact
+    int rad_splits = -1, anti_splits = 0;
     if (cut) {
         // AvplBvh(avpls, false) per frame (cudaGather.cu:858)
         ARC_TRY(arc::bvh_build_avpls(g->bvh, a, nullptr, p->seed, (flags & ARC_TRUST_INPUT) == 0));
@@ -720,24 +582,32 @@ int arc_gather_run(arc_gather* g, const arc_avpls* avpls, arc_float3 camPos, flo
                            p->clusterRefinementThreshold, p->stackSize, p->writeSentinel, rad, counters);
         ARC_LAUNCHED(ctx);
         if (profile) ARC_HIP(ctx, hipEventRecord(ev[3], ctx->stream));
-        if (n > 0) dispatch_exact<false, true>(g, p->antiradianceModel, a, chunk, splits, cam, radius, cos_half);
-        ARC_LAUNCHED(ctx);
-        if (profile) ARC_HIP(ctx, hipEventRecord(ev[4], ctx->stream));
-        hipLaunchKernelGGL(k_finalize, dim3(blocks(npix)), dim3(kThreads), 0, ctx->stream, g->part.as<float>(),
-                           n > 0 ? splits : 0, npix, 0, 1, rad, nullptr, anti, res);
-        ARC_LAUNCHED(ctx);
     } else {
         if (profile) ARC_HIP(ctx, hipEventRecord(ev[2], ctx->stream));
-        if (n > 0) dispatch_exact<true, true>(g, p->antiradianceModel, a, chunk, splits, cam, radius, cos_half);
-        ARC_LAUNCHED(ctx);
-        if (profile) {
-            ARC_HIP(ctx, hipEventRecord(ev[3], ctx->stream));
-            ARC_HIP(ctx, hipEventRecord(ev[4], ctx->stream));
+        rad_splits = n > 0 ? splits : 0;
+        if (n > 0 && !clustered) {  // one pass sums both (kernel_radiance_simple + kernel_antiradiance)
+            dispatch_exact<true, true>(g, p->antiradianceModel, a, chunk, splits, cam, radius, cos_half);
+            anti_splits = splits;
+        } else if (n > 0) {
+            dispatch_exact<true, false>(g, p->antiradianceModel, a, chunk, splits, cam, radius, cos_half);
         }
-        hipLaunchKernelGGL(k_finalize, dim3(blocks(npix)), dim3(kThreads), 0, ctx->stream, g->part.as<float>(),
-                           n > 0 ? splits : 0, npix, 1, 1, rad, rad, anti, res);
+        ARC_LAUNCHED(ctx);
+        if (profile) ARC_HIP(ctx, hipEventRecord(ev[3], ctx->stream));
+    }
+    // ---- antiradiance: clustered lists (kernel_antiradiance_clusterred) or exact
+    if (clustered) {
+        ARC_TRY(arc::anti_clustered(g, a, p->antiradianceModel, cam, anti_params(radius, cos_half),
+                                    (double)p->coneHalfAngle, g->part.as<float>(), counters));
+        anti_splits = 1;
+    } else if (cut && n > 0) {
+        dispatch_exact<false, true>(g, p->antiradianceModel, a, chunk, splits, cam, radius, cos_half);
+        anti_splits = splits;
         ARC_LAUNCHED(ctx);
     }
+    if (profile) ARC_HIP(ctx, hipEventRecord(ev[4], ctx->stream));
+    hipLaunchKernelGGL(k_finalize, dim3(blocks(npix)), dim3(kThreads), 0, ctx->stream, g->part.as<float>(),
+                       rad_splits, anti_splits, npix, rad, rad, anti, res);
+    ARC_LAUNCHED(ctx);
     if (profile) ARC_HIP(ctx, hipEventRecord(ev[5], ctx->stream));
     if (stats) {
         stats->numLeafs = cut ? g->bvh->n : 0;

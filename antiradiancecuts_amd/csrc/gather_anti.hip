@@ -1,0 +1,344 @@
+// Clustered antiradiance on gfx950 — the reference's clustered-deferred path
+// (VisiblePointsBvh, bvh.cu:963-1208; kernel_gather_cluster_lights,
+// cudaGather.cu:686-748; kernel_antiradiance_clusterred, :605-659).
+//
+// The reference clusters visible points, lists for every cluster the AVPLs
+// whose incident ray hits the cluster's box grown by 2 photonRadius (ray-box
+// slab test, cudaGather.cu:664-684), and sums each pixel over its cluster's
+// list.  A disk hit needs the ray to pass within the photon radius of the
+// pixel, so the list holds every AVPL that can contribute: the result equals
+// the exact gather.  Here:
+//   k_flag_count / k_scan_blocks / k_compact
+//                  stable compaction of the AVPLs with non-zero antiradiance
+//                  (the rest contribute nothing), no host round trip;
+//   k_cluster_bounds
+//                  one cluster per 32x32 pixel block, box of its G-buffer
+//                  positions (the reference adds depth slices; a block of
+//                  the screen with its tight box is what the list test needs);
+//   k_cluster_lists
+//                  lane = AVPL, loop over clusters with box data read by
+//                  wave-uniform loads; ray-box slab test against the box
+//                  grown by the photon radius plus a margin, and the
+//                  "pixel behind the AVPL's surface" bound; hits appended
+//                  with one atomic per wave and cluster (ballot + popcount);
+//                  the reference's C x N list buffer becomes C x kCap with
+//                  an overflow flag;
+//   k_anti_clustered
+//                  one 8x8 pixel tile per wave; the wave walks its cluster's
+//                  list with wave-uniform (scalar) loads of the entries and
+//                  the packed records and evaluates the guarded pair test
+//                  per pixel.  A cluster whose list overflowed walks every
+//                  antiradiance AVPL instead.
+#include <cmath>
+
+#include "gather_common.hpp"
+
+using arc::f3;
+using namespace arc::gm;
+using namespace arc::gd;
+
+namespace {
+
+constexpr int kClusterPx = 32;   // cluster side in pixels
+constexpr int kCap = 16384;      // list entries per cluster
+constexpr int kCompact = 1024;   // compaction block
+constexpr int kClustersPerBlock = 64;
+
+__device__ inline bool has_anti(const arc::AvplsDev& a, int i) {
+    if (!a.anti) return false;
+    const f3 A = a.anti[i];
+    return A.x != 0.f || A.y != 0.f || A.z != 0.f;
+}
+
+__device__ inline uint32_t block_count(bool flag, uint32_t* sh) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const uint64_t m = __ballot(flag);
+    if (lane == 0) sh[wave] = (uint32_t)__popcll(m);
+    __syncthreads();
+    uint32_t t = 0;
+    for (int w = 0; w < (int)(blockDim.x >> 6); ++w) t += sh[w];
+    return t;
+}
+
+__global__ __launch_bounds__(kCompact) void k_flag_count(arc::AvplsDev a, int* __restrict__ counts) {
+    __shared__ uint32_t sh[kCompact / 64];
+    const int i = blockIdx.x * kCompact + threadIdx.x;
+    const bool f = i < a.n && has_anti(a, i);
+    const uint32_t t = block_count(f, sh);
+    if (threadIdx.x == 0) counts[blockIdx.x] = (int)t;
+}
+
+// Exclusive scan of nb block counts in one block; counts[nb] = total.
+__global__ __launch_bounds__(1024) void k_scan_blocks(int* __restrict__ counts, int nb) {
+    __shared__ int sh[1024];
+    int run = 0;
+    for (int base = 0; base < nb; base += 1024) {
+        const int t = base + (int)threadIdx.x;
+        const int v = t < nb ? counts[t] : 0;
+        sh[threadIdx.x] = v;
+        __syncthreads();
+        for (int off = 1; off < 1024; off <<= 1) {
+            const int u = (int)threadIdx.x >= off ? sh[threadIdx.x - off] : 0;
+            __syncthreads();
+            sh[threadIdx.x] += u;
+            __syncthreads();
+        }
+        if (t < nb) counts[t] = run + sh[threadIdx.x] - v;
+        run += sh[1023];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) counts[nb] = run;
+}
+
+__global__ __launch_bounds__(kCompact) void k_compact(arc::AvplsDev a, const int* __restrict__ offs,
+                                                      int* __restrict__ idx) {
+    __shared__ uint32_t sh[kCompact / 64];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int i = blockIdx.x * kCompact + threadIdx.x;
+    const bool f = i < a.n && has_anti(a, i);
+    const uint64_t m = __ballot(f);
+    if (lane == 0) sh[wave] = (uint32_t)__popcll(m);
+    __syncthreads();
+    uint32_t before = 0;
+    for (int w = 0; w < wave; ++w) before += sh[w];
+    if (f) idx[offs[blockIdx.x] + before + __popcll(m & ((1ull << lane) - 1ull))] = i;
+}
+
+// Box of the G-buffer positions of each 32x32 block: [C][8] = min xyz, -, max xyz, -.
+__global__ __launch_bounds__(256) void k_cluster_bounds(const float4* __restrict__ gpos, int W, int H, int cw,
+                                                        float* __restrict__ cbox) {
+    __shared__ float s[6][256];
+    const int c = blockIdx.x;
+    const int x0 = (c % cw) * kClusterPx, y0 = (c / cw) * kClusterPx;
+    float v[6] = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY};
+    for (int k = threadIdx.x; k < kClusterPx * kClusterPx; k += 256) {
+        const int x = x0 + (k % kClusterPx), y = y0 + (k / kClusterPx);
+        if (x < W && y < H) {
+            const float4 p = gpos[y * W + x];
+            v[0] = fminf(v[0], p.x); v[1] = fminf(v[1], p.y); v[2] = fminf(v[2], p.z);
+            v[3] = fmaxf(v[3], p.x); v[4] = fmaxf(v[4], p.y); v[5] = fmaxf(v[5], p.z);
+        }
+    }
+    for (int k = 0; k < 6; ++k) s[k][threadIdx.x] = v[k];
+    __syncthreads();
+    for (int off = 128; off > 0; off >>= 1) {
+        if ((int)threadIdx.x < off)
+            for (int k = 0; k < 6; ++k)
+                s[k][threadIdx.x] = k < 3 ? fminf(s[k][threadIdx.x], s[k][threadIdx.x + off])
+                                          : fmaxf(s[k][threadIdx.x], s[k][threadIdx.x + off]);
+        __syncthreads();
+    }
+    if (threadIdx.x < 8) {
+        const int k = threadIdx.x;
+        cbox[c * 8 + k] = k == 3 || k == 7 ? 0.f : s[k < 3 ? k : k - 1][0];
+    }
+}
+
+__global__ __launch_bounds__(256) void k_zero(int* __restrict__ p, int n) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i < n) p[i] = 0;
+}
+
+__device__ inline float safe_inv(float w) {
+    // keeps the slab parameters finite (no 0 * inf) for axis-parallel rays
+    const float m = fmaxf(fabsf(w), 1e-20f);
+    return 1.0f / copysignf(m, w);
+}
+
+// Disk model: the forward ray must hit the box grown by the photon radius
+// (a disk hit lies on the ray within that radius of a pixel), and part of the
+// box must lie behind the AVPL's surface.  Cone model: the cone of half-angle
+// theta about w must reach the box's bounding sphere: angle(c - a, w) <=
+// theta + asin(R / |c - a|), with theta slightly enlarged (cos_t, sin_t).
+template <int MODEL>
+__global__ __launch_bounds__(256) void k_cluster_lists(arc::AvplsDev a, const int* __restrict__ idx,
+                                                       const int* __restrict__ n_anti_ptr,
+                                                       const float* __restrict__ cbox, int C, float grow,
+                                                       float cos_t, float sin_t,
+                                                       int* __restrict__ counts, int* __restrict__ lists) {
+    const int n_anti = *n_anti_ptr;
+    const int j = blockIdx.x * 256 + threadIdx.x;
+    const bool valid = j < n_anti;
+    const int lane = threadIdx.x & 63;
+    const int i = valid ? idx[j] : 0;
+    f3 p{0.f, 0.f, 0.f}, w{1.f, 0.f, 0.f}, na{0.f, 0.f, 0.f};
+    if (valid) {
+        p = a.pos[i];
+        w = a.dir[i];
+        na = a.nrm[i];
+    }
+    const float ix = safe_inv(w.x), iy = safe_inv(w.y), iz = safe_inv(w.z);
+    const int c0 = blockIdx.y * kClustersPerBlock, c1 = min(C, c0 + kClustersPerBlock);
+    for (int c = c0; c < c1; ++c) {
+        const float* b = cbox + (size_t)c * 8;
+        const float bx0 = b[0] - grow, by0 = b[1] - grow, bz0 = b[2] - grow;
+        const float bx1 = b[4] + grow, by1 = b[5] + grow, bz1 = b[6] + grow;
+        // ray-box slab test on the forward ray (intersectBB, cudaGather.cu:664-684)
+        const float tx0 = (bx0 - p.x) * ix, tx1 = (bx1 - p.x) * ix;
+        const float ty0 = (by0 - p.y) * iy, ty1 = (by1 - p.y) * iy;
+        const float tz0 = (bz0 - p.z) * iz, tz1 = (bz1 - p.z) * iz;
+        const float tmin = fmaxf(fmaxf(fminf(tx0, tx1), fminf(ty0, ty1)), fminf(tz0, tz1));
+        const float tmax = fminf(fminf(fmaxf(tx0, tx1), fmaxf(ty0, ty1)), fmaxf(tz0, tz1));
+        // some point of the box lies behind the AVPL's surface: min n_a . (x - a) < 0
+        const float behind = fminf(na.x * (bx0 - p.x), na.x * (bx1 - p.x)) +
+                             fminf(na.y * (by0 - p.y), na.y * (by1 - p.y)) +
+                             fminf(na.z * (bz0 - p.z), na.z * (bz1 - p.z));
+        bool hit;
+        if (MODEL == ARC_ANTIRAD_DISK) {
+            hit = valid && tmax >= fmaxf(tmin, 0.f) && behind < grow;
+        } else {
+            const float cx = 0.5f * (b[0] + b[4]) - p.x, cy = 0.5f * (b[1] + b[5]) - p.y,
+                        cz = 0.5f * (b[2] + b[6]) - p.z;
+            const float ex = 0.5f * (b[4] - b[0]), ey = 0.5f * (b[5] - b[1]), ez = 0.5f * (b[6] - b[2]);
+            const float R = sqrtf(ex * ex + ey * ey + ez * ez) + grow;
+            const float D = sqrtf(cx * cx + cy * cy + cz * cz);
+            const float wl = sqrtf(w.x * w.x + w.y * w.y + w.z * w.z);
+            const float cosang = (cx * w.x + cy * w.y + cz * w.z) / fmaxf(D * wl, 1e-30f);
+            const float sb = fminf(R / fmaxf(D, 1e-30f), 1.f), cb = sqrtf(fmaxf(0.f, 1.f - sb * sb));
+            hit = valid && (D <= R || cosang >= cos_t * cb - sin_t * sb - 1e-6f);
+        }
+        const uint64_t m = __ballot(hit);
+        if (m) {
+            const int leader = __ffsll((long long)m) - 1;
+            int base = 0;
+            if (lane == leader) base = atomicAdd(&counts[c], (int)__popcll(m));
+            base = __shfl(base, leader);
+            if (hit) {
+                const int slot = base + (int)__popcll(m & ((1ull << lane) - 1ull));
+                if (slot < kCap) lists[(size_t)c * kCap + slot] = i;
+            }
+        }
+    }
+}
+
+template <int MODEL, bool GENERAL>
+__global__ __launch_bounds__(256) void k_anti_clustered(const float4* __restrict__ rec, arc::AvplsDev raw,
+                                                        const float4* __restrict__ gpos,
+                                                        const float4* __restrict__ gnrm,
+                                                        const float4* __restrict__ galb,
+                                                        const arc_material* __restrict__ mats, int W, int H,
+                                                        int cw, float camx, float camy, float camz, AntiParams ap,
+                                                        const int* __restrict__ counts,
+                                                        const int* __restrict__ lists,
+                                                        const int* __restrict__ idx,
+                                                        const int* __restrict__ n_anti_ptr,
+                                                        float* __restrict__ part,
+                                                        unsigned long long* __restrict__ counters) {
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const int x = blockIdx.x * 16 + (wave & 1) * 8 + (lane & 7);
+    const int y = blockIdx.y * 16 + (wave >> 1) * 8 + (lane >> 3);
+    const bool live = x < W && y < H;
+    const int npix = W * H;
+    const int q = live ? y * W + x : 0;
+    const PixelF P = load_pixel(gpos, gnrm, galb, mats, q);
+    // the wave's 8x8 tile lies inside one 32x32 cluster
+    const int c = (blockIdx.y * 16 / kClusterPx) * cw + (blockIdx.x * 16 / kClusterPx);
+    const int listed = counts[c];
+    const bool overflow = listed > kCap;
+    const int count = overflow ? *n_anti_ptr : listed;
+    const int* list = overflow ? idx : lists + (size_t)c * kCap;
+    Kahan ka[3];
+    for (int base = 0; base < count; base += kRun) {
+        const int end = min(count, base + kRun);
+        float s0 = 0.f, s1 = 0.f, s2 = 0.f;
+        for (int j = base; j < end; ++j) {
+            const int i = list[j];
+            const float4* r = rec + (size_t)kRecF4 * i;
+            const float4 r0 = r[0], r1 = r[1], r2v = r[2], r4 = r[4];
+            const float dx = r0.x - P.px, dy = r0.y - P.py, dz = r0.z - P.pz;
+            const float dd = dx * dx + dy * dy + dz * dz;
+            const float inv = rsq(dd);
+            const float cn = P.nx * dx + P.ny * dy + P.nz * dz;
+            const float ca = -(r1.x * dx + r1.y * dy + r1.z * dz);
+            const float m =
+                anti_weight<MODEL>(P, r0, r1, r2v, r4.w, dx, dy, dz, dd, inv, cn, ca, ap, gpos, gnrm, q, raw, i);
+            if (GENERAL) {
+                if (m != 0.f) {
+                    using V3 = V<float>;
+                    Mat<float> pm = material<float>(mats, P.mat);
+                    if (galb) pm.diffuse = mk(P.br / kOneOverPi, P.bg / kOneOverPi, P.bb / kOneOverPi);
+                    const V3 b = f_r3(mk(r0.x, r0.y, r0.z), mk(P.px, P.py, P.pz), mk(camx, camy, camz),
+                                      mk(P.nx, P.ny, P.nz), pm) * m;
+                    s0 += r4.x * b.x; s1 += r4.y * b.y; s2 += r4.z * b.z;
+                }
+            } else {
+                s0 += r4.x * m; s1 += r4.y * m; s2 += r4.z * m;
+            }
+        }
+        ka[0].add(s0); ka[1].add(s1); ka[2].add(s2);
+    }
+    if (counters && lane == 0) {
+        const int alive = __popcll(__ballot(live));
+        atomicAdd(&counters[C_ANTI_EVALUATED], (unsigned long long)count * alive);
+    }
+    if (!live) return;
+    float* o = part + q;
+    if (GENERAL) {
+        o[3 * (size_t)npix] = ka[0].s; o[4 * (size_t)npix] = ka[1].s; o[5 * (size_t)npix] = ka[2].s;
+    } else {
+        o[3 * (size_t)npix] = ka[0].s * P.br; o[4 * (size_t)npix] = ka[1].s * P.bg;
+        o[5 * (size_t)npix] = ka[2].s * P.bb;
+    }
+}
+
+template <int MODEL, bool GENERAL>
+void launch_pixels(arc_gather* g, const arc::AvplsDev& a, int cw, float3 cam, const AntiParams& ap,
+                   const int* n_anti_ptr, float* part, unsigned long long* counters) {
+    const dim3 grid((g->width + 15) / 16, (g->height + 15) / 16);
+    hipLaunchKernelGGL((k_anti_clustered<MODEL, GENERAL>), grid, dim3(256), 0, g->ctx->stream, g->rec.as<float4>(), a,
+                       g->gpos, g->gnrm, g->galb, g->mats.as<arc_material>(), g->width, g->height, cw, cam.x, cam.y,
+                       cam.z, ap, g->ccount.as<int>(), g->clist.as<int>(), g->anti_idx.as<int>(), n_anti_ptr, part,
+                       counters);
+}
+
+}  // namespace
+
+namespace arc {
+
+int anti_clustered(arc_gather* g, const AvplsDev& a, int model, float3 cam, const AntiParams& ap,
+                   double half_angle, float* part, unsigned long long* counters) {
+    arc_ctx* ctx = g->ctx;
+    const int n = a.n;
+    const int nb = (n + kCompact - 1) / kCompact;
+    const int cw = (g->width + kClusterPx - 1) / kClusterPx, ch = (g->height + kClusterPx - 1) / kClusterPx;
+    const int C = cw * ch;
+    // anti_idx: [n] compacted indices, then [nb + 1] block offsets (last = count)
+    ARC_TRY(ensure(ctx, g->anti_idx, sizeof(int) * ((size_t)n + nb + 1)));
+    ARC_TRY(ensure(ctx, g->cbox, sizeof(float) * 8 * (size_t)C));
+    ARC_TRY(ensure(ctx, g->ccount, sizeof(int) * (size_t)C));
+    ARC_TRY(ensure(ctx, g->clist, sizeof(int) * (size_t)C * kCap));
+    int* idx = g->anti_idx.as<int>();
+    int* offs = idx + n;
+    hipLaunchKernelGGL(k_flag_count, dim3(nb), dim3(kCompact), 0, ctx->stream, a, offs);
+    hipLaunchKernelGGL(k_scan_blocks, dim3(1), dim3(1024), 0, ctx->stream, offs, nb);
+    hipLaunchKernelGGL(k_compact, dim3(nb), dim3(kCompact), 0, ctx->stream, a, offs, idx);
+    hipLaunchKernelGGL(k_cluster_bounds, dim3(C), dim3(256), 0, ctx->stream, g->gpos, g->width, g->height, cw,
+                       g->cbox.as<float>());
+    hipLaunchKernelGGL(k_zero, dim3((C + 255) / 256), dim3(256), 0, ctx->stream, g->ccount.as<int>(), C);
+    ARC_LAUNCHED(ctx);
+    // The list test must keep every pair the FP32 / FP64 pair test accepts: the
+    // box grows by the photon radius plus a relative and an absolute margin.
+    const float grow = ap.radius * 1.001f + 1e-5f;
+    const double th = std::min(M_PI, half_angle * 1.001 + 1e-4);
+    const float cos_t = (float)std::cos(th), sin_t = (float)std::sin(th);
+    const int* n_anti = offs + nb;
+    const dim3 lgrid((n + 255) / 256, (C + kClustersPerBlock - 1) / kClustersPerBlock);
+    if (model == ARC_ANTIRAD_DISK)
+        hipLaunchKernelGGL(k_cluster_lists<ARC_ANTIRAD_DISK>, lgrid, dim3(256), 0, ctx->stream, a, idx, n_anti,
+                           g->cbox.as<float>(), C, grow, cos_t, sin_t, g->ccount.as<int>(), g->clist.as<int>());
+    else
+        hipLaunchKernelGGL(k_cluster_lists<ARC_ANTIRAD_CONE>, lgrid, dim3(256), 0, ctx->stream, a, idx, n_anti,
+                           g->cbox.as<float>(), C, grow, cos_t, sin_t, g->ccount.as<int>(), g->clist.as<int>());
+    ARC_LAUNCHED(ctx);
+    if (model == ARC_ANTIRAD_DISK) {
+        if (g->general) launch_pixels<ARC_ANTIRAD_DISK, true>(g, a, cw, cam, ap, n_anti, part, counters);
+        else launch_pixels<ARC_ANTIRAD_DISK, false>(g, a, cw, cam, ap, n_anti, part, counters);
+    } else {
+        if (g->general) launch_pixels<ARC_ANTIRAD_CONE, true>(g, a, cw, cam, ap, n_anti, part, counters);
+        else launch_pixels<ARC_ANTIRAD_CONE, false>(g, a, cw, cam, ap, n_anti, part, counters);
+    }
+    ARC_LAUNCHED(ctx);
+    return ARC_OK;
+}
+
+}  // namespace arc

@@ -1,0 +1,201 @@
+// Shared pieces of the gather translation units: the gather object, the
+// packed AVPL record layout, compensated sums, pixel loads and the guarded
+// antiradiance decisions (cudaGather.cu:109-144, Shaders/Gather.frag:149-188).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+
+#include "arc_internal.hpp"
+#include "gather_math.hpp"
+
+struct arc_gather {
+    arc_ctx* ctx = nullptr;
+    int width = 0, height = 0;
+    arc::DevBuf mats;
+    int num_mats = 0;
+    bool general = false;  // some material has a specular lobe
+    const float4* gpos = nullptr;
+    const float4* gnrm = nullptr;
+    const float4* galb = nullptr;
+    float4* out_res = nullptr;
+    float4* out_rad = nullptr;
+    float4* out_anti = nullptr;
+    arc::DevBuf int_rad, int_anti, int_res;
+    arc::DevBuf rec, part, counters;
+    arc::DevBuf stage;  // AVPL SoA staged from host memory
+    // clustered antiradiance: compacted AVPL indices (+ block offsets and
+    // count), cluster boxes, list lengths and lists
+    arc::DevBuf anti_idx, cbox, ccount, clist;
+    arc_bvh* bvh = nullptr;
+    // per-frame phase events: frame slot, then 6 marks (start, upload done,
+    // bvh done, radiance done, antiradiance done, end)
+    static constexpr int kRing = 64;
+    hipEvent_t ev[kRing][6] = {};
+    int ring_next = 0, ring_count = 0;
+};
+
+
+namespace arc {
+namespace gd {
+
+using namespace arc::gm;
+
+constexpr int kThreads = 256;
+constexpr int kRecF4 = 5;  // float4 per packed AVPL record
+constexpr int kRun = 64;   // pairs summed plainly before a compensated add
+// Relative guard band of the FP32 antiradiance decisions (FP32 rounding of
+// these quantities is below 1e-6 relative); pairs inside it are re-decided
+// in FP64.
+constexpr float kGuard = 1e-5f;
+
+enum Counter { C_VISITED = 0, C_EVALUATED = 1, C_ANTI_VISITED = 2, C_ANTI_EVALUATED = 3, C_ERRORS = 4, C_NUM = 8 };
+
+struct Kahan {
+    float s = 0.f, c = 0.f;
+    __device__ inline void add(float x) {
+        const float y = x - c;
+        const float t = s + y;
+        c = (t - s) - y;
+        s = t;
+    }
+};
+
+__device__ inline f3 f3of(float4 v) { return f3{v.x, v.y, v.z}; }
+
+
+// ----------------------------------------------------------- pixels ----
+struct PixelF {
+    float px, py, pz;  // position
+    float nx, ny, nz;  // unit normal
+    float br, bg, bb;  // diffuse / pi (Lambertian pixel BRDF)
+    int mat;
+};
+
+__device__ inline PixelF load_pixel(const float4* gpos, const float4* gnrm, const float4* galb,
+                                    const arc_material* mats, int pix) {
+    PixelF P;
+    const float4 p = gpos[pix], n = gnrm[pix];
+    P.px = p.x; P.py = p.y; P.pz = p.z;
+    const float s = rsq(n.x * n.x + n.y * n.y + n.z * n.z);
+    P.nx = n.x * s; P.ny = n.y * s; P.nz = n.z * s;
+    P.mat = (int)n.w;
+    float dr, dg, db;
+    if (galb) {
+        const float4 a = galb[pix];
+        dr = a.x; dg = a.y; db = a.z;
+    } else {
+        const arc_material& m = mats[mat_slot(P.mat)];
+        dr = m.diffuse.x; dg = m.diffuse.y; db = m.diffuse.z;
+    }
+    P.br = kOneOverPi * dr; P.bg = kOneOverPi * dg; P.bb = kOneOverPi * db;
+    return P;
+}
+
+
+// ------------------------------------------------------- decisions ----
+// The antiradiance tests are discontinuous (inside / outside a disk or cone,
+// sign tests of dot products).  A decision is made in FP32 when the FP32
+// value is farther from its threshold than a guard that bounds the FP32
+// rounding error (with a wide margin); pairs inside the guard band are
+// re-decided in FP64 from the raw inputs with the reference's formulas, which
+// is the decision the FP64 oracle makes.  Pairs in the band are rare except
+// for coplanar pixel / AVPL pairs, whose signs the reference leaves to
+// rounding noise.
+
+// getAntiradiance's tests (cudaGather.cu:116-140) in FP64.
+__device__ __attribute__((noinline)) inline bool disk_accept_f64(float4 p4, float4 n4, f3 a, f3 an, f3 w, float radius) {
+    using V3 = V<double>;
+    const V3 pos = mk<double>(p4.x, p4.y, p4.z), nrm = mk<double>(n4.x, n4.y, n4.z);
+    const V3 apos = ld3<double>(a), anrm = ld3<double>(an), aw = ld3<double>(w);
+    const double denom = dot(aw, nrm);
+    if (denom == 0.0) return false;
+    const double t = dot(pos - apos, nrm) / denom;
+    if (t <= 0.0) return false;
+    const V3 hit = apos + t * normalize(aw);
+    if (length(hit - pos) >= (double)radius) return false;
+    if (dot(normalize(hit - apos), aw) <= 0.0) return false;
+    if (dot(nrm, aw) >= 0.0) return false;
+    if (dot(normalize(pos - apos), anrm) >= 0.0) return false;
+    return true;
+}
+
+// GetAntiradiance's tests (Shaders/Gather.frag:155-166) in FP64.
+__device__ __attribute__((noinline)) inline bool cone_accept_f64(float4 p4, float4 n4, f3 a, f3 w, double cos_half) {
+    using V3 = V<double>;
+    const V3 pos = mk<double>(p4.x, p4.y, p4.z), nrm = mk<double>(n4.x, n4.y, n4.z);
+    const V3 apos = ld3<double>(a), aw = ld3<double>(w);
+    const V3 to_pos = normalize(pos - apos);
+    const double c = dot(to_pos, normalize(aw));
+    if (c <= (double)kConeEps) return false;
+    if (dot(nrm, -to_pos) <= (double)kConeEps) return false;
+    return c > cos_half;
+}
+
+
+struct AntiParams {
+    float radius, r2;        // disk radius and its square
+    float cos_half, cone_lo; // cone: cos(half-angle), lower edge of the guard band
+    double cos_half_d;
+};
+
+__host__ inline AntiParams anti_params(float radius, double cos_half) {
+    AntiParams a;
+    a.radius = radius;
+    a.r2 = radius * radius;
+    a.cos_half = (float)cos_half;
+    a.cone_lo = std::max(kConeEps, (float)cos_half) - kGuard;
+    a.cos_half_d = cos_half;
+    return a;
+}
+
+// Antiradiance weight of one (pixel, AVPL) pair with d = a - p, dd = |d|^2,
+// inv = 1/|d|, cn = n_p . d and ca = -n_a . d: 0 or 1 for the disk model
+// (the record already holds A / (pi r^2)), cos_p / |d|^2 for the cone model.
+// FP32 decisions outside the guard band; FP64 re-decision inside it.
+template <int MODEL>
+__device__ inline float anti_weight(const PixelF& P, const float4& r0, const float4& r1, const float4& r2v,
+                                    float guard_w, float dx, float dy, float dz, float dd, float inv, float cn,
+                                    float ca, const AntiParams& ap, const float4* gpos, const float4* gnrm, int q,
+                                    const AvplsDev& raw, int i) {
+    if (MODEL == ARC_ANTIRAD_DISK) {
+        // line a + t w hits the tangent plane at t = -cn / (w . n) > 0, within the
+        // disk, on the far side of the AVPL's surface, pixel facing against w.
+        // With h = d + t w_unit, |h| < r  <=>  |h denom|^2 < r^2 denom^2, and
+        // h denom = d denom - cn w_unit: no division needed.
+        const float denom = r2v.x * P.nx + r2v.y * P.ny + r2v.z * P.nz;
+        const float hx = dx * denom - cn * r0.w;
+        const float hy = dy * denom - cn * r1.w;
+        const float hz = dz * denom - cn * r2v.w;
+        const float h2 = hx * hx + hy * hy + hz * hz;
+        const float dn2 = denom * denom;
+        const float lim = ap.r2 * dn2;
+        const float g = kGuard * (dd * dn2 + cn * cn);
+        const float dl = kGuard * (dd * inv);
+        bool in = denom < 0.f && cn > 0.f && h2 < lim && ca < 0.f;
+        const bool amb = h2 < lim + g &&
+                         (!(fabsf(h2 - lim) > g) || !(fabsf(cn) > dl) || !(fabsf(ca) > dl) ||
+                          !(fabsf(denom) > guard_w));
+        if (amb) in = disk_accept_f64(gpos[q], gnrm[q], raw.pos[i], raw.nrm[i], raw.dir[i], ap.radius);
+        return in ? 1.f : 0.f;
+    } else {
+        // cone of half-angle acos(cos_half) about w, cosine at the pixel / dist^2
+        const float c = -(dx * r0.w + dy * r1.w + dz * r2v.w) * inv;
+        const float cp = cn * inv;
+        bool in = c > kConeEps && cp > kConeEps && c > ap.cos_half;
+        const bool amb = c > ap.cone_lo && cp > kConeEps - kGuard &&
+                         (!(fabsf(c - ap.cos_half) > kGuard) || !(fabsf(cp - kConeEps) > kGuard) ||
+                          !(fabsf(c - kConeEps) > kGuard));
+        if (amb) in = cone_accept_f64(gpos[q], gnrm[q], raw.pos[i], raw.dir[i], ap.cos_half_d);
+        return in ? cp * (inv * inv) : 0.f;
+    }
+}
+
+}  // namespace gd
+
+// Clustered antiradiance (gather_anti.hip).
+int anti_clustered(arc_gather* g, const AvplsDev& a, int model, float3 cam, const gd::AntiParams& ap,
+                   double half_angle, float* part, unsigned long long* counters);
+
+}  // namespace arc

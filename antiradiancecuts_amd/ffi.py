@@ -29,6 +29,8 @@ ARC_CUT_SOLID_ANGLE = 0
 ARC_CUT_ERROR_BOUND = 1
 ARC_ANTIRAD_DISK = 0
 ARC_ANTIRAD_CONE = 1
+ARC_ANTI_EXACT = 0
+ARC_ANTI_CLUSTERED = 1
 ARC_RUN_PROFILE = 1
 ARC_RUN_COUNTERS = 2
 ARC_RUN_TIMELINE = 4
@@ -73,7 +75,7 @@ class arc_avpl_node_data(C.Structure):
 class arc_gather_params(C.Structure):
     _fields_ = [("useLightCuts", C.c_int), ("cutMode", C.c_int),
                 ("clusterRefinementThreshold", C.c_float), ("errorBound", C.c_float),
-                ("antiradianceModel", C.c_int), ("antiradianceCut", C.c_int),
+                ("antiradianceModel", C.c_int), ("antiradianceMode", C.c_int),
                 ("photonRadiusScale", C.c_float), ("coneHalfAngle", C.c_float),
                 ("writeSentinel", C.c_int), ("stackSize", C.c_int), ("seed", C.c_uint64)]
 

@@ -162,7 +162,7 @@ def main() -> int:
         "config": {
             "workload": f"clutter_{sc.width}x{sc.height}: Cornell walls + {args.tris} random triangles, "
                         f"{n} AVPLs (bounce 1-4, L log-uniform [1e-3,10], 30% antiradiance); AvplBvh rebuilt "
-                        "per frame; light-cut radiance (solid angle < pi/10); exact disk antiradiance",
+                        "per frame; light-cut radiance (solid angle < pi/10); disk antiradiance over per-cluster AVPL lists (exact)",
             "width": sc.width, "height": sc.height, "avpls": n, "parallelism": "single GPU",
             "phase_ms": {k: round(v, 4) for k, v in phase_ms.items()},
             "cut": {"visited_nodes": int(st.visitedNodes), "evaluated_nodes": int(st.evaluatedNodes),

@@ -183,6 +183,10 @@ typedef struct arc_gather arc_gather;
 #define ARC_ANTIRAD_DISK 0      /* getAntiradiance, cudaGather.cu:109-144                 */
 #define ARC_ANTIRAD_CONE 1      /* GetAntiradiance, Shaders/Gather.frag:149-188 x BRDF    */
 
+/* How antiradiance is summed; both give the exact sum over all AVPLs. */
+#define ARC_ANTI_EXACT 0        /* every AVPL per pixel (kernel_antiradiance, cudaGather.cu:446-557)     */
+#define ARC_ANTI_CLUSTERED 1    /* per screen-cluster AVPL lists (useClusteredDeferred, :605-748, 950-1011) */
+
 /* The CONF_VARS fields the hot path reads (CConfigManager.h:6-52), plus the
  * extension switches.  arc_gather_params_default fills the reference's
  * defaults (CConfigManager.cpp:14-37). */
@@ -192,7 +196,7 @@ typedef struct arc_gather_params {
     float clusterRefinementThreshold; /* solid-angle threshold, default pi/10        */
     float errorBound;                 /* ARC_CUT_ERROR_BOUND relative bound, 0.01    */
     int antiradianceModel;            /* ARC_ANTIRAD_*                               */
-    int antiradianceCut;              /* 1: antiradiance over the LBVH (ERROR_BOUND) */
+    int antiradianceMode;             /* ARC_ANTI_*                                  */
     float photonRadiusScale;          /* disk radius = scale * sceneExtent / 100     */
     float coneHalfAngle;              /* radians, CONE model                         */
     int writeSentinel;                /* 1: (1e5, 0, 1e5, 1) on traversal error      */
