@@ -3,6 +3,9 @@
 libarc.so        HIP kernels + C ABI (include/arc.h), gfx950 only
 libarc_scene.so  host C++ synthetic scene / AVPL / point generators
                  (include/arc_scene.h)
+_build/arc_facade_test
+                 host program over the C++ facade (include/arc_cudagather.hpp),
+                 run by the GPU tests
 
 Run as ``python -m antiradiancecuts_amd.build`` or through
 ``__graft_entry__.build()``.  Every HIP translation unit is compiled with
@@ -31,6 +34,7 @@ HIP_SOURCES = {
     "lbvh.hip": ["-ffp-contract=off"],
     "gather.hip": [],
     "gather_anti.hip": [],
+    "gather_cut.hip": [],
 }
 
 HIPCC_FLAGS = [
@@ -87,9 +91,24 @@ def build_scene(verbose: bool = False) -> Path:
     return out
 
 
+def build_facade_test(verbose: bool = False) -> Path:
+    """Compile tests/cpp/facade_test.cpp against the two libraries."""
+    OBJDIR.mkdir(exist_ok=True)
+    out = OBJDIR / "arc_facade_test"
+    tmp = OBJDIR / "arc_facade_test.tmp"
+    _run(["g++", "-O2", "-std=c++17", "-Wall", f"-I{INCLUDE}", "-o", str(tmp),
+          str(ROOT / "tests" / "cpp" / "facade_test.cpp"), f"-L{PKG}", "-larc", "-larc_scene",
+          "-Wl,-rpath,$ORIGIN/.."])
+    os.replace(tmp, out)
+    if verbose:
+        print(f"built {out}")
+    return out
+
+
 def build_all(verbose: bool = False) -> None:
     build_scene(verbose)
     build_hip(verbose)
+    build_facade_test(verbose)
 
 
 if __name__ == "__main__":

@@ -87,7 +87,8 @@ struct arc_bvh {
     arc::DevBuf codes, perm, nodes, parents, ranges, positions, normals;
     arc::DevBuf in_pos, in_nrm;                 // staged host input
     arc::DevBuf nd_size, nd_mat, nd_rand, nd_pos, nd_nrm, nd_rad, nd_dir;
-    arc::DevBuf nd_anti;                        // per-slot antiradiance sums (cut extension)
+    arc::DevBuf nd_radb;                        // per-slot sums of L x diffuse / pi (error-bound cut)
+    arc::DevBuf nd_mom, nd_nmom;                // per-slot weighted position / normal moments
     arc::DevBuf bounds;                         // 12 floats + validation flag
     arc::DevBuf partials;                       // reduction partials
     arc::DevBuf ready;                          // refit: nodes spanning blocks, ready to merge
@@ -102,6 +103,7 @@ struct AvplsDev {
     int n;
 };
 int bvh_build_points(arc_bvh* b, const f3* pos, const f3* nrm, int n);
-int bvh_build_avpls(arc_bvh* b, const AvplsDev& a, const float* rand, uint64_t seed, bool validate);
+int bvh_build_avpls(arc_bvh* b, const AvplsDev& a, const float* rand, uint64_t seed, bool validate,
+                    const arc_material* mats = nullptr);
 int random_numbers(arc_ctx* ctx, uint64_t seed, int count, float* out);
 }  // namespace arc

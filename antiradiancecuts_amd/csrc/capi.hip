@@ -156,7 +156,7 @@ void arc_bvh_destroy(arc_bvh* b) {
     if (b->ctx) (void)hipStreamSynchronize(b->ctx->stream);
     arc::DevBuf* bufs[] = {&b->codes,   &b->perm,    &b->nodes,  &b->parents, &b->ranges,   &b->positions,
                            &b->normals, &b->in_pos,  &b->in_nrm, &b->nd_size, &b->nd_mat,   &b->nd_rand,
-                           &b->nd_pos,  &b->nd_nrm,  &b->nd_rad, &b->nd_dir,  &b->nd_anti,  &b->bounds,
+                           &b->nd_pos,  &b->nd_nrm,  &b->nd_rad, &b->nd_dir,  &b->nd_radb, &b->nd_mom, &b->nd_nmom,  &b->bounds,
                            &b->partials, &b->ready, &b->sort.keys_alt, &b->sort.vals_alt, &b->sort.counts};
     for (arc::DevBuf* d : bufs) arc::release(*d);
     delete b;
@@ -177,6 +177,7 @@ int arc_bvh_build_points(arc_bvh* b, const arc_float3* positions, const arc_floa
             ARC_HIP(ctx, hipMemcpyAsync(b->in_nrm.p, normals, bytes, hipMemcpyHostToDevice, ctx->stream));
             nrm = b->in_nrm.as<f3>();
         }
+        ARC_HIP(ctx, hipStreamSynchronize(ctx->stream));  // host arrays are free to reuse on return
     }
     return arc::bvh_build_points(b, pos, nrm, n);
 }
@@ -209,6 +210,7 @@ int arc_bvh_build_avpls(arc_bvh* b, const arc_avpls* a, const float* rand, uint6
             if (src[k] && sz[k])
                 ARC_HIP(ctx, hipMemcpyAsync(dst[k], src[k], sz[k], hipMemcpyHostToDevice, ctx->stream));
         }
+        ARC_HIP(ctx, hipStreamSynchronize(ctx->stream));  // host arrays are free to reuse on return
         d.pos = reinterpret_cast<const f3*>(dst[0]);
         d.nrm = reinterpret_cast<const f3*>(dst[1]);
         d.dir = reinterpret_cast<const f3*>(dst[2]);

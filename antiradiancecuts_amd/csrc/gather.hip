@@ -400,6 +400,8 @@ int stage_avpls(arc_gather* g, const arc_avpls* in, int flags, arc::AvplsDev& ou
         off += sz[k];
         if (src[k]) ARC_HIP(ctx, hipMemcpyAsync(dst[k], src[k], sz[k], hipMemcpyHostToDevice, ctx->stream));
     }
+    // the caller may free or reuse its host arrays once the call returns
+    ARC_HIP(ctx, hipStreamSynchronize(ctx->stream));
     out.pos = reinterpret_cast<const f3*>(dst[0]);
     out.nrm = reinterpret_cast<const f3*>(dst[1]);
     out.dir = reinterpret_cast<const f3*>(dst[2]);
@@ -449,6 +451,11 @@ void arc_gather_params_default(arc_gather_params* p) {
     p->writeSentinel = 1;
     p->stackSize = 32;                          // STACK_SIZE, cudaGather.cu:19
     p->seed = 5489;
+    p->cutEstimateDepth = 6;
+    p->cutBudget = 1 << 16;
+    p->cutPerTile = 0;
+    p->cutCriterion = ARC_CRITERION_ESTIMATE;
+    p->cutImageFloor = 1.0f;
 }
 
 int arc_gather_create(arc_ctx* ctx, int width, int height, const arc_material* materials, int num_materials,
@@ -502,6 +509,11 @@ void arc_gather_destroy(arc_gather* g) {
     arc::release(g->part);
     arc::release(g->counters);
     arc::release(g->stage);
+    arc::release(g->anti_idx);
+    arc::release(g->cbox);
+    arc::release(g->ccount);
+    arc::release(g->clist);
+    arc::release(g->crec);
     arc_bvh_destroy(g->bvh);
     delete g;
 }
@@ -564,22 +576,36 @@ int arc_gather_run(arc_gather* g, const arc_avpls* avpls, arc_float3 camPos, flo
 
     const bool cut = p->useLightCuts && n > 1;
     const bool clustered = p->antiradianceMode == ARC_ANTI_CLUSTERED && n > 0;
-    if (p->useLightCuts && p->cutMode != ARC_CUT_SOLID_ANGLE)
-        return arc::fail(ctx, ARC_E_UNSUPPORTED, "cutMode not available in this build");
+    if (p->useLightCuts && p->cutMode != ARC_CUT_SOLID_ANGLE && p->cutMode != ARC_CUT_ERROR_BOUND)
+        return arc::fail(ctx, ARC_E_INVALID, "cutMode");
+    if (p->useLightCuts && p->cutMode == ARC_CUT_ERROR_BOUND && g->general)
+        return arc::fail(ctx, ARC_E_UNSUPPORTED, "the error-bound cut needs Lambertian materials");
+    if (p->cutEstimateDepth < 0 || p->cutEstimateDepth > 40) return arc::fail(ctx, ARC_E_INVALID, "cutEstimateDepth");
+    if (!(p->cutImageFloor >= 0.f)) return arc::fail(ctx, ARC_E_INVALID, "cutImageFloor");
+    if (p->cutCriterion != ARC_CRITERION_BOUND && p->cutCriterion != ARC_CRITERION_ESTIMATE)
+        return arc::fail(ctx, ARC_E_INVALID, "cutCriterion");
+    if (p->cutBudget < 1) return arc::fail(ctx, ARC_E_INVALID, "cutBudget");
     // ---- radiance: light cut over this frame's AvplBvh (cudaGather.cu:854-923) or exact
     int rad_splits = -1, anti_splits = 0;
     if (cut) {
         // AvplBvh(avpls, false) per frame (cudaGather.cu:858)
-        ARC_TRY(arc::bvh_build_avpls(g->bvh, a, nullptr, p->seed, (flags & ARC_TRUST_INPUT) == 0));
+        ARC_TRY(arc::bvh_build_avpls(g->bvh, a, nullptr, p->seed, (flags & ARC_TRUST_INPUT) == 0,
+                                     g->mats.as<arc_material>()));
         if (profile) ARC_HIP(ctx, hipEventRecord(ev[2], ctx->stream));
         arc_bvh* b = g->bvh;
-        CutTree t{b->nodes.as<arc_bvh_node>(), b->nd_mat.as<int>(), b->nd_pos.as<f3>(), b->nd_nrm.as<f3>(),
-                  b->nd_rad.as<f3>(), b->nd_dir.as<f3>(), b->n};
-        dim3 grid((g->width + 15) / 16, (g->height + 15) / 16);
-        const size_t lds = sizeof(int) * (size_t)p->stackSize * kThreads;
-        hipLaunchKernelGGL(k_cut_sa, grid, dim3(kThreads), lds, ctx->stream, t, g->gpos, g->gnrm, g->galb,
-                           g->mats.as<arc_material>(), g->width, g->height, cam.x, cam.y, cam.z,
-                           p->clusterRefinementThreshold, p->stackSize, p->writeSentinel, rad, counters);
+        if (p->cutMode == ARC_CUT_ERROR_BOUND) {
+            const arc::CutEBParams cp{p->errorBound, p->cutEstimateDepth, 1e-3f * sceneExtent, p->cutBudget,
+                                          p->cutPerTile, p->cutCriterion, p->cutImageFloor};
+            ARC_TRY(arc::cut_error_bound(g, b, cp, rad, counters));
+        } else {
+            CutTree t{b->nodes.as<arc_bvh_node>(), b->nd_mat.as<int>(), b->nd_pos.as<f3>(), b->nd_nrm.as<f3>(),
+                      b->nd_rad.as<f3>(), b->nd_dir.as<f3>(), b->n};
+            dim3 grid((g->width + 15) / 16, (g->height + 15) / 16);
+            const size_t lds = sizeof(int) * (size_t)p->stackSize * kThreads;
+            hipLaunchKernelGGL(k_cut_sa, grid, dim3(kThreads), lds, ctx->stream, t, g->gpos, g->gnrm, g->galb,
+                               g->mats.as<arc_material>(), g->width, g->height, cam.x, cam.y, cam.z,
+                               p->clusterRefinementThreshold, p->stackSize, p->writeSentinel, rad, counters);
+        }
         ARC_LAUNCHED(ctx);
         if (profile) ARC_HIP(ctx, hipEventRecord(ev[3], ctx->stream));
     } else {
@@ -620,6 +646,7 @@ int arc_gather_run(arc_gather* g, const arc_avpls* avpls, arc_float3 camPos, flo
             stats->antiVisitedNodes = (long long)c[C_ANTI_VISITED];
             stats->antiEvaluatedNodes = (long long)c[C_ANTI_EVALUATED];
             stats->errorPixels = (long long)c[C_ERRORS];
+            stats->budgetHits = (long long)c[C_BUDGET];
         }
         if (flags & ARC_RUN_PROFILE) {
             ARC_HIP(ctx, hipEventSynchronize(ev[5]));

@@ -28,6 +28,8 @@ struct arc_gather {
     // clustered antiradiance: compacted AVPL indices (+ block offsets and
     // count), cluster boxes, list lengths and lists
     arc::DevBuf anti_idx, cbox, ccount, clist;
+    arc::DevBuf crec;  // error-bound cut: packed node records
+    arc::DevBuf cest;  // error-bound cut: per-pixel estimates and their sum
     arc_bvh* bvh = nullptr;
     // per-frame phase events: frame slot, then 6 marks (start, upload done,
     // bvh done, radiance done, antiradiance done, end)
@@ -50,7 +52,7 @@ constexpr int kRun = 64;   // pairs summed plainly before a compensated add
 // in FP64.
 constexpr float kGuard = 1e-5f;
 
-enum Counter { C_VISITED = 0, C_EVALUATED = 1, C_ANTI_VISITED = 2, C_ANTI_EVALUATED = 3, C_ERRORS = 4, C_NUM = 8 };
+enum Counter { C_VISITED = 0, C_EVALUATED = 1, C_ANTI_VISITED = 2, C_ANTI_EVALUATED = 3, C_ERRORS = 4, C_BUDGET = 5, C_NUM = 8 };
 
 struct Kahan {
     float s = 0.f, c = 0.f;
@@ -193,6 +195,18 @@ __device__ inline float anti_weight(const PixelF& P, const float4& r0, const flo
 }
 
 }  // namespace gd
+
+// Error-bounded light cut (gather_cut.hip).
+struct CutEBParams {
+    float eps;          // errorBound
+    int d0;             // cutEstimateDepth
+    float delta;        // distance floor of the bounds
+    int budget;         // cutBudget
+    int per_tile;       // cutPerTile
+    int criterion;      // cutCriterion
+    float image_floor;  // cutImageFloor
+};
+int cut_error_bound(arc_gather* g, arc_bvh* b, const CutEBParams& cp, float4* rad, unsigned long long* counters);
 
 // Clustered antiradiance (gather_anti.hip).
 int anti_clustered(arc_gather* g, const AvplsDev& a, int model, float3 cam, const gd::AntiParams& ap,

@@ -1,0 +1,387 @@
+// Error-bounded light-cut radiance on gfx950 (ARC_CUT_ERROR_BOUND).
+//
+// The reference refines a cluster while its solid angle exceeds a threshold
+// (kernel_bvh, cudaGather.cu:405-414).  This mode uses the Lightcuts
+// criterion instead (Walter et al., SIGGRAPH 2005): a cluster is kept when an
+// upper bound of its contribution is at most errorBound times the pixel's
+// radiance estimate, and refined otherwise.
+//
+//   cluster estimate  I * G~   with I the sum of L * diffuse / pi over the
+//                     node's AVPLs (Lambertian AVPLs; the pixel BRDF is
+//                     applied once per pixel) and G~ evaluated at the
+//                     w-weighted centroid c of the node (w = mean of the
+//                     three channels of L * diffuse / pi):
+//                       G~ = cos_p(c) * cos~ / |c - p|^2,
+//                       cos~ = (1 - |m|) / 4 + max(0, -m . (p - c)^),
+//                     m the w-weighted mean unit normal.  cos~ is exact for
+//                     one AVPL or a node whose normals agree (|m| = 1) and
+//                     matches the average clamped cosine of isotropic normals
+//                     (|m| = 0, mean 1/4); the reference's random
+//                     representative (bvh.cu:726-744) is unbiased but has a
+//                     spread of the order of the node's contribution when
+//                     normals vary, which the deterministic moment estimate
+//                     removes.
+//   bound             max(I) * min(1, h_max / d_min) / d_min^2 over the node's
+//                     box: d_min is the distance from p to the box, floored at
+//                     delta (1e-3 of the scene extent), h_max the largest
+//                     height of the box over the pixel's tangent plane (<= 0:
+//                     the whole node is behind the pixel and contributes
+//                     exactly 0); the AVPL-side cosine is bounded by 1.
+//   threshold         errorBound * the pixel's estimate from the cut at depth
+//                     d0 (phase 1).
+//
+// Two traversals share the node records:
+//   k_cut_eb_px   one pixel per lane, its own depth-first cut (the reference's
+//                 per-pixel traversal), stack in LDS;
+//   k_cut_eb_tile one 8x8 tile per wave: one depth-first cut for the tile,
+//                 refined when any lane's bound exceeds its threshold (ballot),
+//                 wave-uniform stack and scalar node loads.  It pays off where
+//                 neighbouring pixels see the AVPLs alike (smooth surfaces far
+//                 from the AVPLs); with AVPLs around the pixels the tile cut
+//                 is the union of 64 different near fields.
+#include <cmath>
+
+#include "gather_common.hpp"
+
+using arc::f3;
+using namespace arc::gm;
+using namespace arc::gd;
+
+namespace {
+
+constexpr int kTileStack = 96;  // per-wave stack of the tile traversal
+constexpr int kPxStack = 48;    // per-lane stack of the pixel traversal (LDS)
+constexpr int kCutThreads = 256;
+constexpr int kEstStack = 48;   // per-wave stack of the depth-d0 estimate cut (d0 <= 40)
+
+struct CutEB {
+    const arc_bvh_node* nodes;  // n - 1 internal nodes
+    const float4* crec;         // 3 float4 per slot: (centroid, max I), (m, |m|), (I, AVPL count)
+    int n;
+};
+
+__global__ __launch_bounds__(kCutThreads) void k_pack_cut(int n, const f3* __restrict__ pos,
+                                                          const f3* __restrict__ nrm,
+                                                          const f3* __restrict__ radb,
+                                                          const float4* __restrict__ mom,
+                                                          const f3* __restrict__ nmom, const int* __restrict__ size,
+                                                          float4* __restrict__ crec) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= 2 * n - 1) return;
+    const f3 I = radb[s];
+    const float4 mo = mom[s];
+    f3 c, m;
+    if (mo.w > 0.f) {
+        const float iw = 1.0f / mo.w;
+        c = f3{mo.x * iw, mo.y * iw, mo.z * iw};
+        const f3 nm = nmom[s];
+        m = f3{nm.x * iw, nm.y * iw, nm.z * iw};
+    } else {  // no intensity: the representative's data (the node contributes 0 anyway)
+        c = pos[s];
+        const f3 q = nrm[s];
+        const float l = q.x * q.x + q.y * q.y + q.z * q.z;
+        const float iq = l > 0.f ? 1.0f / sqrtf(l) : 0.f;
+        m = f3{q.x * iq, q.y * iq, q.z * iq};
+    }
+    const float ml = fminf(1.f, sqrtf(m.x * m.x + m.y * m.y + m.z * m.z));
+    crec[3 * (size_t)s + 0] = make_float4(c.x, c.y, c.z, fmaxf(I.x, fmaxf(I.y, I.z)));
+    crec[3 * (size_t)s + 1] = make_float4(m.x, m.y, m.z, ml);
+    crec[3 * (size_t)s + 2] = make_float4(I.x, I.y, I.z, (float)size[s]);
+}
+
+// G~(p, node) (cudaGather.cu:45-57 with the node's moments); for a leaf the
+// exact G of the AVPL.
+__device__ inline float geom_est(const PixelF& P, const float4& r0, const float4& r1) {
+    const float dx = r0.x - P.px, dy = r0.y - P.py, dz = r0.z - P.pz;
+    const float dd = dx * dx + dy * dy + dz * dz;
+    const float inv = dd > 0.f ? rsq(dd) : 0.f;  // coincident points: no contribution (not 0 * inf)
+    const float c1 = fminf(fmaxf((P.nx * dx + P.ny * dy + P.nz * dz) * inv, 0.f), 1.f);
+    const float c2 = fminf(0.25f * (1.f - r1.w) + fmaxf(-(r1.x * dx + r1.y * dy + r1.z * dz) * inv, 0.f), 1.f);
+    return c1 * c2 * (inv * inv);
+}
+
+__device__ inline float node_bound(const PixelF& P, const arc_bvh_node& b, float ilum, float delta2) {
+    const float ex = fmaxf(fmaxf(b.bbMin.x - P.px, P.px - b.bbMax.x), 0.f);
+    const float ey = fmaxf(fmaxf(b.bbMin.y - P.py, P.py - b.bbMax.y), 0.f);
+    const float ez = fmaxf(fmaxf(b.bbMin.z - P.pz, P.pz - b.bbMax.z), 0.f);
+    const float dmin2 = fmaxf(ex * ex + ey * ey + ez * ez, delta2);
+    const float hmax = P.nx * (P.nx > 0.f ? b.bbMax.x - P.px : b.bbMin.x - P.px) +
+                       P.ny * (P.ny > 0.f ? b.bbMax.y - P.py : b.bbMin.y - P.py) +
+                       P.nz * (P.nz > 0.f ? b.bbMax.z - P.pz : b.bbMin.z - P.pz);
+    // 0 when the node lies behind the pixel's tangent plane
+    return hmax > 0.f ? ilum * fminf(1.f, hmax * rsq(dmin2)) / dmin2 : 0.f;
+}
+
+// Estimated error of keeping the node instead of its AVPLs (criterion 1): the
+// contribution bound seen from the centroid, max(I) cos_max / (d_c - s)^2
+// with s the half-diagonal of the box, times the estimator's relative error:
+// (s / d_c)^2 for the position (centroid, second order) plus
+// 1.2 (1 - |m|) / sqrt(count) for the spread of the clamped cosines of
+// differing normals (their mean over `count` normals has that relative
+// spread).  A pixel within 2 s of the centroid always refines.
+__device__ inline float node_error(const PixelF& P, const arc_bvh_node& b, const float4& r0, const float4& r1,
+                                   const float4& r2, float delta2) {
+    const float ex = b.bbMax.x - b.bbMin.x, ey = b.bbMax.y - b.bbMin.y, ez = b.bbMax.z - b.bbMin.z;
+    const float s2 = 0.25f * (ex * ex + ey * ey + ez * ez);
+    const float dx = r0.x - P.px, dy = r0.y - P.py, dz = r0.z - P.pz;
+    const float dc2 = dx * dx + dy * dy + dz * dz;
+    if (dc2 <= 4.f * s2) return INFINITY;
+    const float hmax = P.nx * (P.nx > 0.f ? b.bbMax.x - P.px : b.bbMin.x - P.px) +
+                       P.ny * (P.ny > 0.f ? b.bbMax.y - P.py : b.bbMin.y - P.py) +
+                       P.nz * (P.nz > 0.f ? b.bbMax.z - P.pz : b.bbMin.z - P.pz);
+    if (!(hmax > 0.f)) return 0.f;  // behind the pixel's tangent plane: exactly 0
+    const float dc = sqrtf(dc2), sn = sqrtf(s2);
+    const float near = fmaxf(dc - sn, sqrtf(delta2));
+    const float inv_near = 1.0f / near;
+    const float cb = fminf(1.f, hmax * inv_near);
+    const float rel = s2 / dc2 + 1.2f * (1.f - r1.w) * rsq(fmaxf(r2.w, 1.f));
+    return r0.w * cb * (inv_near * inv_near) * rel;
+}
+
+__device__ inline int uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
+
+struct Acc {
+    Kahan k[3];
+    float s0 = 0.f, s1 = 0.f, s2 = 0.f;
+    int run = 0;
+    __device__ inline void add(const float4& I, float g) {
+        s0 += I.x * g;
+        s1 += I.y * g;
+        s2 += I.z * g;
+        if (++run == kRun) flush();
+    }
+    __device__ inline void flush() {
+        k[0].add(s0); k[1].add(s1); k[2].add(s2);
+        s0 = s1 = s2 = 0.f;
+        run = 0;
+    }
+};
+
+// Pixel of lane `lane` of wave `wave` in a 16x16 block: four 8x8 tiles.
+__device__ inline int2 tile_pixel(int wave, int lane) {
+    return make_int2(blockIdx.x * 16 + (wave & 1) * 8 + (lane & 7), blockIdx.y * 16 + (wave >> 1) * 8 + (lane >> 3));
+}
+
+struct Threshold {
+    const float* est;  // per-pixel estimate of the depth-d0 cut
+    const float* sum;  // its sum over the image
+    float eps, floor_frac, inv_npix;
+    // eps * max(estimate, floor_frac * image-mean estimate)
+    __device__ inline float at(int q) const {
+        return fmaxf(eps * fmaxf(est[q], floor_frac * sum[0] * inv_npix), 1e-30f);
+    }
+};
+
+__global__ void k_cut_zero(float* __restrict__ sum) {
+    if (threadIdx.x == 0) sum[0] = 0.f;
+}
+
+// Phase 1: the pixel's estimate I * G~ summed over the cut made of every node
+// at depth d0 (and the leaves above it).  The cut is the same for every
+// pixel, so the wave walks it uniformly; each lane sums its own pixel.
+__global__ __launch_bounds__(kCutThreads) void k_cut_estimate(CutEB t, const float4* __restrict__ gpos,
+                                                              const float4* __restrict__ gnrm,
+                                                              const float4* __restrict__ galb,
+                                                              const arc_material* __restrict__ mats, int W, int H,
+                                                              int d0, float* __restrict__ est,
+                                                              float* __restrict__ sum) {
+    __shared__ int stk[kCutThreads / 64][kEstStack];
+    __shared__ float red[kCutThreads / 64];
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const int2 xy = tile_pixel(wave, lane);
+    const bool live = xy.x < W && xy.y < H;
+    const int q = live ? xy.y * W + xy.x : 0;
+    const PixelF P = load_pixel(gpos, gnrm, galb, mats, q);
+    int* S = stk[wave];
+    const int n = t.n;
+    float e = 0.f;
+    int sp = 1;
+    S[0] = -64;  // node * 64 + depth
+    while (sp > 0) {
+        const int v = uniform(S[--sp]);
+        const int node = v >> 6, dep = v & 63;
+        if (node >= 0 || dep >= d0) {
+            const float4* r = t.crec + 3 * (size_t)(node >= 0 ? node : n + (-(node + 1)));
+            e += r[0].w * geom_est(P, r[0], r[1]);
+            continue;
+        }
+        const arc_bvh_node& b = t.nodes[-(node + 1)];
+        S[sp] = b.left * 64 + dep + 1;
+        S[sp + 1] = b.right * 64 + dep + 1;
+        sp += 2;
+    }
+    if (live) est[q] = e;
+    float w = live ? e : 0.f;
+    for (int off = 32; off > 0; off >>= 1) w += __shfl_down(w, off);
+    if (lane == 0) red[wave] = w;
+    __syncthreads();
+    if (tid == 0) atomicAdd(sum, red[0] + red[1] + red[2] + red[3]);
+}
+
+// ------------------------------------------------------------ per pixel --
+__global__ __launch_bounds__(kCutThreads) void k_cut_eb_px(CutEB t, const float4* __restrict__ gpos,
+                                                           const float4* __restrict__ gnrm,
+                                                           const float4* __restrict__ galb,
+                                                           const arc_material* __restrict__ mats, int W, int H,
+                                                           Threshold th, float delta2, int budget, int criterion,
+                                                           float4* __restrict__ out_rad,
+                                                           unsigned long long* __restrict__ counters) {
+    __shared__ int stk[kPxStack * kCutThreads];
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const int2 xy = tile_pixel(wave, lane);
+    const bool live = xy.x < W && xy.y < H;
+    unsigned long long nvis = 0, nev = 0, nerr = 0, nbud = 0;
+    if (live) {
+        const int q = xy.y * W + xy.x;
+        const PixelF P = load_pixel(gpos, gnrm, galb, mats, q);
+        const int n = t.n;
+        int* S = stk + tid;  // entry j at S[j * kCutThreads]
+        const float thr = th.at(q);
+        // depth first: the right child waits on the stack while the left one
+        // is processed at once
+        Acc acc;
+        int sp = 0;
+        int node = -1;
+        for (;;) {
+            int slot = node;
+            if (node < 0) {
+                const int k = -(node + 1);
+                const arc_bvh_node& b = t.nodes[k];
+                slot = n + k;
+                const float4* r = t.crec + 3 * (size_t)slot;
+                const float crit = criterion ? node_error(P, b, r[0], r[1], r[2], delta2)
+                                             : node_bound(P, b, r[0].w, delta2);
+                if (crit > thr) {
+                    if ((int)nev >= budget) {
+                        ++nbud;  // cut budget spent: keep the cluster
+                    } else if (sp < kPxStack) {
+                        S[(sp++) * kCutThreads] = b.right;
+                        node = b.left;
+                        ++nvis;
+                        continue;
+                    } else {
+                        ++nerr;  // stack full: keep the cluster
+                    }
+                }
+            }
+            const float4* r = t.crec + 3 * (size_t)slot;
+            acc.add(r[2], geom_est(P, r[0], r[1]));
+            ++nev;
+            if (sp == 0) break;
+            node = S[(--sp) * kCutThreads];
+        }
+        acc.flush();
+        out_rad[q] = make_float4(acc.k[0].s * P.br, acc.k[1].s * P.bg, acc.k[2].s * P.bb, 1.f);
+    }
+    if (counters) {
+        for (int off = 32; off > 0; off >>= 1) {
+            nvis += __shfl_down(nvis, off);
+            nev += __shfl_down(nev, off);
+            nerr += __shfl_down(nerr, off);
+            nbud += __shfl_down(nbud, off);
+        }
+        if (lane == 0) {
+            atomicAdd(&counters[C_VISITED], nvis);
+            atomicAdd(&counters[C_EVALUATED], nev);
+            atomicAdd(&counters[C_ERRORS], nerr);
+            atomicAdd(&counters[C_BUDGET], nbud);
+        }
+    }
+}
+
+// ------------------------------------------------------------- per tile --
+__global__ __launch_bounds__(kCutThreads) void k_cut_eb_tile(CutEB t, const float4* __restrict__ gpos,
+                                                             const float4* __restrict__ gnrm,
+                                                             const float4* __restrict__ galb,
+                                                             const arc_material* __restrict__ mats, int W, int H,
+                                                             Threshold th, float delta2, int budget, int criterion,
+                                                             float4* __restrict__ out_rad,
+                                                             unsigned long long* __restrict__ counters) {
+    __shared__ int stk[kCutThreads / 64][kTileStack];
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const int2 xy = tile_pixel(wave, lane);
+    const bool live = xy.x < W && xy.y < H;
+    const int q = live ? xy.y * W + xy.x : 0;
+    const PixelF P = load_pixel(gpos, gnrm, galb, mats, q);
+    int* S = stk[wave];
+    const int n = t.n;
+    const float thr = live ? th.at(q) : INFINITY;
+    Acc acc;
+    unsigned long long nvis = 0, nev = 0, nerr = 0, nbud = 0;
+    int sp = 0;
+    int node = -1;
+    for (;;) {
+        int slot = node;
+        if (node < 0) {
+            const int k = -(node + 1);
+            const arc_bvh_node& b = t.nodes[k];
+            slot = n + k;
+            const float4* r = t.crec + 3 * (size_t)slot;
+            const float crit = criterion ? node_error(P, b, r[0], r[1], r[2], delta2)
+                                         : node_bound(P, b, r[0].w, delta2);
+            if (__ballot(crit > thr)) {
+                if ((int)nev >= budget) {
+                    ++nbud;
+                } else if (sp < kTileStack) {
+                    S[sp++] = b.right;
+                    node = uniform(b.left);
+                    ++nvis;
+                    continue;
+                } else {
+                    ++nerr;
+                }
+            }
+        }
+        const float4* r = t.crec + 3 * (size_t)slot;
+        acc.add(r[2], geom_est(P, r[0], r[1]));
+        ++nev;
+        if (sp == 0) break;
+        node = uniform(S[--sp]);
+    }
+    acc.flush();
+    if (counters && lane == 0) {
+        const int alive = __popcll(__ballot(live));
+        atomicAdd(&counters[C_VISITED], nvis * alive);
+        atomicAdd(&counters[C_EVALUATED], nev * alive);
+        atomicAdd(&counters[C_ERRORS], nerr);
+        atomicAdd(&counters[C_BUDGET], nbud);
+    }
+    if (live) out_rad[q] = make_float4(acc.k[0].s * P.br, acc.k[1].s * P.bg, acc.k[2].s * P.bb, 1.f);
+}
+
+}  // namespace
+
+namespace arc {
+
+int cut_error_bound(arc_gather* g, arc_bvh* b, const CutEBParams& cp, float4* rad, unsigned long long* counters) {
+    arc_ctx* ctx = g->ctx;
+    const int n = b->n;
+    const int npix = g->width * g->height;
+    if (n > (1 << 25)) return fail(ctx, ARC_E_UNSUPPORTED, "error-bound cut: more than 2^25 AVPLs");
+    ARC_TRY(ensure(ctx, g->crec, sizeof(float4) * 3 * (size_t)(2 * n - 1)));
+    ARC_TRY(ensure(ctx, g->cest, sizeof(float) * ((size_t)npix + 64)));
+    float* est = g->cest.as<float>();
+    float* sum = est + ((size_t)npix + 63) / 64 * 64;
+    hipLaunchKernelGGL(k_pack_cut, dim3((2 * n - 1 + kCutThreads - 1) / kCutThreads), dim3(kCutThreads), 0,
+                       ctx->stream, n, b->nd_pos.as<f3>(), b->nd_nrm.as<f3>(), b->nd_radb.as<f3>(),
+                       b->nd_mom.as<float4>(), b->nd_nmom.as<f3>(), b->nd_size.as<int>(), g->crec.as<float4>());
+    CutEB t{b->nodes.as<arc_bvh_node>(), g->crec.as<float4>(), n};
+    const dim3 grid((g->width + 15) / 16, (g->height + 15) / 16);
+    hipLaunchKernelGGL(k_cut_zero, dim3(1), dim3(64), 0, ctx->stream, sum);
+    hipLaunchKernelGGL(k_cut_estimate, grid, dim3(kCutThreads), 0, ctx->stream, t, g->gpos, g->gnrm, g->galb,
+                       g->mats.as<arc_material>(), g->width, g->height, cp.d0, est, sum);
+    const Threshold th{est, sum, cp.eps, cp.image_floor, 1.0f / (float)npix};
+    const float delta2 = cp.delta * cp.delta;
+    if (cp.per_tile)
+        hipLaunchKernelGGL(k_cut_eb_tile, grid, dim3(kCutThreads), 0, ctx->stream, t, g->gpos, g->gnrm, g->galb,
+                           g->mats.as<arc_material>(), g->width, g->height, th, delta2, cp.budget, cp.criterion,
+                           rad, counters);
+    else
+        hipLaunchKernelGGL(k_cut_eb_px, grid, dim3(kCutThreads), 0, ctx->stream, t, g->gpos, g->gnrm, g->galb,
+                           g->mats.as<arc_material>(), g->width, g->height, th, delta2, cp.budget, cp.criterion,
+                           rad, counters);
+    ARC_LAUNCHED(ctx);
+    return ARC_OK;
+}
+
+}  // namespace arc

@@ -169,12 +169,15 @@ struct NodeData {  // AvplBvhNodeDataParam over 2n-1 slots
     f3* nrm;
     f3* rad;
     f3* dir;
-    f3* anti;  // antiradiance sums (extension; may be null)
+    f3* radb;  // sums of L x diffuse / pi, the BRDF-weighted intensity (extension; may be null)
+    float4* mom;  // with radb: sums of w x position and w (w = mean of L x diffuse / pi)
+    f3* nmom;     // with radb: sums of w x unit normal
 };
 
 __global__ __launch_bounds__(kBlock) void k_permute_avpls(const int* __restrict__ perm,
                                                           arc::AvplsDev a, f3* __restrict__ spos,
-                                                          f3* __restrict__ snrm, NodeData nd) {
+                                                          f3* __restrict__ snrm, NodeData nd,
+                                                          const arc_material* __restrict__ mats) {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= a.n) return;
     const int o = perm[s];
@@ -187,7 +190,24 @@ __global__ __launch_bounds__(kBlock) void k_permute_avpls(const int* __restrict_
     nd.nrm[s] = q;
     nd.rad[s] = a.rad[o];
     nd.dir[s] = a.dir[o];
-    if (nd.anti) nd.anti[s] = a.anti ? a.anti[o] : f3{0.f, 0.f, 0.f};
+    if (nd.radb) {
+        // L x f_r of the AVPL's Lambertian material (f_r = 1 for a light-source
+        // AVPL, w = 0, cudaGather.cu:89-90); the table holds 256 entries
+        const f3 L = a.rad[o], w = a.dir[o];
+        f3 r = L;
+        if (mats && (w.x != 0.f || w.y != 0.f || w.z != 0.f)) {
+            const arc_material& m = mats[a.mat[o] & 255];
+            r = f3{L.x * (0.318309886f * m.diffuse.x), L.y * (0.318309886f * m.diffuse.y),
+                   L.z * (0.318309886f * m.diffuse.z)};
+        }
+        nd.radb[s] = r;
+        const f3 q = a.nrm[o], x = a.pos[o];
+        const float w8 = (r.x + r.y + r.z) * (1.0f / 3.0f);
+        const float ql = q.x * q.x + q.y * q.y + q.z * q.z;
+        const float iq = ql > 0.f ? 1.0f / sqrtf(ql) : 0.f;
+        nd.mom[s] = make_float4(w8 * x.x, w8 * x.y, w8 * x.z, w8);
+        nd.nmom[s] = f3{w8 * q.x * iq, w8 * q.y * iq, w8 * q.z * iq};
+    }
 }
 
 __global__ __launch_bounds__(kBlock) void k_random(uint64_t seed, int count, float* __restrict__ out) {
@@ -267,9 +287,13 @@ __device__ inline void cluster(int target, int left, int right, int n, const Nod
     nd.dir[tid] = nd.dir[repr];
     nd.pos[tid] = nd.pos[repr];
     nd.nrm[tid] = nd.nrm[repr];
-    if (nd.anti) {
-        const f3 al = nd.anti[lid], ar = nd.anti[rid];
-        nd.anti[tid] = f3{al.x + ar.x, al.y + ar.y, al.z + ar.z};
+    if (nd.radb) {
+        const f3 al = nd.radb[lid], ar = nd.radb[rid];
+        nd.radb[tid] = f3{al.x + ar.x, al.y + ar.y, al.z + ar.z};
+        const float4 ml = nd.mom[lid], mr = nd.mom[rid];
+        nd.mom[tid] = make_float4(ml.x + mr.x, ml.y + mr.y, ml.z + mr.z, ml.w + mr.w);
+        const f3 nl = nd.nmom[lid], nr = nd.nmom[rid];
+        nd.nmom[tid] = f3{nl.x + nr.x, nl.y + nr.y, nl.z + nr.z};
     }
 }
 
@@ -454,7 +478,8 @@ int bvh_build_points(arc_bvh* b, const f3* pos, const f3* nrm, int n) {
     return ARC_OK;
 }
 
-int bvh_build_avpls(arc_bvh* b, const AvplsDev& a, const float* rand, uint64_t seed, bool validate) {
+int bvh_build_avpls(arc_bvh* b, const AvplsDev& a, const float* rand, uint64_t seed, bool validate,
+                    const arc_material* mats) {
     arc_ctx* ctx = b->ctx;
     const int n = a.n;
     if (n <= 1) {
@@ -470,7 +495,9 @@ int bvh_build_avpls(arc_bvh* b, const AvplsDev& a, const float* rand, uint64_t s
     ARC_TRY(ensure(ctx, b->nd_nrm, sizeof(f3) * slots));
     ARC_TRY(ensure(ctx, b->nd_rad, sizeof(f3) * slots));
     ARC_TRY(ensure(ctx, b->nd_dir, sizeof(f3) * slots));
-    ARC_TRY(ensure(ctx, b->nd_anti, sizeof(f3) * slots));
+    ARC_TRY(ensure(ctx, b->nd_radb, sizeof(f3) * slots));
+    ARC_TRY(ensure(ctx, b->nd_mom, sizeof(float4) * slots));
+    ARC_TRY(ensure(ctx, b->nd_nmom, sizeof(f3) * slots));
     if (validate) {
         int* flag = reinterpret_cast<int*>(b->bounds.as<float>() + 12);
         ARC_HIP(ctx, hipMemsetAsync(flag, 0, sizeof(int), ctx->stream));
@@ -494,9 +521,10 @@ int bvh_build_avpls(arc_bvh* b, const AvplsDev& a, const float* rand, uint64_t s
         ARC_TRY(random_numbers(ctx, seed, (int)slots, b->nd_rand.as<float>()));
     }
     NodeData nd{b->nd_size.as<int>(), b->nd_mat.as<int>(), b->nd_rand.as<float>(), b->nd_pos.as<f3>(),
-                b->nd_nrm.as<f3>(),   b->nd_rad.as<f3>(), b->nd_dir.as<f3>(),    b->nd_anti.as<f3>()};
+                b->nd_nrm.as<f3>(),   b->nd_rad.as<f3>(), b->nd_dir.as<f3>(),    b->nd_radb.as<f3>(),
+                b->nd_mom.as<float4>(), b->nd_nmom.as<f3>()};
     hipLaunchKernelGGL(k_permute_avpls, dim3(blocks_for(n)), dim3(kBlock), 0, ctx->stream, b->perm.as<int>(),
-                       a, b->positions.as<f3>(), b->normals.as<f3>(), nd);
+                       a, b->positions.as<f3>(), b->normals.as<f3>(), nd, mats);
     ARC_LAUNCHED(ctx);
     ARC_TRY(build_tree(b, n, &nd));
     b->n = n;

@@ -27,6 +27,8 @@ ARC_INPUT_HOST = 256
 ARC_TRUST_INPUT = 512
 ARC_CUT_SOLID_ANGLE = 0
 ARC_CUT_ERROR_BOUND = 1
+ARC_CRITERION_BOUND = 0
+ARC_CRITERION_ESTIMATE = 1
 ARC_ANTIRAD_DISK = 0
 ARC_ANTIRAD_CONE = 1
 ARC_ANTI_EXACT = 0
@@ -77,7 +79,9 @@ class arc_gather_params(C.Structure):
                 ("clusterRefinementThreshold", C.c_float), ("errorBound", C.c_float),
                 ("antiradianceModel", C.c_int), ("antiradianceMode", C.c_int),
                 ("photonRadiusScale", C.c_float), ("coneHalfAngle", C.c_float),
-                ("writeSentinel", C.c_int), ("stackSize", C.c_int), ("seed", C.c_uint64)]
+                ("writeSentinel", C.c_int), ("stackSize", C.c_int), ("seed", C.c_uint64),
+                ("cutEstimateDepth", C.c_int), ("cutBudget", C.c_int), ("cutPerTile", C.c_int),
+                ("cutCriterion", C.c_int), ("cutImageFloor", C.c_float)]
 
 
 class arc_gather_stats(C.Structure):
@@ -85,7 +89,7 @@ class arc_gather_stats(C.Structure):
                 ("msAntiradiance", C.c_float), ("msCombine", C.c_float), ("msTotal", C.c_float),
                 ("visitedNodes", C.c_longlong), ("evaluatedNodes", C.c_longlong),
                 ("antiVisitedNodes", C.c_longlong), ("antiEvaluatedNodes", C.c_longlong),
-                ("errorPixels", C.c_longlong), ("numLeafs", C.c_int)]
+                ("errorPixels", C.c_longlong), ("numLeafs", C.c_int), ("budgetHits", C.c_longlong)]
 
 
 class arc_scene_desc(C.Structure):

@@ -127,7 +127,7 @@ int arc_copy_to_host(arc_ctx* ctx, void* host_dst, const void* src, size_t bytes
 typedef struct arc_bvh arc_bvh;
 
 #define ARC_BVH_CONSIDER_NORMALS 1   /* 6D position+normal codes, truncated to 32 bits (bvh.h:17) */
-#define ARC_INPUT_HOST 256           /* input pointers are host memory (copied in)               */
+#define ARC_INPUT_HOST 256           /* input pointers are host memory, copied before the call returns */
 #define ARC_TRUST_INPUT 512          /* skip the synchronous bounce > 0 check of AVPL inputs      */
 
 /* Bvh(bool considerNormals) (bvh.h:56). */
@@ -180,6 +180,10 @@ typedef struct arc_gather arc_gather;
 #define ARC_CUT_SOLID_ANGLE 0   /* kernel_bvh: Omega < clusterRefinementThreshold (cudaGather.cu:410-414) */
 #define ARC_CUT_ERROR_BOUND 1   /* Lightcuts-style: refine while bound > errorBound * estimate           */
 
+/* What the error-bound cut compares with errorBound * estimate. */
+#define ARC_CRITERION_BOUND 0     /* upper bound of the node's contribution (Lightcuts)       */
+#define ARC_CRITERION_ESTIMATE 1  /* contribution bound x expected relative error of the node */
+
 #define ARC_ANTIRAD_DISK 0      /* getAntiradiance, cudaGather.cu:109-144                 */
 #define ARC_ANTIRAD_CONE 1      /* GetAntiradiance, Shaders/Gather.frag:149-188 x BRDF    */
 
@@ -202,6 +206,12 @@ typedef struct arc_gather_params {
     int writeSentinel;                /* 1: (1e5, 0, 1e5, 1) on traversal error      */
     int stackSize;                    /* traversal stack entries, 32 in kernel_bvh   */
     uint64_t seed;                    /* cluster-representative generator seed       */
+    int cutEstimateDepth;             /* ARC_CUT_ERROR_BOUND: depth of the estimate cut */
+    int cutBudget;                    /* ARC_CUT_ERROR_BOUND: most nodes one cut evaluates */
+    int cutPerTile;                   /* ARC_CUT_ERROR_BOUND: 1 = one cut per 8x8 tile, 0 = per pixel */
+    int cutCriterion;                 /* ARC_CUT_ERROR_BOUND: ARC_CRITERION_*                         */
+    float cutImageFloor;              /* ARC_CUT_ERROR_BOUND: the threshold is errorBound * max(pixel
+                                         estimate, cutImageFloor * image-mean estimate); 0 = per pixel */
 } arc_gather_params;
 
 void arc_gather_params_default(arc_gather_params* p);
@@ -214,8 +224,10 @@ typedef struct arc_gather_stats {
     long long evaluatedNodes;         /* (pixel, node or leaf) radiance evaluations */
     long long antiVisitedNodes;
     long long antiEvaluatedNodes;
-    long long errorPixels;            /* traversal stack overflows                 */
+    long long errorPixels;            /* traversal stack overflows (node kept unrefined)   */
     int numLeafs;                     /* AVPLs in the LBVH of this frame            */
+    long long budgetHits;             /* ARC_CUT_ERROR_BOUND: nodes kept because the
+                                         pixel's cutBudget was spent               */
 } arc_gather_stats;
 
 #define ARC_RUN_PROFILE 1      /* time each phase with HIP events (synchronises)  */

@@ -59,3 +59,37 @@ def test_struct_layouts_match_header():
     assert ctypes.sizeof(ffi.arc_material) == 40
     assert ctypes.sizeof(ffi.arc_float3) == 12
     assert ctypes.sizeof(ffi.arc_avpls) == 8 * 7 + 8
+
+
+def _c_layout(struct: str, fields):
+    """sizeof and offsetof as the C compiler lays the header's struct out."""
+    import subprocess
+    import tempfile
+
+    body = "".join(f'    printf("%zu\\n", offsetof({struct}, {f}));\n' for f in fields)
+    src = ("#include <stddef.h>\n#include <stdio.h>\n#include \"arc.h\"\nint main(void) {\n"
+           f'    printf("%zu\\n", sizeof({struct}));\n{body}    return 0;\n}}\n')
+    with tempfile.TemporaryDirectory() as td:
+        c, exe = Path(td) / "lay.c", Path(td) / "lay"
+        c.write_text(src)
+        subprocess.run(["gcc", "-std=c99", f"-I{ROOT / 'include'}", "-o", str(exe), str(c)], check=True)
+        out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split()
+    return [int(v) for v in out]
+
+
+@pytest.mark.parametrize("struct", ["arc_gather_params", "arc_gather_stats", "arc_avpls", "arc_bvh_param",
+                                    "arc_bvh_data", "arc_avpl_node_data", "arc_bvh_node", "arc_material"])
+def test_ctypes_layout_equals_c(struct):
+    ct = getattr(ffi, struct)
+    fields = [f[0] for f in ct._fields_]
+    got = _c_layout(struct, fields)
+    assert got[0] == ctypes.sizeof(ct)
+    assert got[1:] == [getattr(ct, f).offset for f in fields]
+
+
+def test_cpp_facade_compiles():
+    """include/arc_cudagather.hpp is self-contained C++17."""
+    import subprocess
+
+    subprocess.run(["g++", "-std=c++17", "-Wall", "-Werror", "-fsyntax-only", f"-I{ROOT / 'include'}",
+                    str(ROOT / "tests" / "cpp" / "facade_test.cpp")], check=True)
