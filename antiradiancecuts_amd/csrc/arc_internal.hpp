@@ -84,6 +84,7 @@ struct arc_bvh {
     int consider_normals = 0;
     int n = 0;               // leaves of the current build (0: none)
     bool has_node_data = false;
+    bool has_moments = false;
     arc::DevBuf codes, perm, nodes, parents, ranges, positions, normals;
     arc::DevBuf in_pos, in_nrm;                 // staged host input
     arc::DevBuf nd_size, nd_mat, nd_rand, nd_pos, nd_nrm, nd_rad, nd_dir;
@@ -103,7 +104,9 @@ struct AvplsDev {
     int n;
 };
 int bvh_build_points(arc_bvh* b, const f3* pos, const f3* nrm, int n);
+// moments: also sum the BRDF-weighted intensity and its position / normal
+// moments per node (the error-bound cut's node estimates), using mats.
 int bvh_build_avpls(arc_bvh* b, const AvplsDev& a, const float* rand, uint64_t seed, bool validate,
-                    const arc_material* mats = nullptr);
+                    const arc_material* mats = nullptr, bool moments = false);
 int random_numbers(arc_ctx* ctx, uint64_t seed, int count, float* out);
 }  // namespace arc

@@ -590,7 +590,7 @@ int arc_gather_run(arc_gather* g, const arc_avpls* avpls, arc_float3 camPos, flo
     if (cut) {
         // AvplBvh(avpls, false) per frame (cudaGather.cu:858)
         ARC_TRY(arc::bvh_build_avpls(g->bvh, a, nullptr, p->seed, (flags & ARC_TRUST_INPUT) == 0,
-                                     g->mats.as<arc_material>()));
+                                     g->mats.as<arc_material>(), p->cutMode == ARC_CUT_ERROR_BOUND));
         if (profile) ARC_HIP(ctx, hipEventRecord(ev[2], ctx->stream));
         arc_bvh* b = g->bvh;
         if (p->cutMode == ARC_CUT_ERROR_BOUND) {

@@ -43,6 +43,7 @@ constexpr int kClusterPx = 32;   // cluster side in pixels
 constexpr int kCap = 16384;      // list entries per cluster
 constexpr int kCompact = 1024;   // compaction block
 constexpr int kClustersPerBlock = 64;
+constexpr int kSub = 2048;       // per-wave tile sub-list entries (LDS)
 
 __device__ inline bool has_anti(const arc::AvplsDev& a, int i) {
     if (!a.anti) return false;
@@ -145,17 +146,57 @@ __device__ inline float safe_inv(float w) {
     return 1.0f / copysignf(m, w);
 }
 
-// Disk model: the forward ray must hit the box grown by the photon radius
-// (a disk hit lies on the ray within that radius of a pixel), and part of the
-// box must lie behind the AVPL's surface.  Cone model: the cone of half-angle
-// theta about w must reach the box's bounding sphere: angle(c - a, w) <=
-// theta + asin(R / |c - a|), with theta slightly enlarged (cos_t, sin_t).
+// Whether AVPL ray (p, w) with surface normal na can reach a pixel inside box
+// b = (min xyz, max xyz).  Disk model: the forward ray must hit the box grown
+// by the photon radius (a disk hit lies on the ray within that radius of a
+// pixel), and part of the box must lie behind the AVPL's surface.  Cone
+// model: the cone of half-angle theta about w must reach the box's bounding
+// sphere: angle(c - a, w) <= theta + asin(R / |c - a|), with theta slightly
+// enlarged (cos_t, sin_t).  Used for cluster boxes and for tile boxes: any
+// box that contains the pixels' positions keeps every accepted pair.
+template <int MODEL>
+__device__ inline bool ray_box_hit(const f3& p, const f3& w, const f3& na, float ix, float iy, float iz,
+                                   const float* b, float grow, float cos_t, float sin_t) {
+    const float bx0 = b[0] - grow, by0 = b[1] - grow, bz0 = b[2] - grow;
+    const float bx1 = b[3] + grow, by1 = b[4] + grow, bz1 = b[5] + grow;
+    if (MODEL == ARC_ANTIRAD_DISK) {
+        // ray-box slab test on the forward ray (intersectBB, cudaGather.cu:664-684)
+        const float tx0 = (bx0 - p.x) * ix, tx1 = (bx1 - p.x) * ix;
+        const float ty0 = (by0 - p.y) * iy, ty1 = (by1 - p.y) * iy;
+        const float tz0 = (bz0 - p.z) * iz, tz1 = (bz1 - p.z) * iz;
+        const float tmin = fmaxf(fmaxf(fminf(tx0, tx1), fminf(ty0, ty1)), fminf(tz0, tz1));
+        const float tmax = fminf(fminf(fmaxf(tx0, tx1), fmaxf(ty0, ty1)), fmaxf(tz0, tz1));
+        // some point of the box lies behind the AVPL's surface: min n_a . (x - a) < 0
+        const float behind = fminf(na.x * (bx0 - p.x), na.x * (bx1 - p.x)) +
+                             fminf(na.y * (by0 - p.y), na.y * (by1 - p.y)) +
+                             fminf(na.z * (bz0 - p.z), na.z * (bz1 - p.z));
+        return tmax >= fmaxf(tmin, 0.f) && behind < grow;
+    } else {
+        const float cx = 0.5f * (b[0] + b[3]) - p.x, cy = 0.5f * (b[1] + b[4]) - p.y,
+                    cz = 0.5f * (b[2] + b[5]) - p.z;
+        const float ex = 0.5f * (b[3] - b[0]), ey = 0.5f * (b[4] - b[1]), ez = 0.5f * (b[5] - b[2]);
+        const float R = sqrtf(ex * ex + ey * ey + ez * ez) + grow;
+        const float D = sqrtf(cx * cx + cy * cy + cz * cz);
+        const float wl = sqrtf(w.x * w.x + w.y * w.y + w.z * w.z);
+        const float cosang = (cx * w.x + cy * w.y + cz * w.z) / fmaxf(D * wl, 1e-30f);
+        const float sb = fminf(R / fmaxf(D, 1e-30f), 1.f), cb = sqrtf(fmaxf(0.f, 1.f - sb * sb));
+        return D <= R || cosang >= cos_t * cb - sin_t * sb - 1e-6f;
+    }
+}
+
 template <int MODEL>
 __global__ __launch_bounds__(256) void k_cluster_lists(arc::AvplsDev a, const int* __restrict__ idx,
                                                        const int* __restrict__ n_anti_ptr,
                                                        const float* __restrict__ cbox, int C, float grow,
                                                        float cos_t, float sin_t,
                                                        int* __restrict__ counts, int* __restrict__ lists) {
+    __shared__ float boxes[kClustersPerBlock][6];
+    const int c0 = blockIdx.y * kClustersPerBlock, c1 = min(C, c0 + kClustersPerBlock);
+    for (int k = threadIdx.x; k < (c1 - c0) * 6; k += 256) {
+        const int c = k / 6, e = k % 6;
+        boxes[c][e] = cbox[(size_t)(c0 + c) * 8 + (e < 3 ? e : e + 1)];
+    }
+    __syncthreads();
     const int n_anti = *n_anti_ptr;
     const int j = blockIdx.x * 256 + threadIdx.x;
     const bool valid = j < n_anti;
@@ -168,35 +209,8 @@ __global__ __launch_bounds__(256) void k_cluster_lists(arc::AvplsDev a, const in
         na = a.nrm[i];
     }
     const float ix = safe_inv(w.x), iy = safe_inv(w.y), iz = safe_inv(w.z);
-    const int c0 = blockIdx.y * kClustersPerBlock, c1 = min(C, c0 + kClustersPerBlock);
     for (int c = c0; c < c1; ++c) {
-        const float* b = cbox + (size_t)c * 8;
-        const float bx0 = b[0] - grow, by0 = b[1] - grow, bz0 = b[2] - grow;
-        const float bx1 = b[4] + grow, by1 = b[5] + grow, bz1 = b[6] + grow;
-        // ray-box slab test on the forward ray (intersectBB, cudaGather.cu:664-684)
-        const float tx0 = (bx0 - p.x) * ix, tx1 = (bx1 - p.x) * ix;
-        const float ty0 = (by0 - p.y) * iy, ty1 = (by1 - p.y) * iy;
-        const float tz0 = (bz0 - p.z) * iz, tz1 = (bz1 - p.z) * iz;
-        const float tmin = fmaxf(fmaxf(fminf(tx0, tx1), fminf(ty0, ty1)), fminf(tz0, tz1));
-        const float tmax = fminf(fminf(fmaxf(tx0, tx1), fmaxf(ty0, ty1)), fmaxf(tz0, tz1));
-        // some point of the box lies behind the AVPL's surface: min n_a . (x - a) < 0
-        const float behind = fminf(na.x * (bx0 - p.x), na.x * (bx1 - p.x)) +
-                             fminf(na.y * (by0 - p.y), na.y * (by1 - p.y)) +
-                             fminf(na.z * (bz0 - p.z), na.z * (bz1 - p.z));
-        bool hit;
-        if (MODEL == ARC_ANTIRAD_DISK) {
-            hit = valid && tmax >= fmaxf(tmin, 0.f) && behind < grow;
-        } else {
-            const float cx = 0.5f * (b[0] + b[4]) - p.x, cy = 0.5f * (b[1] + b[5]) - p.y,
-                        cz = 0.5f * (b[2] + b[6]) - p.z;
-            const float ex = 0.5f * (b[4] - b[0]), ey = 0.5f * (b[5] - b[1]), ez = 0.5f * (b[6] - b[2]);
-            const float R = sqrtf(ex * ex + ey * ey + ez * ez) + grow;
-            const float D = sqrtf(cx * cx + cy * cy + cz * cz);
-            const float wl = sqrtf(w.x * w.x + w.y * w.y + w.z * w.z);
-            const float cosang = (cx * w.x + cy * w.y + cz * w.z) / fmaxf(D * wl, 1e-30f);
-            const float sb = fminf(R / fmaxf(D, 1e-30f), 1.f), cb = sqrtf(fmaxf(0.f, 1.f - sb * sb));
-            hit = valid && (D <= R || cosang >= cos_t * cb - sin_t * sb - 1e-6f);
-        }
+        const bool hit = valid && ray_box_hit<MODEL>(p, w, na, ix, iy, iz, boxes[c - c0], grow, cos_t, sin_t);
         const uint64_t m = __ballot(hit);
         if (m) {
             const int leader = __ffsll((long long)m) - 1;
@@ -221,9 +235,10 @@ __global__ __launch_bounds__(256) void k_anti_clustered(const float4* __restrict
                                                         const int* __restrict__ counts,
                                                         const int* __restrict__ lists,
                                                         const int* __restrict__ idx,
-                                                        const int* __restrict__ n_anti_ptr,
-                                                        float* __restrict__ part,
+                                                        const int* __restrict__ n_anti_ptr, float grow,
+                                                        float cos_t, float sin_t, float* __restrict__ part,
                                                         unsigned long long* __restrict__ counters) {
+    __shared__ int sub[kThreads / 64][kSub];
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const int x = blockIdx.x * 16 + (wave & 1) * 8 + (lane & 7);
     const int y = blockIdx.y * 16 + (wave >> 1) * 8 + (lane >> 3);
@@ -235,14 +250,46 @@ __global__ __launch_bounds__(256) void k_anti_clustered(const float4* __restrict
     const int c = (blockIdx.y * 16 / kClusterPx) * cw + (blockIdx.x * 16 / kClusterPx);
     const int listed = counts[c];
     const bool overflow = listed > kCap;
-    const int count = overflow ? *n_anti_ptr : listed;
+    // a tile wholly outside the image (bottom / right edge blocks) has nothing to sum
+    const int count = __ballot(live) == 0 ? 0 : overflow ? *n_anti_ptr : listed;
     const int* list = overflow ? idx : lists + (size_t)c * kCap;
+    // Tile filter: the wave keeps, in list order, the entries whose ray can
+    // reach its 8x8 tile's box (the same conservative test as the cluster
+    // lists, on a smaller box).
+    float tb[6];
+    {
+        const float big = INFINITY;
+        tb[0] = live ? P.px : big; tb[1] = live ? P.py : big; tb[2] = live ? P.pz : big;
+        tb[3] = live ? P.px : -big; tb[4] = live ? P.py : -big; tb[5] = live ? P.pz : -big;
+        for (int off = 32; off > 0; off >>= 1) {
+            for (int k = 0; k < 3; ++k) tb[k] = fminf(tb[k], __shfl_xor(tb[k], off));
+            for (int k = 3; k < 6; ++k) tb[k] = fmaxf(tb[k], __shfl_xor(tb[k], off));
+        }
+    }
+    int nsub = 0;
+    for (int j0 = 0; j0 < count; j0 += 64) {
+        const int j = j0 + lane;
+        bool hit = false;
+        int i = 0;
+        if (j < count) {
+            i = list[j];
+            const f3 p = raw.pos[i], w = raw.dir[i], na = raw.nrm[i];
+            hit = ray_box_hit<MODEL>(p, w, na, safe_inv(w.x), safe_inv(w.y), safe_inv(w.z), tb, grow, cos_t, sin_t);
+        }
+        const uint64_t m = __ballot(hit);
+        const int slot = nsub + (int)__popcll(m & ((1ull << lane) - 1ull));
+        if (hit && slot < kSub) sub[wave][slot] = i;
+        nsub += (int)__popcll(m);
+    }
+    __syncthreads();  // the sub-lists written by each wave's lanes are read by all of its lanes
+    const bool use_sub = nsub <= kSub;
+    const int walk = use_sub ? nsub : count;
     Kahan ka[3];
-    for (int base = 0; base < count; base += kRun) {
-        const int end = min(count, base + kRun);
+    for (int base = 0; base < walk; base += kRun) {
+        const int end = min(walk, base + kRun);
         float s0 = 0.f, s1 = 0.f, s2 = 0.f;
         for (int j = base; j < end; ++j) {
-            const int i = list[j];
+            const int i = use_sub ? __builtin_amdgcn_readfirstlane(sub[wave][j]) : list[j];
             const float4* r = rec + (size_t)kRecF4 * i;
             const float4 r0 = r[0], r1 = r[1], r2v = r[2], r4 = r[4];
             const float dx = r0.x - P.px, dy = r0.y - P.py, dz = r0.z - P.pz;
@@ -269,7 +316,8 @@ __global__ __launch_bounds__(256) void k_anti_clustered(const float4* __restrict
     }
     if (counters && lane == 0) {
         const int alive = __popcll(__ballot(live));
-        atomicAdd(&counters[C_ANTI_EVALUATED], (unsigned long long)count * alive);
+        atomicAdd(&counters[C_ANTI_VISITED], (unsigned long long)count);
+        atomicAdd(&counters[C_ANTI_EVALUATED], (unsigned long long)walk * alive);
     }
     if (!live) return;
     float* o = part + q;
@@ -283,12 +331,13 @@ __global__ __launch_bounds__(256) void k_anti_clustered(const float4* __restrict
 
 template <int MODEL, bool GENERAL>
 void launch_pixels(arc_gather* g, const arc::AvplsDev& a, int cw, float3 cam, const AntiParams& ap,
-                   const int* n_anti_ptr, float* part, unsigned long long* counters) {
+                   const int* n_anti_ptr, float grow, float cos_t, float sin_t, float* part,
+                   unsigned long long* counters) {
     const dim3 grid((g->width + 15) / 16, (g->height + 15) / 16);
     hipLaunchKernelGGL((k_anti_clustered<MODEL, GENERAL>), grid, dim3(256), 0, g->ctx->stream, g->rec.as<float4>(), a,
                        g->gpos, g->gnrm, g->galb, g->mats.as<arc_material>(), g->width, g->height, cw, cam.x, cam.y,
-                       cam.z, ap, g->ccount.as<int>(), g->clist.as<int>(), g->anti_idx.as<int>(), n_anti_ptr, part,
-                       counters);
+                       cam.z, ap, g->ccount.as<int>(), g->clist.as<int>(), g->anti_idx.as<int>(), n_anti_ptr, grow,
+                       cos_t, sin_t, part, counters);
 }
 
 }  // namespace
@@ -331,11 +380,11 @@ int anti_clustered(arc_gather* g, const AvplsDev& a, int model, float3 cam, cons
                            g->cbox.as<float>(), C, grow, cos_t, sin_t, g->ccount.as<int>(), g->clist.as<int>());
     ARC_LAUNCHED(ctx);
     if (model == ARC_ANTIRAD_DISK) {
-        if (g->general) launch_pixels<ARC_ANTIRAD_DISK, true>(g, a, cw, cam, ap, n_anti, part, counters);
-        else launch_pixels<ARC_ANTIRAD_DISK, false>(g, a, cw, cam, ap, n_anti, part, counters);
+        if (g->general) launch_pixels<ARC_ANTIRAD_DISK, true>(g, a, cw, cam, ap, n_anti, grow, cos_t, sin_t, part, counters);
+        else launch_pixels<ARC_ANTIRAD_DISK, false>(g, a, cw, cam, ap, n_anti, grow, cos_t, sin_t, part, counters);
     } else {
-        if (g->general) launch_pixels<ARC_ANTIRAD_CONE, true>(g, a, cw, cam, ap, n_anti, part, counters);
-        else launch_pixels<ARC_ANTIRAD_CONE, false>(g, a, cw, cam, ap, n_anti, part, counters);
+        if (g->general) launch_pixels<ARC_ANTIRAD_CONE, true>(g, a, cw, cam, ap, n_anti, grow, cos_t, sin_t, part, counters);
+        else launch_pixels<ARC_ANTIRAD_CONE, false>(g, a, cw, cam, ap, n_anti, grow, cos_t, sin_t, part, counters);
     }
     ARC_LAUNCHED(ctx);
     return ARC_OK;

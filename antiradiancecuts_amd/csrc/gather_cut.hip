@@ -358,6 +358,7 @@ int cut_error_bound(arc_gather* g, arc_bvh* b, const CutEBParams& cp, float4* ra
     const int n = b->n;
     const int npix = g->width * g->height;
     if (n > (1 << 25)) return fail(ctx, ARC_E_UNSUPPORTED, "error-bound cut: more than 2^25 AVPLs");
+    if (!b->has_moments) return fail(ctx, ARC_E_STATE, "error-bound cut: LBVH built without node moments");
     ARC_TRY(ensure(ctx, g->crec, sizeof(float4) * 3 * (size_t)(2 * n - 1)));
     ARC_TRY(ensure(ctx, g->cest, sizeof(float) * ((size_t)npix + 64)));
     float* est = g->cest.as<float>();

@@ -475,11 +475,12 @@ int bvh_build_points(arc_bvh* b, const f3* pos, const f3* nrm, int n) {
     ARC_TRY(build_tree(b, n, nullptr));
     b->n = n;
     b->has_node_data = false;
+    b->has_moments = false;
     return ARC_OK;
 }
 
 int bvh_build_avpls(arc_bvh* b, const AvplsDev& a, const float* rand, uint64_t seed, bool validate,
-                    const arc_material* mats) {
+                    const arc_material* mats, bool moments) {
     arc_ctx* ctx = b->ctx;
     const int n = a.n;
     if (n <= 1) {
@@ -495,9 +496,11 @@ int bvh_build_avpls(arc_bvh* b, const AvplsDev& a, const float* rand, uint64_t s
     ARC_TRY(ensure(ctx, b->nd_nrm, sizeof(f3) * slots));
     ARC_TRY(ensure(ctx, b->nd_rad, sizeof(f3) * slots));
     ARC_TRY(ensure(ctx, b->nd_dir, sizeof(f3) * slots));
-    ARC_TRY(ensure(ctx, b->nd_radb, sizeof(f3) * slots));
-    ARC_TRY(ensure(ctx, b->nd_mom, sizeof(float4) * slots));
-    ARC_TRY(ensure(ctx, b->nd_nmom, sizeof(f3) * slots));
+    if (moments) {
+        ARC_TRY(ensure(ctx, b->nd_radb, sizeof(f3) * slots));
+        ARC_TRY(ensure(ctx, b->nd_mom, sizeof(float4) * slots));
+        ARC_TRY(ensure(ctx, b->nd_nmom, sizeof(f3) * slots));
+    }
     if (validate) {
         int* flag = reinterpret_cast<int*>(b->bounds.as<float>() + 12);
         ARC_HIP(ctx, hipMemsetAsync(flag, 0, sizeof(int), ctx->stream));
@@ -521,14 +524,16 @@ int bvh_build_avpls(arc_bvh* b, const AvplsDev& a, const float* rand, uint64_t s
         ARC_TRY(random_numbers(ctx, seed, (int)slots, b->nd_rand.as<float>()));
     }
     NodeData nd{b->nd_size.as<int>(), b->nd_mat.as<int>(), b->nd_rand.as<float>(), b->nd_pos.as<f3>(),
-                b->nd_nrm.as<f3>(),   b->nd_rad.as<f3>(), b->nd_dir.as<f3>(),    b->nd_radb.as<f3>(),
-                b->nd_mom.as<float4>(), b->nd_nmom.as<f3>()};
+                b->nd_nrm.as<f3>(),   b->nd_rad.as<f3>(), b->nd_dir.as<f3>(),
+                moments ? b->nd_radb.as<f3>() : nullptr, moments ? b->nd_mom.as<float4>() : nullptr,
+                moments ? b->nd_nmom.as<f3>() : nullptr};
     hipLaunchKernelGGL(k_permute_avpls, dim3(blocks_for(n)), dim3(kBlock), 0, ctx->stream, b->perm.as<int>(),
                        a, b->positions.as<f3>(), b->normals.as<f3>(), nd, mats);
     ARC_LAUNCHED(ctx);
     ARC_TRY(build_tree(b, n, &nd));
     b->n = n;
     b->has_node_data = true;
+    b->has_moments = moments;
     return ARC_OK;
 }
 
