@@ -15,18 +15,24 @@
 //                  one cluster per 32x32 pixel block, box of its G-buffer
 //                  positions (the reference adds depth slices; a block of
 //                  the screen with its tight box is what the list test needs);
-//   k_cluster_lists
-//                  lane = AVPL, loop over clusters with box data read by
-//                  wave-uniform loads; ray-box slab test against the box
-//                  grown by the photon radius plus a margin, and the
-//                  "pixel behind the AVPL's surface" bound; hits appended
-//                  with one atomic per wave and cluster (ballot + popcount);
-//                  the reference's C x N list buffer becomes C x kCap with
-//                  an overflow flag;
+//   k_cluster_tree
+//                  a screen-space quadtree over the cluster boxes (2x2
+//                  merges up to one root), one workgroup;
+//   k_cluster_lists_tree
+//                  lane = antiradiance AVPL, depth-first descent of the
+//                  quadtree with a conservative test per box: ray-box slab
+//                  test against the box grown by the photon radius plus a
+//                  margin and the "pixel behind the AVPL's surface" bound
+//                  (disk), or the cone against the box's bounding sphere;
+//                  a ray meets ~10 of ~2000 clusters, so the descent tests
+//                  a few dozen boxes where a flat loop tests all of them.
+//                  The reference's C x N list buffer becomes C x kCap with an
+//                  overflow flag;
 //   k_anti_clustered
-//                  one 8x8 pixel tile per wave; the wave walks its cluster's
-//                  list with wave-uniform (scalar) loads of the entries and
-//                  the packed records and evaluates the guarded pair test
+//                  one 8x8 pixel tile per wave; the wave first filters its
+//                  cluster's list against the tile's own box into an LDS
+//                  sub-list, then walks it with wave-uniform (scalar) loads
+//                  of the packed records and evaluates the guarded pair test
 //                  per pixel.  A cluster whose list overflowed walks every
 //                  antiradiance AVPL instead.
 #include <cmath>
@@ -42,8 +48,9 @@ namespace {
 constexpr int kClusterPx = 32;   // cluster side in pixels
 constexpr int kCap = 16384;      // list entries per cluster
 constexpr int kCompact = 1024;   // compaction block
-constexpr int kClustersPerBlock = 64;
 constexpr int kSub = 2048;       // per-wave tile sub-list entries (LDS)
+constexpr int kTreeMaxLevels = 10;  // cluster quadtree levels: up to 512 x 512 clusters
+constexpr int kTreeStack = 32;      // >= 1 + 3 (levels - 1) entries of the tree descent
 
 __device__ inline bool has_anti(const arc::AvplsDev& a, int i) {
     if (!a.anti) return false;
@@ -184,44 +191,78 @@ __device__ inline bool ray_box_hit(const f3& p, const f3& w, const f3& na, float
     }
 }
 
+// Quadtree over the cluster grid: level 0 is the clusters (cw x ch boxes,
+// the k_cluster_bounds output), level l + 1 merges 2x2 nodes of level l, up
+// to one root.  Boxes are [8] = min xyz, -, max xyz, -.
+struct ClusterTree {
+    int levels;
+    int off[kTreeMaxLevels];  // first node of each level
+    int w[kTreeMaxLevels], h[kTreeMaxLevels];
+};
+
+__global__ __launch_bounds__(1024) void k_cluster_tree(float* __restrict__ tree, ClusterTree t) {
+    for (int l = 1; l < t.levels; ++l) {
+        const int W = t.w[l], H = t.h[l], cw = t.w[l - 1], ch = t.h[l - 1];
+        for (int k = threadIdx.x; k < W * H; k += blockDim.x) {
+            const int x = k % W, y = k / W;
+            float b[6] = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY};
+            for (int cy = 2 * y; cy <= min(2 * y + 1, ch - 1); ++cy)
+                for (int cx = 2 * x; cx <= min(2 * x + 1, cw - 1); ++cx) {
+                    const float* c = tree + 8 * (size_t)(t.off[l - 1] + cy * cw + cx);
+                    for (int e = 0; e < 3; ++e) {
+                        b[e] = fminf(b[e], c[e]);
+                        b[3 + e] = fmaxf(b[3 + e], c[4 + e]);
+                    }
+                }
+            float* o = tree + 8 * (size_t)(t.off[l] + k);
+            o[0] = b[0]; o[1] = b[1]; o[2] = b[2]; o[3] = 0.f;
+            o[4] = b[3]; o[5] = b[4]; o[6] = b[5]; o[7] = 0.f;
+        }
+        __syncthreads();  // level l complete before level l + 1 reads it
+    }
+}
+
+// Lists by tree descent: lane = antiradiance AVPL, depth-first through the
+// quadtree with the same conservative test at every level (a parent box
+// contains its children, so a rejected parent rejects every cluster below).
+// Each lane appends its own hits.
 template <int MODEL>
-__global__ __launch_bounds__(256) void k_cluster_lists(arc::AvplsDev a, const int* __restrict__ idx,
-                                                       const int* __restrict__ n_anti_ptr,
-                                                       const float* __restrict__ cbox, int C, float grow,
-                                                       float cos_t, float sin_t,
-                                                       int* __restrict__ counts, int* __restrict__ lists) {
-    __shared__ float boxes[kClustersPerBlock][6];
-    const int c0 = blockIdx.y * kClustersPerBlock, c1 = min(C, c0 + kClustersPerBlock);
-    for (int k = threadIdx.x; k < (c1 - c0) * 6; k += 256) {
-        const int c = k / 6, e = k % 6;
-        boxes[c][e] = cbox[(size_t)(c0 + c) * 8 + (e < 3 ? e : e + 1)];
+__global__ __launch_bounds__(256) void k_cluster_lists_tree(arc::AvplsDev a, const int* __restrict__ idx,
+                                                            const int* __restrict__ n_anti_ptr,
+                                                            const float* __restrict__ tree, ClusterTree t,
+                                                            float grow, float cos_t, float sin_t,
+                                                            int* __restrict__ counts, int* __restrict__ lists) {
+    __shared__ int s_off[kTreeMaxLevels], s_w[kTreeMaxLevels], s_h[kTreeMaxLevels];
+    __shared__ int stk[kTreeStack][256];
+    if (threadIdx.x < kTreeMaxLevels) {
+        s_off[threadIdx.x] = t.off[threadIdx.x];
+        s_w[threadIdx.x] = t.w[threadIdx.x];
+        s_h[threadIdx.x] = t.h[threadIdx.x];
     }
     __syncthreads();
-    const int n_anti = *n_anti_ptr;
     const int j = blockIdx.x * 256 + threadIdx.x;
-    const bool valid = j < n_anti;
-    const int lane = threadIdx.x & 63;
-    const int i = valid ? idx[j] : 0;
-    f3 p{0.f, 0.f, 0.f}, w{1.f, 0.f, 0.f}, na{0.f, 0.f, 0.f};
-    if (valid) {
-        p = a.pos[i];
-        w = a.dir[i];
-        na = a.nrm[i];
-    }
+    if (j >= *n_anti_ptr) return;  // no barrier follows
+    const int i = idx[j];
+    const f3 p = a.pos[i], w = a.dir[i], na = a.nrm[i];
     const float ix = safe_inv(w.x), iy = safe_inv(w.y), iz = safe_inv(w.z);
-    for (int c = c0; c < c1; ++c) {
-        const bool hit = valid && ray_box_hit<MODEL>(p, w, na, ix, iy, iz, boxes[c - c0], grow, cos_t, sin_t);
-        const uint64_t m = __ballot(hit);
-        if (m) {
-            const int leader = __ffsll((long long)m) - 1;
-            int base = 0;
-            if (lane == leader) base = atomicAdd(&counts[c], (int)__popcll(m));
-            base = __shfl(base, leader);
-            if (hit) {
-                const int slot = base + (int)__popcll(m & ((1ull << lane) - 1ull));
-                if (slot < kCap) lists[(size_t)c * kCap + slot] = i;
-            }
+    const int tid = threadIdx.x;
+    int sp = 0;
+    stk[sp++][tid] = (t.levels - 1) << 24;  // root: (level, y, x) = (top, 0, 0)
+    while (sp > 0) {
+        const int v = stk[--sp][tid];
+        const int l = v >> 24, y = (v >> 12) & 4095, x = v & 4095;
+        const float* b8 = tree + 8 * (size_t)(s_off[l] + y * s_w[l] + x);
+        const float b[6] = {b8[0], b8[1], b8[2], b8[4], b8[5], b8[6]};
+        if (!ray_box_hit<MODEL>(p, w, na, ix, iy, iz, b, grow, cos_t, sin_t)) continue;
+        if (l == 0) {
+            const int c = y * s_w[0] + x;
+            const int slot = atomicAdd(&counts[c], 1);
+            if (slot < kCap) lists[(size_t)c * kCap + slot] = i;
+            continue;
         }
+        const int cw = s_w[l - 1], ch = s_h[l - 1];
+        for (int cy = min(2 * y + 1, ch - 1); cy >= 2 * y; --cy)
+            for (int cx = min(2 * x + 1, cw - 1); cx >= 2 * x; --cx) stk[sp++][tid] = ((l - 1) << 24) | (cy << 12) | cx;
     }
 }
 
@@ -353,7 +394,20 @@ int anti_clustered(arc_gather* g, const AvplsDev& a, int model, float3 cam, cons
     const int C = cw * ch;
     // anti_idx: [n] compacted indices, then [nb + 1] block offsets (last = count)
     ARC_TRY(ensure(ctx, g->anti_idx, sizeof(int) * ((size_t)n + nb + 1)));
-    ARC_TRY(ensure(ctx, g->cbox, sizeof(float) * 8 * (size_t)C));
+    ClusterTree tree{};
+    tree.w[0] = cw;
+    tree.h[0] = ch;
+    int nodes = C;
+    tree.levels = 1;
+    while (tree.w[tree.levels - 1] > 1 || tree.h[tree.levels - 1] > 1) {
+        if (tree.levels == kTreeMaxLevels) return fail(ctx, ARC_E_UNSUPPORTED, "image too large for the cluster tree");
+        const int l = tree.levels++;
+        tree.off[l] = nodes;
+        tree.w[l] = (tree.w[l - 1] + 1) / 2;
+        tree.h[l] = (tree.h[l - 1] + 1) / 2;
+        nodes += tree.w[l] * tree.h[l];
+    }
+    ARC_TRY(ensure(ctx, g->cbox, sizeof(float) * 8 * (size_t)nodes));
     ARC_TRY(ensure(ctx, g->ccount, sizeof(int) * (size_t)C));
     ARC_TRY(ensure(ctx, g->clist, sizeof(int) * (size_t)C * kCap));
     int* idx = g->anti_idx.as<int>();
@@ -371,13 +425,15 @@ int anti_clustered(arc_gather* g, const AvplsDev& a, int model, float3 cam, cons
     const double th = std::min(M_PI, half_angle * 1.001 + 1e-4);
     const float cos_t = (float)std::cos(th), sin_t = (float)std::sin(th);
     const int* n_anti = offs + nb;
-    const dim3 lgrid((n + 255) / 256, (C + kClustersPerBlock - 1) / kClustersPerBlock);
+    if (tree.levels > 1)
+        hipLaunchKernelGGL(k_cluster_tree, dim3(1), dim3(1024), 0, ctx->stream, g->cbox.as<float>(), tree);
+    const dim3 lgrid((n + 255) / 256);
     if (model == ARC_ANTIRAD_DISK)
-        hipLaunchKernelGGL(k_cluster_lists<ARC_ANTIRAD_DISK>, lgrid, dim3(256), 0, ctx->stream, a, idx, n_anti,
-                           g->cbox.as<float>(), C, grow, cos_t, sin_t, g->ccount.as<int>(), g->clist.as<int>());
+        hipLaunchKernelGGL(k_cluster_lists_tree<ARC_ANTIRAD_DISK>, lgrid, dim3(256), 0, ctx->stream, a, idx, n_anti,
+                           g->cbox.as<float>(), tree, grow, cos_t, sin_t, g->ccount.as<int>(), g->clist.as<int>());
     else
-        hipLaunchKernelGGL(k_cluster_lists<ARC_ANTIRAD_CONE>, lgrid, dim3(256), 0, ctx->stream, a, idx, n_anti,
-                           g->cbox.as<float>(), C, grow, cos_t, sin_t, g->ccount.as<int>(), g->clist.as<int>());
+        hipLaunchKernelGGL(k_cluster_lists_tree<ARC_ANTIRAD_CONE>, lgrid, dim3(256), 0, ctx->stream, a, idx, n_anti,
+                           g->cbox.as<float>(), tree, grow, cos_t, sin_t, g->ccount.as<int>(), g->clist.as<int>());
     ARC_LAUNCHED(ctx);
     if (model == ARC_ANTIRAD_DISK) {
         if (g->general) launch_pixels<ARC_ANTIRAD_DISK, true>(g, a, cw, cam, ap, n_anti, grow, cos_t, sin_t, part, counters);

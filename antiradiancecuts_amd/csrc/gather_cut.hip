@@ -27,8 +27,13 @@
 //                     height of the box over the pixel's tangent plane (<= 0:
 //                     the whole node is behind the pixel and contributes
 //                     exactly 0); the AVPL-side cosine is bounded by 1.
-//   threshold         errorBound * the pixel's estimate from the cut at depth
-//                     d0 (phase 1).
+//   threshold         errorBound * max(e, cutImageFloor * mean of e over the
+//                     image), e the pixel's estimate from the cut made of the
+//                     nodes at depth d0 (k_cut_estimate).  The floor keeps
+//                     pixels that see almost nothing from refining to the
+//                     leaves: the image error is an L2 norm, so a dark pixel
+//                     needs no finer absolute error than an average one, and
+//                     the longest cut sets the kernel's time.
 //
 // Two traversals share the node records:
 //   k_cut_eb_px   one pixel per lane, its own depth-first cut (the reference's
@@ -271,7 +276,9 @@ __global__ __launch_bounds__(kCutThreads) void k_cut_eb_px(CutEB t, const float4
             node = S[(--sp) * kCutThreads];
         }
         acc.flush();
-        out_rad[q] = make_float4(acc.k[0].s * P.br, acc.k[1].s * P.bg, acc.k[2].s * P.bb, 1.f);
+        // with counters, w carries the pixel's cut size (kernel_bvh's debug output role)
+        out_rad[q] = make_float4(acc.k[0].s * P.br, acc.k[1].s * P.bg, acc.k[2].s * P.bb,
+                                 counters ? (float)nev : 1.f);
     }
     if (counters) {
         for (int off = 32; off > 0; off >>= 1) {
@@ -346,7 +353,9 @@ __global__ __launch_bounds__(kCutThreads) void k_cut_eb_tile(CutEB t, const floa
         atomicAdd(&counters[C_ERRORS], nerr);
         atomicAdd(&counters[C_BUDGET], nbud);
     }
-    if (live) out_rad[q] = make_float4(acc.k[0].s * P.br, acc.k[1].s * P.bg, acc.k[2].s * P.bb, 1.f);
+    if (live)
+        out_rad[q] = make_float4(acc.k[0].s * P.br, acc.k[1].s * P.bg, acc.k[2].s * P.bb,
+                                 counters ? (float)nev : 1.f);
 }
 
 }  // namespace

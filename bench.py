@@ -31,27 +31,41 @@ sys.path.insert(0, str(ROOT))
 METRIC = "pixel-AVPL pairs/s + ms/frame at 1080p×1M AVPLs; rel-L2 vs FP64 oracle"
 PEAK_FP32_TFLOPS = 157.3   # MI355X vector FP32 (published)
 PEAK_HBM_GBS = 8000.0
-# Algorithmic FLOP per unit (FMA = 2): radiance pair 31, disk antiradiance
-# pair 37, cone antiradiance pair 33, visited inner cut node 32, evaluated
-# cut node +31.
+# Algorithmic FLOP per unit (FMA = 2, transcendental = 1):
+#   radiance pair 31; disk antiradiance pair 37; cone antiradiance pair 33;
+#   solid-angle cut: visited inner node 32, evaluated node 31 + 31;
+#   error-bound cut: node test (criterion) 45, node estimate 37, and the
+#   depth-d0 estimate cut 32 per node and pixel;
+#   clustered antiradiance: tile-filter ray-box test 35 per list entry and
+#   tile, plus the pair tests.
 FLOP_RAD_PAIR, FLOP_DISK_PAIR, FLOP_CONE_PAIR = 31, 37, 33
 FLOP_VISIT, FLOP_EVAL = 32, 31
+FLOP_EB_TEST, FLOP_EB_EST, FLOP_EB_PHASE1 = 45, 37, 32
+FLOP_TILE_FILTER = 35
+# AvplBvh rebuild, algorithmic HBM bytes per leaf: codes, perm, sorted
+# position / normal, nodes, parents, ranges, node data and random numbers
+# (276), plus the error-bound cut's node moments and packed records (80 + 48)
+BYTES_BVH_LEAF, BYTES_MOMENTS_LEAF = 276, 128
 PHASES = ("upload", "bvh", "radiance", "antiradiance", "combine", "total")
 
 
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
-    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--width", type=int, default=1920)
     ap.add_argument("--height", type=int, default=1080)
     ap.add_argument("--avpls", type=int, default=1_000_000)
     ap.add_argument("--tris", type=int, default=200_000)
     ap.add_argument("--error-stride", type=int, default=4,
                     help="FP64 error on every k-th pixel in x and y (0 = skip)")
-    ap.add_argument("--cpu-sample", type=int, default=1024,
+    ap.add_argument("--cpu-sample", type=int, default=256,
                     help="CPU baseline on 1/k of the pixels (0 = skip)")
+    ap.add_argument("--cut", choices=("error-bound", "solid-angle"), default="error-bound",
+                    help="light cut of the radiance: the analytic error bound or kernel_bvh's solid angle")
+    ap.add_argument("--error-bound", type=float, default=0.01, help="relative error bound of the cut")
+    ap.add_argument("--per-tile", type=int, default=1, help="error-bound cut: one cut per 8x8 tile (1) or per pixel (0)")
     return ap.parse_args()
 
 
@@ -74,7 +88,10 @@ def main() -> int:
     g = Gather(ctx, sc.width, sc.height, sc.materials)
     g.set_gbuffer(device_array(sc.pos4), device_array(sc.nrm4), device_array(sc.albedo4))
     av = DeviceAvpls.from_scene(sc)
-    params = default_params(useLightCuts=1, antiradianceModel=ffi.ARC_ANTIRAD_DISK)
+    eb = args.cut == "error-bound"
+    params = default_params(useLightCuts=1, antiradianceModel=ffi.ARC_ANTIRAD_DISK,
+                            cutMode=ffi.ARC_CUT_ERROR_BOUND if eb else ffi.ARC_CUT_SOLID_ANGLE,
+                            errorBound=args.error_bound, cutPerTile=args.per_tile)
     flags = ffi.ARC_RUN_TIMELINE | ffi.ARC_TRUST_INPUT
 
     for _ in range(args.warmup):
@@ -114,32 +131,41 @@ def main() -> int:
         diff = result[pix] - ref
         err = {"rel_l2": float(np.linalg.norm(diff) / np.linalg.norm(ref)),
                "max_abs": float(np.abs(diff).max()),
+               "max_rel_vs_rms": float(np.abs(diff).max() / np.sqrt(np.mean(ref ** 2))),
                "ref_norm_rms": float(np.sqrt(np.mean(ref ** 2))),
                "pixels": int(len(pix)), "sample": f"every {k}th pixel in x and y", "f64_seconds": round(t_f64, 2)}
         gs.close()
 
-    # roofline of the dominant kernel (phase timed on the library's stream)
+    # roofline of the dominant phase (timed on the library's stream with HIP
+    # events; its kernels are listed by the rocprofv3 summaries in profiles/)
     npix, n = sc.npix, sc.num_avpls
     dom = max(("bvh", "radiance", "antiradiance"), key=lambda k: phase_ms.get(k, 0.0))
     if dom == "antiradiance":
-        flop = float(npix) * n * FLOP_DISK_PAIR
-        roof = {"kernel": "k_exact (antiradiance, disk)", "bound": "valu"}
+        flop = float(st.antiEvaluatedNodes) * FLOP_DISK_PAIR + float(st.antiVisitedNodes) * FLOP_TILE_FILTER
+        roof = {"kernel": "k_anti_clustered (clustered disk antiradiance)", "bound": "valu"}
+    elif dom == "radiance" and eb:
+        flop = (float(st.visitedNodes + st.evaluatedNodes) * FLOP_EB_TEST + float(st.evaluatedNodes) * FLOP_EB_EST
+                + float(npix) * (2 ** params.cutEstimateDepth) * FLOP_EB_PHASE1)
+        roof = {"kernel": "k_cut_eb_tile (error-bound light cut)" if args.per_tile else
+                "k_cut_eb_px (error-bound light cut)", "bound": "valu"}
     elif dom == "radiance":
         flop = float(st.visitedNodes) * FLOP_VISIT + float(st.evaluatedNodes) * (FLOP_EVAL + FLOP_RAD_PAIR)
-        roof = {"kernel": "k_cut_sa (light-cut radiance)", "bound": "valu"}
+        roof = {"kernel": "k_cut_sa (solid-angle light cut)", "bound": "valu"}
     else:
         flop = None
-        roof = {"kernel": "LBVH rebuild", "bound": "hbm"}
+        roof = {"kernel": "AvplBvh rebuild (sort, Karras, refit)", "bound": "hbm"}
     t_dom = phase_ms.get(dom, float("nan"))
     if flop is not None:
         ach = flop / (t_dom * 1e-3) / 1e12
         roof.update({"achieved": round(ach, 3), "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
-                     "frac": round(ach / PEAK_FP32_TFLOPS, 4), "traffic": None, "ms": round(t_dom, 4)})
+                     "frac": round(ach / PEAK_FP32_TFLOPS, 4), "traffic": None, "ms": round(t_dom, 4),
+                     "phase": dom})
     else:
-        bytes_ = 276.0 * n
+        bytes_ = float(BYTES_BVH_LEAF + (BYTES_MOMENTS_LEAF if eb else 0)) * n
         ach = bytes_ / (t_dom * 1e-3) / 1e9
         roof.update({"achieved": round(ach, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                     "frac": round(ach / PEAK_HBM_GBS, 4), "traffic": None, "ms": round(t_dom, 4)})
+                     "frac": round(ach / PEAK_HBM_GBS, 4), "traffic": None, "ms": round(t_dom, 4),
+                     "phase": dom})
 
     cpu = cpu_baseline(sc, params, args.cpu_sample) if args.cpu_sample > 0 else None
 
@@ -154,6 +180,7 @@ def main() -> int:
         "ms_per_step": round(ms_per_step, 4),
         "ms_per_frame_p50": round(float(np.percentile(frame_ms, 50)), 4),
         "ms_per_frame_p99": round(float(np.percentile(frame_ms, 99)), 4),
+        "evaluated_pairs_per_s": (float(st.evaluatedNodes) + float(st.antiEvaluatedNodes)) / (ms_per_step * 1e-3),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -162,11 +189,16 @@ def main() -> int:
         "config": {
             "workload": f"clutter_{sc.width}x{sc.height}: Cornell walls + {args.tris} random triangles, "
                         f"{n} AVPLs (bounce 1-4, L log-uniform [1e-3,10], 30% antiradiance); AvplBvh rebuilt "
-                        "per frame; light-cut radiance (solid angle < pi/10); disk antiradiance over per-cluster AVPL lists (exact)",
+                        "per frame; light-cut radiance ("
+                        + (f"error bound {args.error_bound:g}, {'per 8x8 tile' if args.per_tile else 'per pixel'}"
+                           if eb else "solid angle < pi/10")
+                        + "); disk antiradiance over per-cluster AVPL lists (exact)",
             "width": sc.width, "height": sc.height, "avpls": n, "parallelism": "single GPU",
             "phase_ms": {k: round(v, 4) for k, v in phase_ms.items()},
-            "cut": {"visited_nodes": int(st.visitedNodes), "evaluated_nodes": int(st.evaluatedNodes),
-                    "error_pixels": int(st.errorPixels)},
+            "cut": {"mode": args.cut, "error_bound": args.error_bound if eb else None,
+                    "visited_nodes": int(st.visitedNodes), "evaluated_nodes": int(st.evaluatedNodes),
+                    "stack_overflows": int(st.errorPixels), "budget_hits": int(st.budgetHits)},
+            "antiradiance": {"pair_tests": int(st.antiEvaluatedNodes), "tile_filter_tests": int(st.antiVisitedNodes)},
             "scene_build_s": round(t_scene, 2),
         },
         "error_vs_f64": err,

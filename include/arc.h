@@ -231,7 +231,9 @@ typedef struct arc_gather_stats {
 } arc_gather_stats;
 
 #define ARC_RUN_PROFILE 1      /* time each phase with HIP events (synchronises)  */
-#define ARC_RUN_COUNTERS 2     /* count visited / evaluated nodes                  */
+#define ARC_RUN_COUNTERS 2     /* count visited / evaluated nodes; the error-bound
+                                  cut then writes each pixel's evaluated-node count
+                                  into the radiance output's w (debug)             */
 #define ARC_RUN_TIMELINE 4     /* record phase events without synchronising       */
 /* ARC_INPUT_HOST (256): the arc_avpls pointers are host memory, uploaded as
  * AvplsGpu does (data_gpu.cu:5-52). */
