@@ -29,12 +29,15 @@
 //                  The reference's C x N list buffer becomes C x kCap with an
 //                  overflow flag;
 //   k_anti_clustered
-//                  one 8x8 pixel tile per wave; the wave first filters its
-//                  cluster's list against the tile's own box into an LDS
-//                  sub-list, then walks it with wave-uniform (scalar) loads
-//                  of the packed records and evaluates the guarded pair test
-//                  per pixel.  A cluster whose list overflowed walks every
-//                  antiradiance AVPL instead.
+//                  one 1024-thread block per cluster, one wave per 8x8 tile;
+//                  the block reads the cluster's list once and sorts its
+//                  entries into per-tile sub-lists in LDS (ray against the
+//                  tile's own box), then each wave walks its sub-list with
+//                  wave-uniform (scalar) loads of the packed records and
+//                  evaluates the guarded pair test per pixel.  A cluster
+//                  whose list overflowed walks every antiradiance AVPL; the
+//                  summation order within a pixel follows the list order,
+//                  which the appends leave unspecified.
 #include <cmath>
 
 #include "gather_common.hpp"
@@ -48,7 +51,9 @@ namespace {
 constexpr int kClusterPx = 32;   // cluster side in pixels
 constexpr int kCap = 16384;      // list entries per cluster
 constexpr int kCompact = 1024;   // compaction block
-constexpr int kSub = 2048;       // per-wave tile sub-list entries (LDS)
+constexpr int kTiles = (kClusterPx / 8) * (kClusterPx / 8);  // 8x8 tiles per cluster
+constexpr int kAntiThreads = 64 * kTiles;                      // one wave per tile
+constexpr int kSub = 1536;       // tile sub-list entries in LDS (more: the tile walks the cluster list)
 constexpr int kTreeMaxLevels = 10;  // cluster quadtree levels: up to 512 x 512 clusters
 constexpr int kTreeStack = 32;      // >= 1 + 3 (levels - 1) entries of the tree descent
 
@@ -266,71 +271,78 @@ __global__ __launch_bounds__(256) void k_cluster_lists_tree(arc::AvplsDev a, con
     }
 }
 
+// One block per 32x32 cluster, one wave per 8x8 tile.  The block reads the
+// cluster's list once: each thread takes an entry and tests its ray against
+// the 16 tile boxes (the same conservative test as the lists, on smaller
+// boxes), appending hits to the tile's sub-list in LDS.  Each wave then walks
+// its tile's sub-list with wave-uniform (scalar) loads of the packed records.
 template <int MODEL, bool GENERAL>
-__global__ __launch_bounds__(256) void k_anti_clustered(const float4* __restrict__ rec, arc::AvplsDev raw,
-                                                        const float4* __restrict__ gpos,
-                                                        const float4* __restrict__ gnrm,
-                                                        const float4* __restrict__ galb,
-                                                        const arc_material* __restrict__ mats, int W, int H,
-                                                        int cw, float camx, float camy, float camz, AntiParams ap,
-                                                        const int* __restrict__ counts,
-                                                        const int* __restrict__ lists,
-                                                        const int* __restrict__ idx,
-                                                        const int* __restrict__ n_anti_ptr, float grow,
-                                                        float cos_t, float sin_t, float* __restrict__ part,
-                                                        unsigned long long* __restrict__ counters) {
-    __shared__ int sub[kThreads / 64][kSub];
+__global__ __launch_bounds__(kAntiThreads) void k_anti_clustered(
+    const float4* __restrict__ rec, arc::AvplsDev raw, const float4* __restrict__ gpos,
+    const float4* __restrict__ gnrm, const float4* __restrict__ galb, const arc_material* __restrict__ mats, int W,
+    int H, int cw, float camx, float camy, float camz, AntiParams ap, const int* __restrict__ counts,
+    const int* __restrict__ lists, const int* __restrict__ idx, const int* __restrict__ n_anti_ptr, float grow,
+    float cos_t, float sin_t, float* __restrict__ part, unsigned long long* __restrict__ counters) {
+    __shared__ float tbox[kTiles][6];
+    __shared__ int tlive[kTiles];
+    __shared__ int tcount[kTiles];
+    __shared__ int sub[kTiles][kSub];
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
-    const int x = blockIdx.x * 16 + (wave & 1) * 8 + (lane & 7);
-    const int y = blockIdx.y * 16 + (wave >> 1) * 8 + (lane >> 3);
+    const int c = xcd_block(blockIdx.x, gridDim.x);
+    const int x = (c % cw) * kClusterPx + (wave & 3) * 8 + (lane & 7);
+    const int y = (c / cw) * kClusterPx + (wave >> 2) * 8 + (lane >> 3);
     const bool live = x < W && y < H;
     const int npix = W * H;
     const int q = live ? y * W + x : 0;
     const PixelF P = load_pixel(gpos, gnrm, galb, mats, q);
-    // the wave's 8x8 tile lies inside one 32x32 cluster
-    const int c = (blockIdx.y * 16 / kClusterPx) * cw + (blockIdx.x * 16 / kClusterPx);
-    const int listed = counts[c];
-    const bool overflow = listed > kCap;
-    // a tile wholly outside the image (bottom / right edge blocks) has nothing to sum
-    const int count = __ballot(live) == 0 ? 0 : overflow ? *n_anti_ptr : listed;
-    const int* list = overflow ? idx : lists + (size_t)c * kCap;
-    // Tile filter: the wave keeps, in list order, the entries whose ray can
-    // reach its 8x8 tile's box (the same conservative test as the cluster
-    // lists, on a smaller box).
-    float tb[6];
-    {
+    {  // the tile's box
         const float big = INFINITY;
-        tb[0] = live ? P.px : big; tb[1] = live ? P.py : big; tb[2] = live ? P.pz : big;
-        tb[3] = live ? P.px : -big; tb[4] = live ? P.py : -big; tb[5] = live ? P.pz : -big;
+        float tb[6] = {live ? P.px : big, live ? P.py : big, live ? P.pz : big,
+                       live ? P.px : -big, live ? P.py : -big, live ? P.pz : -big};
         for (int off = 32; off > 0; off >>= 1) {
             for (int k = 0; k < 3; ++k) tb[k] = fminf(tb[k], __shfl_xor(tb[k], off));
             for (int k = 3; k < 6; ++k) tb[k] = fmaxf(tb[k], __shfl_xor(tb[k], off));
         }
-    }
-    int nsub = 0;
-    for (int j0 = 0; j0 < count; j0 += 64) {
-        const int j = j0 + lane;
-        bool hit = false;
-        int i = 0;
-        if (j < count) {
-            i = list[j];
-            const f3 p = raw.pos[i], w = raw.dir[i], na = raw.nrm[i];
-            hit = ray_box_hit<MODEL>(p, w, na, safe_inv(w.x), safe_inv(w.y), safe_inv(w.z), tb, grow, cos_t, sin_t);
+        const bool any = __ballot(live) != 0;
+        if (lane == 0) {
+            for (int k = 0; k < 6; ++k) tbox[wave][k] = tb[k];
+            tlive[wave] = any ? 1 : 0;  // a tile wholly outside the image has nothing to sum
+            tcount[wave] = 0;
         }
-        const uint64_t m = __ballot(hit);
-        const int slot = nsub + (int)__popcll(m & ((1ull << lane) - 1ull));
-        if (hit && slot < kSub) sub[wave][slot] = i;
-        nsub += (int)__popcll(m);
     }
-    __syncthreads();  // the sub-lists written by each wave's lanes are read by all of its lanes
-    const bool use_sub = nsub <= kSub;
-    const int walk = use_sub ? nsub : count;
+    __syncthreads();
+    const int listed = counts[c];
+    const bool overflow = listed > kCap;
+    const int count = overflow ? *n_anti_ptr : listed;
+    const int* list = overflow ? idx : lists + (size_t)c * kCap;
+    for (int j = tid; j < count; j += kAntiThreads) {
+        const int i = list[j];
+        const f3 p = raw.pos[i], w = raw.dir[i], na = raw.nrm[i];
+        const float ix = safe_inv(w.x), iy = safe_inv(w.y), iz = safe_inv(w.z);
+        for (int t = 0; t < kTiles; ++t) {
+            if (tlive[t] && ray_box_hit<MODEL>(p, w, na, ix, iy, iz, tbox[t], grow, cos_t, sin_t)) {
+                const int slot = atomicAdd(&tcount[t], 1);
+                if (slot < kSub) sub[t][slot] = i;
+            }
+        }
+    }
+    __syncthreads();
+    const int nsub = tlive[wave] ? tcount[wave] : 0;
+    const bool use_sub = nsub <= kSub;  // else the tile walks the whole list
+    const int walk = !tlive[wave] ? 0 : use_sub ? nsub : count;
+    // Entries are fetched 64 at a time, one per lane, and broadcast with
+    // readlane, so the record loads of consecutive entries do not wait on
+    // one another.  Each chunk is one plain FP32 run, Kahan-added.
+    static_assert(kRun == 64, "one compensated add per 64-entry chunk");
     Kahan ka[3];
-    for (int base = 0; base < walk; base += kRun) {
-        const int end = min(walk, base + kRun);
+    for (int base = 0; base < walk; base += 64) {
+        const int jj = base + lane;
+        const int mine = jj < walk ? (use_sub ? sub[wave][jj] : list[jj]) : 0;
+        const int cnt = min(64, walk - base);
         float s0 = 0.f, s1 = 0.f, s2 = 0.f;
-        for (int j = base; j < end; ++j) {
-            const int i = use_sub ? __builtin_amdgcn_readfirstlane(sub[wave][j]) : list[j];
+#pragma unroll 4
+        for (int u = 0; u < cnt; ++u) {
+            const int i = __builtin_amdgcn_readlane(mine, u);
             const float4* r = rec + (size_t)kRecF4 * i;
             const float4 r0 = r[0], r1 = r[1], r2v = r[2], r4 = r[4];
             const float dx = r0.x - P.px, dy = r0.y - P.py, dz = r0.z - P.pz;
@@ -355,9 +367,9 @@ __global__ __launch_bounds__(256) void k_anti_clustered(const float4* __restrict
         }
         ka[0].add(s0); ka[1].add(s1); ka[2].add(s2);
     }
+    const int alive = __popcll(__ballot(live));  // all lanes take part in the ballot
     if (counters && lane == 0) {
-        const int alive = __popcll(__ballot(live));
-        atomicAdd(&counters[C_ANTI_VISITED], (unsigned long long)count);
+        if (wave == 0) atomicAdd(&counters[C_ANTI_VISITED], (unsigned long long)count * kTiles);
         atomicAdd(&counters[C_ANTI_EVALUATED], (unsigned long long)walk * alive);
     }
     if (!live) return;
@@ -371,14 +383,13 @@ __global__ __launch_bounds__(256) void k_anti_clustered(const float4* __restrict
 }
 
 template <int MODEL, bool GENERAL>
-void launch_pixels(arc_gather* g, const arc::AvplsDev& a, int cw, float3 cam, const AntiParams& ap,
+void launch_pixels(arc_gather* g, const arc::AvplsDev& a, int cw, int C, float3 cam, const AntiParams& ap,
                    const int* n_anti_ptr, float grow, float cos_t, float sin_t, float* part,
                    unsigned long long* counters) {
-    const dim3 grid((g->width + 15) / 16, (g->height + 15) / 16);
-    hipLaunchKernelGGL((k_anti_clustered<MODEL, GENERAL>), grid, dim3(256), 0, g->ctx->stream, g->rec.as<float4>(), a,
-                       g->gpos, g->gnrm, g->galb, g->mats.as<arc_material>(), g->width, g->height, cw, cam.x, cam.y,
-                       cam.z, ap, g->ccount.as<int>(), g->clist.as<int>(), g->anti_idx.as<int>(), n_anti_ptr, grow,
-                       cos_t, sin_t, part, counters);
+    hipLaunchKernelGGL((k_anti_clustered<MODEL, GENERAL>), dim3(C), dim3(kAntiThreads), 0, g->ctx->stream,
+                       g->rec.as<float4>(), a, g->gpos, g->gnrm, g->galb, g->mats.as<arc_material>(), g->width,
+                       g->height, cw, cam.x, cam.y, cam.z, ap, g->ccount.as<int>(), g->clist.as<int>(),
+                       g->anti_idx.as<int>(), n_anti_ptr, grow, cos_t, sin_t, part, counters);
 }
 
 }  // namespace
@@ -436,11 +447,11 @@ int anti_clustered(arc_gather* g, const AvplsDev& a, int model, float3 cam, cons
                            g->cbox.as<float>(), tree, grow, cos_t, sin_t, g->ccount.as<int>(), g->clist.as<int>());
     ARC_LAUNCHED(ctx);
     if (model == ARC_ANTIRAD_DISK) {
-        if (g->general) launch_pixels<ARC_ANTIRAD_DISK, true>(g, a, cw, cam, ap, n_anti, grow, cos_t, sin_t, part, counters);
-        else launch_pixels<ARC_ANTIRAD_DISK, false>(g, a, cw, cam, ap, n_anti, grow, cos_t, sin_t, part, counters);
+        if (g->general) launch_pixels<ARC_ANTIRAD_DISK, true>(g, a, cw, C, cam, ap, n_anti, grow, cos_t, sin_t, part, counters);
+        else launch_pixels<ARC_ANTIRAD_DISK, false>(g, a, cw, C, cam, ap, n_anti, grow, cos_t, sin_t, part, counters);
     } else {
-        if (g->general) launch_pixels<ARC_ANTIRAD_CONE, true>(g, a, cw, cam, ap, n_anti, grow, cos_t, sin_t, part, counters);
-        else launch_pixels<ARC_ANTIRAD_CONE, false>(g, a, cw, cam, ap, n_anti, grow, cos_t, sin_t, part, counters);
+        if (g->general) launch_pixels<ARC_ANTIRAD_CONE, true>(g, a, cw, C, cam, ap, n_anti, grow, cos_t, sin_t, part, counters);
+        else launch_pixels<ARC_ANTIRAD_CONE, false>(g, a, cw, C, cam, ap, n_anti, grow, cos_t, sin_t, part, counters);
     }
     ARC_LAUNCHED(ctx);
     return ARC_OK;

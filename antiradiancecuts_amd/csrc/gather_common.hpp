@@ -45,6 +45,19 @@ namespace gd {
 using namespace arc::gm;
 
 constexpr int kThreads = 256;
+
+// Workgroups of a launch are dealt to the 8 XCDs round robin (workgroup b runs
+// on XCD b % 8), and each XCD has its own L2.  xcd_block maps the launch index
+// to a logical block so that each XCD runs one contiguous range of logical
+// blocks: neighbouring screen blocks, which read the same tree nodes and AVPL
+// lists, then share an L2.  A bijection on [0, nblocks).
+__device__ inline int xcd_block(int b, int nblocks) {
+    constexpr int kXcds = 8;
+    const int x = b % kXcds, k = b / kXcds;
+    int off = 0;
+    for (int y = 0; y < x; ++y) off += (nblocks - y + kXcds - 1) / kXcds;
+    return off + k;
+}
 constexpr int kRecF4 = 5;  // float4 per packed AVPL record
 constexpr int kRun = 64;   // pairs summed plainly before a compensated add
 // Relative guard band of the FP32 antiradiance decisions (FP32 rounding of
@@ -167,6 +180,11 @@ __device__ inline float anti_weight(const PixelF& P, const float4& r0, const flo
         // With h = d + t w_unit, |h| < r  <=>  |h denom|^2 < r^2 denom^2, and
         // h denom = d denom - cn w_unit: no division needed.
         const float denom = r2v.x * P.nx + r2v.y * P.ny + r2v.z * P.nz;
+        const float dl = kGuard * (dd * inv);
+        // Outside the guard bands the signs are certain, and a pair that
+        // fails one of them contributes 0 in FP32 and in FP64 alike: skip the
+        // rest (most pairs of a tile fail the same sign test together).
+        if (!(denom < guard_w && cn > -dl && ca < dl)) return 0.f;
         const float hx = dx * denom - cn * r0.w;
         const float hy = dy * denom - cn * r1.w;
         const float hz = dz * denom - cn * r2v.w;
@@ -174,7 +192,6 @@ __device__ inline float anti_weight(const PixelF& P, const float4& r0, const flo
         const float dn2 = denom * denom;
         const float lim = ap.r2 * dn2;
         const float g = kGuard * (dd * dn2 + cn * cn);
-        const float dl = kGuard * (dd * inv);
         bool in = denom < 0.f && cn > 0.f && h2 < lim && ca < 0.f;
         const bool amb = h2 < lim + g &&
                          (!(fabsf(h2 - lim) > g) || !(fabsf(cn) > dl) || !(fabsf(ca) > dl) ||

@@ -57,19 +57,25 @@ namespace {
 constexpr int kTileStack = 96;  // per-wave stack of the tile traversal
 constexpr int kPxStack = 48;    // per-lane stack of the pixel traversal (LDS)
 constexpr int kCutThreads = 256;
+constexpr int kCrec = 4;        // float4 per node record
 constexpr int kEstStack = 48;   // per-wave stack of the depth-d0 estimate cut (d0 <= 40)
 
 struct CutEB {
     const arc_bvh_node* nodes;  // n - 1 internal nodes
-    const float4* crec;         // 3 float4 per slot: (centroid, max I), (m, |m|), (I, AVPL count)
+    const float4* crec;         // kCrec float4 per slot, see k_pack_cut
     int n;
 };
 
+// Node records, one 64-byte line per slot (leaves, then internal nodes):
+//   r0 = (centroid, max(I)), r1 = (m, (1 - |m|) / 4), r2 = (I, 1.2 (1 - |m|) / sqrt(count)),
+//   r3 = (s^2, s, -, -) with s the half-diagonal of the node's box (0 for a leaf).
+// Everything of the tests that does not depend on the pixel is here.
 __global__ __launch_bounds__(kCutThreads) void k_pack_cut(int n, const f3* __restrict__ pos,
                                                           const f3* __restrict__ nrm,
                                                           const f3* __restrict__ radb,
                                                           const float4* __restrict__ mom,
                                                           const f3* __restrict__ nmom, const int* __restrict__ size,
+                                                          const arc_bvh_node* __restrict__ nodes,
                                                           float4* __restrict__ crec) {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= 2 * n - 1) return;
@@ -89,9 +95,18 @@ __global__ __launch_bounds__(kCutThreads) void k_pack_cut(int n, const f3* __res
         m = f3{q.x * iq, q.y * iq, q.z * iq};
     }
     const float ml = fminf(1.f, sqrtf(m.x * m.x + m.y * m.y + m.z * m.z));
-    crec[3 * (size_t)s + 0] = make_float4(c.x, c.y, c.z, fmaxf(I.x, fmaxf(I.y, I.z)));
-    crec[3 * (size_t)s + 1] = make_float4(m.x, m.y, m.z, ml);
-    crec[3 * (size_t)s + 2] = make_float4(I.x, I.y, I.z, (float)size[s]);
+    float s2 = 0.f;
+    if (s >= n) {
+        const arc_bvh_node& b = nodes[s - n];
+        const float ex = b.bbMax.x - b.bbMin.x, ey = b.bbMax.y - b.bbMin.y, ez = b.bbMax.z - b.bbMin.z;
+        s2 = 0.25f * (ex * ex + ey * ey + ez * ez);
+    }
+    const float spread = 1.2f * (1.f - ml) / sqrtf(fmaxf((float)size[s], 1.f));
+    float4* o = crec + kCrec * (size_t)s;
+    o[0] = make_float4(c.x, c.y, c.z, fmaxf(I.x, fmaxf(I.y, I.z)));
+    o[1] = make_float4(m.x, m.y, m.z, 0.25f * (1.f - ml));
+    o[2] = make_float4(I.x, I.y, I.z, spread);
+    o[3] = make_float4(s2, sqrtf(s2), 0.f, 0.f);
 }
 
 // G~(p, node) (cudaGather.cu:45-57 with the node's moments); for a leaf the
@@ -99,9 +114,10 @@ __global__ __launch_bounds__(kCutThreads) void k_pack_cut(int n, const f3* __res
 __device__ inline float geom_est(const PixelF& P, const float4& r0, const float4& r1) {
     const float dx = r0.x - P.px, dy = r0.y - P.py, dz = r0.z - P.pz;
     const float dd = dx * dx + dy * dy + dz * dz;
-    const float inv = dd > 0.f ? rsq(dd) : 0.f;  // coincident points: no contribution (not 0 * inf)
+    // coincident points (closer than 1e-15): no contribution, not 0 * inf
+    const float inv = dd > 1e-30f ? __builtin_amdgcn_rsqf(dd) : 0.f;
     const float c1 = fminf(fmaxf((P.nx * dx + P.ny * dy + P.nz * dz) * inv, 0.f), 1.f);
-    const float c2 = fminf(0.25f * (1.f - r1.w) + fmaxf(-(r1.x * dx + r1.y * dy + r1.z * dz) * inv, 0.f), 1.f);
+    const float c2 = fminf(r1.w + fmaxf(-(r1.x * dx + r1.y * dy + r1.z * dz) * inv, 0.f), 1.f);
     return c1 * c2 * (inv * inv);
 }
 
@@ -114,7 +130,7 @@ __device__ inline float node_bound(const PixelF& P, const arc_bvh_node& b, float
                        P.ny * (P.ny > 0.f ? b.bbMax.y - P.py : b.bbMin.y - P.py) +
                        P.nz * (P.nz > 0.f ? b.bbMax.z - P.pz : b.bbMin.z - P.pz);
     // 0 when the node lies behind the pixel's tangent plane
-    return hmax > 0.f ? ilum * fminf(1.f, hmax * rsq(dmin2)) / dmin2 : 0.f;
+    return hmax > 0.f ? ilum * fminf(1.f, hmax * __builtin_amdgcn_rsqf(dmin2)) * __builtin_amdgcn_rcpf(dmin2) : 0.f;
 }
 
 // Estimated error of keeping the node instead of its AVPLs (criterion 1): the
@@ -124,22 +140,20 @@ __device__ inline float node_bound(const PixelF& P, const arc_bvh_node& b, float
 // 1.2 (1 - |m|) / sqrt(count) for the spread of the clamped cosines of
 // differing normals (their mean over `count` normals has that relative
 // spread).  A pixel within 2 s of the centroid always refines.
-__device__ inline float node_error(const PixelF& P, const arc_bvh_node& b, const float4& r0, const float4& r1,
-                                   const float4& r2, float delta2) {
-    const float ex = b.bbMax.x - b.bbMin.x, ey = b.bbMax.y - b.bbMin.y, ez = b.bbMax.z - b.bbMin.z;
-    const float s2 = 0.25f * (ex * ex + ey * ey + ez * ez);
+__device__ inline float node_error(const PixelF& P, const arc_bvh_node& b, const float4& r0, const float4& r2,
+                                   const float4& r3, float delta) {
     const float dx = r0.x - P.px, dy = r0.y - P.py, dz = r0.z - P.pz;
     const float dc2 = dx * dx + dy * dy + dz * dz;
-    if (dc2 <= 4.f * s2) return INFINITY;
+    if (dc2 <= 4.f * r3.x) return INFINITY;
     const float hmax = P.nx * (P.nx > 0.f ? b.bbMax.x - P.px : b.bbMin.x - P.px) +
                        P.ny * (P.ny > 0.f ? b.bbMax.y - P.py : b.bbMin.y - P.py) +
                        P.nz * (P.nz > 0.f ? b.bbMax.z - P.pz : b.bbMin.z - P.pz);
     if (!(hmax > 0.f)) return 0.f;  // behind the pixel's tangent plane: exactly 0
-    const float dc = sqrtf(dc2), sn = sqrtf(s2);
-    const float near = fmaxf(dc - sn, sqrtf(delta2));
-    const float inv_near = 1.0f / near;
+    // a heuristic: hardware square root and reciprocal (about 1 ulp) suffice
+    const float near = fmaxf(__builtin_amdgcn_sqrtf(dc2) - r3.y, delta);
+    const float inv_near = __builtin_amdgcn_rcpf(near);
     const float cb = fminf(1.f, hmax * inv_near);
-    const float rel = s2 / dc2 + 1.2f * (1.f - r1.w) * rsq(fmaxf(r2.w, 1.f));
+    const float rel = r3.x * __builtin_amdgcn_rcpf(dc2) + r2.w;
     return r0.w * cb * (inv_near * inv_near) * rel;
 }
 
@@ -163,8 +177,10 @@ struct Acc {
 };
 
 // Pixel of lane `lane` of wave `wave` in a 16x16 block: four 8x8 tiles.
-__device__ inline int2 tile_pixel(int wave, int lane) {
-    return make_int2(blockIdx.x * 16 + (wave & 1) * 8 + (lane & 7), blockIdx.y * 16 + (wave >> 1) * 8 + (lane >> 3));
+// The launch is a 1D grid of gx x gy such blocks, taken in XCD-contiguous order.
+__device__ inline int2 tile_pixel(int wave, int lane, int gx, int gy) {
+    const int b = xcd_block(blockIdx.x, gx * gy);
+    return make_int2((b % gx) * 16 + (wave & 1) * 8 + (lane & 7), (b / gx) * 16 + (wave >> 1) * 8 + (lane >> 3));
 }
 
 struct Threshold {
@@ -193,7 +209,7 @@ __global__ __launch_bounds__(kCutThreads) void k_cut_estimate(CutEB t, const flo
     __shared__ int stk[kCutThreads / 64][kEstStack];
     __shared__ float red[kCutThreads / 64];
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
-    const int2 xy = tile_pixel(wave, lane);
+    const int2 xy = tile_pixel(wave, lane, (W + 15) / 16, (H + 15) / 16);
     const bool live = xy.x < W && xy.y < H;
     const int q = live ? xy.y * W + xy.x : 0;
     const PixelF P = load_pixel(gpos, gnrm, galb, mats, q);
@@ -206,7 +222,7 @@ __global__ __launch_bounds__(kCutThreads) void k_cut_estimate(CutEB t, const flo
         const int v = uniform(S[--sp]);
         const int node = v >> 6, dep = v & 63;
         if (node >= 0 || dep >= d0) {
-            const float4* r = t.crec + 3 * (size_t)(node >= 0 ? node : n + (-(node + 1)));
+            const float4* r = t.crec + kCrec * (size_t)(node >= 0 ? node : n + (-(node + 1)));
             e += r[0].w * geom_est(P, r[0], r[1]);
             continue;
         }
@@ -228,12 +244,12 @@ __global__ __launch_bounds__(kCutThreads) void k_cut_eb_px(CutEB t, const float4
                                                            const float4* __restrict__ gnrm,
                                                            const float4* __restrict__ galb,
                                                            const arc_material* __restrict__ mats, int W, int H,
-                                                           Threshold th, float delta2, int budget, int criterion,
+                                                           Threshold th, float delta, int budget, int criterion,
                                                            float4* __restrict__ out_rad,
                                                            unsigned long long* __restrict__ counters) {
     __shared__ int stk[kPxStack * kCutThreads];
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
-    const int2 xy = tile_pixel(wave, lane);
+    const int2 xy = tile_pixel(wave, lane, (W + 15) / 16, (H + 15) / 16);
     const bool live = xy.x < W && xy.y < H;
     unsigned long long nvis = 0, nev = 0, nerr = 0, nbud = 0;
     if (live) {
@@ -253,9 +269,9 @@ __global__ __launch_bounds__(kCutThreads) void k_cut_eb_px(CutEB t, const float4
                 const int k = -(node + 1);
                 const arc_bvh_node& b = t.nodes[k];
                 slot = n + k;
-                const float4* r = t.crec + 3 * (size_t)slot;
-                const float crit = criterion ? node_error(P, b, r[0], r[1], r[2], delta2)
-                                             : node_bound(P, b, r[0].w, delta2);
+                const float4* r = t.crec + kCrec * (size_t)slot;
+                const float crit = criterion ? node_error(P, b, r[0], r[2], r[3], delta)
+                                             : node_bound(P, b, r[0].w, delta * delta);
                 if (crit > thr) {
                     if ((int)nev >= budget) {
                         ++nbud;  // cut budget spent: keep the cluster
@@ -269,7 +285,7 @@ __global__ __launch_bounds__(kCutThreads) void k_cut_eb_px(CutEB t, const float4
                     }
                 }
             }
-            const float4* r = t.crec + 3 * (size_t)slot;
+            const float4* r = t.crec + kCrec * (size_t)slot;
             acc.add(r[2], geom_est(P, r[0], r[1]));
             ++nev;
             if (sp == 0) break;
@@ -297,22 +313,27 @@ __global__ __launch_bounds__(kCutThreads) void k_cut_eb_px(CutEB t, const float4
 }
 
 // ------------------------------------------------------------- per tile --
+// One depth-first cut per 8x8 tile (one wave): a node is refined when any of
+// the tile's pixels asks for it (ballot).  The refined node's left child is
+// taken next without a round trip through the stack; its right child waits
+// there.  Node indices, stack and records are wave-uniform (scalar loads).
 __global__ __launch_bounds__(kCutThreads) void k_cut_eb_tile(CutEB t, const float4* __restrict__ gpos,
                                                              const float4* __restrict__ gnrm,
                                                              const float4* __restrict__ galb,
                                                              const arc_material* __restrict__ mats, int W, int H,
-                                                             Threshold th, float delta2, int budget, int criterion,
+                                                             Threshold th, float delta, int budget, int criterion,
                                                              float4* __restrict__ out_rad,
                                                              unsigned long long* __restrict__ counters) {
     __shared__ int stk[kCutThreads / 64][kTileStack];
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
-    const int2 xy = tile_pixel(wave, lane);
+    const int2 xy = tile_pixel(wave, lane, (W + 15) / 16, (H + 15) / 16);
     const bool live = xy.x < W && xy.y < H;
     const int q = live ? xy.y * W + xy.x : 0;
     const PixelF P = load_pixel(gpos, gnrm, galb, mats, q);
     int* S = stk[wave];
     const int n = t.n;
     const float thr = live ? th.at(q) : INFINITY;
+    const float delta2 = delta * delta;
     Acc acc;
     unsigned long long nvis = 0, nev = 0, nerr = 0, nbud = 0;
     int sp = 0;
@@ -323,31 +344,31 @@ __global__ __launch_bounds__(kCutThreads) void k_cut_eb_tile(CutEB t, const floa
             const int k = -(node + 1);
             const arc_bvh_node& b = t.nodes[k];
             slot = n + k;
-            const float4* r = t.crec + 3 * (size_t)slot;
-            const float crit = criterion ? node_error(P, b, r[0], r[1], r[2], delta2)
+            const float4* r = t.crec + kCrec * (size_t)slot;
+            const float crit = criterion ? node_error(P, b, r[0], r[2], r[3], delta)
                                          : node_bound(P, b, r[0].w, delta2);
             if (__ballot(crit > thr)) {
                 if ((int)nev >= budget) {
-                    ++nbud;
+                    ++nbud;  // cut budget spent: keep the cluster
                 } else if (sp < kTileStack) {
                     S[sp++] = b.right;
                     node = uniform(b.left);
                     ++nvis;
                     continue;
                 } else {
-                    ++nerr;
+                    ++nerr;  // stack full: keep the cluster
                 }
             }
         }
-        const float4* r = t.crec + 3 * (size_t)slot;
+        const float4* r = t.crec + kCrec * (size_t)slot;
         acc.add(r[2], geom_est(P, r[0], r[1]));
         ++nev;
         if (sp == 0) break;
         node = uniform(S[--sp]);
     }
     acc.flush();
+    const int alive = __popcll(__ballot(live));  // all lanes take part in the ballot
     if (counters && lane == 0) {
-        const int alive = __popcll(__ballot(live));
         atomicAdd(&counters[C_VISITED], nvis * alive);
         atomicAdd(&counters[C_EVALUATED], nev * alive);
         atomicAdd(&counters[C_ERRORS], nerr);
@@ -368,27 +389,27 @@ int cut_error_bound(arc_gather* g, arc_bvh* b, const CutEBParams& cp, float4* ra
     const int npix = g->width * g->height;
     if (n > (1 << 25)) return fail(ctx, ARC_E_UNSUPPORTED, "error-bound cut: more than 2^25 AVPLs");
     if (!b->has_moments) return fail(ctx, ARC_E_STATE, "error-bound cut: LBVH built without node moments");
-    ARC_TRY(ensure(ctx, g->crec, sizeof(float4) * 3 * (size_t)(2 * n - 1)));
+    ARC_TRY(ensure(ctx, g->crec, sizeof(float4) * kCrec * (size_t)(2 * n - 1)));
     ARC_TRY(ensure(ctx, g->cest, sizeof(float) * ((size_t)npix + 64)));
     float* est = g->cest.as<float>();
     float* sum = est + ((size_t)npix + 63) / 64 * 64;
     hipLaunchKernelGGL(k_pack_cut, dim3((2 * n - 1 + kCutThreads - 1) / kCutThreads), dim3(kCutThreads), 0,
                        ctx->stream, n, b->nd_pos.as<f3>(), b->nd_nrm.as<f3>(), b->nd_radb.as<f3>(),
-                       b->nd_mom.as<float4>(), b->nd_nmom.as<f3>(), b->nd_size.as<int>(), g->crec.as<float4>());
+                       b->nd_mom.as<float4>(), b->nd_nmom.as<f3>(), b->nd_size.as<int>(), b->nodes.as<arc_bvh_node>(),
+                       g->crec.as<float4>());
     CutEB t{b->nodes.as<arc_bvh_node>(), g->crec.as<float4>(), n};
-    const dim3 grid((g->width + 15) / 16, (g->height + 15) / 16);
+    const dim3 grid(((g->width + 15) / 16) * ((g->height + 15) / 16));
     hipLaunchKernelGGL(k_cut_zero, dim3(1), dim3(64), 0, ctx->stream, sum);
     hipLaunchKernelGGL(k_cut_estimate, grid, dim3(kCutThreads), 0, ctx->stream, t, g->gpos, g->gnrm, g->galb,
                        g->mats.as<arc_material>(), g->width, g->height, cp.d0, est, sum);
     const Threshold th{est, sum, cp.eps, cp.image_floor, 1.0f / (float)npix};
-    const float delta2 = cp.delta * cp.delta;
     if (cp.per_tile)
         hipLaunchKernelGGL(k_cut_eb_tile, grid, dim3(kCutThreads), 0, ctx->stream, t, g->gpos, g->gnrm, g->galb,
-                           g->mats.as<arc_material>(), g->width, g->height, th, delta2, cp.budget, cp.criterion,
+                           g->mats.as<arc_material>(), g->width, g->height, th, cp.delta, cp.budget, cp.criterion,
                            rad, counters);
     else
         hipLaunchKernelGGL(k_cut_eb_px, grid, dim3(kCutThreads), 0, ctx->stream, t, g->gpos, g->gnrm, g->galb,
-                           g->mats.as<arc_material>(), g->width, g->height, th, delta2, cp.budget, cp.criterion,
+                           g->mats.as<arc_material>(), g->width, g->height, th, cp.delta, cp.budget, cp.criterion,
                            rad, counters);
     ARC_LAUNCHED(ctx);
     return ARC_OK;
