@@ -35,6 +35,7 @@ HIP_SOURCES = {
     "gather.hip": [],
     "gather_anti.hip": [],
     "gather_cut.hip": [],
+    "output.hip": [],
 }
 
 HIPCC_FLAGS = [

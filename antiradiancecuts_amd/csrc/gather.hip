@@ -483,7 +483,7 @@ int arc_gather_create(arc_ctx* ctx, int width, int height, const arc_material* m
     }
     if (st == ARC_OK) st = arc::ensure(ctx, g->counters, sizeof(unsigned long long) * C_NUM);
     for (int f = 0; st == ARC_OK && f < arc_gather::kRing; ++f)
-        for (int i = 0; st == ARC_OK && i < 6; ++i) {
+        for (int i = 0; st == ARC_OK && i < arc_gather::kMarks; ++i) {
             hipError_t e = hipEventCreate(&g->ev[f][i]);
             if (e != hipSuccess) st = arc::hip_fail(ctx, e, "hipEventCreate", __FILE__, __LINE__);
         }
@@ -499,7 +499,7 @@ int arc_gather_create(arc_ctx* ctx, int width, int height, const arc_material* m
 void arc_gather_destroy(arc_gather* g) {
     if (!g) return;
     for (int f = 0; f < arc_gather::kRing; ++f)
-        for (int i = 0; i < 6; ++i)
+        for (int i = 0; i < arc_gather::kMarks; ++i)
             if (g->ev[f][i]) (void)hipEventDestroy(g->ev[f][i]);
     arc::release(g->mats);
     arc::release(g->int_rad);
@@ -596,8 +596,9 @@ int arc_gather_run(arc_gather* g, const arc_avpls* avpls, arc_float3 camPos, flo
         if (p->cutMode == ARC_CUT_ERROR_BOUND) {
             const arc::CutEBParams cp{p->errorBound, p->cutEstimateDepth, 1e-3f * sceneExtent, p->cutBudget,
                                           p->cutPerTile, p->cutCriterion, p->cutImageFloor};
-            ARC_TRY(arc::cut_error_bound(g, b, cp, rad, counters));
+            ARC_TRY(arc::cut_error_bound(g, b, cp, rad, counters, profile ? ev[6] : nullptr));
         } else {
+            if (profile) ARC_HIP(ctx, hipEventRecord(ev[6], ctx->stream));
             CutTree t{b->nodes.as<arc_bvh_node>(), b->nd_mat.as<int>(), b->nd_pos.as<f3>(), b->nd_nrm.as<f3>(),
                       b->nd_rad.as<f3>(), b->nd_dir.as<f3>(), b->n};
             dim3 grid((g->width + 15) / 16, (g->height + 15) / 16);
@@ -610,6 +611,7 @@ int arc_gather_run(arc_gather* g, const arc_avpls* avpls, arc_float3 camPos, flo
         if (profile) ARC_HIP(ctx, hipEventRecord(ev[3], ctx->stream));
     } else {
         if (profile) ARC_HIP(ctx, hipEventRecord(ev[2], ctx->stream));
+        if (profile) ARC_HIP(ctx, hipEventRecord(ev[6], ctx->stream));
         rad_splits = n > 0 ? splits : 0;
         if (n > 0 && !clustered) {  // one pass sums both (kernel_radiance_simple + kernel_antiradiance)
             dispatch_exact<true, true>(g, p->antiradianceModel, a, chunk, splits, cam, radius, cos_half);
@@ -623,12 +625,16 @@ int arc_gather_run(arc_gather* g, const arc_avpls* avpls, arc_float3 camPos, flo
     // ---- antiradiance: clustered lists (kernel_antiradiance_clusterred) or exact
     if (clustered) {
         ARC_TRY(arc::anti_clustered(g, a, p->antiradianceModel, cam, anti_params(radius, cos_half),
-                                    (double)p->coneHalfAngle, g->part.as<float>(), counters));
+                                    (double)p->coneHalfAngle, g->part.as<float>(), counters,
+                                    profile ? ev[7] : nullptr));
         anti_splits = 1;
     } else if (cut && n > 0) {
+        if (profile) ARC_HIP(ctx, hipEventRecord(ev[7], ctx->stream));
         dispatch_exact<false, true>(g, p->antiradianceModel, a, chunk, splits, cam, radius, cos_half);
         anti_splits = splits;
         ARC_LAUNCHED(ctx);
+    } else if (profile) {
+        ARC_HIP(ctx, hipEventRecord(ev[7], ctx->stream));  // summed with the radiance: no pass of its own
     }
     if (profile) ARC_HIP(ctx, hipEventRecord(ev[4], ctx->stream));
     hipLaunchKernelGGL(k_finalize, dim3(blocks(npix)), dim3(kThreads), 0, ctx->stream, g->part.as<float>(),
@@ -677,6 +683,20 @@ int arc_gather_profile_history(arc_gather* g, int frames, float* out) {
         ARC_HIP(ctx, hipEventSynchronize(ev[5]));
         for (int i = 0; i < 5; ++i) ARC_HIP(ctx, hipEventElapsedTime(&out[6 * k + i], ev[i], ev[i + 1]));
         ARC_HIP(ctx, hipEventElapsedTime(&out[6 * k + 5], ev[0], ev[5]));
+    }
+    return n;
+}
+
+int arc_gather_kernel_history(arc_gather* g, int frames, float* out) {
+    if (!g || frames < 0 || (frames > 0 && !out)) return ARC_E_INVALID;
+    arc_ctx* ctx = g->ctx;
+    const int n = std::min(frames, g->ring_count);
+    for (int k = 0; k < n; ++k) {
+        const int slot = (g->ring_next - n + k + arc_gather::kRing) % arc_gather::kRing;
+        hipEvent_t* ev = g->ev[slot];
+        ARC_HIP(ctx, hipEventSynchronize(ev[5]));
+        ARC_HIP(ctx, hipEventElapsedTime(&out[2 * k + 0], ev[6], ev[3]));
+        ARC_HIP(ctx, hipEventElapsedTime(&out[2 * k + 1], ev[7], ev[4]));
     }
     return n;
 }

@@ -397,7 +397,7 @@ void launch_pixels(arc_gather* g, const arc::AvplsDev& a, int cw, int C, float3 
 namespace arc {
 
 int anti_clustered(arc_gather* g, const AvplsDev& a, int model, float3 cam, const AntiParams& ap,
-                   double half_angle, float* part, unsigned long long* counters) {
+                   double half_angle, float* part, unsigned long long* counters, hipEvent_t mark) {
     arc_ctx* ctx = g->ctx;
     const int n = a.n;
     const int nb = (n + kCompact - 1) / kCompact;
@@ -446,6 +446,7 @@ int anti_clustered(arc_gather* g, const AvplsDev& a, int model, float3 cam, cons
         hipLaunchKernelGGL(k_cluster_lists_tree<ARC_ANTIRAD_CONE>, lgrid, dim3(256), 0, ctx->stream, a, idx, n_anti,
                            g->cbox.as<float>(), tree, grow, cos_t, sin_t, g->ccount.as<int>(), g->clist.as<int>());
     ARC_LAUNCHED(ctx);
+    if (mark) ARC_HIP(ctx, hipEventRecord(mark, ctx->stream));
     if (model == ARC_ANTIRAD_DISK) {
         if (g->general) launch_pixels<ARC_ANTIRAD_DISK, true>(g, a, cw, C, cam, ap, n_anti, grow, cos_t, sin_t, part, counters);
         else launch_pixels<ARC_ANTIRAD_DISK, false>(g, a, cw, C, cam, ap, n_anti, grow, cos_t, sin_t, part, counters);

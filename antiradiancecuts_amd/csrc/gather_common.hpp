@@ -31,10 +31,12 @@ struct arc_gather {
     arc::DevBuf crec;  // error-bound cut: packed node records
     arc::DevBuf cest;  // error-bound cut: per-pixel estimates and their sum
     arc_bvh* bvh = nullptr;
-    // per-frame phase events: frame slot, then 6 marks (start, upload done,
-    // bvh done, radiance done, antiradiance done, end)
+    // per-frame phase events: frame slot, then 8 marks (start, upload done,
+    // bvh done, radiance done, antiradiance done, end, start of the main
+    // radiance kernel, start of the main antiradiance kernel)
     static constexpr int kRing = 64;
-    hipEvent_t ev[kRing][6] = {};
+    static constexpr int kMarks = 8;
+    hipEvent_t ev[kRing][kMarks] = {};
     int ring_next = 0, ring_count = 0;
 };
 
@@ -223,10 +225,13 @@ struct CutEBParams {
     int criterion;      // cutCriterion
     float image_floor;  // cutImageFloor
 };
-int cut_error_bound(arc_gather* g, arc_bvh* b, const CutEBParams& cp, float4* rad, unsigned long long* counters);
+// `mark` (may be null) is recorded right before the main traversal kernel.
+int cut_error_bound(arc_gather* g, arc_bvh* b, const CutEBParams& cp, float4* rad, unsigned long long* counters,
+                    hipEvent_t mark);
 
 // Clustered antiradiance (gather_anti.hip).
+// `mark` (may be null) is recorded right before the per-pixel kernel.
 int anti_clustered(arc_gather* g, const AvplsDev& a, int model, float3 cam, const gd::AntiParams& ap,
-                   double half_angle, float* part, unsigned long long* counters);
+                   double half_angle, float* part, unsigned long long* counters, hipEvent_t mark);
 
 }  // namespace arc

@@ -383,7 +383,8 @@ __global__ __launch_bounds__(kCutThreads) void k_cut_eb_tile(CutEB t, const floa
 
 namespace arc {
 
-int cut_error_bound(arc_gather* g, arc_bvh* b, const CutEBParams& cp, float4* rad, unsigned long long* counters) {
+int cut_error_bound(arc_gather* g, arc_bvh* b, const CutEBParams& cp, float4* rad, unsigned long long* counters,
+                    hipEvent_t mark) {
     arc_ctx* ctx = g->ctx;
     const int n = b->n;
     const int npix = g->width * g->height;
@@ -403,6 +404,7 @@ int cut_error_bound(arc_gather* g, arc_bvh* b, const CutEBParams& cp, float4* ra
     hipLaunchKernelGGL(k_cut_estimate, grid, dim3(kCutThreads), 0, ctx->stream, t, g->gpos, g->gnrm, g->galb,
                        g->mats.as<arc_material>(), g->width, g->height, cp.d0, est, sum);
     const Threshold th{est, sum, cp.eps, cp.image_floor, 1.0f / (float)npix};
+    if (mark) ARC_HIP(ctx, hipEventRecord(mark, ctx->stream));
     if (cp.per_tile)
         hipLaunchKernelGGL(k_cut_eb_tile, grid, dim3(kCutThreads), 0, ctx->stream, t, g->gpos, g->gnrm, g->galb,
                            g->mats.as<arc_material>(), g->width, g->height, th, cp.delta, cp.budget, cp.criterion,

@@ -102,6 +102,11 @@ class arc_scene_desc(C.Structure):
 
 P = C.POINTER
 # (name, restype, argtypes) for every symbol of include/arc.h
+class arc_image_error_t(C.Structure):
+    _fields_ = [("referenceError", C.c_float), ("rmse", C.c_float * 3), ("relL2", C.c_double),
+                ("maxAbs", C.c_double)]
+
+
 ARC_SYMBOLS = [
     ("arc_version", C.c_char_p, []),
     ("arc_status_string", C.c_char_p, [C.c_int]),
@@ -136,7 +141,13 @@ ARC_SYMBOLS = [
     ("arc_gather_exact_f64", C.c_int, [vp, P(arc_avpls), arc_float3, C.c_float, P(arc_gather_params),
                                        C.c_int, vp, vp]),
     ("arc_gather_profile_history", C.c_int, [vp, C.c_int, vp]),
+    ("arc_gather_kernel_history", C.c_int, [vp, C.c_int, vp]),
     ("arc_gather_get_bvh", vp, [vp]),
+    ("arc_image_accumulate", C.c_int, [vp, vp, vp, C.c_int]),
+    ("arc_image_normalize", C.c_int, [vp, vp, vp, C.c_int, C.c_int]),
+    ("arc_image_error", C.c_int, [vp, vp, vp, C.c_int, P(arc_image_error_t)]),
+    ("arc_write_pfm", C.c_int, [C.c_char_p, vp, C.c_int, C.c_int]),
+    ("arc_read_pfm", C.c_int, [C.c_char_p, vp, C.c_int, P(C.c_int), P(C.c_int)]),
 ]
 
 SCENE_SYMBOLS = [

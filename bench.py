@@ -34,13 +34,12 @@ PEAK_HBM_GBS = 8000.0
 # Algorithmic FLOP per unit (FMA = 2, transcendental = 1):
 #   radiance pair 31; disk antiradiance pair 37; cone antiradiance pair 33;
 #   solid-angle cut: visited inner node 32, evaluated node 31 + 31;
-#   error-bound cut: node test (criterion) 45, node estimate 37, and the
-#   depth-d0 estimate cut 32 per node and pixel;
+#   error-bound cut: node test (criterion) 45, node estimate 37;
 #   clustered antiradiance: tile-filter ray-box test 35 per list entry and
 #   tile, plus the pair tests.
 FLOP_RAD_PAIR, FLOP_DISK_PAIR, FLOP_CONE_PAIR = 31, 37, 33
 FLOP_VISIT, FLOP_EVAL = 32, 31
-FLOP_EB_TEST, FLOP_EB_EST, FLOP_EB_PHASE1 = 45, 37, 32
+FLOP_EB_TEST, FLOP_EB_EST = 45, 37
 FLOP_TILE_FILTER = 35
 # AvplBvh rebuild, algorithmic HBM bytes per leaf: codes, perm, sorted
 # position / normal, nodes, parents, ranges, node data and random numbers
@@ -109,6 +108,12 @@ def main() -> int:
     hist = hist[:max(got, 0)]
     phase_ms = {k: float(hist[:, i].mean()) for i, k in enumerate(PHASES)} if len(hist) else {}
     frame_ms = hist[:, 5] if len(hist) else np.array([ms_per_step])
+    # the two main kernels of each timed frame, timed with HIP events on the
+    # library's stream right around their launches
+    kern = np.zeros((args.steps, 2), np.float32)
+    got = g.lib.arc_gather_kernel_history(g.h, args.steps, kern.ctypes.data)
+    kern = kern[:max(got, 0)]
+    kernel_ms = {"radiance": float(kern[:, 0].mean()), "antiradiance": float(kern[:, 1].mean())} if len(kern) else {}
 
     # untimed: work counters and the error against the exact FP64 gather
     st = g.run(av, sc.cam, sc.extent, params, ffi.ARC_RUN_COUNTERS | ffi.ARC_TRUST_INPUT)
@@ -136,16 +141,18 @@ def main() -> int:
                "pixels": int(len(pix)), "sample": f"every {k}th pixel in x and y", "f64_seconds": round(t_f64, 2)}
         gs.close()
 
-    # roofline of the dominant phase (timed on the library's stream with HIP
-    # events; its kernels are listed by the rocprofv3 summaries in profiles/)
+    # roofline of the dominant kernel: its algorithmic FLOPs (from the
+    # kernel's own counters) over its HIP-event duration; the LBVH rebuild,
+    # many small HBM-bound kernels, is taken as one unit
     npix, n = sc.npix, sc.num_avpls
-    dom = max(("bvh", "radiance", "antiradiance"), key=lambda k: phase_ms.get(k, 0.0))
+    cand = {"radiance": kernel_ms.get("radiance", 0.0), "antiradiance": kernel_ms.get("antiradiance", 0.0),
+            "bvh": phase_ms.get("bvh", 0.0)}
+    dom = max(cand, key=cand.get)
     if dom == "antiradiance":
         flop = float(st.antiEvaluatedNodes) * FLOP_DISK_PAIR + float(st.antiVisitedNodes) * FLOP_TILE_FILTER
         roof = {"kernel": "k_anti_clustered (clustered disk antiradiance)", "bound": "valu"}
     elif dom == "radiance" and eb:
-        flop = (float(st.visitedNodes + st.evaluatedNodes) * FLOP_EB_TEST + float(st.evaluatedNodes) * FLOP_EB_EST
-                + float(npix) * (2 ** params.cutEstimateDepth) * FLOP_EB_PHASE1)
+        flop = float(st.visitedNodes + st.evaluatedNodes) * FLOP_EB_TEST + float(st.evaluatedNodes) * FLOP_EB_EST
         roof = {"kernel": "k_cut_eb_tile (error-bound light cut)" if args.per_tile else
                 "k_cut_eb_px (error-bound light cut)", "bound": "valu"}
     elif dom == "radiance":
@@ -154,18 +161,16 @@ def main() -> int:
     else:
         flop = None
         roof = {"kernel": "AvplBvh rebuild (sort, Karras, refit)", "bound": "hbm"}
-    t_dom = phase_ms.get(dom, float("nan"))
+    t_dom = cand[dom]
     if flop is not None:
         ach = flop / (t_dom * 1e-3) / 1e12
         roof.update({"achieved": round(ach, 3), "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
-                     "frac": round(ach / PEAK_FP32_TFLOPS, 4), "traffic": None, "ms": round(t_dom, 4),
-                     "phase": dom})
+                     "frac": round(ach / PEAK_FP32_TFLOPS, 4), "traffic": None, "ms": round(t_dom, 4)})
     else:
         bytes_ = float(BYTES_BVH_LEAF + (BYTES_MOMENTS_LEAF if eb else 0)) * n
         ach = bytes_ / (t_dom * 1e-3) / 1e9
         roof.update({"achieved": round(ach, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                     "frac": round(ach / PEAK_HBM_GBS, 4), "traffic": None, "ms": round(t_dom, 4),
-                     "phase": dom})
+                     "frac": round(ach / PEAK_HBM_GBS, 4), "traffic": None, "ms": round(t_dom, 4)})
 
     cpu = cpu_baseline(sc, params, args.cpu_sample) if args.cpu_sample > 0 else None
 
@@ -195,6 +200,7 @@ def main() -> int:
                         + "); disk antiradiance over per-cluster AVPL lists (exact)",
             "width": sc.width, "height": sc.height, "avpls": n, "parallelism": "single GPU",
             "phase_ms": {k: round(v, 4) for k, v in phase_ms.items()},
+            "kernel_ms": {k: round(v, 4) for k, v in kernel_ms.items()},
             "cut": {"mode": args.cut, "error_bound": args.error_bound if eb else None,
                     "visited_nodes": int(st.visitedNodes), "evaluated_nodes": int(st.evaluatedNodes),
                     "stack_overflows": int(st.errorPixels), "budget_hits": int(st.budgetHits)},

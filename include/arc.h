@@ -274,8 +274,44 @@ int arc_gather_exact_f64(arc_gather* g, const arc_avpls* avpls, arc_float3 camPo
  * Replaces the CudaTimer prints of run() (cudaGather.cu:845-1049). */
 int arc_gather_profile_history(arc_gather* g, int frames, float* out);
 
+/* Durations (ms) of each recorded frame's main kernels, oldest first, 2
+ * floats per frame: the radiance traversal / exact kernel and the
+ * antiradiance per-pixel kernel (0 when the exact pass summed both), each
+ * timed with HIP events on the context's stream right around its launch.
+ * Same frames and return value as arc_gather_profile_history. */
+int arc_gather_kernel_history(arc_gather* g, int frames, float* out);
+
 /* The AvplBvh of the last cut frame (borrowed; NULL before the first). */
 arc_bvh* arc_gather_get_bvh(arc_gather* g);
+
+/* ------------------------------------------------------------ output side */
+/* Renderer::Add(target, source) (Render.cpp:788-806, Shaders/Add.frag):
+ * accum.rgb += frame.rgb, accum.w += 1 (float4 images of npix pixels, device
+ * memory).  The reference adds the result texture of each gathered frame. */
+int arc_image_accumulate(arc_ctx* ctx, float* accum4, const float* frame4, int npix);
+
+/* Renderer::Normalize (Render.cpp:642-667, Shaders/Normalize.frag):
+ * out.rgb = accum.rgb * (1.f / num_paths) (0 when num_paths == 0), out.w = 1. */
+int arc_image_normalize(arc_ctx* ctx, float* out4, const float* accum4, int npix, int num_paths);
+
+typedef struct arc_image_error_t {
+    float referenceError;  /* CReferenceImage::GetError (CReferenceImage.cpp:28-51):
+                              (e.x * e.y + e.z) / 3 with e the per-channel root mean of
+                              min(100, d^2), the reference's formula as written        */
+    float rmse[3];         /* that e per channel                                        */
+    double relL2;          /* |img - ref| / |ref| over RGB                               */
+    double maxAbs;         /* max |img - ref| over RGB                                   */
+} arc_image_error_t;
+
+/* Error of img4 against ref4 (device float4 images).  Synchronous. */
+int arc_image_error(arc_ctx* ctx, const float* ref4, const float* img4, int npix, arc_image_error_t* out);
+
+/* CExport::ExportPFM (Utils/CExport.cpp:22-65): "PF\n<w> <h>\n-1.0\n" and the RGB
+ * floats of a host float4 image in memory row order.  Host only. */
+int arc_write_pfm(const char* path, const float* rgba_host, int width, int height);
+/* Reads such a file into a host float4 image (w = 1).  With rgba_host NULL
+ * only the size is returned. */
+int arc_read_pfm(const char* path, float* rgba_host, int max_pixels, int* width, int* height);
 
 #ifdef __cplusplus
 }

@@ -659,3 +659,37 @@ int orc_gather_cut_solid_angle(const orc_scene* s, int precision, int n_leaves,
 }
 
 }  // extern "C"
+
+// ---- output side --------------------------------------------------------
+extern "C" {
+
+void orc_image_accumulate(float* accum4, const float* frame4, int npix) {  // Render.cpp:788-806
+    for (int i = 0; i < npix; ++i) {
+        for (int c = 0; c < 3; ++c) accum4[4 * i + c] = accum4[4 * i + c] + frame4[4 * i + c];
+        accum4[4 * i + 3] = accum4[4 * i + 3] + 1.f;
+    }
+}
+
+void orc_image_normalize(float* out4, const float* accum4, int npix, int num_paths) {  // Render.cpp:642-667
+    const float inv = num_paths != 0 ? 1.f / (float)num_paths : 0.f;
+    for (int i = 0; i < npix; ++i) {
+        for (int c = 0; c < 3; ++c) out4[4 * i + c] = accum4[4 * i + c] * inv;
+        out4[4 * i + 3] = 1.f;
+    }
+}
+
+float orc_image_error(const float* ref4, const float* img4, int npix, float* e3) {  // CReferenceImage.cpp:28-51
+    float e[3] = {0.f, 0.f, 0.f};
+    for (int i = 0; i < npix; ++i)
+        for (int c = 0; c < 3; ++c) {
+            const float d = ref4[4 * i + c] - img4[4 * i + c];
+            e[c] += std::min(100.f, d * d);
+        }
+    const float inv = 1.f / (float)npix;
+    for (int c = 0; c < 3; ++c) e[c] = std::sqrt(inv * e[c]);
+    if (e3)
+        for (int c = 0; c < 3; ++c) e3[c] = e[c];
+    return 1.f / 3.f * (e[0] * e[1] + e[2]);
+}
+
+}  // extern "C"

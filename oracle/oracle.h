@@ -136,6 +136,20 @@ int orc_gather_cut_solid_angle(const orc_scene* s, int precision, int n_leaves,
                                float ref_thresh, const int* pixels, int npix,
                                double* rad3, int* err, long long* visited, long long* evaluated);
 
+/* ---- output side (Render.cpp, CReferenceImage.cpp) ----------------------
+ * Renderer::Add (Render.cpp:788-806): accum.rgb += frame.rgb, accum.w += 1,
+ * in FP32.  Renderer::Normalize (Render.cpp:642-667): out.rgb = accum.rgb *
+ * (1.f / paths) (0 when paths == 0), out.w = 1.  Pinned by exact FP32
+ * arithmetic (numpy float32 gives the same bits). */
+void orc_image_accumulate(float* accum4, const float* frame4, int npix);
+void orc_image_normalize(float* out4, const float* accum4, int npix, int num_paths);
+/* CReferenceImage::GetError (CReferenceImage.cpp:28-51) as written: FP32
+ * per-channel sums of min(100, d^2) in pixel order, scaled by 1.f/(w*h),
+ * square roots, then (e.x * e.y + e.z) / 3.  e3 receives the per-channel
+ * values.  Pinned by closed forms (constant differences, the clamp at 100,
+ * the product-plus-sum combination). */
+float orc_image_error(const float* ref4, const float* img4, int npix, float* e3);
+
 #ifdef __cplusplus
 }
 #endif
