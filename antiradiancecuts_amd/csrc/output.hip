@@ -62,9 +62,12 @@ __global__ __launch_bounds__(kThreads) void k_error_partial(const float4* __rest
         acc[0] += (double)fminf(100.f, dx * dx);
         acc[1] += (double)fminf(100.f, dy * dy);
         acc[2] += (double)fminf(100.f, dz * dz);
-        acc[3] += (double)dx * dx + (double)dy * dy + (double)dz * dz;
+        // the reference's clamped terms use the FP32 differences; the
+        // relative L2 and max |d| use exact (FP64) ones
+        const double ex = (double)r.x - u.x, ey = (double)r.y - u.y, ez = (double)r.z - u.z;
+        acc[3] += ex * ex + ey * ey + ez * ez;
         acc[4] += (double)r.x * r.x + (double)r.y * r.y + (double)r.z * r.z;
-        acc[5] = fmax(acc[5], (double)fmaxf(fabsf(dx), fmaxf(fabsf(dy), fabsf(dz))));
+        acc[5] = fmax(acc[5], fmax(fabs(ex), fmax(fabs(ey), fabs(ez))));
     }
     for (int c = 0; c < 6; ++c) sh[c][threadIdx.x] = acc[c];
     __syncthreads();
