@@ -36,6 +36,7 @@ HIP_SOURCES = {
     "gather_anti.hip": [],
     "gather_cut.hip": [],
     "output.hip": [],
+    "trace.hip": ["-ffp-contract=off"],
 }
 
 HIPCC_FLAGS = [

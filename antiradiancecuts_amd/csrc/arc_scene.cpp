@@ -486,4 +486,63 @@ int arc_scene_normals(uint64_t seed, int n, float* nrm3) {
     return 0;
 }
 
+int arc_scene_num_triangles(const arc_scene_desc* d) {
+    if (!valid(d)) return -1;
+    int n = 0;
+    for (const Tri& t : build_scene(*d)) n += t.two_sided ? 2 : 1;
+    return n;
+}
+
+int arc_scene_triangles(const arc_scene_desc* d, float* p0, float* p1, float* p2, int* material, int max_tris) {
+    if (!valid(d) || !p0 || !p1 || !p2 || !material) return -1;
+    const std::vector<Tri> tris = build_scene(*d);
+    int k = 0;
+    auto put = [&](V3 a, V3 b, V3 c, int mat) {
+        if (k >= max_tris) return;
+        const V3 v[3] = {a, b, c};
+        float* o[3] = {p0, p1, p2};
+        for (int j = 0; j < 3; ++j) {
+            o[j][3 * k + 0] = (float)v[j].x;
+            o[j][3 * k + 1] = (float)v[j].y;
+            o[j][3 * k + 2] = (float)v[j].z;
+        }
+        material[k] = mat;
+        ++k;
+    };
+    for (const Tri& t : tris) {
+        // the front face's normal is normalize(cross(p2 - p0, p1 - p0))
+        const bool keep = dot(cross(t.v2 - t.v0, t.v1 - t.v0), t.n) > 0.0;
+        if (keep) put(t.v0, t.v1, t.v2, t.material);
+        else put(t.v0, t.v2, t.v1, t.material);
+        if (t.two_sided) {  // the back face as a triangle of its own
+            if (keep) put(t.v0, t.v2, t.v1, t.material);
+            else put(t.v0, t.v1, t.v2, t.material);
+        }
+    }
+    return k;
+}
+
+int arc_scene_light(const arc_scene_desc* d, float* center3, float* front3, float* up3, float* size2,
+                    float* radiance3, float* material10) {
+    if (!valid(d)) return -1;
+    // a Cornell-style ceiling light just below the ceiling, facing down
+    const float c[3] = {0.5f, 0.999f, 0.5f}, f[3] = {0.f, -1.f, 0.f}, u[3] = {0.f, 0.f, 1.f};
+    const float rad[3] = {17.f, 12.f, 4.f};
+    for (int k = 0; k < 3; ++k) {
+        if (center3) center3[k] = c[k];
+        if (front3) front3[k] = f[k];
+        if (up3) up3[k] = u[k];
+        if (radiance3) radiance3[k] = rad[k];
+    }
+    if (size2) {
+        size2[0] = 130.f / 556.f;
+        size2[1] = 105.f / 559.2f;
+    }
+    if (material10) {  // MATERIAL with emissive = radiance (AreaLight.h, "LightSource")
+        for (int k = 0; k < 10; ++k) material10[k] = 0.f;
+        for (int k = 0; k < 3; ++k) material10[k] = rad[k];
+    }
+    return 0;
+}
+
 }  // extern "C"

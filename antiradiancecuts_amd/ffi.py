@@ -28,6 +28,7 @@ ARC_TRUST_INPUT = 512
 ARC_CUT_SOLID_ANGLE = 0
 ARC_CUT_ERROR_BOUND = 1
 ARC_CRITERION_BOUND = 0
+ARC_SHOOT_NO_EXPLICIT_DIRECT = 1
 ARC_CRITERION_ESTIMATE = 1
 ARC_ANTIRAD_DISK = 0
 ARC_ANTIRAD_CONE = 1
@@ -107,6 +108,11 @@ class arc_image_error_t(C.Structure):
                 ("maxAbs", C.c_double)]
 
 
+class arc_area_light(C.Structure):
+    _fields_ = [("center", arc_float3), ("front", arc_float3), ("up", arc_float3), ("width", C.c_float),
+                ("height", C.c_float), ("radiance", arc_float3), ("materialIndex", C.c_int)]
+
+
 ARC_SYMBOLS = [
     ("arc_version", C.c_char_p, []),
     ("arc_status_string", C.c_char_p, [C.c_int]),
@@ -143,6 +149,12 @@ ARC_SYMBOLS = [
     ("arc_gather_profile_history", C.c_int, [vp, C.c_int, vp]),
     ("arc_gather_kernel_history", C.c_int, [vp, C.c_int, vp]),
     ("arc_gather_get_bvh", vp, [vp]),
+    ("arc_trace_scene_create", C.c_int, [vp, vp, vp, vp, vp, C.c_int, vp, C.c_int, C.c_int, P(vp)]),
+    ("arc_trace_scene_destroy", None, [vp]),
+    ("arc_trace_scene_info", C.c_int, [vp, P(C.c_int), P(C.c_int)]),
+    ("arc_trace_intersect", C.c_int, [vp, vp, vp, C.c_int, vp, vp]),
+    ("arc_trace_shoot", C.c_int, [vp, P(arc_area_light), C.c_int, C.c_uint64, C.c_int, P(arc_avpls), P(C.c_int)]),
+    ("arc_trace_gbuffer", C.c_int, [vp, arc_float3, arc_float3, arc_float3, C.c_float, C.c_int, C.c_int, vp, vp]),
     ("arc_image_accumulate", C.c_int, [vp, vp, vp, C.c_int]),
     ("arc_image_normalize", C.c_int, [vp, vp, vp, C.c_int, C.c_int]),
     ("arc_image_error", C.c_int, [vp, vp, vp, C.c_int, P(arc_image_error_t)]),
@@ -158,6 +170,9 @@ SCENE_SYMBOLS = [
     ("arc_scene_avpls", C.c_int, [P(arc_scene_desc), vp, vp, vp, vp, vp, vp, vp]),
     ("arc_scene_points", C.c_int, [C.c_uint64, C.c_int, C.c_int, C.c_float, vp]),
     ("arc_scene_normals", C.c_int, [C.c_uint64, C.c_int, vp]),
+    ("arc_scene_num_triangles", C.c_int, [P(arc_scene_desc)]),
+    ("arc_scene_triangles", C.c_int, [P(arc_scene_desc), vp, vp, vp, vp, C.c_int]),
+    ("arc_scene_light", C.c_int, [P(arc_scene_desc), vp, vp, vp, vp, vp, vp]),
 ]
 
 _lib = None

@@ -251,3 +251,77 @@ def photon_radius(params, extent: float) -> float:
 
 
 CONE_10DEG = math.pi / 18.0
+
+
+class TraceScene:
+    """arc_trace_scene: triangle BVH, AVPL shooter and G-buffer ray caster."""
+
+    def __init__(self, ctx: Context, ts):
+        self.ctx, self.lib, self.ts = ctx, ctx.lib, ts
+        mats = materials_struct(ts.materials)
+        self._mats = mats
+        h = C.c_void_p()
+        st = self.lib.arc_trace_scene_create(ctx.h, ts.p0.ctypes.data, ts.p1.ctypes.data, ts.p2.ctypes.data,
+                                             ts.material.ctypes.data, int(len(ts.material)), mats,
+                                             int(len(ts.materials)), ffi.ARC_INPUT_HOST, C.byref(h))
+        check(st, "arc_trace_scene_create", ctx.h)
+        self.h = h
+        L = ts.light
+        self.light = ffi.arc_area_light(ffi.arc_float3(*map(float, L["center"])), ffi.arc_float3(*map(float, L["front"])),
+                                        ffi.arc_float3(*map(float, L["up"])), L["width"], L["height"],
+                                        ffi.arc_float3(*map(float, L["radiance"])), int(L["material"]))
+
+    def close(self) -> None:
+        if self.h:
+            self.lib.arc_trace_scene_destroy(self.h)
+            self.h = None
+
+    def info(self):
+        n, d = C.c_int(0), C.c_int(0)
+        check(self.lib.arc_trace_scene_info(self.h, C.byref(n), C.byref(d)), "arc_trace_scene_info", self.ctx.h)
+        return n.value, d.value
+
+    def intersect(self, org, dirs):
+        torch = _torch()
+        o, d = device_array(org, self.ctx.device), device_array(dirs, self.ctx.device)
+        n = int(org.shape[0])
+        t = torch.empty(n, dtype=torch.float32, device=o.device)
+        tri = torch.empty(n, dtype=torch.int32, device=o.device)
+        check(self.lib.arc_trace_intersect(self.h, C.c_void_p(o.data_ptr()), C.c_void_p(d.data_ptr()), n,
+                                           C.c_void_p(t.data_ptr()), C.c_void_p(tri.data_ptr())),
+              "arc_trace_intersect", self.ctx.h)
+        return t, tri
+
+    def shoot(self, num_paths: int, seed: int, flags: int = 0) -> DeviceAvpls:
+        """AVPLs of num_paths paths; the arrays are borrowed from the scene
+        object (valid until the next shoot)."""
+        s = ffi.arc_avpls()
+        n = C.c_int(0)
+        check(self.lib.arc_trace_shoot(self.h, C.byref(self.light), int(num_paths), int(seed), int(flags), C.byref(s),
+                                       C.byref(n)), "arc_trace_shoot", self.ctx.h)
+        return DeviceAvpls({}, s)
+
+    def gbuffer(self, cam, forward, up, fovy: float, width: int, height: int):
+        torch = _torch()
+        dev = f"cuda:{self.ctx.device}"
+        pos4 = torch.empty((width * height, 4), dtype=torch.float32, device=dev)
+        nrm4 = torch.empty((width * height, 4), dtype=torch.float32, device=dev)
+        check(self.lib.arc_trace_gbuffer(self.h, ffi.arc_float3(*map(float, cam)), ffi.arc_float3(*map(float, forward)),
+                                         ffi.arc_float3(*map(float, up)), float(fovy), int(width), int(height),
+                                         C.c_void_p(pos4.data_ptr()), C.c_void_p(nrm4.data_ptr())),
+              "arc_trace_gbuffer", self.ctx.h)
+        return pos4, nrm4
+
+
+def avpls_to_host(avpls: DeviceAvpls, ctx: Context) -> dict:
+    """Copies an AVPL SoA (device pointers) to numpy arrays."""
+    s = avpls.struct
+    n = s.numAvpls
+    out = {}
+    for name, ptr, cols, dt in (("pos", s.position, 3, np.float32), ("nrm", s.normal, 3, np.float32),
+                                ("dir", s.incDirection, 3, np.float32), ("rad", s.incRadiance, 3, np.float32),
+                                ("anti", s.antiradiance, 3, np.float32), ("mat", s.materialIndex, 1, np.int32),
+                                ("bounce", s.bounce, 1, np.int32)):
+        a = ctx.to_host(ptr or 0, dt, n * cols)
+        out[name] = a.reshape(n, 3) if cols == 3 else a
+    return out

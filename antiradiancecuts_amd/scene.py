@@ -129,3 +129,56 @@ SMOKE_POINTS = np.array([
     [555.476, 471.466, 510.595], [438.946, 548.9, 506.097], [434.071, 501.603, 559.3],
     [452.146, 548.9, 529.52], [504.6, 509.978, 559.3]], dtype=np.float64).astype(np.float32)
 SMOKE_NORMALS = np.tile(np.array([0.0, 1.0, 0.0], np.float32), (8, 1))
+
+
+@dataclass
+class TriangleScene:
+    """The scene's triangles for the shooter and the G-buffer ray caster
+    (Triangle convention: front-face normal normalize(cross(p2 - p0, p1 - p0))),
+    the ceiling area light, and the material table with the light's emissive
+    material appended."""
+    p0: np.ndarray
+    p1: np.ndarray
+    p2: np.ndarray
+    material: np.ndarray
+    materials: np.ndarray      # (num_materials + 1, 10); the last one is the light's
+    light: dict                # center, front, up, width, height, radiance, material
+    light_axes: tuple          # unit side axes (s, u) of the light quad
+
+
+def triangle_scene(d: ffi.arc_scene_desc, light_quad: bool = True) -> TriangleScene:
+    lib = ffi.scene_lib()
+    n = lib.arc_scene_num_triangles(C.byref(d))
+    if n < 0:
+        raise ValueError("arc_scene_num_triangles rejected the description")
+    p0, p1, p2 = (np.zeros((n, 3), np.float32) for _ in range(3))
+    mat = np.zeros(n, np.int32)
+    got = lib.arc_scene_triangles(C.byref(d), _ptr(p0), _ptr(p1), _ptr(p2), _ptr(mat), n)
+    assert got == n
+    nm = lib.arc_scene_num_materials()
+    mats = np.zeros((nm + 1, 10), np.float32)
+    lib.arc_scene_materials(_ptr(mats), nm)
+    c, f, u, size, rad = (np.zeros(k, np.float32) for k in (3, 3, 3, 2, 3))
+    lmat = np.zeros(10, np.float32)
+    lib.arc_scene_light(C.byref(d), _ptr(c), _ptr(f), _ptr(u), _ptr(size), _ptr(rad), _ptr(lmat))
+    mats[nm] = lmat
+    # the quad's axes as the shooter forms them (lookAt towards -front)
+    fw = -f / np.linalg.norm(f)
+    s = np.cross(fw, u)
+    s = (s / np.linalg.norm(s)).astype(np.float32)
+    uu = np.cross(s, fw).astype(np.float32)
+    light = dict(center=c, front=f, up=u, width=float(size[0]), height=float(size[1]), radiance=rad, material=nm)
+    if light_quad:  # two emissive triangles facing along front
+        hw, hh = 0.5 * size[0], 0.5 * size[1]
+        q = [c - hw * s - hh * uu, c + hw * s - hh * uu, c + hw * s + hh * uu, c - hw * s + hh * uu]
+        tri = [(q[0], q[1], q[2]), (q[0], q[2], q[3])]
+        lp = []
+        for a, b, e in tri:
+            if np.dot(np.cross(e - a, b - a), f) < 0:  # orient the front face along front
+                b, e = e, b
+            lp.append((a, b, e))
+        p0 = np.concatenate([p0, np.array([t[0] for t in lp], np.float32)])
+        p1 = np.concatenate([p1, np.array([t[1] for t in lp], np.float32)])
+        p2 = np.concatenate([p2, np.array([t[2] for t in lp], np.float32)])
+        mat = np.concatenate([mat, np.full(2, nm, np.int32)])
+    return TriangleScene(p0, p1, p2, mat, mats, light, (s, uu))

@@ -284,6 +284,63 @@ int arc_gather_kernel_history(arc_gather* g, int frames, float* out);
 /* The AvplBvh of the last cut frame (borrowed; NULL before the first). */
 arc_bvh* arc_gather_get_bvh(arc_gather* g);
 
+/* ------------------------------------------------------------ ray tracing */
+/* The triangle scene the AVPLs are shot into: Scene with its kd-tree
+ * (Scene.cpp:175-255, KdTreeAccelerator.h:17-59).  Triangles follow Triangle
+ * (Triangle.h): vertices p0, p1, p2, normal normalize(cross(p2 - p0, p1 - p0)),
+ * a front face seen against the normal; a two-sided surface is two triangles.
+ * Hits on triangles whose material is emissive count as misses
+ * (Scene.cpp:247-252). */
+typedef struct arc_trace_scene arc_trace_scene;
+
+/* AreaLight (AreaLight.h): the emitting rectangle is centred at `center`,
+ * faces `front`, has its height along `up` projected into its plane.  (The
+ * reference places the quad at centerPosition + normalize(frontDirection);
+ * pass that point as center.) */
+typedef struct arc_area_light {
+    arc_float3 center, front, up;
+    float width, height;
+    arc_float3 radiance;
+    int materialIndex;
+} arc_area_light;
+
+#define ARC_SHOOT_NO_EXPLICIT_DIRECT 1 /* keep the antiradiance of bounce-1 AVPLs
+                                           (explicitDirectIllum off, AvplShooter.h:115-117) */
+
+/* Builds the scene's triangle BVH (flags: ARC_INPUT_HOST).  Materials are
+ * host memory, at most 256 entries. */
+int arc_trace_scene_create(arc_ctx* ctx, const arc_float3* p0, const arc_float3* p1, const arc_float3* p2,
+                           const int* material, int ntri, const arc_material* materials, int num_materials,
+                           int flags, arc_trace_scene** out);
+void arc_trace_scene_destroy(arc_trace_scene* scene);
+/* Triangle count and depth of the scene's BVH. */
+int arc_trace_scene_info(arc_trace_scene* scene, int* num_triangles, int* tree_depth);
+
+/* Scene::IntersectRayScene(ray, &t, &isect, FRONT_FACE) for n rays (device
+ * arrays): the closest front-face hit with t > 0 (t = -1 and triangle -1 when
+ * none or emissive).  Triangle indices are input indices. */
+int arc_trace_intersect(arc_trace_scene* scene, const arc_float3* org, const arc_float3* dir, int n, float* t,
+                        int* tri_index);
+
+/* AvplShooter::shoot(direct, indirect, numPaths) (AvplShooter.h:36-142): one
+ * path per index from the area light, bounces with Russian roulette 0.8.
+ * Writes the indirect AVPLs (bounce >= 1, what the gather takes,
+ * Render.cpp:297) into device buffers owned by the scene, path by path in
+ * index order, and points *out at them (valid until the next shoot or
+ * destroy).  Random numbers: a counter-based hash of (seed, path, draw).
+ * Synchronous (the AVPL count is read back). */
+int arc_trace_shoot(arc_trace_scene* scene, const arc_area_light* light, int num_paths, uint64_t seed, int flags,
+                    arc_avpls* out, int* num_indirect);
+
+/* The G-buffer (Shaders/CreateGBuffer.frag) by ray casting: pixel (px, py),
+ * row 0 at the top, looks along normalize(F + sx * t * aspect * R + sy * t * U)
+ * with F = normalize(forward), R = normalize(cross(up, F)), U = cross(F, R),
+ * t = tan(fovy / 2), sx = 2 (px + 0.5) / width - 1, sy = 1 - 2 (py + 0.5) /
+ * height.  Writes float4 (x, y, z, 1) positions and (n, material) normals of
+ * the closest front face, (0,0,0,0) / (0,0,-1,1) where nothing is hit. */
+int arc_trace_gbuffer(arc_trace_scene* scene, arc_float3 camPos, arc_float3 forward, arc_float3 up, float fovy,
+                      int width, int height, float* pos4, float* nrm4);
+
 /* ------------------------------------------------------------ output side */
 /* Renderer::Add(target, source) (Render.cpp:788-806, Shaders/Add.frag):
  * accum.rgb += frame.rgb, accum.w += 1 (float4 images of npix pixels, device

@@ -72,6 +72,18 @@ int arc_scene_points(uint64_t seed, int n, int num_gaussians, float sigma, float
 /* Unit normals drawn uniformly on the sphere (for SimpleBvh inputs). */
 int arc_scene_normals(uint64_t seed, int n, float* nrm3);
 
+/* The scene's triangles in the Triangle convention of the AVPL shooter
+ * (front face normal = normalize(cross(p2 - p0, p1 - p0))); a two-sided
+ * random triangle is written twice, once per face.  Returns the count
+ * (at most max_tris written); arc_scene_num_triangles gives it up front. */
+int arc_scene_num_triangles(const arc_scene_desc* d);
+int arc_scene_triangles(const arc_scene_desc* d, float* p0, float* p1, float* p2, int* material, int max_tris);
+
+/* A ceiling area light for the shooter: centre, front, up, width and height,
+ * radiance, and its emissive MATERIAL (10 floats).  Any pointer may be NULL. */
+int arc_scene_light(const arc_scene_desc* d, float* center3, float* front3, float* up3, float* size2,
+                    float* radiance3, float* material10);
+
 #ifdef __cplusplus
 }
 #endif

@@ -693,3 +693,212 @@ float orc_image_error(const float* ref4, const float* img4, int npix, float* e3)
 }
 
 }  // extern "C"
+
+// ---- ray tracing and the AVPL shooter ----------------------------------
+namespace {
+
+struct T3 {
+    float x, y, z;
+};
+inline T3 t3(const float* p) { return T3{p[0], p[1], p[2]}; }
+inline T3 operator+(T3 a, T3 b) { return T3{a.x + b.x, a.y + b.y, a.z + b.z}; }
+inline T3 operator-(T3 a, T3 b) { return T3{a.x - b.x, a.y - b.y, a.z - b.z}; }
+inline T3 operator*(float s, T3 a) { return T3{s * a.x, s * a.y, s * a.z}; }
+inline T3 mul(T3 a, T3 b) { return T3{a.x * b.x, a.y * b.y, a.z * b.z}; }
+inline float dot3(T3 a, T3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+inline T3 cross3(T3 a, T3 b) { return T3{a.y * b.z - b.y * a.z, a.z * b.x - b.z * a.x, a.x * b.y - b.x * a.y}; }
+inline float len3(T3 a) { return std::sqrt(dot3(a, a)); }
+inline T3 norm3(T3 a) { return (1.0f / len3(a)) * a; }
+
+bool tri_intersect(T3 P0, T3 P1, T3 P2, T3 o, T3 d, float det_eps, float& t) {  // Triangle.h, FRONT_FACE
+    const T3 e1 = P2 - P0, e2 = P1 - P0;
+    const T3 p = cross3(d, e2);
+    const float det = dot3(e1, p);
+    if (det < det_eps && det > -det_eps) return false;
+    const T3 tt = o - P0;
+    const float u = dot3(tt, p);
+    if (u < 0.0f || u > det) return false;
+    const T3 q = cross3(tt, e1);
+    const float v = dot3(d, q);
+    if (v < 0.0f || u + v > det) return false;
+    t = 1.f / det * dot3(e2, q);
+    return true;
+}
+
+// the library's counter-based uniform floats: (mix64 hash >> 40) * 2^-24
+inline uint64_t mix64(uint64_t z) {
+    z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ULL;
+    z = (z ^ (z >> 27)) * 0x94d049bb133111ebULL;
+    return z ^ (z >> 31);
+}
+inline float hash_uniform(uint64_t seed, uint64_t i) {
+    const uint64_t h = mix64(seed * 0x9e3779b97f4a7c15ULL + mix64(i + 0x632be59bd9b4e019ULL));
+    return (float)(h >> 40) * (1.0f / 16777216.0f);
+}
+
+const float kTwoPiF = 6.28318530718f, kOneOverTwoPiF = 0.15915494309f, kOneOverPiF = 0.318309886f;
+
+T3 sample_cos_cone(T3 n, float u1, float u2, float order, float& pdf) {  // Sampler.h:13-35
+    const T3 tt{n.y, n.z, -n.x};
+    const T3 U = norm3(cross3(tt, n));
+    const T3 W = norm3(cross3(U, n));
+    // order 1: cos(theta) = sqrt(u1), sin(theta) = sqrt(1 - u1) (the value of
+    // the reference's acos / pow / sin / cos up to rounding)
+    float ct, st;
+    if (order == 1.f) {
+        ct = std::sqrt(u1);
+        st = std::sqrt(std::max(0.f, 1.f - u1));
+    } else {
+        const float theta = std::acos(std::pow(u1, 1.0f / (order + 1.0f)));
+        ct = std::cos(theta);
+        st = std::sin(theta);
+    }
+    const float phi = kTwoPiF * u2;
+    const T3 s = norm3(T3{st * std::cos(phi), st * std::sin(phi), ct});
+    pdf = (order + 1.f) * kOneOverTwoPiF * std::pow(s.z, order);
+    return norm3(T3{U.x * s.x + W.x * s.y + n.x * s.z, U.y * s.x + W.y * s.y + n.y * s.z,
+                    U.z * s.x + W.z * s.y + n.z * s.z});
+}
+
+float lum(T3 v) { return 0.2126f * v.x + 0.7152f * v.y + 0.0722f * v.z; }
+
+T3 phong_eval(T3 wi, T3 wo, T3 n, const float* m) {  // Brdf.h:9-22
+    if (dot3(wi, n) <= 0.f || dot3(wo, n) <= 0.f) return T3{0, 0, 0};
+    T3 res{0, 0, 0};
+    const T3 spec = t3(m + 6);
+    if (len3(spec) > 0.f) {
+        const T3 r = norm3((2.f * dot3(wi, n)) * n - wi);
+        const float c = std::max(0.f, dot3(r, wo));
+        res = res + (kOneOverTwoPiF * (m[9] + 2.f) * std::pow(c, m[9])) * spec;
+    }
+    return res + kOneOverPiF * t3(m + 3);
+}
+
+T3 phong_sample(T3 wi, T3 n, const float* m, float s0, float s1, float& pdf) {  // Brdf.h:54-82
+    const bool hd = len3(t3(m + 3)) > 0.f, hs = len3(t3(m + 6)) > 0.f;
+    bool spec = hs;
+    if (hd && hs) {
+        const float a = lum(t3(m + 6)) / (lum(t3(m + 6)) + lum(t3(m + 3)));
+        if (s0 <= a) {
+            s0 /= a;
+        } else {
+            s0 = (s0 - a) / (1.f - a);
+            spec = false;
+        }
+    }
+    if (spec) {
+        const T3 dir = sample_cos_cone(wi, s0, s1, m[9], pdf);
+        if (dot3(dir, n) <= 0.f) return T3{0, 0, 0};
+    }
+    return sample_cos_cone(n, s0, s1, 1.f, pdf);
+}
+
+}  // namespace
+
+extern "C" {
+
+int orc_tri_intersect(const float* p0, const float* p1, const float* p2, const float* org, const float* dir,
+                      float det_eps, float* t) {
+    float tt = 0.f;
+    const bool h = tri_intersect(t3(p0), t3(p1), t3(p2), t3(org), t3(dir), det_eps, tt);
+    if (h && t) *t = tt;
+    return h ? 1 : 0;
+}
+
+int orc_trace_closest(int ntri, const float* p0, const float* p1, const float* p2, const int* material,
+                      const int* emissive, const float* org, const float* dir, float det_eps, float* t) {
+    int best = -1;
+    float tb = INFINITY;
+    const T3 o = t3(org), d = t3(dir);
+    for (int i = 0; i < ntri; ++i) {
+        float tt;
+        if (tri_intersect(t3(p0 + 3 * i), t3(p1 + 3 * i), t3(p2 + 3 * i), o, d, det_eps, tt) && tt > 0.f &&
+            tt < tb) {
+            tb = tt;
+            best = i;
+        }
+    }
+    if (best >= 0 && emissive && material && emissive[material[best] & 255]) best = -1;
+    if (t) *t = best >= 0 ? tb : -1.f;
+    return best;
+}
+
+int orc_trace_shoot(int ntri, const float* p0, const float* p1, const float* p2, const int* material,
+                    const float* mats, int num_mats, const float* light, uint64_t seed, int first, int count,
+                    int explicit_direct, int* counts, float* pos3, float* nrm3, float* dir3, float* rad3,
+                    float* anti3, int* mat_out, int* bounce_out, int max_avpls) {
+    std::vector<int> emissive(256, 0);
+    for (int k = 0; k < num_mats && k < 256; ++k)
+        emissive[k] = (mats[10 * k] * mats[10 * k] + mats[10 * k + 1] * mats[10 * k + 1] +
+                       mats[10 * k + 2] * mats[10 * k + 2]) > 0.f;
+    std::vector<float> zero(10, 0.f);
+    auto material_of = [&](int m) { return (m & 255) < num_mats ? mats + 10 * (m & 255) : zero.data(); };
+    const T3 lc = t3(light), lf = t3(light + 3), ls = t3(light + 6), lu = t3(light + 9);
+    const float hw = light[12], hh = light[13];
+    const T3 lrad = t3(light + 14);
+    const int lmat = (int)light[17];
+    const float area = (2.f * hw) * (2.f * hh);
+    int total = 0;
+    for (int path = first; path < first + count; ++path) {
+        const uint64_t base = (uint64_t)path << 24;
+        uint32_t k = 0;
+        auto rnd = [&]() { return hash_uniform(seed, base | (uint64_t)(k++)); };
+        const float sx = -1.f + 2.f * rnd();
+        const float sy = -1.f + 2.f * rnd();
+        T3 pos = lc + (sx * hw) * ls + (sy * hh) * lu;
+        float pdf = 1.f / area;
+        T3 nrm = lf;
+        T3 rad = (1.f / pdf) * lrad;
+        T3 in_dir{0, 0, 0};
+        int bounce = 0, mat = lmat;
+        const float u1 = rnd(), u2 = rnd();
+        T3 direction = sample_cos_cone(nrm, u1, u2, 1.f, pdf);
+        int n = 0;
+        for (;;) {
+            const T3 org = pos + 1e-4f * nrm;
+            float t;
+            const float o3[3] = {org.x, org.y, org.z}, d3[3] = {direction.x, direction.y, direction.z};
+            const int s = orc_trace_closest(ntri, p0, p1, p2, material, emissive.data(), o3, d3, 1e-4f, &t);
+            if (s < 0) break;
+            const float cos_theta = dot3(nrm, direction);
+            T3 contrib{rad.x * cos_theta / pdf, rad.y * cos_theta / pdf, rad.z * cos_theta / pdf};
+            if (bounce > 0) contrib = mul(contrib, phong_eval(-1.f * in_dir, direction, nrm, material_of(mat)));
+            const T3 P0 = t3(p0 + 3 * s), P1 = t3(p1 + 3 * s), P2 = t3(p2 + 3 * s);
+            T3 hn = cross3(P2 - P0, P1 - P0);
+            if (len3(hn) > 0.f) hn = norm3(hn);
+            const T3 hit = org + t * direction;
+            pos = hit - 1e-4f * hn;
+            nrm = hn;
+            rad = contrib;
+            T3 anti = contrib;
+            in_dir = direction;
+            ++bounce;
+            mat = material[s];
+            if (bounce == 1 && explicit_direct) anti = T3{0, 0, 0};
+            const bool stop = rnd() > 0.8f;
+            rad = stop ? T3{0, 0, 0} : (1.f / 0.8f) * rad;
+            if (pos3 && total < max_avpls) {
+                const T3 vals[5] = {pos, nrm, in_dir, rad, anti};
+                float* outs[5] = {pos3, nrm3, dir3, rad3, anti3};
+                for (int a = 0; a < 5; ++a) {
+                    outs[a][3 * total + 0] = vals[a].x;
+                    outs[a][3 * total + 1] = vals[a].y;
+                    outs[a][3 * total + 2] = vals[a].z;
+                }
+                mat_out[total] = mat;
+                bounce_out[total] = bounce;
+            }
+            ++total;
+            ++n;
+            if (stop) break;
+            const float s0 = rnd(), s1 = rnd();
+            direction = phong_sample(-1.f * direction, nrm, material_of(mat), s0, s1, pdf);
+            if (len3(direction) <= 0.f) break;
+            if (pdf < 1e-4f) break;
+        }
+        if (counts) counts[path - first] = n;
+    }
+    return total;
+}
+
+}  // extern "C"

@@ -150,6 +150,31 @@ void orc_image_normalize(float* out4, const float* accum4, int npix, int num_pat
  * the product-plus-sum combination). */
 float orc_image_error(const float* ref4, const float* img4, int npix, float* e3);
 
+/* ---- ray tracing and the AVPL shooter (Triangle.h, AvplShooter.h) -------
+ * Triangle::intersect, FRONT_FACE mode (Triangle.h): e1 = p2 - p0,
+ * e2 = p1 - p0, Moller-Trumbore on unnormalised barycentrics, |det| <
+ * det_eps rejected (the reference's EPS = 1e-4), FP32 in glm's operation
+ * order.  Returns 1 and *t on a hit (any sign of t).  Pinned by closed forms
+ * (head-on hit distance, back face, edge and outside rays, the det floor). */
+int orc_tri_intersect(const float* p0, const float* p1, const float* p2, const float* org, const float* dir,
+                      float det_eps, float* t);
+/* The closest hit with t > 0 over all triangles by brute force (ties: the
+ * lower index); a triangle whose material is flagged emissive ends the ray
+ * with no hit (Scene.cpp:247-252).  Returns the triangle or -1. */
+int orc_trace_closest(int ntri, const float* p0, const float* p1, const float* p2, const int* material,
+                      const int* emissive, const float* org, const float* dir, float det_eps, float* t);
+/* AvplShooter::createPath (AvplShooter.h:77-142) for paths [first,
+ * first + count) with the library's counter-based random numbers, brute-force
+ * intersection.  light: center3, front3, s3, u3 (the quad's unit side axes),
+ * half_w, half_h, radiance3, material (as floats).  mats: 10 floats per
+ * material.  Writes each path's indirect AVPL count to counts[] and, when
+ * pos3 is not NULL, its AVPLs consecutively (at most max_avpls); returns the
+ * total count. */
+int orc_trace_shoot(int ntri, const float* p0, const float* p1, const float* p2, const int* material,
+                    const float* mats, int num_mats, const float* light, uint64_t seed, int first, int count,
+                    int explicit_direct, int* counts, float* pos3, float* nrm3, float* dir3, float* rad3,
+                    float* anti3, int* mat, int* bounce, int max_avpls);
+
 #ifdef __cplusplus
 }
 #endif

@@ -66,6 +66,10 @@ def lib() -> C.CDLL:
             "orc_pair": (i32, [C.POINTER(orc_scene), i32, i32, i32, vp, vp]),
             "orc_gather_exact": (i32, [C.POINTER(orc_scene), i32, vp, i32, vp, vp]),
             "orc_image_accumulate": (None, [vp, vp, i32]),
+            "orc_tri_intersect": (i32, [vp, vp, vp, vp, vp, f32, vp]),
+            "orc_trace_closest": (i32, [i32, vp, vp, vp, vp, vp, vp, vp, f32, vp]),
+            "orc_trace_shoot": (i32, [i32, vp, vp, vp, vp, vp, i32, vp, u64, i32, i32, i32, vp, vp, vp, vp, vp,
+                                      vp, vp, vp, i32]),
             "orc_image_normalize": (None, [vp, vp, i32, i32]),
             "orc_image_error": (f32, [vp, vp, i32, vp]),
             "orc_gather_exact_parts": (i32, [C.POINTER(orc_scene), i32, vp, i32, vp, vp, i32]),
