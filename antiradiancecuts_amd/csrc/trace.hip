@@ -51,8 +51,12 @@ struct arc_trace_scene {
     arc::DevBuf emissive;             // per material: emissive != 0
     int max_depth = 0;                // deepest internal node (root = 1)
     // shooter output
-    arc::DevBuf path_count, path_offset, scan_blocks, out;
-    size_t out_capacity = 0;
+    arc::DevBuf state, active, stage, path_offset, scan_blocks, out;
+    struct {
+        f3 *pos, *nrm, *rad, *in_dir, *dir;
+        float* pdf;
+        int *bounce, *mat, *draws, *count;
+    } ps{};
 };
 
 namespace {
@@ -301,83 +305,130 @@ struct AvplOut {
     int *mat, *bounce;
 };
 
-// createPath (AvplShooter.h:77-142).  WRITE = false counts the indirect
-// AVPLs of the path; WRITE = true stores them from `at`.
-template <bool WRITE>
-__device__ int create_path(const SceneDev& S, const Light& L, Rng& rng, int explicit_direct, int* stk,
-                           const AvplOut& o, int at) {
+// Wavefront form of createPath (AvplShooter.h:77-142): the state of every
+// path lives in HBM between launches, one launch per bounce runs the paths
+// still alive (no lane waits on another lane's longer path), and each
+// launch stages the AVPLs it creates; a final scatter puts every path's
+// AVPLs together in path order.  Draw order per path as in createPath:
+// light position (2), first direction (2), then per bounce roulette (1) and
+// the next direction (2).
+struct PathState {
+    f3 *pos, *nrm, *rad, *in_dir, *dir;
+    float* pdf;
+    int *bounce, *mat, *draws, *count;
+};
+
+struct Staged {  // AVPLs of all bounces, in the order the bounces created them
+    AvplOut a;
+    int* path;
+};
+
+__global__ __launch_bounds__(kThreads) void k_path_start(Light L, uint64_t seed, int paths, PathState P,
+                                                         int* __restrict__ active) {
+    const int i = blockIdx.x * kThreads + threadIdx.x;
+    if (i >= paths) return;
+    Rng rng{seed, (uint64_t)i << 24, 0};
     // AreaLight::samplePos: glm::linearRand over [-1, 1]^2, x then y
     const float sx = -1.f + 2.f * rng.next();
     const float sy = -1.f + 2.f * rng.next();
-    V pos = L.center + (sx * L.half_w) * L.s + (sy * L.half_h) * L.u;
-    float pdf = 1.f / L.area;
-    V nrm = L.front;
-    V rad = (1.f / pdf) * L.radiance;  // rad / pdf
-    V in_dir = mk(0.f, 0.f, 0.f);
-    int bounce = 0, mat = L.mat;
+    const V pos = L.center + (sx * L.half_w) * L.s + (sy * L.half_h) * L.u;
+    const float pdf0 = 1.f / L.area;
+    const V rad = (1.f / pdf0) * L.radiance;  // rad / pdf
     const float u1 = rng.next(), u2 = rng.next();
-    V direction = sample_cos_cone(nrm, u1, u2, 1.f, pdf);
-    int count = 0;
-    for (;;) {
-        const V org = pos + kEps * nrm;
-        float t;
-        const int s = scene_hit(S, org, direction, t, stk);
-        if (s < 0) break;
-        const float cos_theta = dot(nrm, direction);
-        // getIncidentRadiance() * cos_theta / pdf, per component
-        V contrib = mk(rad.x * cos_theta / pdf, rad.y * cos_theta / pdf, rad.z * cos_theta / pdf);
-        if (bounce > 0) contrib = contrib * phong_eval(-1.f * in_dir, direction, nrm, S.mats[mat & 255]);
-        const float4* T = S.tri + 4 * (size_t)s;
-        const V hit_n = mk(T[3].x, T[3].y, T[3].z);
-        const V hit = org + t * direction;  // Intersection::setPosition(origin + dist * dir)
-        pos = hit - kEps * hit_n;
-        nrm = hit_n;
-        rad = contrib;
-        V anti = contrib;
-        in_dir = direction;
-        ++bounce;
-        mat = __float_as_int(T[0].w);
-        if (bounce == 1 && explicit_direct) anti = mk(0.f, 0.f, 0.f);
-        const bool stop = rng.next() > kRr;  // Russian roulette
-        if (stop) rad = mk(0.f, 0.f, 0.f);
-        else rad = (1.f / kRr) * rad;
-        if (WRITE) {
-            const int j = at + count;
-            o.pos[j] = f3{pos.x, pos.y, pos.z};
-            o.nrm[j] = f3{nrm.x, nrm.y, nrm.z};
-            o.dir[j] = f3{in_dir.x, in_dir.y, in_dir.z};
-            o.rad[j] = f3{rad.x, rad.y, rad.z};
-            o.anti[j] = f3{anti.x, anti.y, anti.z};
-            o.mat[j] = mat;
-            o.bounce[j] = bounce;
-        }
-        ++count;
-        if (stop) break;
-        const float s0 = rng.next(), s1 = rng.next();
-        direction = phong_sample(-1.f * direction, nrm, S.mats[mat & 255], s0, s1, pdf);
-        if (length(direction) <= 0.f) break;
-        if (pdf < 1e-4f) break;
+    float pdf;
+    const V dir = sample_cos_cone(L.front, u1, u2, 1.f, pdf);
+    P.pos[i] = f3{pos.x, pos.y, pos.z};
+    P.nrm[i] = f3{L.front.x, L.front.y, L.front.z};
+    P.rad[i] = f3{rad.x, rad.y, rad.z};
+    P.in_dir[i] = f3{0.f, 0.f, 0.f};
+    P.dir[i] = f3{dir.x, dir.y, dir.z};
+    P.pdf[i] = pdf;
+    P.bounce[i] = 0;
+    P.mat[i] = L.mat;
+    P.draws[i] = (int)rng.k;
+    P.count[i] = 0;
+    active[i] = i;
+}
+
+// One bounce of every live path: trace, create the AVPL, roulette, sample
+// the next direction.  Survivors are appended to next_active.
+__global__ __launch_bounds__(kThreads) void k_path_step(SceneDev S, uint64_t seed, int explicit_direct, PathState P,
+                                                        const int* __restrict__ active, int n_active,
+                                                        int* __restrict__ next_active, int* __restrict__ n_next,
+                                                        Staged st, int stage_base) {
+    __shared__ int stk[kTraceStack * kThreads];
+    const int j = blockIdx.x * kThreads + threadIdx.x;
+    if (j >= n_active) return;
+    const int i = active[j];
+    const V nrm = ld(P.nrm[i]), direction = ld(P.dir[i]);
+    const V org = ld(P.pos[i]) + kEps * nrm;
+    float t;
+    const int s = scene_hit(S, org, direction, t, stk + threadIdx.x);
+    // every active path stages one AVPL slot; a miss leaves it unused (bounce 0)
+    const int slot = stage_base + j;
+    if (s < 0) {
+        st.path[slot] = -1;
+        return;
     }
-    return count;
+    const V rad0 = ld(P.rad[i]);
+    const float pdf0 = P.pdf[i];
+    const float cos_theta = dot(nrm, direction);
+    // getIncidentRadiance() * cos_theta / pdf, per component
+    V contrib = mk(rad0.x * cos_theta / pdf0, rad0.y * cos_theta / pdf0, rad0.z * cos_theta / pdf0);
+    const int bounce0 = P.bounce[i];
+    if (bounce0 > 0) contrib = contrib * phong_eval(-1.f * ld(P.in_dir[i]), direction, nrm, S.mats[P.mat[i] & 255]);
+    const float4* T = S.tri + 4 * (size_t)s;
+    const V hit_n = mk(T[3].x, T[3].y, T[3].z);
+    const V hit = org + t * direction;  // Intersection::setPosition(origin + dist * dir)
+    const V pos = hit - kEps * hit_n;
+    const int bounce = bounce0 + 1, mat = __float_as_int(T[0].w);
+    V anti = contrib;
+    if (bounce == 1 && explicit_direct) anti = mk(0.f, 0.f, 0.f);
+    Rng rng{seed, (uint64_t)i << 24, (uint32_t)P.draws[i]};
+    const bool stop = rng.next() > kRr;  // Russian roulette
+    const V rad = stop ? mk(0.f, 0.f, 0.f) : (1.f / kRr) * contrib;
+    st.a.pos[slot] = f3{pos.x, pos.y, pos.z};
+    st.a.nrm[slot] = f3{hit_n.x, hit_n.y, hit_n.z};
+    st.a.dir[slot] = f3{direction.x, direction.y, direction.z};
+    st.a.rad[slot] = f3{rad.x, rad.y, rad.z};
+    st.a.anti[slot] = f3{anti.x, anti.y, anti.z};
+    st.a.mat[slot] = mat;
+    st.a.bounce[slot] = bounce;
+    st.path[slot] = i;
+    P.count[i] += 1;
+    if (stop) return;
+    const float s0 = rng.next(), s1 = rng.next();
+    float pdf;
+    const V next = phong_sample(-1.f * direction, hit_n, S.mats[mat & 255], s0, s1, pdf);
+    if (length(next) <= 0.f || pdf < 1e-4f) return;
+    P.pos[i] = f3{pos.x, pos.y, pos.z};
+    P.nrm[i] = f3{hit_n.x, hit_n.y, hit_n.z};
+    P.rad[i] = f3{rad.x, rad.y, rad.z};
+    P.in_dir[i] = f3{direction.x, direction.y, direction.z};
+    P.dir[i] = f3{next.x, next.y, next.z};
+    P.pdf[i] = pdf;
+    P.bounce[i] = bounce;
+    P.mat[i] = mat;
+    P.draws[i] = (int)rng.k;
+    next_active[atomicAdd(n_next, 1)] = i;
 }
 
-__global__ __launch_bounds__(kThreads) void k_count_paths(SceneDev S, Light L, uint64_t seed, int paths,
-                                                          int explicit_direct, int* __restrict__ counts) {
-    __shared__ int stk[kTraceStack * kThreads];
-    const int i = blockIdx.x * kThreads + threadIdx.x;
-    if (i >= paths) return;
-    Rng rng{seed, (uint64_t)i << 24, 0};
-    counts[i] = create_path<false>(S, L, rng, explicit_direct, stk + threadIdx.x, AvplOut{}, 0);
-}
-
-__global__ __launch_bounds__(kThreads) void k_write_paths(SceneDev S, Light L, uint64_t seed, int paths,
-                                                          int explicit_direct, const int* __restrict__ offsets,
-                                                          AvplOut o) {
-    __shared__ int stk[kTraceStack * kThreads];
-    const int i = blockIdx.x * kThreads + threadIdx.x;
-    if (i >= paths) return;
-    Rng rng{seed, (uint64_t)i << 24, 0};
-    create_path<true>(S, L, rng, explicit_direct, stk + threadIdx.x, o, offsets[i]);
+// Staged AVPL -> offset[path] + bounce - 1: each path's AVPLs together, in
+// path order and bounce order.
+__global__ __launch_bounds__(kThreads) void k_path_scatter(Staged st, int n_staged, const int* __restrict__ offsets,
+                                                           AvplOut o) {
+    const int k = blockIdx.x * kThreads + threadIdx.x;
+    if (k >= n_staged) return;
+    const int i = st.path[k];
+    if (i < 0) return;
+    const int dst = offsets[i] + st.a.bounce[k] - 1;
+    o.pos[dst] = st.a.pos[k];
+    o.nrm[dst] = st.a.nrm[k];
+    o.dir[dst] = st.a.dir[k];
+    o.rad[dst] = st.a.rad[k];
+    o.anti[dst] = st.a.anti[k];
+    o.mat[dst] = st.a.mat[k];
+    o.bounce[dst] = st.a.bounce[k];
 }
 
 // Exclusive scan of n ints: per-block scan (kScan per block), block totals,
@@ -696,7 +747,9 @@ void arc_trace_scene_destroy(arc_trace_scene* sc) {
     arc::release(sc->counters);
     arc::release(sc->mats);
     arc::release(sc->emissive);
-    arc::release(sc->path_count);
+    arc::release(sc->state);
+    arc::release(sc->active);
+    arc::release(sc->stage);
     arc::release(sc->path_offset);
     arc::release(sc->scan_blocks);
     arc::release(sc->out);
@@ -756,14 +809,82 @@ int arc_trace_shoot(arc_trace_scene* sc, const arc_area_light* light, int num_pa
     L.mat = light->materialIndex;
     const int explicit_direct = (flags & ARC_SHOOT_NO_EXPLICIT_DIRECT) ? 0 : 1;
     if (num_paths == 0) return ARC_OK;
-    const int nb = (num_paths + kScan - 1) / kScan;
-    ARC_TRY(arc::ensure(ctx, sc->path_count, sizeof(int) * (size_t)num_paths));
-    ARC_TRY(arc::ensure(ctx, sc->path_offset, sizeof(int) * ((size_t)num_paths + 1)));
-    ARC_TRY(arc::ensure(ctx, sc->scan_blocks, sizeof(int) * ((size_t)nb + 1)));
+    const size_t np = (size_t)num_paths;
+    // path state, 80 bytes per path
+    ARC_TRY(arc::ensure(ctx, sc->state, np * (5 * sizeof(f3) + 5 * sizeof(int))));
+    {
+        char* b = sc->state.as<char>();
+        f3* v[5];
+        for (int k = 0; k < 5; ++k) v[k] = reinterpret_cast<f3*>(b + (size_t)k * np * sizeof(f3));
+        int* w = reinterpret_cast<int*>(b + 5 * np * sizeof(f3));
+        sc->ps = {v[0], v[1], v[2], v[3], v[4], reinterpret_cast<float*>(w), w + np, w + 2 * np,
+                               w + 3 * np, w + 4 * np};
+    }
+    const PathState P{sc->ps.pos, sc->ps.nrm, sc->ps.rad, sc->ps.in_dir, sc->ps.dir, sc->ps.pdf, sc->ps.bounce,
+                      sc->ps.mat, sc->ps.draws, sc->ps.count};
+    ARC_TRY(arc::ensure(ctx, sc->active, sizeof(int) * (2 * np + 2)));
+    int* act[2] = {sc->active.as<int>(), sc->active.as<int>() + np};
+    int* n_next = sc->active.as<int>() + 2 * np;
     const SceneDev S = scene_dev(sc);
-    hipLaunchKernelGGL(k_count_paths, dim3(blocks(num_paths)), dim3(kThreads), 0, ctx->stream, S, L, seed, num_paths,
-                       explicit_direct, sc->path_count.as<int>());
-    hipLaunchKernelGGL(k_scan_local, dim3(nb), dim3(kScan), 0, ctx->stream, sc->path_count.as<int>(), num_paths,
+    hipLaunchKernelGGL(k_path_start, dim3(blocks(num_paths)), dim3(kThreads), 0, ctx->stream, L, seed, num_paths, P,
+                       act[0]);
+    ARC_LAUNCHED(ctx);
+    // one launch per bounce over the live paths
+    size_t cap = 0, staged = 0;
+    Staged st{};
+    auto stage_at = [&](arc::DevBuf& buf, size_t c) {
+        char* b = buf.as<char>();
+        Staged r;
+        r.a.pos = reinterpret_cast<f3*>(b);
+        r.a.nrm = r.a.pos + c;
+        r.a.dir = r.a.nrm + c;
+        r.a.rad = r.a.dir + c;
+        r.a.anti = r.a.rad + c;
+        r.a.mat = reinterpret_cast<int*>(r.a.anti + c);
+        r.a.bounce = r.a.mat + c;
+        r.path = r.a.bounce + c;
+        return r;
+    };
+    const size_t slot_bytes = 5 * sizeof(f3) + 3 * sizeof(int);
+    if (sc->stage.p) {  // reuse the last call's staging area
+        cap = sc->stage.bytes / slot_bytes;
+        st = stage_at(sc->stage, cap);
+    }
+    int n_active = num_paths, cur = 0;
+    for (int step = 0; n_active > 0; ++step) {
+        if (step > 4096) return arc::fail(ctx, ARC_E_STATE, "path tracing did not terminate");
+        if (staged + (size_t)n_active > cap) {  // grow the staging area, keeping what it holds
+            const size_t ncap = std::max(staged + (size_t)n_active, std::max((size_t)8 * np, 2 * cap));
+            arc::DevBuf nb;
+            ARC_TRY(arc::ensure(ctx, nb, ncap * slot_bytes));
+            const Staged ns = stage_at(nb, ncap);
+            if (staged > 0) {
+                const void* src[8] = {st.a.pos, st.a.nrm, st.a.dir, st.a.rad, st.a.anti, st.a.mat, st.a.bounce, st.path};
+                void* dst[8] = {ns.a.pos, ns.a.nrm, ns.a.dir, ns.a.rad, ns.a.anti, ns.a.mat, ns.a.bounce, ns.path};
+                const size_t sz[8] = {sizeof(f3), sizeof(f3), sizeof(f3), sizeof(f3), sizeof(f3), 4, 4, 4};
+                for (int k = 0; k < 8; ++k)
+                    ARC_HIP(ctx, hipMemcpyAsync(dst[k], src[k], staged * sz[k], hipMemcpyDeviceToDevice, ctx->stream));
+            }
+            ARC_HIP(ctx, hipStreamSynchronize(ctx->stream));
+            arc::release(sc->stage);
+            sc->stage = nb;
+            cap = ncap;
+            st = ns;
+        }
+        hipLaunchKernelGGL(k_zero_ints, dim3(1), dim3(kThreads), 0, ctx->stream, n_next, 1);
+        hipLaunchKernelGGL(k_path_step, dim3(blocks(n_active)), dim3(kThreads), 0, ctx->stream, S, seed,
+                           explicit_direct, P, act[cur], n_active, act[1 - cur], n_next, st, (int)staged);
+        ARC_LAUNCHED(ctx);
+        staged += (size_t)n_active;
+        ARC_HIP(ctx, hipMemcpyAsync(&n_active, n_next, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+        ARC_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        cur = 1 - cur;
+    }
+    // per-path counts -> offsets -> AVPLs in path order
+    const int nb = (num_paths + kScan - 1) / kScan;
+    ARC_TRY(arc::ensure(ctx, sc->path_offset, sizeof(int) * (np + 1)));
+    ARC_TRY(arc::ensure(ctx, sc->scan_blocks, sizeof(int) * ((size_t)nb + 1)));
+    hipLaunchKernelGGL(k_scan_local, dim3(nb), dim3(kScan), 0, ctx->stream, P.count, num_paths,
                        sc->path_offset.as<int>(), sc->scan_blocks.as<int>());
     hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(kScan), 0, ctx->stream, sc->scan_blocks.as<int>(), nb);
     hipLaunchKernelGGL(k_scan_add, dim3(nb), dim3(kScan), 0, ctx->stream, sc->path_offset.as<int>(), num_paths,
@@ -779,9 +900,9 @@ int arc_trace_shoot(arc_trace_scene* sc, const arc_area_light* light, int num_pa
     AvplOut o{reinterpret_cast<f3*>(base), reinterpret_cast<f3*>(base + v3), reinterpret_cast<f3*>(base + 2 * v3),
               reinterpret_cast<f3*>(base + 3 * v3), reinterpret_cast<f3*>(base + 4 * v3),
               reinterpret_cast<int*>(base + 5 * v3), reinterpret_cast<int*>(base + 5 * v3 + vi)};
-    if (total > 0) {
-        hipLaunchKernelGGL(k_write_paths, dim3(blocks(num_paths)), dim3(kThreads), 0, ctx->stream, S, L, seed,
-                           num_paths, explicit_direct, sc->path_offset.as<int>(), o);
+    if (staged > 0) {
+        hipLaunchKernelGGL(k_path_scatter, dim3(blocks((int)staged)), dim3(kThreads), 0, ctx->stream, st, (int)staged,
+                           sc->path_offset.as<int>(), o);
         ARC_LAUNCHED(ctx);
     }
     out->position = reinterpret_cast<const arc_float3*>(o.pos);
