@@ -65,6 +65,10 @@ def parse():
                     help="light cut of the radiance: the analytic error bound or kernel_bvh's solid angle")
     ap.add_argument("--error-bound", type=float, default=0.01, help="relative error bound of the cut")
     ap.add_argument("--per-tile", type=int, default=1, help="error-bound cut: one cut per 8x8 tile (1) or per pixel (0)")
+    ap.add_argument("--component", choices=("frame", "trace"), default="frame",
+                    help="frame: the gather frame (the headline metric); trace: the GPU AVPL shooter and "
+                         "G-buffer ray caster on the same scene (a side measurement)")
+    ap.add_argument("--paths", type=int, default=250_000, help="trace: paths shot per step")
     return ap.parse_args()
 
 
@@ -74,6 +78,8 @@ def main() -> int:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
               "(the gather shards by pixel tiles with no exchange; see the design notes)", file=sys.stderr)
         return 2
+    if args.component == "trace":
+        return bench_trace(args)
     import torch
 
     from antiradiancecuts_amd import ffi
@@ -213,6 +219,53 @@ def main() -> int:
     }
     print(json.dumps(line), flush=True)
     g.close()
+    ctx.close()
+    return 0
+
+
+def bench_trace(args) -> int:
+    """AvplShooter::shoot and the G-buffer on the GPU, on the frame's scene
+    (Cornell walls + 200K random triangles, both faces: 400,012 triangles):
+    `steps` shoots of --paths paths, then `steps` 1080p G-buffer casts."""
+    import math
+
+    import torch
+
+    from antiradiancecuts_amd import scene as S
+    from antiradiancecuts_amd.runtime import Context, TraceScene
+
+    ctx = Context(0)
+    d = S.desc(2, args.width, args.height, 0, 1, 4, 1e-3, 10.0, True, 0.3, False, args.tris, 1e-4, 1e-2)
+    ts = S.triangle_scene(d)
+    sc = TraceScene(ctx, ts)
+    for k in range(args.warmup):
+        sc.shoot(args.paths, 1000 + k)
+    torch.cuda.synchronize()
+    t0 = time.time()
+    n = 0
+    for k in range(args.steps):
+        n += sc.shoot(args.paths, k).struct.numAvpls
+    torch.cuda.synchronize()
+    shoot_ms = (time.time() - t0) * 1e3 / args.steps
+    cam = (0.5, 0.5, -0.49 / math.tan(math.pi / 8) / max(args.width / args.height, 1.0))
+    for _ in range(args.warmup):
+        sc.gbuffer(cam, (0, 0, 1), (0, 1, 0), math.pi / 4, args.width, args.height)
+    torch.cuda.synchronize()
+    t0 = time.time()
+    for _ in range(args.steps):
+        sc.gbuffer(cam, (0, 0, 1), (0, 1, 0), math.pi / 4, args.width, args.height)
+    torch.cuda.synchronize()
+    gb_ms = (time.time() - t0) * 1e3 / args.steps
+    line = {"metric": "AVPLs/s shot on the GPU (side measurement; not the headline metric)",
+            "value": n / args.steps / (shoot_ms * 1e-3), "unit": "AVPLs/s", "n_gpus": 1, "steps": args.steps,
+            "warmup": args.warmup, "ms_per_step": round(shoot_ms, 3), "higher_is_better": True,
+            "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+            "config": {"workload": f"AvplShooter::shoot, {args.paths} paths, {len(ts.material)} triangles "
+                                   f"(tree depth {sc.info()[1]}); G-buffer {args.width}x{args.height}",
+                       "avpls_per_step": n / args.steps, "gbuffer_ms": round(gb_ms, 3),
+                       "gbuffer_rays_per_s": args.width * args.height / (gb_ms * 1e-3)}}
+    print(json.dumps(line), flush=True)
+    sc.close()
     ctx.close()
     return 0
 
