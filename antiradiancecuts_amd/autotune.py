@@ -1,0 +1,54 @@
+"""Per-view autotuner of the gather's schedule knobs.
+
+A view's frame time depends on knobs that leave its result (nearly) alone:
+the error-bound cut's traversal (one cut per 8x8 tile, or per pixel), the
+depth of the estimate cut that sets each pixel's threshold, and the
+traversal stack of the solid-angle cut.  The tuner times every candidate on
+the view itself (HIP events on the library's stream, median of `reps`
+frames after one warm-up) and keeps the fastest.  The accuracy knobs
+(errorBound, clusterRefinementThreshold) are the caller's and are not
+touched.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import itertools
+
+import numpy as np
+
+from . import ffi
+from .runtime import default_params
+
+
+def candidates(base: ffi.arc_gather_params):
+    """The schedule variants of `base` the tuner tries."""
+    out = []
+    if base.useLightCuts and base.cutMode == ffi.ARC_CUT_ERROR_BOUND:
+        for per_tile, d0 in itertools.product((1, 0), (4, 6, 8)):
+            p = _copy(base)
+            p.cutPerTile, p.cutEstimateDepth = per_tile, d0
+            out.append(p)
+    else:
+        out.append(_copy(base))
+    return out
+
+
+def _copy(p: ffi.arc_gather_params) -> ffi.arc_gather_params:
+    q = ffi.arc_gather_params()
+    C.memmove(C.addressof(q), C.addressof(p), C.sizeof(ffi.arc_gather_params))
+    return q
+
+
+def tune(gather, avpls, cam, extent, base: ffi.arc_gather_params | None = None, reps: int = 3):
+    """Returns (best params, [(cutPerTile, cutEstimateDepth, median ms), ...])."""
+    base = base or default_params()
+    flags = ffi.ARC_RUN_PROFILE | ffi.ARC_TRUST_INPUT
+    table = []
+    best, best_ms = None, np.inf
+    for p in candidates(base):
+        gather.run(avpls, cam, extent, p, ffi.ARC_TRUST_INPUT)
+        ms = float(np.median([gather.run(avpls, cam, extent, p, flags).msTotal for _ in range(reps)]))
+        table.append((p.cutPerTile, p.cutEstimateDepth, ms))
+        if ms < best_ms:
+            best, best_ms = p, ms
+    return best, table

@@ -37,6 +37,7 @@ HIP_SOURCES = {
     "gather_cut.hip": [],
     "output.hip": [],
     "trace.hip": ["-ffp-contract=off"],
+    "batch.hip": [],
 }
 
 HIPCC_FLAGS = [

@@ -687,6 +687,8 @@ int arc_gather_profile_history(arc_gather* g, int frames, float* out) {
     return n;
 }
 
+arc_ctx* arc_gather_ctx(arc_gather* g) { return g ? g->ctx : nullptr; }
+
 int arc_gather_kernel_history(arc_gather* g, int frames, float* out) {
     if (!g || frames < 0 || (frames > 0 && !out)) return ARC_E_INVALID;
     arc_ctx* ctx = g->ctx;

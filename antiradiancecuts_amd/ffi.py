@@ -113,6 +113,11 @@ class arc_area_light(C.Structure):
                 ("height", C.c_float), ("radiance", arc_float3), ("materialIndex", C.c_int)]
 
 
+class arc_batch_view(C.Structure):
+    _fields_ = [("gather", C.c_void_p), ("avpls", arc_avpls), ("camPos", arc_float3), ("sceneExtent", C.c_float),
+                ("params", arc_gather_params)]
+
+
 ARC_SYMBOLS = [
     ("arc_version", C.c_char_p, []),
     ("arc_status_string", C.c_char_p, [C.c_int]),
@@ -149,6 +154,12 @@ ARC_SYMBOLS = [
     ("arc_gather_profile_history", C.c_int, [vp, C.c_int, vp]),
     ("arc_gather_kernel_history", C.c_int, [vp, C.c_int, vp]),
     ("arc_gather_get_bvh", vp, [vp]),
+    ("arc_gather_ctx", vp, [vp]),
+    ("arc_batch_create", C.c_int, [vp, C.c_int, P(vp)]),
+    ("arc_batch_lane", vp, [vp, C.c_int]),
+    ("arc_batch_capture", C.c_int, [vp, vp, C.c_int]),
+    ("arc_batch_launch", C.c_int, [vp]),
+    ("arc_batch_destroy", None, [vp]),
     ("arc_trace_scene_create", C.c_int, [vp, vp, vp, vp, vp, C.c_int, vp, C.c_int, C.c_int, P(vp)]),
     ("arc_trace_scene_destroy", None, [vp]),
     ("arc_trace_scene_info", C.c_int, [vp, P(C.c_int), P(C.c_int)]),

@@ -325,3 +325,50 @@ def avpls_to_host(avpls: DeviceAvpls, ctx: Context) -> dict:
         a = ctx.to_host(ptr or 0, dt, n * cols)
         out[name] = a.reshape(n, 3) if cols == 3 else a
     return out
+
+
+class LaneContext:
+    """A batch lane's arc_ctx (owned by its Batch); gathers of that lane are
+    created on it."""
+
+    def __init__(self, lib, h, device: int):
+        self.lib, self.h, self.device = lib, h, device
+
+    def sync(self) -> None:
+        check(self.lib.arc_ctx_synchronize(self.h), "arc_ctx_synchronize", self.h)
+
+
+class Batch:
+    """arc_batch: views captured into one HIP graph, one launch per batch."""
+
+    def __init__(self, ctx: Context, lanes: int = 4):
+        self.ctx, self.lib = ctx, ctx.lib
+        h = C.c_void_p()
+        check(self.lib.arc_batch_create(ctx.h, int(lanes), C.byref(h)), "arc_batch_create", ctx.h)
+        self.h = h
+        self.lanes = [LaneContext(self.lib, C.c_void_p(self.lib.arc_batch_lane(h, i)), ctx.device)
+                      for i in range(lanes)]
+        self._views = None
+
+    def lane(self, i: int) -> LaneContext:
+        return self.lanes[i % len(self.lanes)]
+
+    def capture(self, views) -> None:
+        """views: (Gather, DeviceAvpls, cam, extent, params) per view."""
+        arr = (ffi.arc_batch_view * len(views))()
+        for k, (g, av, cam, extent, params) in enumerate(views):
+            arr[k].gather = g.h.value if isinstance(g.h, C.c_void_p) else g.h
+            arr[k].avpls = av.struct
+            arr[k].camPos = ffi.arc_float3(*map(float, cam))
+            arr[k].sceneExtent = float(extent)
+            arr[k].params = params
+        self._views = (arr, views)  # keep the inputs alive
+        check(self.lib.arc_batch_capture(self.h, arr, len(views)), "arc_batch_capture", self.ctx.h)
+
+    def launch(self) -> None:
+        check(self.lib.arc_batch_launch(self.h), "arc_batch_launch", self.ctx.h)
+
+    def close(self) -> None:
+        if self.h:
+            self.lib.arc_batch_destroy(self.h)
+            self.h = None

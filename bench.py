@@ -65,10 +65,15 @@ def parse():
                     help="light cut of the radiance: the analytic error bound or kernel_bvh's solid angle")
     ap.add_argument("--error-bound", type=float, default=0.01, help="relative error bound of the cut")
     ap.add_argument("--per-tile", type=int, default=1, help="error-bound cut: one cut per 8x8 tile (1) or per pixel (0)")
-    ap.add_argument("--component", choices=("frame", "trace"), default="frame",
+    ap.add_argument("--component", choices=("frame", "trace", "batch"), default="frame",
                     help="frame: the gather frame (the headline metric); trace: the GPU AVPL shooter and "
-                         "G-buffer ray caster on the same scene (a side measurement)")
+                         "G-buffer ray caster on the same scene; batch: many views in one graph launch "
+                         "(side measurements)")
     ap.add_argument("--paths", type=int, default=250_000, help="trace: paths shot per step")
+    ap.add_argument("--views", type=int, default=64, help="batch: views per graph launch")
+    ap.add_argument("--lanes", type=int, default=4, help="batch: streams the views are dealt over")
+    ap.add_argument("--batch-seed", type=int, default=7, help="batch: seed of the view sizes")
+    ap.add_argument("--tune", type=int, default=1, help="batch: autotune each view's schedule (1) or not (0)")
     return ap.parse_args()
 
 
@@ -80,6 +85,8 @@ def main() -> int:
         return 2
     if args.component == "trace":
         return bench_trace(args)
+    if args.component == "batch":
+        return bench_batch(args)
     import torch
 
     from antiradiancecuts_amd import ffi
@@ -266,6 +273,82 @@ def bench_trace(args) -> int:
                        "gbuffer_rays_per_s": args.width * args.height / (gb_ms * 1e-3)}}
     print(json.dumps(line), flush=True)
     sc.close()
+    ctx.close()
+    return 0
+
+
+def bench_batch(args) -> int:
+    """Batched views in one HIP graph: --views per-view seeded variants of the
+    frame workload (Cornell walls + --tris random triangles), each with a
+    width and height drawn uniformly from [256, 1920] x [256, 1080] (rounded
+    to 8) and an AVPL count log-uniform over [16K, 1M].  Each view's
+    schedule (error-bound cut per tile or per pixel, estimate-cut depth) is
+    picked by the autotuner on the view itself; then all views are captured
+    into one graph over --lanes streams and a step is one hipGraphLaunch of
+    the whole batch.  Per-view ms are the same views run alone (HIP events,
+    median of 3)."""
+    import torch
+
+    from antiradiancecuts_amd import autotune, ffi
+    from antiradiancecuts_amd import scene as S
+    from antiradiancecuts_amd.runtime import Batch, Context, DeviceAvpls, Gather, default_params, device_array
+
+    rng = np.random.default_rng(args.batch_seed)
+    ctx = Context(0)
+    b = Batch(ctx, lanes=args.lanes)
+    base = default_params(useLightCuts=1, antiradianceModel=ffi.ARC_ANTIRAD_DISK,
+                          cutMode=ffi.ARC_CUT_ERROR_BOUND, errorBound=args.error_bound, cutPerTile=args.per_tile)
+    views, alone_ms, pairs, tuned = [], [], 0.0, {}
+    t0 = time.time()
+    for k in range(args.views):
+        w = int(rng.integers(256, 1921)) // 8 * 8
+        h = int(rng.integers(256, 1081)) // 8 * 8
+        n = int(round(math.exp(rng.uniform(math.log(16384), math.log(1_000_000)))))
+        sc = S.clutter_1080p(seed=1000 + k, width=w, height=h, num_avpls=n, num_tris=args.tris)
+        g = Gather(b.lane(k), w, h, sc.materials)
+        g.set_gbuffer(device_array(sc.pos4), device_array(sc.nrm4), device_array(sc.albedo4))
+        av = DeviceAvpls.from_scene(sc)
+        torch.cuda.synchronize()
+        p = base
+        if args.tune:
+            p, table = autotune.tune(g, av, sc.cam, sc.extent, base)
+            key = f"tile={p.cutPerTile},d0={p.cutEstimateDepth}"
+            tuned[key] = tuned.get(key, 0) + 1
+        ms = [g.run(av, sc.cam, sc.extent, p, ffi.ARC_RUN_PROFILE | ffi.ARC_TRUST_INPUT).msTotal for _ in range(3)]
+        alone_ms.append(float(np.median(ms)))
+        pairs += float(w * h) * n
+        views.append((g, av, sc.cam, sc.extent, p))
+        print(f"[batch] view {k}: {w}x{h}, {n} AVPLs, alone {alone_ms[-1]:.3f} ms", file=sys.stderr, flush=True)
+    t_setup = time.time() - t0
+    b.capture(views)
+    for _ in range(args.warmup):
+        b.launch()
+    torch.cuda.synchronize()
+    start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    start.record()
+    for _ in range(args.steps):
+        b.launch()
+    stop.record()
+    torch.cuda.synchronize()
+    ms_per_step = start.elapsed_time(stop) / args.steps
+    alone = np.array(alone_ms)
+    line = {"metric": "aggregate pixel-AVPL pairs/s of a batch of views in one graph launch (side measurement)",
+            "value": pairs / (ms_per_step * 1e-3), "unit": "pixel-AVPL pairs/s", "n_gpus": 1,
+            "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_per_step, 3),
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+            "config": {"workload": f"{args.views} views, W,H uniform [256,1920]x[256,1080], AVPLs log-uniform "
+                                   f"[16K,1M], Cornell walls + {args.tris} random triangles per view (seeded "
+                                   f"variants), error-bound cut {args.error_bound:g}, {args.lanes} lanes",
+                       "views": args.views, "lanes": args.lanes, "pairs_per_batch": pairs,
+                       "view_ms_alone_p50": round(float(np.percentile(alone, 50)), 4),
+                       "view_ms_alone_p99": round(float(np.percentile(alone, 99)), 4),
+                       "sum_view_ms_alone": round(float(alone.sum()), 3),
+                       "tuned_schedules": tuned if args.tune else None, "setup_s": round(t_setup, 1)}}
+    print(json.dumps(line), flush=True)
+    ctx.sync()
+    for g, *_ in views:
+        g.close()
+    b.close()
     ctx.close()
     return 0
 

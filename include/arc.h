@@ -284,6 +284,36 @@ int arc_gather_kernel_history(arc_gather* g, int frames, float* out);
 /* The AvplBvh of the last cut frame (borrowed; NULL before the first). */
 arc_bvh* arc_gather_get_bvh(arc_gather* g);
 
+/* The context a gather was created on. */
+arc_ctx* arc_gather_ctx(arc_gather* g);
+
+/* -------------------------------------------------------- batched views */
+/* Not in the reference (one view per CudaGather::run on the default stream):
+ * many views recorded once into one HIP graph, replayed with one launch.
+ * Views are dealt over `lanes` streams of the batch's own, forked from and
+ * joined back to ctx's stream, so views on different lanes run together. */
+typedef struct arc_batch arc_batch;
+
+typedef struct arc_batch_view {
+    arc_gather* gather;          /* created on a lane context (arc_batch_lane), G-buffer and
+                                    outputs bound                                              */
+    arc_avpls avpls;             /* device AVPLs of the view                                  */
+    arc_float3 camPos;
+    float sceneExtent;
+    arc_gather_params params;
+} arc_batch_view;
+
+int arc_batch_create(arc_ctx* ctx, int lanes, arc_batch** out);   /* lanes: 1..8 */
+/* The context of lane `lane`, to create that lane's gathers on. */
+arc_ctx* arc_batch_lane(arc_batch* b, int lane);
+/* Runs every view once (its buffers reach their sizes), then records all of
+ * them into one graph.  Synchronous.  The views' inputs and outputs must stay
+ * where they are while the graph is in use; their contents may change. */
+int arc_batch_capture(arc_batch* b, const arc_batch_view* views, int nviews);
+/* One graph launch on ctx's stream: a frame of every view.  Asynchronous. */
+int arc_batch_launch(arc_batch* b);
+void arc_batch_destroy(arc_batch* b);
+
 /* ------------------------------------------------------------ ray tracing */
 /* The triangle scene the AVPLs are shot into: Scene with its kd-tree
  * (Scene.cpp:175-255, KdTreeAccelerator.h:17-59).  Triangles follow Triangle

@@ -77,7 +77,7 @@ def _c_layout(struct: str, fields):
     return [int(v) for v in out]
 
 
-@pytest.mark.parametrize("struct", ["arc_gather_params", "arc_gather_stats", "arc_image_error_t", "arc_avpls", "arc_bvh_param",
+@pytest.mark.parametrize("struct", ["arc_gather_params", "arc_gather_stats", "arc_image_error_t", "arc_batch_view", "arc_area_light", "arc_avpls", "arc_bvh_param",
                                     "arc_bvh_data", "arc_avpl_node_data", "arc_bvh_node", "arc_material"])
 def test_ctypes_layout_equals_c(struct):
     ct = getattr(ffi, struct)
