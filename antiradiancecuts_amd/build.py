@@ -38,6 +38,7 @@ HIP_SOURCES = {
     "output.hip": [],
     "trace.hip": ["-ffp-contract=off"],
     "batch.hip": [],
+    "cutnet.hip": [],
 }
 
 HIPCC_FLAGS = [

@@ -21,6 +21,7 @@
 #include <cstring>
 #include <vector>
 
+#include "cutnet.hpp"
 #include "gather_common.hpp"
 
 using arc::DevBuf;
@@ -456,6 +457,7 @@ void arc_gather_params_default(arc_gather_params* p) {
     p->cutPerTile = 0;
     p->cutCriterion = ARC_CRITERION_ESTIMATE;
     p->cutImageFloor = 1.0f;
+    p->cutConfidence = 0.9f;
 }
 
 int arc_gather_create(arc_ctx* ctx, int width, int height, const arc_material* materials, int num_materials,
@@ -576,10 +578,16 @@ int arc_gather_run(arc_gather* g, const arc_avpls* avpls, arc_float3 camPos, flo
 
     const bool cut = p->useLightCuts && n > 1;
     const bool clustered = p->antiradianceMode == ARC_ANTI_CLUSTERED && n > 0;
-    if (p->useLightCuts && p->cutMode != ARC_CUT_SOLID_ANGLE && p->cutMode != ARC_CUT_ERROR_BOUND)
+    if (p->useLightCuts && p->cutMode != ARC_CUT_SOLID_ANGLE && p->cutMode != ARC_CUT_ERROR_BOUND &&
+        p->cutMode != ARC_CUT_LEARNED)
         return arc::fail(ctx, ARC_E_INVALID, "cutMode");
-    if (p->useLightCuts && p->cutMode == ARC_CUT_ERROR_BOUND && g->general)
-        return arc::fail(ctx, ARC_E_UNSUPPORTED, "the error-bound cut needs Lambertian materials");
+    const bool moments = p->cutMode != ARC_CUT_SOLID_ANGLE;
+    if (p->useLightCuts && moments && g->general)
+        return arc::fail(ctx, ARC_E_UNSUPPORTED, "the error-bound and learned cuts need Lambertian materials");
+    if (p->useLightCuts && p->cutMode == ARC_CUT_LEARNED && !(g->net && g->net->ready))
+        return arc::fail(ctx, ARC_E_STATE, "learned cut without a trained arc_cutnet (arc_gather_set_cutnet)");
+    if (p->useLightCuts && p->cutMode == ARC_CUT_LEARNED && !(p->cutConfidence >= 0.5f && p->cutConfidence <= 1.f))
+        return arc::fail(ctx, ARC_E_INVALID, "cutConfidence");
     if (p->cutEstimateDepth < 0 || p->cutEstimateDepth > 40) return arc::fail(ctx, ARC_E_INVALID, "cutEstimateDepth");
     if (!(p->cutImageFloor >= 0.f)) return arc::fail(ctx, ARC_E_INVALID, "cutImageFloor");
     if (p->cutCriterion != ARC_CRITERION_BOUND && p->cutCriterion != ARC_CRITERION_ESTIMATE)
@@ -590,13 +598,16 @@ int arc_gather_run(arc_gather* g, const arc_avpls* avpls, arc_float3 camPos, flo
     if (cut) {
         // AvplBvh(avpls, false) per frame (cudaGather.cu:858)
         ARC_TRY(arc::bvh_build_avpls(g->bvh, a, nullptr, p->seed, (flags & ARC_TRUST_INPUT) == 0,
-                                     g->mats.as<arc_material>(), p->cutMode == ARC_CUT_ERROR_BOUND));
+                                     g->mats.as<arc_material>(), moments));
         if (profile) ARC_HIP(ctx, hipEventRecord(ev[2], ctx->stream));
         arc_bvh* b = g->bvh;
-        if (p->cutMode == ARC_CUT_ERROR_BOUND) {
+        if (moments) {
             const arc::CutEBParams cp{p->errorBound, p->cutEstimateDepth, 1e-3f * sceneExtent, p->cutBudget,
-                                          p->cutPerTile, p->cutCriterion, p->cutImageFloor};
-            ARC_TRY(arc::cut_error_bound(g, b, cp, rad, counters, profile ? ev[6] : nullptr));
+                                      p->cutPerTile, p->cutCriterion, p->cutImageFloor, p->cutConfidence};
+            if (p->cutMode == ARC_CUT_LEARNED)
+                ARC_TRY(arc::cut_learned(g, b, cp, rad, counters, profile ? ev[6] : nullptr));
+            else
+                ARC_TRY(arc::cut_error_bound(g, b, cp, rad, counters, profile ? ev[6] : nullptr));
         } else {
             if (profile) ARC_HIP(ctx, hipEventRecord(ev[6], ctx->stream));
             CutTree t{b->nodes.as<arc_bvh_node>(), b->nd_mat.as<int>(), b->nd_pos.as<f3>(), b->nd_nrm.as<f3>(),
@@ -653,6 +664,7 @@ int arc_gather_run(arc_gather* g, const arc_avpls* avpls, arc_float3 camPos, flo
             stats->antiEvaluatedNodes = (long long)c[C_ANTI_EVALUATED];
             stats->errorPixels = (long long)c[C_ERRORS];
             stats->budgetHits = (long long)c[C_BUDGET];
+            stats->fallbackNodes = (long long)c[C_FALLBACK];
         }
         if (flags & ARC_RUN_PROFILE) {
             ARC_HIP(ctx, hipEventSynchronize(ev[5]));
@@ -701,6 +713,42 @@ int arc_gather_kernel_history(arc_gather* g, int frames, float* out) {
         ARC_HIP(ctx, hipEventElapsedTime(&out[2 * k + 1], ev[7], ev[4]));
     }
     return n;
+}
+
+int arc_gather_set_cutnet(arc_gather* g, arc_cutnet* net) {
+    if (!g) return ARC_E_INVALID;
+    if (net && net->ctx->device != g->ctx->device)
+        return arc::fail(g->ctx, ARC_E_INVALID, "cutnet on another device");
+    g->net = net;
+    return ARC_OK;
+}
+
+int arc_gather_cut_samples(arc_gather* g, const arc_avpls* avpls, arc_float3 camPos, float sceneExtent,
+                           const arc_gather_params* p, int flags, float* X, float* y, int capacity, uint64_t seed,
+                           float keep, int* count) {
+    (void)camPos;
+    ARC_TRY(validate_params(g, avpls, p));
+    arc_ctx* ctx = g->ctx;
+    if (!X || !y || capacity < 0 || !count || !(keep > 0.f && keep <= 1.f)) return ARC_E_INVALID;
+    if (g->general) return arc::fail(ctx, ARC_E_UNSUPPORTED, "the learned cut needs Lambertian materials");
+    if (p->cutEstimateDepth < 0 || p->cutEstimateDepth > 40) return arc::fail(ctx, ARC_E_INVALID, "cutEstimateDepth");
+    if (!(p->cutImageFloor >= 0.f)) return arc::fail(ctx, ARC_E_INVALID, "cutImageFloor");
+    *count = 0;
+    const int n = avpls->numAvpls;
+    if (n < 2) return ARC_OK;
+    arc::AvplsDev a{};
+    ARC_TRY(stage_avpls(g, avpls, flags, a));
+    ARC_TRY(arc::bvh_build_avpls(g->bvh, a, nullptr, p->seed, (flags & ARC_TRUST_INPUT) == 0,
+                                 g->mats.as<arc_material>(), true));
+    ARC_TRY(arc::ensure(ctx, g->counters, sizeof(unsigned long long) * C_NUM));
+    int* dcount = reinterpret_cast<int*>(g->counters.p);
+    ARC_HIP(ctx, hipMemsetAsync(dcount, 0, sizeof(int), ctx->stream));
+    const arc::CutEBParams cp{p->errorBound, p->cutEstimateDepth, 1e-3f * sceneExtent, p->cutBudget,
+                              1, p->cutCriterion, p->cutImageFloor, p->cutConfidence};
+    ARC_TRY(arc::cut_samples(g, g->bvh, cp, X, y, capacity, seed, keep, dcount));
+    ARC_HIP(ctx, hipMemcpyAsync(count, dcount, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+    ARC_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return ARC_OK;
 }
 
 int arc_gather_exact_f64(arc_gather* g, const arc_avpls* avpls, arc_float3 camPos, float sceneExtent,

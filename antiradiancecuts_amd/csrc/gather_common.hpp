@@ -30,6 +30,7 @@ struct arc_gather {
     arc::DevBuf anti_idx, cbox, ccount, clist;
     arc::DevBuf crec;  // error-bound cut: packed node records
     arc::DevBuf cest;  // error-bound cut: per-pixel estimates and their sum
+    arc_cutnet* net = nullptr;  // learned cut: the predictor (borrowed)
     arc_bvh* bvh = nullptr;
     // per-frame phase events: frame slot, then 8 marks (start, upload done,
     // bvh done, radiance done, antiradiance done, end, start of the main
@@ -67,7 +68,10 @@ constexpr int kRun = 64;   // pairs summed plainly before a compensated add
 // in FP64.
 constexpr float kGuard = 1e-5f;
 
-enum Counter { C_VISITED = 0, C_EVALUATED = 1, C_ANTI_VISITED = 2, C_ANTI_EVALUATED = 3, C_ERRORS = 4, C_BUDGET = 5, C_NUM = 8 };
+enum Counter {
+    C_VISITED = 0, C_EVALUATED = 1, C_ANTI_VISITED = 2, C_ANTI_EVALUATED = 3, C_ERRORS = 4, C_BUDGET = 5,
+    C_FALLBACK = 6, C_NUM = 8
+};
 
 struct Kahan {
     float s = 0.f, c = 0.f;
@@ -224,10 +228,18 @@ struct CutEBParams {
     int per_tile;       // cutPerTile
     int criterion;      // cutCriterion
     float image_floor;  // cutImageFloor
+    float confidence;   // cutConfidence (learned cut)
 };
 // `mark` (may be null) is recorded right before the main traversal kernel.
 int cut_error_bound(arc_gather* g, arc_bvh* b, const CutEBParams& cp, float4* rad, unsigned long long* counters,
                     hipEvent_t mark);
+// The learned cut: per tile, refine / keep from g->net, the error-bound
+// criterion where the prediction is not confident.
+int cut_learned(arc_gather* g, arc_bvh* b, const CutEBParams& cp, float4* rad, unsigned long long* counters,
+                hipEvent_t mark);
+// Training rows with exact labels (arc_gather_cut_samples); count: one device int.
+int cut_samples(arc_gather* g, arc_bvh* b, const CutEBParams& cp, float* X, float* y, int capacity, uint64_t seed,
+                float keep, int* count);
 
 // Clustered antiradiance (gather_anti.hip).
 // `mark` (may be null) is recorded right before the per-pixel kernel.

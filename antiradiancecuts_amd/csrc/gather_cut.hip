@@ -46,11 +46,13 @@
 //                 is the union of 64 different near fields.
 #include <cmath>
 
+#include "cutnet.hpp"
 #include "gather_common.hpp"
 
 using arc::f3;
 using namespace arc::gm;
 using namespace arc::gd;
+namespace net = arc::net;
 
 namespace {
 
@@ -68,7 +70,7 @@ struct CutEB {
 
 // Node records, one 64-byte line per slot (leaves, then internal nodes):
 //   r0 = (centroid, max(I)), r1 = (m, (1 - |m|) / 4), r2 = (I, 1.2 (1 - |m|) / sqrt(count)),
-//   r3 = (s^2, s, -, -) with s the half-diagonal of the node's box (0 for a leaf).
+//   r3 = (s^2, s, count, -) with s the half-diagonal of the node's box (0 for a leaf).
 // Everything of the tests that does not depend on the pixel is here.
 __global__ __launch_bounds__(kCutThreads) void k_pack_cut(int n, const f3* __restrict__ pos,
                                                           const f3* __restrict__ nrm,
@@ -106,7 +108,7 @@ __global__ __launch_bounds__(kCutThreads) void k_pack_cut(int n, const f3* __res
     o[0] = make_float4(c.x, c.y, c.z, fmaxf(I.x, fmaxf(I.y, I.z)));
     o[1] = make_float4(m.x, m.y, m.z, 0.25f * (1.f - ml));
     o[2] = make_float4(I.x, I.y, I.z, spread);
-    o[3] = make_float4(s2, sqrtf(s2), 0.f, 0.f);
+    o[3] = make_float4(s2, sqrtf(s2), (float)size[s], 0.f);
 }
 
 // G~(p, node) (cudaGather.cu:45-57 with the node's moments); for a leaf the
@@ -379,12 +381,325 @@ __global__ __launch_bounds__(kCutThreads) void k_cut_eb_tile(CutEB t, const floa
                                  counters ? (float)nev : 1.f);
 }
 
+// ------------------------------------------------------- learned cut --
+// The learned cut decides per (8x8 tile, node) from features seen at the
+// tile's centre; the same feature function serves the labelled traversal
+// that makes training rows and the inference traversal.
+
+constexpr int kLearnStack = 1024;  // per-wave stack: a batch pops 16 nodes and may push 32
+constexpr int kLearnBatch = 16;   // nodes decided together: one MFMA sample tile
+
+__device__ inline float wave_sum(float v) {
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+    return v;
+}
+__device__ inline float wave_max(float v) {
+    for (int off = 32; off > 0; off >>= 1) v = fmaxf(v, __shfl_xor(v, off));
+    return v;
+}
+__device__ inline float wave_min(float v) {
+    for (int off = 32; off > 0; off >>= 1) v = fminf(v, __shfl_xor(v, off));
+    return v;
+}
+
+// Wave-uniform statistics of a tile's live pixels.
+struct TileStats {
+    PixelF c;         // centre: mean position, normalised mean normal
+    float nspread;    // 1 - |mean normal|
+    float radius;     // largest distance of a pixel from the centre
+    float thr_min, thr_max;
+    float live_frac;
+};
+
+__device__ inline TileStats tile_stats(const PixelF& P, bool live, float thr) {
+    TileStats T;
+    const float cnt = wave_sum(live ? 1.f : 0.f);
+    T.live_frac = cnt * (1.f / 64.f);
+    const float ic = cnt > 0.f ? 1.f / cnt : 0.f;
+    T.c.px = wave_sum(live ? P.px : 0.f) * ic;
+    T.c.py = wave_sum(live ? P.py : 0.f) * ic;
+    T.c.pz = wave_sum(live ? P.pz : 0.f) * ic;
+    const float mx = wave_sum(live ? P.nx : 0.f) * ic, my = wave_sum(live ? P.ny : 0.f) * ic,
+                mz = wave_sum(live ? P.nz : 0.f) * ic;
+    const float ml = sqrtf(mx * mx + my * my + mz * mz);
+    T.nspread = 1.f - fminf(ml, 1.f);
+    const float im = ml > 0.f ? 1.f / ml : 0.f;
+    T.c.nx = mx * im;
+    T.c.ny = my * im;
+    T.c.nz = ml > 0.f ? mz * im : 1.f;
+    T.c.br = T.c.bg = T.c.bb = 0.f;
+    T.c.mat = 0;
+    const float dx = P.px - T.c.px, dy = P.py - T.c.py, dz = P.pz - T.c.pz;
+    T.radius = wave_max(live ? sqrtf(dx * dx + dy * dy + dz * dz) : 0.f);
+    T.thr_min = wave_min(live ? thr : INFINITY);
+    T.thr_max = wave_max(live ? thr : 0.f);
+    return T;
+}
+
+__device__ inline float lg2(float x) { return __builtin_amdgcn_logf(fmaxf(x, 1e-30f)); }
+__device__ inline float clampf(float x, float lo, float hi) { return fminf(fmaxf(x, lo), hi); }
+
+// The 24 features of (tile, internal node).  Scale-free: distances enter as
+// ratios, intensities relative to the tile's most demanding threshold.
+__device__ inline void cut_features(const TileStats& T, const arc_bvh_node& b, const float4& r0, const float4& r1,
+                                    const float4& r2, const float4& r3, float eps, float delta,
+                                    float f[net::kIn]) {
+    const PixelF& C = T.c;
+    const float dx = r0.x - C.px, dy = r0.y - C.py, dz = r0.z - C.pz;
+    const float dc2 = fmaxf(dx * dx + dy * dy + dz * dz, 1e-30f);
+    const float inv = __builtin_amdgcn_rsqf(dc2), dc = dc2 * inv;
+    const float ux = dx * inv, uy = dy * inv, uz = dz * inv;
+    const float c1 = C.nx * ux + C.ny * uy + C.nz * uz;
+    const float c2 = -(r1.x * ux + r1.y * uy + r1.z * uz);
+    const float hmax = C.nx * (C.nx > 0.f ? b.bbMax.x - C.px : b.bbMin.x - C.px) +
+                       C.ny * (C.ny > 0.f ? b.bbMax.y - C.py : b.bbMin.y - C.py) +
+                       C.nz * (C.nz > 0.f ? b.bbMax.z - C.pz : b.bbMin.z - C.pz);
+    const float est = r0.w * geom_est(C, r0, r1);
+    const float err = node_error(C, b, r0, r2, r3, delta);
+    const float bnd = node_bound(C, b, r0.w, delta * delta);
+    const float itm = 1.f / fmaxf(T.thr_min, 1e-30f);
+    const float ex = b.bbMax.x - b.bbMin.x, ey = b.bbMax.y - b.bbMin.y, ez = b.bbMax.z - b.bbMin.z;
+    f[0] = fminf(r3.y * inv, 8.f);
+    f[1] = fminf(T.radius * inv, 8.f);
+    f[2] = c1;
+    f[3] = c2;
+    f[4] = 1.f - 4.f * r1.w;
+    f[5] = T.nspread;
+    f[6] = lg2(r3.z) * (1.f / 16.f);
+    f[7] = clampf(lg2(est * itm) * (1.f / 16.f), -4.f, 4.f);
+    f[8] = err == INFINITY ? 4.f : clampf(lg2(err * itm) * (1.f / 16.f), -4.f, 4.f);
+    f[9] = clampf(lg2(bnd * itm) * (1.f / 16.f), -4.f, 4.f);
+    f[10] = clampf(hmax * inv, -4.f, 4.f);
+    f[11] = clampf(lg2(dc2 / (4.f * r3.x + 1e-30f)) * 0.125f, -4.f, 4.f);
+    f[12] = r2.w;
+    f[13] = lg2(T.thr_max * itm) * 0.125f;
+    f[14] = lg2(eps) * 0.125f;
+    f[15] = T.live_frac;
+    f[16] = clampf(lg2(dc / delta) * (1.f / 16.f), -4.f, 4.f);
+    f[17] = C.nx * r1.x + C.ny * r1.y + C.nz * r1.z;
+    f[18] = (r2.x + r2.y + r2.z) / (3.f * r0.w + 1e-30f);
+    f[19] = fmaxf(ex, fmaxf(ey, ez)) / (2.f * r3.y + 1e-30f);
+    f[20] = clampf(c1, 0.f, 1.f);
+    f[21] = fminf(1.f, r1.w + fmaxf(c2, 0.f));
+    f[22] = hmax > 0.f ? fminf(1.f, hmax / fmaxf(dc - r3.y, delta)) : 0.f;
+    f[23] = dc2 <= 4.f * r3.x ? 1.f : 0.f;
+}
+
+__device__ inline uint32_t mix32(uint64_t x) {
+    x ^= x >> 33;
+    x *= 0xff51afd7ed558ccdull;
+    x ^= x >> 33;
+    x *= 0xc4ceb9fe1a85ec53ull;
+    x ^= x >> 33;
+    return (uint32_t)x;
+}
+
+struct LabelOut {
+    float* X;
+    float* y;
+    int* count;
+    int capacity;
+    uint64_t seed;
+    uint32_t keep;  // keep a row when its hash is below this
+};
+
+// The labelled traversal: one depth-first cut per tile whose refine
+// decisions are the labels.  A node's label is "refine" when some live pixel
+// of the tile sees the node's estimate I * G~ off the exact sum of the
+// node's leaves by more than the pixel's threshold (largest channel).
+__global__ __launch_bounds__(kCutThreads) void k_cut_label_tile(CutEB t, const int* __restrict__ ranges,
+                                                                const float4* __restrict__ gpos,
+                                                                const float4* __restrict__ gnrm,
+                                                                const float4* __restrict__ galb,
+                                                                const arc_material* __restrict__ mats, int W, int H,
+                                                                Threshold th, float eps, float delta, LabelOut o) {
+    __shared__ int stk[kCutThreads / 64][kTileStack];
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const int gx = (W + 15) / 16, gy = (H + 15) / 16;
+    const int2 xy = tile_pixel(wave, lane, gx, gy);
+    const bool live = xy.x < W && xy.y < H;
+    const int q = live ? xy.y * W + xy.x : 0;
+    const PixelF P = load_pixel(gpos, gnrm, galb, mats, q);
+    const float thr = live ? th.at(q) : INFINITY;
+    const TileStats T = tile_stats(P, live, thr);
+    if (T.live_frac == 0.f || t.n < 2) return;  // wave-uniform
+    const uint64_t tile = (uint64_t)xcd_block(blockIdx.x, gx * gy) * 4 + wave;
+    int* S = stk[wave];
+    const int n = t.n;
+    int sp = 0;
+    int k = 0;  // internal node
+    for (;;) {
+        const arc_bvh_node& b = t.nodes[k];
+        const float4* r = t.crec + kCrec * (size_t)(n + k);
+        const float4 r0 = r[0], r1 = r[1], r2 = r[2], r3 = r[3];
+        float e0 = 0.f, e1 = 0.f, e2 = 0.f;
+        const int first = uniform(ranges[2 * k]), last = uniform(ranges[2 * k + 1]);
+        for (int s = first; s <= last; ++s) {
+            const float4* lr = t.crec + kCrec * (size_t)s;
+            const float ge = geom_est(P, lr[0], lr[1]);
+            const float4 li = lr[2];
+            e0 += li.x * ge;
+            e1 += li.y * ge;
+            e2 += li.z * ge;
+        }
+        const float ge = geom_est(P, r0, r1);
+        const float err = fmaxf(fabsf(e0 - r2.x * ge), fmaxf(fabsf(e1 - r2.y * ge), fabsf(e2 - r2.z * ge)));
+        const bool refine = __ballot(live && err > thr) != 0;
+        if (mix32(o.seed ^ (tile * 0x9E3779B97F4A7C15ull) ^ ((uint64_t)k << 1)) < o.keep) {
+            float f[net::kIn];
+            cut_features(T, b, r0, r1, r2, r3, eps, delta, f);
+            if (lane == 0) {
+                const int row = atomicAdd(o.count, 1);
+                if (row < o.capacity) {
+                    float* x = o.X + (size_t)row * net::kIn;
+#pragma unroll
+                    for (int j = 0; j < net::kIn; ++j) x[j] = f[j];
+                    o.y[row] = refine ? 1.f : 0.f;
+                }
+            }
+        }
+        int next = 0;
+        bool have = false;
+        if (refine) {
+            if (b.right < 0 && sp < kTileStack) S[sp++] = -(b.right + 1);
+            if (b.left < 0) {
+                next = -(b.left + 1);
+                have = true;
+            }
+        }
+        if (!have) {
+            if (sp == 0) break;
+            next = S[--sp];
+        }
+        k = uniform(next);
+    }
+}
+
+// The learned traversal: per tile, up to 16 nodes popped together are
+// decided by one MFMA pass of the network (all 64 lanes take part; lane
+// i % 16 computes node i's features); a prediction p is taken when
+// p >= conf (refine) or p <= 1 - conf (keep), and the error-bound criterion
+// over the tile's pixels decides the rest.  Kept nodes and leaves add their
+// estimate to every pixel.
+__global__ __launch_bounds__(kCutThreads) void k_cut_learned_tile(CutEB t, const float4* __restrict__ gpos,
+                                                                  const float4* __restrict__ gnrm,
+                                                                  const float4* __restrict__ galb,
+                                                                  const arc_material* __restrict__ mats, int W,
+                                                                  int H, Threshold th, float eps, float delta,
+                                                                  int budget, float conf,
+                                                                  const float* __restrict__ params,
+                                                                  const net::bf4* __restrict__ frag,
+                                                                  float4* __restrict__ out_rad,
+                                                                  unsigned long long* __restrict__ counters) {
+    __shared__ int stk[kCutThreads / 64][kLearnStack];
+    __shared__ float sb[3 * net::kHid + 1];
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    for (int j = tid; j < net::kHid; j += kCutThreads) {
+        sb[j] = params[net::kB1 + j];
+        sb[net::kHid + j] = params[net::kB2 + j];
+        sb[2 * net::kHid + j] = params[net::kW3 + j];
+    }
+    if (tid == 0) sb[3 * net::kHid] = params[net::kB3];
+    __syncthreads();
+    const int2 xy = tile_pixel(wave, lane, (W + 15) / 16, (H + 15) / 16);
+    const bool live = xy.x < W && xy.y < H;
+    const int q = live ? xy.y * W + xy.x : 0;
+    const PixelF P = load_pixel(gpos, gnrm, galb, mats, q);
+    const float thr = live ? th.at(q) : INFINITY;
+    const TileStats T = tile_stats(P, live, thr);
+    net::Frags F;
+    net::load_frags(frag, lane, F);
+    const int i = lane & 15, g = lane >> 4;
+    const int n = t.n;
+    int* S = stk[wave];
+    Acc acc;
+    unsigned long long nvis = 0, nev = 0, nerr = 0, nbud = 0, nfb = 0;
+    int sp = 0;
+    if (T.live_frac > 0.f) S[sp++] = n >= 2 ? -1 : 0;
+    while (sp > 0) {
+        const int cnt = min(kLearnBatch, sp);
+        sp -= cnt;
+        const int mine = S[sp + min(i, cnt - 1)];
+        const bool have = i < cnt;
+        const bool internal = have && mine < 0;
+        float f[net::kIn];
+        if (internal) {
+            const int k = -(mine + 1);
+            const float4* r = t.crec + kCrec * (size_t)(n + k);
+            cut_features(T, t.nodes[k], r[0], r[1], r[2], r[3], eps, delta, f);
+        } else {
+#pragma unroll
+            for (int j = 0; j < net::kIn; ++j) f[j] = 0.f;
+        }
+        net::bf4 x0, x1;
+        net::input_frags(f, params, g, x0, x1);
+        const float z = net::forward(F, sb, x0, x1, g);
+        const float pr = 1.f / (1.f + __expf(-z));
+        const bool head = lane < kLearnBatch;
+        uint64_t R = __ballot(head && internal && pr >= conf);
+        uint64_t A = __ballot(head && have && (!internal || pr <= 1.f - conf));
+        uint64_t U = __ballot(head && internal && pr < conf && pr > 1.f - conf);
+        while (U) {  // not confident: the error-bound criterion over the tile's pixels
+            const int j = __builtin_ctzll(U);
+            U &= U - 1;
+            const int k = -(__builtin_amdgcn_readlane(mine, j) + 1);
+            const float4* r = t.crec + kCrec * (size_t)(n + k);
+            const float crit = node_error(P, t.nodes[k], r[0], r[2], r[3], delta);
+            if (__ballot(crit > thr)) R |= 1ull << j;
+            else A |= 1ull << j;
+            ++nfb;
+        }
+        if (R && (int)nev >= budget) {  // cut budget spent: keep the clusters
+            nbud += __popcll(R);
+            A |= R;
+            R = 0;
+        }
+        while (R && sp + 2 * __popcll(R) > kLearnStack) {  // stack full: keep the last ones
+            const uint64_t top = 1ull << (63 - __builtin_clzll(R));
+            R &= ~top;
+            A |= top;
+            ++nerr;
+        }
+        if (head && ((R >> lane) & 1)) {
+            const arc_bvh_node& b = t.nodes[-(mine + 1)];
+            const int pos = sp + 2 * __popcll(R & ((1ull << lane) - 1));
+            S[pos] = b.right;
+            S[pos + 1] = b.left;
+        }
+        sp += 2 * __popcll(R);
+        nvis += __popcll(R);
+        while (A) {
+            const int j = __builtin_ctzll(A);
+            A &= A - 1;
+            const int node = __builtin_amdgcn_readlane(mine, j);
+            const int slot = node >= 0 ? node : n + (-(node + 1));
+            const float4* r = t.crec + kCrec * (size_t)slot;
+            acc.add(r[2], geom_est(P, r[0], r[1]));
+            ++nev;
+        }
+    }
+    acc.flush();
+    const int alive = __popcll(__ballot(live));
+    if (counters && lane == 0) {
+        atomicAdd(&counters[C_VISITED], nvis * alive);
+        atomicAdd(&counters[C_EVALUATED], nev * alive);
+        atomicAdd(&counters[C_ERRORS], nerr);
+        atomicAdd(&counters[C_BUDGET], nbud);
+        atomicAdd(&counters[C_FALLBACK], nfb);
+    }
+    if (live)
+        out_rad[q] = make_float4(acc.k[0].s * P.br, acc.k[1].s * P.bg, acc.k[2].s * P.bb,
+                                 counters ? (float)nev : 1.f);
+}
+
 }  // namespace
 
 namespace arc {
 
-int cut_error_bound(arc_gather* g, arc_bvh* b, const CutEBParams& cp, float4* rad, unsigned long long* counters,
-                    hipEvent_t mark) {
+namespace {
+
+// Node records, per-pixel estimates of the depth-d0 cut and the thresholds.
+int prepare_cut(arc_gather* g, arc_bvh* b, const CutEBParams& cp, CutEB& t, Threshold& th) {
     arc_ctx* ctx = g->ctx;
     const int n = b->n;
     const int npix = g->width * g->height;
@@ -398,12 +713,24 @@ int cut_error_bound(arc_gather* g, arc_bvh* b, const CutEBParams& cp, float4* ra
                        ctx->stream, n, b->nd_pos.as<f3>(), b->nd_nrm.as<f3>(), b->nd_radb.as<f3>(),
                        b->nd_mom.as<float4>(), b->nd_nmom.as<f3>(), b->nd_size.as<int>(), b->nodes.as<arc_bvh_node>(),
                        g->crec.as<float4>());
-    CutEB t{b->nodes.as<arc_bvh_node>(), g->crec.as<float4>(), n};
+    t = CutEB{b->nodes.as<arc_bvh_node>(), g->crec.as<float4>(), n};
     const dim3 grid(((g->width + 15) / 16) * ((g->height + 15) / 16));
     hipLaunchKernelGGL(k_cut_zero, dim3(1), dim3(64), 0, ctx->stream, sum);
     hipLaunchKernelGGL(k_cut_estimate, grid, dim3(kCutThreads), 0, ctx->stream, t, g->gpos, g->gnrm, g->galb,
                        g->mats.as<arc_material>(), g->width, g->height, cp.d0, est, sum);
-    const Threshold th{est, sum, cp.eps, cp.image_floor, 1.0f / (float)npix};
+    th = Threshold{est, sum, cp.eps, cp.image_floor, 1.0f / (float)npix};
+    return ARC_OK;
+}
+
+}  // namespace
+
+int cut_error_bound(arc_gather* g, arc_bvh* b, const CutEBParams& cp, float4* rad, unsigned long long* counters,
+                    hipEvent_t mark) {
+    arc_ctx* ctx = g->ctx;
+    CutEB t{};
+    Threshold th{};
+    ARC_TRY(prepare_cut(g, b, cp, t, th));
+    const dim3 grid(((g->width + 15) / 16) * ((g->height + 15) / 16));
     if (mark) ARC_HIP(ctx, hipEventRecord(mark, ctx->stream));
     if (cp.per_tile)
         hipLaunchKernelGGL(k_cut_eb_tile, grid, dim3(kCutThreads), 0, ctx->stream, t, g->gpos, g->gnrm, g->galb,
@@ -413,6 +740,37 @@ int cut_error_bound(arc_gather* g, arc_bvh* b, const CutEBParams& cp, float4* ra
         hipLaunchKernelGGL(k_cut_eb_px, grid, dim3(kCutThreads), 0, ctx->stream, t, g->gpos, g->gnrm, g->galb,
                            g->mats.as<arc_material>(), g->width, g->height, th, cp.delta, cp.budget, cp.criterion,
                            rad, counters);
+    ARC_LAUNCHED(ctx);
+    return ARC_OK;
+}
+
+int cut_learned(arc_gather* g, arc_bvh* b, const CutEBParams& cp, float4* rad, unsigned long long* counters,
+                hipEvent_t mark) {
+    arc_ctx* ctx = g->ctx;
+    if (!g->net || !g->net->ready) return fail(ctx, ARC_E_STATE, "learned cut without a trained arc_cutnet");
+    CutEB t{};
+    Threshold th{};
+    ARC_TRY(prepare_cut(g, b, cp, t, th));
+    const dim3 grid(((g->width + 15) / 16) * ((g->height + 15) / 16));
+    if (mark) ARC_HIP(ctx, hipEventRecord(mark, ctx->stream));
+    hipLaunchKernelGGL(k_cut_learned_tile, grid, dim3(kCutThreads), 0, ctx->stream, t, g->gpos, g->gnrm, g->galb,
+                       g->mats.as<arc_material>(), g->width, g->height, th, cp.eps, cp.delta, cp.budget,
+                       cp.confidence, g->net->params.as<float>(), g->net->frag.as<net::bf4>(), rad, counters);
+    ARC_LAUNCHED(ctx);
+    return ARC_OK;
+}
+
+int cut_samples(arc_gather* g, arc_bvh* b, const CutEBParams& cp, float* X, float* y, int capacity, uint64_t seed,
+                float keep, int* count) {
+    arc_ctx* ctx = g->ctx;
+    CutEB t{};
+    Threshold th{};
+    ARC_TRY(prepare_cut(g, b, cp, t, th));
+    const dim3 grid(((g->width + 15) / 16) * ((g->height + 15) / 16));
+    const double kd = std::min(1.0, (double)keep) * 4294967296.0;
+    const LabelOut o{X, y, count, capacity, seed, kd >= 4294967295.0 ? 0xffffffffu : (uint32_t)kd};
+    hipLaunchKernelGGL(k_cut_label_tile, grid, dim3(kCutThreads), 0, ctx->stream, t, b->ranges.as<int>(), g->gpos,
+                       g->gnrm, g->galb, g->mats.as<arc_material>(), g->width, g->height, th, cp.eps, cp.delta, o);
     ARC_LAUNCHED(ctx);
     return ARC_OK;
 }

@@ -27,6 +27,9 @@ ARC_INPUT_HOST = 256
 ARC_TRUST_INPUT = 512
 ARC_CUT_SOLID_ANGLE = 0
 ARC_CUT_ERROR_BOUND = 1
+ARC_CUT_LEARNED = 2
+ARC_CUTNET_FEATURES = 24
+ARC_CUTNET_PARAMS = 6400
 ARC_CRITERION_BOUND = 0
 ARC_SHOOT_NO_EXPLICIT_DIRECT = 1
 ARC_CRITERION_ESTIMATE = 1
@@ -82,7 +85,7 @@ class arc_gather_params(C.Structure):
                 ("photonRadiusScale", C.c_float), ("coneHalfAngle", C.c_float),
                 ("writeSentinel", C.c_int), ("stackSize", C.c_int), ("seed", C.c_uint64),
                 ("cutEstimateDepth", C.c_int), ("cutBudget", C.c_int), ("cutPerTile", C.c_int),
-                ("cutCriterion", C.c_int), ("cutImageFloor", C.c_float)]
+                ("cutCriterion", C.c_int), ("cutImageFloor", C.c_float), ("cutConfidence", C.c_float)]
 
 
 class arc_gather_stats(C.Structure):
@@ -90,7 +93,8 @@ class arc_gather_stats(C.Structure):
                 ("msAntiradiance", C.c_float), ("msCombine", C.c_float), ("msTotal", C.c_float),
                 ("visitedNodes", C.c_longlong), ("evaluatedNodes", C.c_longlong),
                 ("antiVisitedNodes", C.c_longlong), ("antiEvaluatedNodes", C.c_longlong),
-                ("errorPixels", C.c_longlong), ("numLeafs", C.c_int), ("budgetHits", C.c_longlong)]
+                ("errorPixels", C.c_longlong), ("numLeafs", C.c_int), ("budgetHits", C.c_longlong),
+                ("fallbackNodes", C.c_longlong)]
 
 
 class arc_scene_desc(C.Structure):
@@ -101,8 +105,6 @@ class arc_scene_desc(C.Structure):
                 ("tri_area_lo", C.c_float), ("tri_area_hi", C.c_float)]
 
 
-P = C.POINTER
-# (name, restype, argtypes) for every symbol of include/arc.h
 class arc_image_error_t(C.Structure):
     _fields_ = [("referenceError", C.c_float), ("rmse", C.c_float * 3), ("relL2", C.c_double),
                 ("maxAbs", C.c_double)]
@@ -118,6 +120,8 @@ class arc_batch_view(C.Structure):
                 ("params", arc_gather_params)]
 
 
+P = C.POINTER
+# (name, restype, argtypes) for every symbol of include/arc.h
 ARC_SYMBOLS = [
     ("arc_version", C.c_char_p, []),
     ("arc_status_string", C.c_char_p, [C.c_int]),
@@ -155,6 +159,16 @@ ARC_SYMBOLS = [
     ("arc_gather_kernel_history", C.c_int, [vp, C.c_int, vp]),
     ("arc_gather_get_bvh", vp, [vp]),
     ("arc_gather_ctx", vp, [vp]),
+    ("arc_cutnet_create", C.c_int, [vp, P(vp)]),
+    ("arc_cutnet_destroy", None, [vp]),
+    ("arc_cutnet_init", C.c_int, [vp, C.c_uint64, vp, vp]),
+    ("arc_cutnet_set_weights", C.c_int, [vp, vp, C.c_int]),
+    ("arc_cutnet_get_weights", C.c_int, [vp, vp, C.c_int]),
+    ("arc_cutnet_forward", C.c_int, [vp, vp, C.c_int, vp]),
+    ("arc_cutnet_train", C.c_int, [vp, vp, vp, vp, C.c_int, C.c_int, C.c_int, C.c_float, C.c_uint64, vp]),
+    ("arc_gather_set_cutnet", C.c_int, [vp, vp]),
+    ("arc_gather_cut_samples", C.c_int, [vp, P(arc_avpls), arc_float3, C.c_float, P(arc_gather_params), C.c_int,
+                                         vp, vp, C.c_int, C.c_uint64, C.c_float, P(C.c_int)]),
     ("arc_batch_create", C.c_int, [vp, C.c_int, P(vp)]),
     ("arc_batch_lane", vp, [vp, C.c_int]),
     ("arc_batch_capture", C.c_int, [vp, vp, C.c_int]),

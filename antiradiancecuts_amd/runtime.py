@@ -237,6 +237,26 @@ class Gather:
         check(st, "arc_gather_exact_f64", self.ctx.h)
         return rad, anti
 
+    def set_cutnet(self, net: "CutNet | None") -> None:
+        check(self.lib.arc_gather_set_cutnet(self.h, net.h if net is not None else None), "arc_gather_set_cutnet",
+              self.ctx.h)
+        self._net = net
+
+    def cut_samples(self, avpls: DeviceAvpls, cam, extent: float, params, capacity: int, seed: int = 1,
+                    keep: float = 1.0, flags: int = 0):
+        """Training rows of this frame (arc_gather_cut_samples): (X, y, rows met)."""
+        torch = _torch()
+        dev = f"cuda:{self.ctx.device}"
+        X = torch.zeros((max(capacity, 1), ffi.ARC_CUTNET_FEATURES), dtype=torch.float32, device=dev)
+        y = torch.zeros(max(capacity, 1), dtype=torch.float32, device=dev)
+        cnt = C.c_int(0)
+        check(self.lib.arc_gather_cut_samples(self.h, C.byref(avpls.struct), ffi.arc_float3(*map(float, cam)),
+                                              float(extent), C.byref(params), flags, C.c_void_p(X.data_ptr()),
+                                              C.c_void_p(y.data_ptr()), int(capacity), int(seed), float(keep),
+                                              C.byref(cnt)), "arc_gather_cut_samples", self.ctx.h)
+        m = min(cnt.value, capacity)
+        return X[:m], y[:m], cnt.value
+
     def bvh_to_host(self) -> dict:
         h = self.lib.arc_gather_get_bvh(self.h)
         if not h:
@@ -372,3 +392,53 @@ class Batch:
         if self.h:
             self.lib.arc_batch_destroy(self.h)
             self.h = None
+
+
+class CutNet:
+    """arc_cutnet: the learned cut's refine / keep predictor."""
+
+    def __init__(self, ctx: Context):
+        self.ctx, self.lib = ctx, ctx.lib
+        h = C.c_void_p()
+        check(self.lib.arc_cutnet_create(ctx.h, C.byref(h)), "arc_cutnet_create", ctx.h)
+        self.h = h
+
+    def close(self) -> None:
+        if self.h:
+            self.lib.arc_cutnet_destroy(self.h)
+            self.h = None
+
+    def init(self, seed: int, mean, std) -> None:
+        m = np.ascontiguousarray(mean, np.float32)
+        sd = np.ascontiguousarray(std, np.float32)
+        assert m.shape == (ffi.ARC_CUTNET_FEATURES,) and sd.shape == m.shape
+        check(self.lib.arc_cutnet_init(self.h, int(seed), m.ctypes.data_as(C.c_void_p),
+                                       sd.ctypes.data_as(C.c_void_p)), "arc_cutnet_init", self.ctx.h)
+
+    def set_weights(self, w) -> None:
+        w = np.ascontiguousarray(w, np.float32)
+        check(self.lib.arc_cutnet_set_weights(self.h, w.ctypes.data_as(C.c_void_p), int(w.size)),
+              "arc_cutnet_set_weights", self.ctx.h)
+
+    def get_weights(self) -> np.ndarray:
+        w = np.zeros(ffi.ARC_CUTNET_PARAMS, np.float32)
+        check(self.lib.arc_cutnet_get_weights(self.h, w.ctypes.data_as(C.c_void_p), int(w.size)),
+              "arc_cutnet_get_weights", self.ctx.h)
+        return w
+
+    def forward(self, X):
+        """Probabilities of the rows of a (n, 24) float32 device tensor."""
+        torch = _torch()
+        n = int(X.shape[0])
+        p = torch.zeros(max(n, 1), dtype=torch.float32, device=X.device)
+        check(self.lib.arc_cutnet_forward(self.h, C.c_void_p(X.data_ptr()), n, C.c_void_p(p.data_ptr())),
+              "arc_cutnet_forward", self.ctx.h)
+        return p[:n]
+
+    def train(self, X, y, w=None, epochs: int = 10, batch: int = 4096, lr: float = 1e-3, seed: int = 1):
+        loss = np.zeros(max(epochs, 1), np.float32)
+        check(self.lib.arc_cutnet_train(self.h, C.c_void_p(X.data_ptr()), C.c_void_p(y.data_ptr()),
+                                        C.c_void_p(w.data_ptr()) if w is not None else None, int(X.shape[0]),
+                                        int(epochs), int(batch), float(lr), int(seed),
+                                        loss.ctypes.data_as(C.c_void_p)), "arc_cutnet_train", self.ctx.h)
+        return loss[:epochs]

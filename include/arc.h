@@ -179,6 +179,8 @@ typedef struct arc_gather arc_gather;
 
 #define ARC_CUT_SOLID_ANGLE 0   /* kernel_bvh: Omega < clusterRefinementThreshold (cudaGather.cu:410-414) */
 #define ARC_CUT_ERROR_BOUND 1   /* Lightcuts-style: refine while bound > errorBound * estimate           */
+#define ARC_CUT_LEARNED 2       /* per 8x8 tile, refine / keep predicted by an arc_cutnet; the error-bound
+                                   criterion decides where the prediction is not confident              */
 
 /* What the error-bound cut compares with errorBound * estimate. */
 #define ARC_CRITERION_BOUND 0     /* upper bound of the node's contribution (Lightcuts)       */
@@ -212,6 +214,8 @@ typedef struct arc_gather_params {
     int cutCriterion;                 /* ARC_CUT_ERROR_BOUND: ARC_CRITERION_*                         */
     float cutImageFloor;              /* ARC_CUT_ERROR_BOUND: the threshold is errorBound * max(pixel
                                          estimate, cutImageFloor * image-mean estimate); 0 = per pixel */
+    float cutConfidence;              /* ARC_CUT_LEARNED: a prediction p is taken when max(p, 1 - p) >=
+                                         cutConfidence (default 0.9), else the error-bound criterion */
 } arc_gather_params;
 
 void arc_gather_params_default(arc_gather_params* p);
@@ -228,6 +232,8 @@ typedef struct arc_gather_stats {
     int numLeafs;                     /* AVPLs in the LBVH of this frame            */
     long long budgetHits;             /* ARC_CUT_ERROR_BOUND: nodes kept because the
                                          pixel's cutBudget was spent               */
+    long long fallbackNodes;          /* ARC_CUT_LEARNED: (tile, node) decisions left to the
+                                         error-bound criterion                      */
 } arc_gather_stats;
 
 #define ARC_RUN_PROFILE 1      /* time each phase with HIP events (synchronises)  */
@@ -286,6 +292,52 @@ arc_bvh* arc_gather_get_bvh(arc_gather* g);
 
 /* The context a gather was created on. */
 arc_ctx* arc_gather_ctx(arc_gather* g);
+
+/* ------------------------------------------------------------ learned cut */
+/* Not in the reference: a refine / keep predictor for the light cut.  For each
+ * (8x8 pixel tile, BVH node) the tile traversal computes ARC_CUTNET_FEATURES
+ * features (node box extent and distance, normal cone, radiance, the
+ * error-bound criterion's terms, the tile's normal, extent and threshold
+ * statistics) and a 24-64-64-1 MLP (ReLU, sigmoid) with BF16 weights and FP32
+ * accumulation on the matrix cores predicts the probability that the node
+ * must be refined.  It is trained on labels made by the exact gather: a node
+ * is labelled "refine" when some pixel of the tile sees it with an error,
+ * against the node's exact sum, above that pixel's threshold.
+ *
+ * The weights are a host array of ARC_CUTNET_PARAMS floats: feature mean[24],
+ * 1/std[24], W1[32][64] (rows 24..31 zero), b1[64], W2[64][64] (W2[in][out]),
+ * b2[64], w3[64], b3, then zero padding. */
+typedef struct arc_cutnet arc_cutnet;
+#define ARC_CUTNET_FEATURES 24
+#define ARC_CUTNET_PARAMS 6400
+
+int arc_cutnet_create(arc_ctx* ctx, arc_cutnet** out);
+void arc_cutnet_destroy(arc_cutnet* net);
+/* Random initial weights (He-normal, from seed) with the given feature
+ * normalisation (host arrays of 24; std entries <= 0 count as 1). */
+int arc_cutnet_init(arc_cutnet* net, uint64_t seed, const float* feature_mean, const float* feature_std);
+int arc_cutnet_set_weights(arc_cutnet* net, const float* weights, int count);   /* host, count = PARAMS */
+int arc_cutnet_get_weights(arc_cutnet* net, float* weights, int count);         /* host, synchronous    */
+/* Probabilities of `n` feature rows (device arrays, X row-major n x 24). */
+int arc_cutnet_forward(arc_cutnet* net, const float* X, int n, float* prob);
+/* Minibatch Adam on the weighted binary cross-entropy (device arrays; w may
+ * be NULL for unit weights), `epochs` passes over a seeded permutation of the
+ * rows.  loss (host, may be NULL) receives each epoch's mean loss.
+ * Synchronous. */
+int arc_cutnet_train(arc_cutnet* net, const float* X, const float* y, const float* w, int n, int epochs,
+                     int batch, float learning_rate, uint64_t seed, float* loss);
+/* The network the gather's ARC_CUT_LEARNED frames use (borrowed; NULL detaches). */
+int arc_gather_set_cutnet(arc_gather* g, arc_cutnet* net);
+/* Training rows of one frame: builds the frame's AvplBvh and thresholds as
+ * arc_gather_run does with params (cutMode is ignored), then walks each tile's
+ * cut with exact labels (a node is refined exactly when its label is
+ * "refine") and writes the features and label of each internal node it
+ * meets, kept with probability `keep`, into X (capacity x 24) and y
+ * (capacity), device memory.  *count receives the number of rows met, which
+ * may exceed capacity (the rest are dropped).  Synchronous. */
+int arc_gather_cut_samples(arc_gather* g, const arc_avpls* avpls, arc_float3 camPos, float sceneExtent,
+                           const arc_gather_params* params, int flags, float* X, float* y, int capacity,
+                           uint64_t seed, float keep, int* count);
 
 /* -------------------------------------------------------- batched views */
 /* Not in the reference (one view per CudaGather::run on the default stream):
