@@ -92,6 +92,27 @@ __device__ inline void input_frags(const float* f /* [kIn] */, const float* __re
     x1 = pack4(v[4], v[5], v[6], v[7]);
 }
 
+// input_frags() with the normalisation read from `norm` (mean[24], then
+// 1/std[24]; LDS or global).
+__device__ inline void input_frags_n(const float* f, const float* norm, int g, bf4& x0, bf4& x1) {
+    float v[8];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const float a0 = f[j], a1 = f[4 + j], a2 = f[8 + j], a3 = f[12 + j];
+        const float c0 = f[16 + j], c1 = f[20 + j];
+        v[j] = g == 0 ? a0 : g == 1 ? a1 : g == 2 ? a2 : a3;
+        v[4 + j] = g == 0 ? c0 : g == 1 ? c1 : 0.f;
+    }
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const int k0 = 4 * g + j, k1 = 16 + 4 * g + j;
+        v[j] = (v[j] - norm[k0]) * norm[kIn + k0];
+        v[4 + j] = k1 < kIn ? (v[4 + j] - norm[k1]) * norm[kIn + k1] : 0.f;
+    }
+    x0 = pack4(v[0], v[1], v[2], v[3]);
+    x1 = pack4(v[4], v[5], v[6], v[7]);
+}
+
 // Forward pass of 16 samples (one per lane % 16); every lane returns the
 // logit of its sample.  sb: b1, b2, w3 (64 each) and b3, in LDS or global.
 // All 64 lanes must be active.
@@ -112,6 +133,36 @@ __device__ inline float forward(const Frags& F, const float* sb, bf4 x0, bf4 x1,
         f4 d = zero;
 #pragma unroll
         for (int kt = 0; kt < 4; ++kt) d = mfma(F.w2[mt][kt], h1[kt], d);
+        const float* b = sb + kHid + 16 * mt + 4 * g;
+        const float* w = sb + 2 * kHid + 16 * mt + 4 * g;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) part += fmaxf(d[j] + b[j], 0.f) * w[j];
+    }
+    part += __shfl_xor(part, 16);
+    part += __shfl_xor(part, 32);
+    return part + sb[3 * kHid];
+}
+
+// forward() with the weight fragments read from LDS as they are used
+// (sfrag: the first kF2b fragments, 64 lanes each), for kernels whose
+// occupancy cannot spare the 48 VGPRs of Frags.
+__device__ inline float forward_lds(const bf4* sfrag, const float* sb, bf4 x0, bf4 x1, int lane, int g) {
+    const f4 zero = {0.f, 0.f, 0.f, 0.f};
+    bf4 h1[4];
+#pragma unroll
+    for (int mt = 0; mt < 4; ++mt) {
+        f4 d = mfma(sfrag[(kF1 + mt * 2) * 64 + lane], x0, zero);
+        d = mfma(sfrag[(kF1 + mt * 2 + 1) * 64 + lane], x1, d);
+        const float* b = sb + 16 * mt + 4 * g;
+        h1[mt] = pack4(fmaxf(d[0] + b[0], 0.f), fmaxf(d[1] + b[1], 0.f), fmaxf(d[2] + b[2], 0.f),
+                       fmaxf(d[3] + b[3], 0.f));
+    }
+    float part = 0.f;
+#pragma unroll
+    for (int mt = 0; mt < 4; ++mt) {
+        f4 d = zero;
+#pragma unroll
+        for (int kt = 0; kt < 4; ++kt) d = mfma(sfrag[(kF2 + mt * 4 + kt) * 64 + lane], h1[kt], d);
         const float* b = sb + kHid + 16 * mt + 4 * g;
         const float* w = sb + 2 * kHid + 16 * mt + 4 * g;
 #pragma unroll

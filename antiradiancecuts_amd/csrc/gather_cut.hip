@@ -433,16 +433,54 @@ __device__ inline TileStats tile_stats(const PixelF& P, bool live, float thr) {
     T.radius = wave_max(live ? sqrtf(dx * dx + dy * dy + dz * dz) : 0.f);
     T.thr_min = wave_min(live ? thr : INFINITY);
     T.thr_max = wave_max(live ? thr : 0.f);
+    // every lane holds the same values: keep them in scalar registers
+    T.c.px = __builtin_amdgcn_readfirstlane(T.c.px);
+    T.c.py = __builtin_amdgcn_readfirstlane(T.c.py);
+    T.c.pz = __builtin_amdgcn_readfirstlane(T.c.pz);
+    T.c.nx = __builtin_amdgcn_readfirstlane(T.c.nx);
+    T.c.ny = __builtin_amdgcn_readfirstlane(T.c.ny);
+    T.c.nz = __builtin_amdgcn_readfirstlane(T.c.nz);
+    T.nspread = __builtin_amdgcn_readfirstlane(T.nspread);
+    T.radius = __builtin_amdgcn_readfirstlane(T.radius);
+    T.thr_min = __builtin_amdgcn_readfirstlane(T.thr_min);
+    T.thr_max = __builtin_amdgcn_readfirstlane(T.thr_max);
+    T.live_frac = __builtin_amdgcn_readfirstlane(T.live_frac);
     return T;
+}
+
+// One corner pixel of the tile per lane group (lanes 16 g .. 16 g + 15):
+// pixels (0,0), (7,0), (0,7), (7,7) of the 8x8 tile; a dead corner has an
+// infinite threshold.
+struct Corner {
+    PixelF p;
+    float thr;
+};
+
+__device__ inline Corner tile_corner(const PixelF& P, float thr, int g) {
+    const int cl = g == 0 ? 0 : g == 1 ? 7 : g == 2 ? 56 : 63;
+    Corner c;
+    c.p.px = __shfl(P.px, cl);
+    c.p.py = __shfl(P.py, cl);
+    c.p.pz = __shfl(P.pz, cl);
+    c.p.nx = __shfl(P.nx, cl);
+    c.p.ny = __shfl(P.ny, cl);
+    c.p.nz = __shfl(P.nz, cl);
+    c.p.br = c.p.bg = c.p.bb = 0.f;
+    c.p.mat = 0;
+    c.thr = __shfl(thr, cl);
+    return c;
 }
 
 __device__ inline float lg2(float x) { return __builtin_amdgcn_logf(fmaxf(x, 1e-30f)); }
 __device__ inline float clampf(float x, float lo, float hi) { return fminf(fmaxf(x, lo), hi); }
 
 // The 24 features of (tile, internal node).  Scale-free: distances enter as
-// ratios, intensities relative to the tile's most demanding threshold.
-__device__ inline void cut_features(const TileStats& T, const arc_bvh_node& b, const float4& r0, const float4& r1,
-                                    const float4& r2, const float4& r3, float eps, float delta,
+// ratios, intensities relative to the tile's most demanding threshold.  Two
+// of them are the error-bound criterion's terms at the tile's four corners
+// (one corner per lane group, the largest over the groups), so all 64 lanes
+// must call this, each group with the same node.
+__device__ inline void cut_features(const TileStats& T, const Corner& K, const arc_bvh_node& b, const float4& r0,
+                                    const float4& r1, const float4& r2, const float4& r3, float eps, float delta,
                                     float f[net::kIn]) {
     const PixelF& C = T.c;
     const float dx = r0.x - C.px, dy = r0.y - C.py, dz = r0.z - C.pz;
@@ -458,7 +496,6 @@ __device__ inline void cut_features(const TileStats& T, const arc_bvh_node& b, c
     const float err = node_error(C, b, r0, r2, r3, delta);
     const float bnd = node_bound(C, b, r0.w, delta * delta);
     const float itm = 1.f / fmaxf(T.thr_min, 1e-30f);
-    const float ex = b.bbMax.x - b.bbMin.x, ey = b.bbMax.y - b.bbMin.y, ez = b.bbMax.z - b.bbMin.z;
     f[0] = fminf(r3.y * inv, 8.f);
     f[1] = fminf(T.radius * inv, 8.f);
     f[2] = c1;
@@ -478,11 +515,20 @@ __device__ inline void cut_features(const TileStats& T, const arc_bvh_node& b, c
     f[16] = clampf(lg2(dc / delta) * (1.f / 16.f), -4.f, 4.f);
     f[17] = C.nx * r1.x + C.ny * r1.y + C.nz * r1.z;
     f[18] = (r2.x + r2.y + r2.z) / (3.f * r0.w + 1e-30f);
-    f[19] = fmaxf(ex, fmaxf(ey, ez)) / (2.f * r3.y + 1e-30f);
     f[20] = clampf(c1, 0.f, 1.f);
     f[21] = fminf(1.f, r1.w + fmaxf(c2, 0.f));
     f[22] = hmax > 0.f ? fminf(1.f, hmax / fmaxf(dc - r3.y, delta)) : 0.f;
-    f[23] = dc2 <= 4.f * r3.x ? 1.f : 0.f;
+    // the criterion at this group's corner, then the largest over the corners
+    const float ek = node_error(K.p, b, r0, r2, r3, delta);
+    const float bk = node_bound(K.p, b, r0.w, delta * delta);
+    float fe = ek == INFINITY ? 4.f : clampf(lg2(ek / K.thr) * (1.f / 16.f), -4.f, 4.f);
+    float fb = clampf(lg2(bk / K.thr) * (1.f / 16.f), -4.f, 4.f);
+    fe = fmaxf(fe, __shfl_xor(fe, 16));
+    fe = fmaxf(fe, __shfl_xor(fe, 32));
+    fb = fmaxf(fb, __shfl_xor(fb, 16));
+    fb = fmaxf(fb, __shfl_xor(fb, 32));
+    f[19] = fe;
+    f[23] = fb;
 }
 
 __device__ inline uint32_t mix32(uint64_t x) {
@@ -522,6 +568,7 @@ __global__ __launch_bounds__(kCutThreads) void k_cut_label_tile(CutEB t, const i
     const PixelF P = load_pixel(gpos, gnrm, galb, mats, q);
     const float thr = live ? th.at(q) : INFINITY;
     const TileStats T = tile_stats(P, live, thr);
+    const Corner K = tile_corner(P, thr, lane >> 4);
     if (T.live_frac == 0.f || t.n < 2) return;  // wave-uniform
     const uint64_t tile = (uint64_t)xcd_block(blockIdx.x, gx * gy) * 4 + wave;
     int* S = stk[wave];
@@ -547,7 +594,7 @@ __global__ __launch_bounds__(kCutThreads) void k_cut_label_tile(CutEB t, const i
         const bool refine = __ballot(live && err > thr) != 0;
         if (mix32(o.seed ^ (tile * 0x9E3779B97F4A7C15ull) ^ ((uint64_t)k << 1)) < o.keep) {
             float f[net::kIn];
-            cut_features(T, b, r0, r1, r2, r3, eps, delta, f);
+            cut_features(T, K, b, r0, r1, r2, r3, eps, delta, f);
             if (lane == 0) {
                 const int row = atomicAdd(o.count, 1);
                 if (row < o.capacity) {
@@ -581,7 +628,7 @@ __global__ __launch_bounds__(kCutThreads) void k_cut_label_tile(CutEB t, const i
 // p >= conf (refine) or p <= 1 - conf (keep), and the error-bound criterion
 // over the tile's pixels decides the rest.  Kept nodes and leaves add their
 // estimate to every pixel.
-__global__ __launch_bounds__(kCutThreads) void k_cut_learned_tile(CutEB t, const float4* __restrict__ gpos,
+__global__ __launch_bounds__(kCutThreads) __attribute__((amdgpu_waves_per_eu(4))) void k_cut_learned_tile(CutEB t, const float4* __restrict__ gpos,
                                                                   const float4* __restrict__ gnrm,
                                                                   const float4* __restrict__ galb,
                                                                   const arc_material* __restrict__ mats, int W,
@@ -593,12 +640,16 @@ __global__ __launch_bounds__(kCutThreads) void k_cut_learned_tile(CutEB t, const
                                                                   unsigned long long* __restrict__ counters) {
     __shared__ int stk[kCutThreads / 64][kLearnStack];
     __shared__ float sb[3 * net::kHid + 1];
+    __shared__ float snorm[2 * net::kIn];
+    __shared__ net::bf4 sfrag[net::kF2b * 64];
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    for (int j = tid; j < net::kF2b * 64; j += kCutThreads) sfrag[j] = frag[j];
     for (int j = tid; j < net::kHid; j += kCutThreads) {
         sb[j] = params[net::kB1 + j];
         sb[net::kHid + j] = params[net::kB2 + j];
         sb[2 * net::kHid + j] = params[net::kW3 + j];
     }
+    if (tid < 2 * net::kIn) snorm[tid] = params[net::kMean + tid];
     if (tid == 0) sb[3 * net::kHid] = params[net::kB3];
     __syncthreads();
     const int2 xy = tile_pixel(wave, lane, (W + 15) / 16, (H + 15) / 16);
@@ -607,33 +658,35 @@ __global__ __launch_bounds__(kCutThreads) void k_cut_learned_tile(CutEB t, const
     const PixelF P = load_pixel(gpos, gnrm, galb, mats, q);
     const float thr = live ? th.at(q) : INFINITY;
     const TileStats T = tile_stats(P, live, thr);
-    net::Frags F;
-    net::load_frags(frag, lane, F);
     const int i = lane & 15, g = lane >> 4;
+    const Corner K = tile_corner(P, thr, g);
     const int n = t.n;
     int* S = stk[wave];
     Acc acc;
-    unsigned long long nvis = 0, nev = 0, nerr = 0, nbud = 0, nfb = 0;
+    unsigned nvis = 0, nev = 0, nerr = 0, nbud = 0, nfb = 0;
     int sp = 0;
     if (T.live_frac > 0.f) S[sp++] = n >= 2 ? -1 : 0;
     while (sp > 0) {
+        // keeps the network's LDS reads inside the loop: held in VGPRs across
+        // it they would halve the occupancy
+        asm volatile("" ::: "memory");
         const int cnt = min(kLearnBatch, sp);
         sp -= cnt;
         const int mine = S[sp + min(i, cnt - 1)];
         const bool have = i < cnt;
         const bool internal = have && mine < 0;
         float f[net::kIn];
-        if (internal) {
-            const int k = -(mine + 1);
+        if (n >= 2) {  // uniform: every lane takes part (lanes without an internal node use the root)
+            const int k = internal ? -(mine + 1) : 0;
             const float4* r = t.crec + kCrec * (size_t)(n + k);
-            cut_features(T, t.nodes[k], r[0], r[1], r[2], r[3], eps, delta, f);
+            cut_features(T, K, t.nodes[k], r[0], r[1], r[2], r[3], eps, delta, f);
         } else {
 #pragma unroll
             for (int j = 0; j < net::kIn; ++j) f[j] = 0.f;
         }
         net::bf4 x0, x1;
-        net::input_frags(f, params, g, x0, x1);
-        const float z = net::forward(F, sb, x0, x1, g);
+        net::input_frags_n(f, snorm, g, x0, x1);
+        const float z = net::forward_lds(sfrag, sb, x0, x1, lane, g);
         const float pr = 1.f / (1.f + __expf(-z));
         const bool head = lane < kLearnBatch;
         uint64_t R = __ballot(head && internal && pr >= conf);
@@ -681,11 +734,11 @@ __global__ __launch_bounds__(kCutThreads) void k_cut_learned_tile(CutEB t, const
     acc.flush();
     const int alive = __popcll(__ballot(live));
     if (counters && lane == 0) {
-        atomicAdd(&counters[C_VISITED], nvis * alive);
-        atomicAdd(&counters[C_EVALUATED], nev * alive);
-        atomicAdd(&counters[C_ERRORS], nerr);
-        atomicAdd(&counters[C_BUDGET], nbud);
-        atomicAdd(&counters[C_FALLBACK], nfb);
+        atomicAdd(&counters[C_VISITED], (unsigned long long)nvis * alive);
+        atomicAdd(&counters[C_EVALUATED], (unsigned long long)nev * alive);
+        atomicAdd(&counters[C_ERRORS], (unsigned long long)nerr);
+        atomicAdd(&counters[C_BUDGET], (unsigned long long)nbud);
+        atomicAdd(&counters[C_FALLBACK], (unsigned long long)nfb);
     }
     if (live)
         out_rad[q] = make_float4(acc.k[0].s * P.br, acc.k[1].s * P.bg, acc.k[2].s * P.bb,

@@ -65,10 +65,10 @@ def parse():
                     help="light cut of the radiance: the analytic error bound or kernel_bvh's solid angle")
     ap.add_argument("--error-bound", type=float, default=0.01, help="relative error bound of the cut")
     ap.add_argument("--per-tile", type=int, default=1, help="error-bound cut: one cut per 8x8 tile (1) or per pixel (0)")
-    ap.add_argument("--component", choices=("frame", "trace", "batch"), default="frame",
+    ap.add_argument("--component", choices=("frame", "trace", "batch", "learned"), default="frame",
                     help="frame: the gather frame (the headline metric); trace: the GPU AVPL shooter and "
-                         "G-buffer ray caster on the same scene; batch: many views in one graph launch "
-                         "(side measurements)")
+                         "G-buffer ray caster on the same scene; batch: many views in one graph launch; "
+                         "learned: the learned cut against the error-bound cut (side measurements)")
     ap.add_argument("--paths", type=int, default=250_000, help="trace: paths shot per step")
     ap.add_argument("--views", type=int, default=64, help="batch: views per graph launch")
     ap.add_argument("--lanes", type=int, default=4, help="batch: streams the views are dealt over")
@@ -87,6 +87,8 @@ def main() -> int:
         return bench_trace(args)
     if args.component == "batch":
         return bench_batch(args)
+    if args.component == "learned":
+        return bench_learned(args)
     import torch
 
     from antiradiancecuts_amd import ffi
@@ -349,6 +351,31 @@ def bench_batch(args) -> int:
     for g, *_ in views:
         g.close()
     b.close()
+    ctx.close()
+    return 0
+
+
+def bench_learned(args) -> int:
+    """The learned cut (committed weights) and the error-bound tile cut on the
+    frame workload at error bounds 0.003, 0.01 and 0.03: radiance-kernel ms
+    (HIP events, median of --steps frames), frame ms and the result's
+    relative L2 against the exact FP64 gather on every 4th pixel in x and y;
+    then, per learned point, the error-bound cut's time at the same error."""
+    from antiradiancecuts_amd import learned_cut as L
+    from antiradiancecuts_amd.runtime import Context
+
+    ctx = Context(0)
+    net = L.load(ctx)
+    out = L.evaluate(ctx, net, width=args.width, height=args.height, avpls=args.avpls, tris=args.tris,
+                     reps=max(args.steps, 1), log=lambda s: print(s, file=sys.stderr, flush=True))
+    eq = [r for r in out["equal_error"] if r["speedup"] is not None]
+    line = {"metric": "learned-cut speedup over the error-bound cut at equal result rel-L2 (side measurement)",
+            "value": eq[0]["speedup"] if eq else None, "unit": "x", "n_gpus": 1, "steps": args.steps,
+            "warmup": 2, "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "bf16+f32",
+            "data": "synthetic", "config": {"workload": out["workload"], "weights": str(L.DEFAULT_WEIGHTS.name),
+                                            "points": out["rows"], "equal_error": out["equal_error"]}}
+    print(json.dumps(line), flush=True)
+    net.close()
     ctx.close()
     return 0
 

@@ -179,3 +179,24 @@ def test_labelled_traversal_edges(gpu_ctx, rig):
     # keep = 0.25 keeps about a quarter of the same rows
     _, _, c4 = g.cut_samples(av, sc.cam, sc.extent, p0, capacity=1, keep=0.25, seed=9)
     assert abs(c4 / cnt - 0.25) < 0.02
+
+
+def test_committed_network_runs_the_learned_cut(gpu_ctx, rig):
+    """The shipped weights load, decide a share of the nodes with confidence
+    and give an image within a loose bound of the exact one (the learned cut
+    trades error for nodes; its accuracy is measured on the frame workload)."""
+    from antiradiancecuts_amd import learned_cut as L
+
+    sc, g, av, r64 = rig
+    net = L.load(gpu_ctx)
+    g.set_cutnet(net)
+    st = g.run(av, sc.cam, sc.extent, learned(errorBound=0.003), ffi.ARC_RUN_COUNTERS)
+    gpu_ctx.sync()
+    e = rel_l2(g.radiance[:, :3].double().cpu().numpy(), r64)
+    decisions = (st.visitedNodes + st.evaluatedNodes) / 64.0
+    print(f"[learned] committed weights: rel-L2 {e:.3e}, evaluated/px {st.evaluatedNodes / sc.npix:.1f}, "
+          f"fallback share {st.fallbackNodes / decisions:.3f}", flush=True)
+    assert 0 < st.fallbackNodes < decisions
+    assert e < 0.2
+    g.set_cutnet(None)
+    net.close()
