@@ -490,6 +490,12 @@ int arc_gather_create(arc_ctx* ctx, int width, int height, const arc_material* m
             if (e != hipSuccess) st = arc::hip_fail(ctx, e, "hipEventCreate", __FILE__, __LINE__);
         }
     if (st == ARC_OK) st = arc_bvh_create(ctx, 0, &g->bvh);
+    if (st == ARC_OK) {
+        hipError_t e = hipStreamCreateWithFlags(&g->aux, hipStreamNonBlocking);
+        if (e == hipSuccess) e = hipEventCreateWithFlags(&g->fork, hipEventDisableTiming);
+        if (e == hipSuccess) e = hipEventCreateWithFlags(&g->join, hipEventDisableTiming);
+        if (e != hipSuccess) st = arc::hip_fail(ctx, e, "aux stream", __FILE__, __LINE__);
+    }
     if (st != ARC_OK) {
         arc_gather_destroy(g);
         return st;
@@ -500,6 +506,12 @@ int arc_gather_create(arc_ctx* ctx, int width, int height, const arc_material* m
 
 void arc_gather_destroy(arc_gather* g) {
     if (!g) return;
+    if (g->aux) {
+        (void)hipStreamSynchronize(g->aux);
+        (void)hipStreamDestroy(g->aux);
+    }
+    if (g->fork) (void)hipEventDestroy(g->fork);
+    if (g->join) (void)hipEventDestroy(g->join);
     for (int f = 0; f < arc_gather::kRing; ++f)
         for (int i = 0; i < arc_gather::kMarks; ++i)
             if (g->ev[f][i]) (void)hipEventDestroy(g->ev[f][i]);
@@ -578,6 +590,13 @@ int arc_gather_run(arc_gather* g, const arc_avpls* avpls, arc_float3 camPos, flo
 
     const bool cut = p->useLightCuts && n > 1;
     const bool clustered = p->antiradianceMode == ARC_ANTI_CLUSTERED && n > 0;
+    // the clustered antiradiance of a cut frame needs neither the LBVH nor the
+    // radiance: it runs on the gather's second stream meanwhile.  Not while
+    // the stream is being captured into a graph (arc_batch_capture): there
+    // the frame stays on one stream.
+    hipStreamCaptureStatus capture = hipStreamCaptureStatusNone;
+    ARC_HIP(ctx, hipStreamIsCapturing(ctx->stream, &capture));
+    const bool overlap = cut && clustered && capture == hipStreamCaptureStatusNone;
     if (p->useLightCuts && p->cutMode != ARC_CUT_SOLID_ANGLE && p->cutMode != ARC_CUT_ERROR_BOUND &&
         p->cutMode != ARC_CUT_LEARNED)
         return arc::fail(ctx, ARC_E_INVALID, "cutMode");
@@ -595,6 +614,22 @@ int arc_gather_run(arc_gather* g, const arc_avpls* avpls, arc_float3 camPos, flo
     if (p->cutBudget < 1) return arc::fail(ctx, ARC_E_INVALID, "cutBudget");
     // ---- radiance: light cut over this frame's AvplBvh (cudaGather.cu:854-923) or exact
     int rad_splits = -1, anti_splits = 0;
+    if (overlap) {
+        ARC_HIP(ctx, hipEventRecord(g->fork, ctx->stream));
+        ARC_HIP(ctx, hipStreamWaitEvent(g->aux, g->fork, 0));
+        hipStream_t main_stream = ctx->stream;
+        ctx->stream = g->aux;  // the antiradiance launches go to the second stream
+        const int sa = arc::anti_clustered(g, a, p->antiradianceModel, cam, anti_params(radius, cos_half),
+                                           (double)p->coneHalfAngle, g->part.as<float>(), counters,
+                                           profile ? ev[7] : nullptr);
+        hipError_t e = hipSuccess;
+        if (sa == ARC_OK && profile) e = hipEventRecord(ev[8], g->aux);
+        if (sa == ARC_OK && e == hipSuccess) e = hipEventRecord(g->join, g->aux);
+        ctx->stream = main_stream;
+        ARC_TRY(sa);
+        if (e != hipSuccess) return arc::hip_fail(ctx, e, "antiradiance stream", __FILE__, __LINE__);
+        anti_splits = 1;
+    }
     if (cut) {
         // AvplBvh(avpls, false) per frame (cudaGather.cu:858)
         ARC_TRY(arc::bvh_build_avpls(g->bvh, a, nullptr, p->seed, (flags & ARC_TRUST_INPUT) == 0,
@@ -634,18 +669,23 @@ int arc_gather_run(arc_gather* g, const arc_avpls* avpls, arc_float3 camPos, flo
         if (profile) ARC_HIP(ctx, hipEventRecord(ev[3], ctx->stream));
     }
     // ---- antiradiance: clustered lists (kernel_antiradiance_clusterred) or exact
-    if (clustered) {
+    if (overlap) {
+        ARC_HIP(ctx, hipStreamWaitEvent(ctx->stream, g->join, 0));  // what the radiance did not hide
+    } else if (clustered) {
         ARC_TRY(arc::anti_clustered(g, a, p->antiradianceModel, cam, anti_params(radius, cos_half),
                                     (double)p->coneHalfAngle, g->part.as<float>(), counters,
                                     profile ? ev[7] : nullptr));
         anti_splits = 1;
+        if (profile) ARC_HIP(ctx, hipEventRecord(ev[8], ctx->stream));
     } else if (cut && n > 0) {
         if (profile) ARC_HIP(ctx, hipEventRecord(ev[7], ctx->stream));
         dispatch_exact<false, true>(g, p->antiradianceModel, a, chunk, splits, cam, radius, cos_half);
         anti_splits = splits;
         ARC_LAUNCHED(ctx);
+        if (profile) ARC_HIP(ctx, hipEventRecord(ev[8], ctx->stream));
     } else if (profile) {
         ARC_HIP(ctx, hipEventRecord(ev[7], ctx->stream));  // summed with the radiance: no pass of its own
+        ARC_HIP(ctx, hipEventRecord(ev[8], ctx->stream));
     }
     if (profile) ARC_HIP(ctx, hipEventRecord(ev[4], ctx->stream));
     hipLaunchKernelGGL(k_finalize, dim3(blocks(npix)), dim3(kThreads), 0, ctx->stream, g->part.as<float>(),
@@ -710,7 +750,7 @@ int arc_gather_kernel_history(arc_gather* g, int frames, float* out) {
         hipEvent_t* ev = g->ev[slot];
         ARC_HIP(ctx, hipEventSynchronize(ev[5]));
         ARC_HIP(ctx, hipEventElapsedTime(&out[2 * k + 0], ev[6], ev[3]));
-        ARC_HIP(ctx, hipEventElapsedTime(&out[2 * k + 1], ev[7], ev[4]));
+        ARC_HIP(ctx, hipEventElapsedTime(&out[2 * k + 1], ev[7], ev[8]));
     }
     return n;
 }

@@ -32,12 +32,16 @@ struct arc_gather {
     arc::DevBuf cest;  // error-bound cut: per-pixel estimates and their sum
     arc_cutnet* net = nullptr;  // learned cut: the predictor (borrowed)
     arc_bvh* bvh = nullptr;
-    // per-frame phase events: frame slot, then 8 marks (start, upload done,
+    // per-frame phase events: frame slot, then 9 marks (start, upload done,
     // bvh done, radiance done, antiradiance done, end, start of the main
-    // radiance kernel, start of the main antiradiance kernel)
+    // radiance kernel, start and end of the main antiradiance kernel)
     static constexpr int kRing = 64;
-    static constexpr int kMarks = 8;
+    static constexpr int kMarks = 9;
     hipEvent_t ev[kRing][kMarks] = {};
+    // the clustered antiradiance of a cut frame runs on a stream of its own,
+    // forked from and joined back to the context's stream
+    hipStream_t aux = nullptr;
+    hipEvent_t fork = nullptr, join = nullptr;
     int ring_next = 0, ring_count = 0;
 };
 
