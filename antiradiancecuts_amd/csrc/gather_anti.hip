@@ -52,7 +52,9 @@ constexpr int kClusterPx = 32;   // cluster side in pixels
 constexpr int kCap = 16384;      // list entries per cluster
 constexpr int kCompact = 1024;   // compaction block
 constexpr int kTiles = (kClusterPx / 8) * (kClusterPx / 8);  // 8x8 tiles per cluster
-constexpr int kAntiThreads = 64 * kTiles;                      // one wave per tile
+constexpr int kParts = 4;                                      // blocks per cluster (rows of 32 x 8 px)
+constexpr int kBlockTiles = kTiles / kParts;                   // tiles per block
+constexpr int kAntiThreads = 64 * kBlockTiles;                 // one wave per tile
 constexpr int kSub = 1536;       // tile sub-list entries in LDS (more: the tile walks the cluster list)
 constexpr int kTreeMaxLevels = 10;  // cluster quadtree levels: up to 512 x 512 clusters
 constexpr int kTreeStack = 32;      // >= 1 + 3 (levels - 1) entries of the tree descent
@@ -271,26 +273,31 @@ __global__ __launch_bounds__(256) void k_cluster_lists_tree(arc::AvplsDev a, con
     }
 }
 
-// One block per 32x32 cluster, one wave per 8x8 tile.  The block reads the
-// cluster's list once: each thread takes an entry and tests its ray against
-// the 16 tile boxes (the same conservative test as the lists, on smaller
-// boxes), appending hits to the tile's sub-list in LDS.  Each wave then walks
-// its tile's sub-list with wave-uniform (scalar) loads of the packed records.
+// Four blocks per 32x32 cluster (its rows of 32x8 pixels), one wave per 8x8
+// tile.  A block reads the cluster's list once: each thread takes an entry
+// and tests its ray against the block's 4 tile boxes (the same conservative
+// test as the lists, on smaller boxes), appending hits to the tile's
+// sub-list in LDS.  Each wave then walks its tile's sub-list with
+// wave-uniform (scalar) loads of the packed records.  Small blocks keep the
+// LDS at 25 KB per block, so the occupancy is set by the registers (5 waves
+// per SIMD) and the blocks share the CUs with the cut kernel that runs
+// beside them on the other stream.
 template <int MODEL, bool GENERAL>
-__global__ __launch_bounds__(kAntiThreads) void k_anti_clustered(
+__global__ __launch_bounds__(kAntiThreads) __attribute__((amdgpu_waves_per_eu(5))) void k_anti_clustered(
     const float4* __restrict__ rec, arc::AvplsDev raw, const float4* __restrict__ gpos,
     const float4* __restrict__ gnrm, const float4* __restrict__ galb, const arc_material* __restrict__ mats, int W,
     int H, int cw, float camx, float camy, float camz, AntiParams ap, const int* __restrict__ counts,
     const int* __restrict__ lists, const int* __restrict__ idx, const int* __restrict__ n_anti_ptr, float grow,
     float cos_t, float sin_t, float* __restrict__ part, unsigned long long* __restrict__ counters) {
-    __shared__ float tbox[kTiles][6];
-    __shared__ int tlive[kTiles];
-    __shared__ int tcount[kTiles];
-    __shared__ int sub[kTiles][kSub];
+    __shared__ float tbox[kBlockTiles][6];
+    __shared__ int tlive[kBlockTiles];
+    __shared__ int tcount[kBlockTiles];
+    __shared__ int sub[kBlockTiles][kSub];
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
-    const int c = xcd_block(blockIdx.x, gridDim.x);
+    const int b = xcd_block(blockIdx.x, gridDim.x);
+    const int c = b / kParts, half = b % kParts;
     const int x = (c % cw) * kClusterPx + (wave & 3) * 8 + (lane & 7);
-    const int y = (c / cw) * kClusterPx + (wave >> 2) * 8 + (lane >> 3);
+    const int y = (c / cw) * kClusterPx + half * (kClusterPx / kParts) + (wave >> 2) * 8 + (lane >> 3);
     const bool live = x < W && y < H;
     const int npix = W * H;
     const int q = live ? y * W + x : 0;
@@ -319,7 +326,7 @@ __global__ __launch_bounds__(kAntiThreads) void k_anti_clustered(
         const int i = list[j];
         const f3 p = raw.pos[i], w = raw.dir[i], na = raw.nrm[i];
         const float ix = safe_inv(w.x), iy = safe_inv(w.y), iz = safe_inv(w.z);
-        for (int t = 0; t < kTiles; ++t) {
+        for (int t = 0; t < kBlockTiles; ++t) {
             if (tlive[t] && ray_box_hit<MODEL>(p, w, na, ix, iy, iz, tbox[t], grow, cos_t, sin_t)) {
                 const int slot = atomicAdd(&tcount[t], 1);
                 if (slot < kSub) sub[t][slot] = i;
@@ -369,7 +376,7 @@ __global__ __launch_bounds__(kAntiThreads) void k_anti_clustered(
     }
     const int alive = __popcll(__ballot(live));  // all lanes take part in the ballot
     if (counters && lane == 0) {
-        if (wave == 0) atomicAdd(&counters[C_ANTI_VISITED], (unsigned long long)count * kTiles);
+        if (wave == 0) atomicAdd(&counters[C_ANTI_VISITED], (unsigned long long)count * kBlockTiles);
         atomicAdd(&counters[C_ANTI_EVALUATED], (unsigned long long)walk * alive);
     }
     if (!live) return;
@@ -386,7 +393,7 @@ template <int MODEL, bool GENERAL>
 void launch_pixels(arc_gather* g, const arc::AvplsDev& a, int cw, int C, float3 cam, const AntiParams& ap,
                    const int* n_anti_ptr, float grow, float cos_t, float sin_t, float* part,
                    unsigned long long* counters) {
-    hipLaunchKernelGGL((k_anti_clustered<MODEL, GENERAL>), dim3(C), dim3(kAntiThreads), 0, g->ctx->stream,
+    hipLaunchKernelGGL((k_anti_clustered<MODEL, GENERAL>), dim3(C * kParts), dim3(kAntiThreads), 0, g->ctx->stream,
                        g->rec.as<float4>(), a, g->gpos, g->gnrm, g->galb, g->mats.as<arc_material>(), g->width,
                        g->height, cw, cam.x, cam.y, cam.z, ap, g->ccount.as<int>(), g->clist.as<int>(),
                        g->anti_idx.as<int>(), n_anti_ptr, grow, cos_t, sin_t, part, counters);
