@@ -177,15 +177,16 @@ def main() -> int:
         flop = None
         roof = {"kernel": "AvplBvh rebuild (sort, Karras, refit)", "bound": "hbm"}
     t_dom = cand[dom]
+    traffic = profiled_traffic(dom, eb, bool(args.per_tile))
     if flop is not None:
         ach = flop / (t_dom * 1e-3) / 1e12
         roof.update({"achieved": round(ach, 3), "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
-                     "frac": round(ach / PEAK_FP32_TFLOPS, 4), "traffic": None, "ms": round(t_dom, 4)})
+                     "frac": round(ach / PEAK_FP32_TFLOPS, 4), "traffic": traffic, "ms": round(t_dom, 4)})
     else:
         bytes_ = float(BYTES_BVH_LEAF + (BYTES_MOMENTS_LEAF if eb else 0)) * n
         ach = bytes_ / (t_dom * 1e-3) / 1e9
         roof.update({"achieved": round(ach, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                     "frac": round(ach / PEAK_HBM_GBS, 4), "traffic": None, "ms": round(t_dom, 4)})
+                     "frac": round(ach / PEAK_HBM_GBS, 4), "traffic": traffic, "ms": round(t_dom, 4)})
 
     cpu = cpu_baseline(sc, params, args.cpu_sample) if args.cpu_sample > 0 else None
 
@@ -230,6 +231,36 @@ def main() -> int:
     g.close()
     ctx.close()
     return 0
+
+
+TRAFFIC_PROFILE = ROOT / "profiles" / "frame_1080p_1M_traffic.txt"
+# LBVH kernels of one rebuild, with their dispatch counts (4 radix passes)
+BVH_KERNELS = {"k_bounds_partial": 1, "k_bounds_final": 1, "k_encode": 1, "k_hist": 4, "k_scan_digit": 4,
+               "k_scatter": 4, "k_permute_avpls": 1, "k_random": 1, "k_karras": 1, "k_refit_local<true>": 1,
+               "k_refit_top<true>": 1}
+
+
+def profiled_traffic(dom: str, eb: bool, per_tile: bool):
+    """HBM bytes per launch of the dominant kernel on the 1080p x 1M frame,
+    from the committed rocprofv3 PMC summary (FETCH_SIZE + WRITE_SIZE scaled
+    by a plain copy's readings); None when the summary does not cover it."""
+    if not TRAFFIC_PROFILE.exists():
+        return None
+    table = {}
+    for line in TRAFFIC_PROFILE.read_text().splitlines():
+        if line.startswith("#") or "\t" not in line:
+            continue
+        name, _, _, b = line.split("\t")
+        table[name] = float(b)
+    if dom == "antiradiance":
+        names = {"k_anti_clustered<0, false>": 1}
+    elif dom == "radiance":
+        names = {("k_cut_eb_tile" if per_tile else "k_cut_eb_px") if eb else "k_cut_sa": 1}
+    else:
+        names = BVH_KERNELS
+    if not all(k in table for k in names if names[k] == 1 and k != "k_bounds_final"):
+        return None
+    return float(sum(table.get(k, 0.0) * c for k, c in names.items()))
 
 
 def bench_trace(args) -> int:
