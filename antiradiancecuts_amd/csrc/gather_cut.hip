@@ -337,7 +337,7 @@ __global__ __launch_bounds__(kCutThreads) void k_cut_eb_tile(CutEB t, const floa
     const float thr = live ? th.at(q) : INFINITY;
     const float delta2 = delta * delta;
     Acc acc;
-    unsigned long long nvis = 0, nev = 0, nerr = 0, nbud = 0;
+    unsigned nvis = 0, nev = 0, nerr = 0, nbud = 0;  // wave-uniform: scalar registers
     int sp = 0;
     int node = -1;
     for (;;) {
@@ -371,10 +371,10 @@ __global__ __launch_bounds__(kCutThreads) void k_cut_eb_tile(CutEB t, const floa
     acc.flush();
     const int alive = __popcll(__ballot(live));  // all lanes take part in the ballot
     if (counters && lane == 0) {
-        atomicAdd(&counters[C_VISITED], nvis * alive);
-        atomicAdd(&counters[C_EVALUATED], nev * alive);
-        atomicAdd(&counters[C_ERRORS], nerr);
-        atomicAdd(&counters[C_BUDGET], nbud);
+        atomicAdd(&counters[C_VISITED], (unsigned long long)nvis * alive);
+        atomicAdd(&counters[C_EVALUATED], (unsigned long long)nev * alive);
+        atomicAdd(&counters[C_ERRORS], (unsigned long long)nerr);
+        atomicAdd(&counters[C_BUDGET], (unsigned long long)nbud);
     }
     if (live)
         out_rad[q] = make_float4(acc.k[0].s * P.br, acc.k[1].s * P.bg, acc.k[2].s * P.bb,
