@@ -755,6 +755,22 @@ int arc_gather_kernel_history(arc_gather* g, int frames, float* out) {
     return n;
 }
 
+int arc_gather_kernel_spans(arc_gather* g, int frames, float* out) {
+    if (!g || frames < 0 || (frames > 0 && !out)) return ARC_E_INVALID;
+    arc_ctx* ctx = g->ctx;
+    const int n = std::min(frames, g->ring_count);
+    for (int k = 0; k < n; ++k) {
+        const int slot = (g->ring_next - n + k + arc_gather::kRing) % arc_gather::kRing;
+        hipEvent_t* ev = g->ev[slot];
+        ARC_HIP(ctx, hipEventSynchronize(ev[5]));
+        ARC_HIP(ctx, hipEventElapsedTime(&out[4 * k + 0], ev[0], ev[6]));
+        ARC_HIP(ctx, hipEventElapsedTime(&out[4 * k + 1], ev[0], ev[3]));
+        ARC_HIP(ctx, hipEventElapsedTime(&out[4 * k + 2], ev[0], ev[7]));
+        ARC_HIP(ctx, hipEventElapsedTime(&out[4 * k + 3], ev[0], ev[8]));
+    }
+    return n;
+}
+
 int arc_gather_set_cutnet(arc_gather* g, arc_cutnet* net) {
     if (!g) return ARC_E_INVALID;
     if (net && net->ctx->device != g->ctx->device)

@@ -157,6 +157,7 @@ ARC_SYMBOLS = [
                                        C.c_int, vp, vp]),
     ("arc_gather_profile_history", C.c_int, [vp, C.c_int, vp]),
     ("arc_gather_kernel_history", C.c_int, [vp, C.c_int, vp]),
+    ("arc_gather_kernel_spans", C.c_int, [vp, C.c_int, vp]),
     ("arc_gather_get_bvh", vp, [vp]),
     ("arc_gather_ctx", vp, [vp]),
     ("arc_cutnet_create", C.c_int, [vp, P(vp)]),

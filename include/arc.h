@@ -287,6 +287,13 @@ int arc_gather_profile_history(arc_gather* g, int frames, float* out);
  * Same frames and return value as arc_gather_profile_history. */
 int arc_gather_kernel_history(arc_gather* g, int frames, float* out);
 
+/* Where those kernels ran within each frame, 4 floats per frame (ms from the
+ * frame's start): radiance kernel start and end, antiradiance kernel start
+ * and end.  In a cut frame with clustered antiradiance the two run
+ * concurrently on two streams, so their spans overlap.  Same frames and
+ * return value as arc_gather_profile_history. */
+int arc_gather_kernel_spans(arc_gather* g, int frames, float* out);
+
 /* The AvplBvh of the last cut frame (borrowed; NULL before the first). */
 arc_bvh* arc_gather_get_bvh(arc_gather* g);
 
