@@ -144,19 +144,23 @@ __device__ inline float node_bound(const PixelF& P, const arc_bvh_node& b, float
 // spread).  A pixel within 2 s of the centroid always refines.
 __device__ inline float node_error(const PixelF& P, const arc_bvh_node& b, const float4& r0, const float4& r2,
                                    const float4& r3, float delta) {
+    // the box corners as values first: selecting between them per lane then
+    // costs a v_cndmask instead of a per-lane (vector) load
+    const float x0 = b.bbMin.x, y0 = b.bbMin.y, z0 = b.bbMin.z;
+    const float x1 = b.bbMax.x, y1 = b.bbMax.y, z1 = b.bbMax.z;
     const float dx = r0.x - P.px, dy = r0.y - P.py, dz = r0.z - P.pz;
     const float dc2 = dx * dx + dy * dy + dz * dz;
-    if (dc2 <= 4.f * r3.x) return INFINITY;
-    const float hmax = P.nx * (P.nx > 0.f ? b.bbMax.x - P.px : b.bbMin.x - P.px) +
-                       P.ny * (P.ny > 0.f ? b.bbMax.y - P.py : b.bbMin.y - P.py) +
-                       P.nz * (P.nz > 0.f ? b.bbMax.z - P.pz : b.bbMin.z - P.pz);
-    if (!(hmax > 0.f)) return 0.f;  // behind the pixel's tangent plane: exactly 0
-    // a heuristic: hardware square root and reciprocal (about 1 ulp) suffice
+    const float hmax = P.nx * ((P.nx > 0.f ? x1 : x0) - P.px) + P.ny * ((P.ny > 0.f ? y1 : y0) - P.py) +
+                       P.nz * ((P.nz > 0.f ? z1 : z0) - P.pz);
+    // branch-free: a heuristic, hardware square root and reciprocal (about
+    // 1 ulp) suffice
     const float near = fmaxf(__builtin_amdgcn_sqrtf(dc2) - r3.y, delta);
     const float inv_near = __builtin_amdgcn_rcpf(near);
     const float cb = fminf(1.f, hmax * inv_near);
     const float rel = r3.x * __builtin_amdgcn_rcpf(dc2) + r2.w;
-    return r0.w * cb * (inv_near * inv_near) * rel;
+    const float e = r0.w * cb * (inv_near * inv_near) * rel;
+    // behind the pixel's tangent plane: exactly 0; within 2 s of the centroid: refine
+    return dc2 <= 4.f * r3.x ? INFINITY : (hmax > 0.f ? e : 0.f);
 }
 
 __device__ inline int uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
