@@ -338,39 +338,56 @@ __global__ __launch_bounds__(kAntiThreads) __attribute__((amdgpu_waves_per_eu(5)
     const bool use_sub = nsub <= kSub;  // else the tile walks the whole list
     const int walk = !tlive[wave] ? 0 : use_sub ? nsub : count;
     // Entries are fetched 64 at a time, one per lane, and broadcast with
-    // readlane, so the record loads of consecutive entries do not wait on
-    // one another.  Each chunk is one plain FP32 run, Kahan-added.
+    // readlane.  Their records are read with scalar loads four entries at a
+    // time: scalar loads complete out of order, so one wait covers a batch
+    // and the four loads overlap.  Each chunk of 64 is one plain FP32 run,
+    // Kahan-added.
     static_assert(kRun == 64, "one compensated add per 64-entry chunk");
+    auto pair = [&](int i, const float4& r0, const float4& r1, const float4& r2v, const float4& r4, float& s0,
+                    float& s1, float& s2) {
+        const float dx = r0.x - P.px, dy = r0.y - P.py, dz = r0.z - P.pz;
+        const float dd = dx * dx + dy * dy + dz * dz;
+        // hardware reciprocal square root (about 1 ulp): it scales the guard
+        // band and the cone's 1 / d^2, never a decision inside the band
+        const float inv = __builtin_amdgcn_rsqf(dd);
+        const float cn = P.nx * dx + P.ny * dy + P.nz * dz;
+        const float ca = -(r1.x * dx + r1.y * dy + r1.z * dz);
+        const float m = anti_weight<MODEL>(P, r0, r1, r2v, r4.w, dx, dy, dz, dd, inv, cn, ca, ap, gpos, gnrm, q, raw, i);
+        if (GENERAL) {
+            if (m != 0.f) {
+                using V3 = V<float>;
+                Mat<float> pm = material<float>(mats, P.mat);
+                if (galb) pm.diffuse = mk(P.br / kOneOverPi, P.bg / kOneOverPi, P.bb / kOneOverPi);
+                const V3 b = f_r3(mk(r0.x, r0.y, r0.z), mk(P.px, P.py, P.pz), mk(camx, camy, camz),
+                                  mk(P.nx, P.ny, P.nz), pm) * m;
+                s0 += r4.x * b.x; s1 += r4.y * b.y; s2 += r4.z * b.z;
+            }
+        } else {
+            s0 += r4.x * m; s1 += r4.y * m; s2 += r4.z * m;
+        }
+    };
     Kahan ka[3];
     for (int base = 0; base < walk; base += 64) {
         const int jj = base + lane;
-        const int mine = jj < walk ? (use_sub ? sub[wave][jj] : list[jj]) : 0;
-        const int cnt = min(64, walk - base);
+        // lanes past the end hold the chunk's first entry (a valid index)
+        const int mine = jj < walk ? (use_sub ? sub[wave][jj] : list[jj]) : (use_sub ? sub[wave][base] : list[base]);
+        const int cnt = __builtin_amdgcn_readfirstlane(min(64, walk - base));
         float s0 = 0.f, s1 = 0.f, s2 = 0.f;
-#pragma unroll 4
-        for (int u = 0; u < cnt; ++u) {
-            const int i = __builtin_amdgcn_readlane(mine, u);
-            const float4* r = rec + (size_t)kRecF4 * i;
-            const float4 r0 = r[0], r1 = r[1], r2v = r[2], r4 = r[4];
-            const float dx = r0.x - P.px, dy = r0.y - P.py, dz = r0.z - P.pz;
-            const float dd = dx * dx + dy * dy + dz * dz;
-            const float inv = rsq(dd);
-            const float cn = P.nx * dx + P.ny * dy + P.nz * dz;
-            const float ca = -(r1.x * dx + r1.y * dy + r1.z * dz);
-            const float m =
-                anti_weight<MODEL>(P, r0, r1, r2v, r4.w, dx, dy, dz, dd, inv, cn, ca, ap, gpos, gnrm, q, raw, i);
-            if (GENERAL) {
-                if (m != 0.f) {
-                    using V3 = V<float>;
-                    Mat<float> pm = material<float>(mats, P.mat);
-                    if (galb) pm.diffuse = mk(P.br / kOneOverPi, P.bg / kOneOverPi, P.bb / kOneOverPi);
-                    const V3 b = f_r3(mk(r0.x, r0.y, r0.z), mk(P.px, P.py, P.pz), mk(camx, camy, camz),
-                                      mk(P.nx, P.ny, P.nz), pm) * m;
-                    s0 += r4.x * b.x; s1 += r4.y * b.y; s2 += r4.z * b.z;
-                }
-            } else {
-                s0 += r4.x * m; s1 += r4.y * m; s2 += r4.z * m;
+        for (int u = 0; u < cnt; u += 4) {
+            int i[4];
+            float4 a0[4], a1[4], a2[4], a4[4];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                i[k] = __builtin_amdgcn_readlane(mine, min(u + k, cnt - 1));
+                const float4* r = rec + (size_t)kRecF4 * i[k];
+                a0[k] = r[0];
+                a1[k] = r[1];
+                a2[k] = r[2];
+                a4[k] = r[4];
             }
+#pragma unroll
+            for (int k = 0; k < 4; ++k)
+                if (k == 0 || u + k < cnt) pair(i[k], a0[k], a1[k], a2[k], a4[k], s0, s1, s2);
         }
         ka[0].add(s0); ka[1].add(s1); ka[2].add(s2);
     }
