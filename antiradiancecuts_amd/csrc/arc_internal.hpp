@@ -90,6 +90,7 @@ struct arc_bvh {
     arc::DevBuf nd_size, nd_mat, nd_rand, nd_pos, nd_nrm, nd_rad, nd_dir;
     arc::DevBuf nd_radb;                        // per-slot sums of L x diffuse / pi (error-bound cut)
     arc::DevBuf nd_mom, nd_nmom;                // per-slot weighted position / normal moments
+    arc::DevBuf staged;                         // AVPLs as 64-byte records for the payload permute
     arc::DevBuf bounds;                         // 12 floats + validation flag
     arc::DevBuf partials;                       // reduction partials
     arc::DevBuf ready;                          // refit: nodes spanning blocks, ready to merge

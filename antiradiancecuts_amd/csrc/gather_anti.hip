@@ -279,7 +279,7 @@ __global__ __launch_bounds__(256) void k_cluster_lists_tree(arc::AvplsDev a, con
 // test as the lists, on smaller boxes), appending hits to the tile's
 // sub-list in LDS.  Each wave then walks its tile's sub-list with
 // wave-uniform (scalar) loads of the packed records.  Small blocks keep the
-// LDS at 25 KB per block, so the occupancy is set by the registers (5 waves
+// LDS at 25 KB per block, so the occupancy is set by the registers (at least 5 waves
 // per SIMD) and the blocks share the CUs with the cut kernel that runs
 // beside them on the other stream.
 template <int MODEL, bool GENERAL>

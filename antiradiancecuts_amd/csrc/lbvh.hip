@@ -174,34 +174,50 @@ struct NodeData {  // AvplBvhNodeDataParam over 2n-1 slots
     f3* nmom;     // with radb: sums of w x unit normal
 };
 
-__global__ __launch_bounds__(kBlock) void k_permute_avpls(const int* __restrict__ perm,
-                                                          arc::AvplsDev a, f3* __restrict__ spos,
-                                                          f3* __restrict__ snrm, NodeData nd,
+// The payload of the sort gathered by the permutation: read at random, four
+// SoA arrays of 12-byte elements cost a cache line each per AVPL, so they
+// are first packed in input order into one 64-byte record per AVPL
+// (position and material, normal, radiance, direction).
+__global__ __launch_bounds__(kBlock) void k_stage_avpls(arc::AvplsDev a, float4* __restrict__ staged) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= a.n) return;
+    const f3 p = a.pos[i], q = a.nrm[i], L = a.rad[i], w = a.dir[i];
+    float4* r = staged + 4 * (size_t)i;
+    r[0] = make_float4(p.x, p.y, p.z, __int_as_float(a.mat[i]));
+    r[1] = make_float4(q.x, q.y, q.z, 0.f);
+    r[2] = make_float4(L.x, L.y, L.z, 0.f);
+    r[3] = make_float4(w.x, w.y, w.z, 0.f);
+}
+
+__global__ __launch_bounds__(kBlock) void k_permute_avpls(const int* __restrict__ perm, int n,
+                                                          const float4* __restrict__ staged,
+                                                          f3* __restrict__ spos, f3* __restrict__ snrm, NodeData nd,
                                                           const arc_material* __restrict__ mats) {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= a.n) return;
-    const int o = perm[s];
-    const f3 p = a.pos[o], q = a.nrm[o];
+    if (s >= n) return;
+    const float4* rec = staged + 4 * (size_t)perm[s];
+    const float4 r0 = rec[0], r1 = rec[1], r2 = rec[2], r3 = rec[3];
+    const f3 p{r0.x, r0.y, r0.z}, q{r1.x, r1.y, r1.z}, L{r2.x, r2.y, r2.z}, w{r3.x, r3.y, r3.z};
+    const int mat = __float_as_int(r0.w);
     spos[s] = p;
     snrm[s] = q;
     nd.size[s] = 1;
-    nd.mat[s] = a.mat[o];
+    nd.mat[s] = mat;
     nd.pos[s] = p;
     nd.nrm[s] = q;
-    nd.rad[s] = a.rad[o];
-    nd.dir[s] = a.dir[o];
+    nd.rad[s] = L;
+    nd.dir[s] = w;
     if (nd.radb) {
         // L x f_r of the AVPL's Lambertian material (f_r = 1 for a light-source
         // AVPL, w = 0, cudaGather.cu:89-90); the table holds 256 entries
-        const f3 L = a.rad[o], w = a.dir[o];
         f3 r = L;
         if (mats && (w.x != 0.f || w.y != 0.f || w.z != 0.f)) {
-            const arc_material& m = mats[a.mat[o] & 255];
+            const arc_material& m = mats[mat & 255];
             r = f3{L.x * (0.318309886f * m.diffuse.x), L.y * (0.318309886f * m.diffuse.y),
                    L.z * (0.318309886f * m.diffuse.z)};
         }
         nd.radb[s] = r;
-        const f3 q = a.nrm[o], x = a.pos[o];
+        const f3 x = p;
         const float w8 = (r.x + r.y + r.z) * (1.0f / 3.0f);
         const float ql = q.x * q.x + q.y * q.y + q.z * q.z;
         const float iq = ql > 0.f ? 1.0f / sqrtf(ql) : 0.f;
@@ -527,8 +543,10 @@ int bvh_build_avpls(arc_bvh* b, const AvplsDev& a, const float* rand, uint64_t s
                 b->nd_nrm.as<f3>(),   b->nd_rad.as<f3>(), b->nd_dir.as<f3>(),
                 moments ? b->nd_radb.as<f3>() : nullptr, moments ? b->nd_mom.as<float4>() : nullptr,
                 moments ? b->nd_nmom.as<f3>() : nullptr};
-    hipLaunchKernelGGL(k_permute_avpls, dim3(blocks_for(n)), dim3(kBlock), 0, ctx->stream, b->perm.as<int>(),
-                       a, b->positions.as<f3>(), b->normals.as<f3>(), nd, mats);
+    ARC_TRY(ensure(ctx, b->staged, sizeof(float4) * 4 * (size_t)n));
+    hipLaunchKernelGGL(k_stage_avpls, dim3(blocks_for(n)), dim3(kBlock), 0, ctx->stream, a, b->staged.as<float4>());
+    hipLaunchKernelGGL(k_permute_avpls, dim3(blocks_for(n)), dim3(kBlock), 0, ctx->stream, b->perm.as<int>(), n,
+                       b->staged.as<float4>(), b->positions.as<f3>(), b->normals.as<f3>(), nd, mats);
     ARC_LAUNCHED(ctx);
     ARC_TRY(build_tree(b, n, &nd));
     b->n = n;
