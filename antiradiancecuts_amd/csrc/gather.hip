@@ -458,6 +458,7 @@ void arc_gather_params_default(arc_gather_params* p) {
     p->cutCriterion = ARC_CRITERION_ESTIMATE;
     p->cutImageFloor = 1.0f;
     p->cutConfidence = 0.9f;
+    p->cutEvalDepth = 0;
 }
 
 int arc_gather_create(arc_ctx* ctx, int width, int height, const arc_material* materials, int num_materials,
@@ -609,6 +610,7 @@ int arc_gather_run(arc_gather* g, const arc_avpls* avpls, arc_float3 camPos, flo
         return arc::fail(ctx, ARC_E_INVALID, "cutConfidence");
     if (p->cutEstimateDepth < 0 || p->cutEstimateDepth > 40) return arc::fail(ctx, ARC_E_INVALID, "cutEstimateDepth");
     if (!(p->cutImageFloor >= 0.f)) return arc::fail(ctx, ARC_E_INVALID, "cutImageFloor");
+    if (p->cutEvalDepth < 0 || p->cutEvalDepth > 4) return arc::fail(ctx, ARC_E_INVALID, "cutEvalDepth");
     if (p->cutCriterion != ARC_CRITERION_BOUND && p->cutCriterion != ARC_CRITERION_ESTIMATE)
         return arc::fail(ctx, ARC_E_INVALID, "cutCriterion");
     if (p->cutBudget < 1) return arc::fail(ctx, ARC_E_INVALID, "cutBudget");
@@ -638,7 +640,8 @@ int arc_gather_run(arc_gather* g, const arc_avpls* avpls, arc_float3 camPos, flo
         arc_bvh* b = g->bvh;
         if (moments) {
             const arc::CutEBParams cp{p->errorBound, p->cutEstimateDepth, 1e-3f * sceneExtent, p->cutBudget,
-                                      p->cutPerTile, p->cutCriterion, p->cutImageFloor, p->cutConfidence};
+                                      p->cutPerTile, p->cutCriterion, p->cutImageFloor, p->cutConfidence,
+                                      p->cutEvalDepth};
             if (p->cutMode == ARC_CUT_LEARNED)
                 ARC_TRY(arc::cut_learned(g, b, cp, rad, counters, profile ? ev[6] : nullptr));
             else
@@ -789,6 +792,7 @@ int arc_gather_cut_samples(arc_gather* g, const arc_avpls* avpls, arc_float3 cam
     if (g->general) return arc::fail(ctx, ARC_E_UNSUPPORTED, "the learned cut needs Lambertian materials");
     if (p->cutEstimateDepth < 0 || p->cutEstimateDepth > 40) return arc::fail(ctx, ARC_E_INVALID, "cutEstimateDepth");
     if (!(p->cutImageFloor >= 0.f)) return arc::fail(ctx, ARC_E_INVALID, "cutImageFloor");
+    if (p->cutEvalDepth < 0 || p->cutEvalDepth > 4) return arc::fail(ctx, ARC_E_INVALID, "cutEvalDepth");
     *count = 0;
     const int n = avpls->numAvpls;
     if (n < 2) return ARC_OK;
@@ -800,7 +804,7 @@ int arc_gather_cut_samples(arc_gather* g, const arc_avpls* avpls, arc_float3 cam
     int* dcount = reinterpret_cast<int*>(g->counters.p);
     ARC_HIP(ctx, hipMemsetAsync(dcount, 0, sizeof(int), ctx->stream));
     const arc::CutEBParams cp{p->errorBound, p->cutEstimateDepth, 1e-3f * sceneExtent, p->cutBudget,
-                              1, p->cutCriterion, p->cutImageFloor, p->cutConfidence};
+                              1, p->cutCriterion, p->cutImageFloor, p->cutConfidence, 0};
     ARC_TRY(arc::cut_samples(g, g->bvh, cp, X, y, capacity, seed, keep, dcount));
     ARC_HIP(ctx, hipMemcpyAsync(count, dcount, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
     ARC_HIP(ctx, hipStreamSynchronize(ctx->stream));

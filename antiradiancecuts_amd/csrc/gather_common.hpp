@@ -233,6 +233,7 @@ struct CutEBParams {
     int criterion;      // cutCriterion
     float image_floor;  // cutImageFloor
     float confidence;   // cutConfidence (learned cut)
+    int eval_depth;     // cutEvalDepth
 };
 // `mark` (may be null) is recorded right before the main traversal kernel.
 int cut_error_bound(arc_gather* g, arc_bvh* b, const CutEBParams& cp, float4* rad, unsigned long long* counters,

@@ -182,6 +182,34 @@ struct Acc {
     }
 };
 
+// A kept node's contribution to the pixel: its own estimate, or the sum of
+// its descendants' estimates D levels down (leaves where the subtree ends
+// sooner).  A finer sum for the same node test: the descendants' centroids
+// sit nearer their AVPLs.  Returns the estimates added.
+template <int D>
+__device__ inline int add_desc(const CutEB& t, const PixelF& P, int v, Acc& acc) {
+    if (D == 0 || v >= 0) {
+        const float4* r = t.crec + kCrec * (size_t)(v >= 0 ? v : t.n + (-(v + 1)));
+        acc.add(r[2], geom_est(P, r[0], r[1]));
+        return 1;
+    }
+    const arc_bvh_node& b = t.nodes[-(v + 1)];
+    const int l = b.left, r = b.right;
+    return add_desc<(D > 0 ? D - 1 : 0)>(t, P, l, acc) + add_desc<(D > 0 ? D - 1 : 0)>(t, P, r, acc);
+}
+
+constexpr int kMaxEvalDepth = 4;
+
+__device__ inline int add_kept(const CutEB& t, const PixelF& P, int node, int depth, Acc& acc) {
+    switch (depth) {
+        case 1: return add_desc<1>(t, P, node, acc);
+        case 2: return add_desc<2>(t, P, node, acc);
+        case 3: return add_desc<3>(t, P, node, acc);
+        case 4: return add_desc<4>(t, P, node, acc);
+        default: return add_desc<0>(t, P, node, acc);
+    }
+}
+
 // Pixel of lane `lane` of wave `wave` in a 16x16 block: four 8x8 tiles.
 // The launch is a 1D grid of gx x gy such blocks, taken in XCD-contiguous order.
 __device__ inline int2 tile_pixel(int wave, int lane, int gx, int gy) {
@@ -251,7 +279,7 @@ __global__ __launch_bounds__(kCutThreads) void k_cut_eb_px(CutEB t, const float4
                                                            const float4* __restrict__ galb,
                                                            const arc_material* __restrict__ mats, int W, int H,
                                                            Threshold th, float delta, int budget, int criterion,
-                                                           float4* __restrict__ out_rad,
+                                                           int eval_depth, float4* __restrict__ out_rad,
                                                            unsigned long long* __restrict__ counters) {
     __shared__ int stk[kPxStack * kCutThreads];
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
@@ -291,9 +319,8 @@ __global__ __launch_bounds__(kCutThreads) void k_cut_eb_px(CutEB t, const float4
                     }
                 }
             }
-            const float4* r = t.crec + kCrec * (size_t)slot;
-            acc.add(r[2], geom_est(P, r[0], r[1]));
-            ++nev;
+            (void)slot;
+            nev += add_kept(t, P, node, eval_depth, acc);
             if (sp == 0) break;
             node = S[(--sp) * kCutThreads];
         }
@@ -323,6 +350,7 @@ __global__ __launch_bounds__(kCutThreads) void k_cut_eb_px(CutEB t, const float4
 // the tile's pixels asks for it (ballot).  The refined node's left child is
 // taken next without a round trip through the stack; its right child waits
 // there.  Node indices, stack and records are wave-uniform (scalar loads).
+template <int D>
 __global__ __launch_bounds__(kCutThreads) void k_cut_eb_tile(CutEB t, const float4* __restrict__ gpos,
                                                              const float4* __restrict__ gnrm,
                                                              const float4* __restrict__ galb,
@@ -366,9 +394,8 @@ __global__ __launch_bounds__(kCutThreads) void k_cut_eb_tile(CutEB t, const floa
                 }
             }
         }
-        const float4* r = t.crec + kCrec * (size_t)slot;
-        acc.add(r[2], geom_est(P, r[0], r[1]));
-        ++nev;
+        (void)slot;
+        nev += add_desc<D>(t, P, node, acc);
         if (sp == 0) break;
         node = uniform(S[--sp]);
     }
@@ -789,14 +816,25 @@ int cut_error_bound(arc_gather* g, arc_bvh* b, const CutEBParams& cp, float4* ra
     ARC_TRY(prepare_cut(g, b, cp, t, th));
     const dim3 grid(((g->width + 15) / 16) * ((g->height + 15) / 16));
     if (mark) ARC_HIP(ctx, hipEventRecord(mark, ctx->stream));
-    if (cp.per_tile)
-        hipLaunchKernelGGL(k_cut_eb_tile, grid, dim3(kCutThreads), 0, ctx->stream, t, g->gpos, g->gnrm, g->galb,
-                           g->mats.as<arc_material>(), g->width, g->height, th, cp.delta, cp.budget, cp.criterion,
-                           rad, counters);
-    else
+    if (cp.per_tile) {
+        // one instantiation per evaluation depth: the deeper sums would cost
+        // the shallow ones registers (occupancy)
+        auto tile = [&](auto kernel) {
+            hipLaunchKernelGGL(kernel, grid, dim3(kCutThreads), 0, ctx->stream, t, g->gpos, g->gnrm, g->galb,
+                               g->mats.as<arc_material>(), g->width, g->height, th, cp.delta, cp.budget,
+                               cp.criterion, rad, counters);
+        };
+        switch (cp.eval_depth) {
+            case 1: tile(k_cut_eb_tile<1>); break;
+            case 2: tile(k_cut_eb_tile<2>); break;
+            case 3: tile(k_cut_eb_tile<3>); break;
+            case 4: tile(k_cut_eb_tile<4>); break;
+            default: tile(k_cut_eb_tile<0>); break;
+        }
+    } else
         hipLaunchKernelGGL(k_cut_eb_px, grid, dim3(kCutThreads), 0, ctx->stream, t, g->gpos, g->gnrm, g->galb,
                            g->mats.as<arc_material>(), g->width, g->height, th, cp.delta, cp.budget, cp.criterion,
-                           rad, counters);
+                           cp.eval_depth, rad, counters);
     ARC_LAUNCHED(ctx);
     return ARC_OK;
 }

@@ -85,7 +85,8 @@ class arc_gather_params(C.Structure):
                 ("photonRadiusScale", C.c_float), ("coneHalfAngle", C.c_float),
                 ("writeSentinel", C.c_int), ("stackSize", C.c_int), ("seed", C.c_uint64),
                 ("cutEstimateDepth", C.c_int), ("cutBudget", C.c_int), ("cutPerTile", C.c_int),
-                ("cutCriterion", C.c_int), ("cutImageFloor", C.c_float), ("cutConfidence", C.c_float)]
+                ("cutCriterion", C.c_int), ("cutImageFloor", C.c_float), ("cutConfidence", C.c_float),
+                ("cutEvalDepth", C.c_int)]
 
 
 class arc_gather_stats(C.Structure):

@@ -63,8 +63,10 @@ def parse():
                     help="CPU baseline on 1/k of the pixels (0 = skip)")
     ap.add_argument("--cut", choices=("error-bound", "solid-angle"), default="error-bound",
                     help="light cut of the radiance: the analytic error bound or kernel_bvh's solid angle")
-    ap.add_argument("--error-bound", type=float, default=0.01, help="relative error bound of the cut")
+    ap.add_argument("--error-bound", type=float, default=0.1, help="relative error bound of the cut")
     ap.add_argument("--per-tile", type=int, default=1, help="error-bound cut: one cut per 8x8 tile (1) or per pixel (0)")
+    ap.add_argument("--eval-depth", type=int, default=3,
+                    help="error-bound cut: a kept node is summed as its descendants this many levels down (0-2)")
     ap.add_argument("--component", choices=("frame", "trace", "batch", "learned"), default="frame",
                     help="frame: the gather frame (the headline metric); trace: the GPU AVPL shooter and "
                          "G-buffer ray caster on the same scene; batch: many views in one graph launch; "
@@ -105,7 +107,7 @@ def main() -> int:
     eb = args.cut == "error-bound"
     params = default_params(useLightCuts=1, antiradianceModel=ffi.ARC_ANTIRAD_DISK,
                             cutMode=ffi.ARC_CUT_ERROR_BOUND if eb else ffi.ARC_CUT_SOLID_ANGLE,
-                            errorBound=args.error_bound, cutPerTile=args.per_tile)
+                            errorBound=args.error_bound, cutPerTile=args.per_tile, cutEvalDepth=args.eval_depth)
     flags = ffi.ARC_RUN_TIMELINE | ffi.ARC_TRUST_INPUT
 
     for _ in range(args.warmup):
@@ -183,7 +185,7 @@ def main() -> int:
         if len(spans) else 0.0
     if t_pair >= t_bvh and t_pair > 0.0:
         ach = (flop_rad + flop_anti) / (t_pair * 1e-3) / 1e12
-        tr = [profiled_traffic(k, eb, bool(args.per_tile)) for k in ("radiance", "antiradiance")]
+        tr = [profiled_traffic(k, eb, bool(args.per_tile), args.eval_depth) for k in ("radiance", "antiradiance")]
         roof = {"kernel": f"{rad_name} + k_anti_clustered (concurrent on two streams)", "bound": "valu",
                 "achieved": round(ach, 3), "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
                 "frac": round(ach / PEAK_FP32_TFLOPS, 4),
@@ -222,13 +224,15 @@ def main() -> int:
             "workload": f"clutter_{sc.width}x{sc.height}: Cornell walls + {args.tris} random triangles, "
                         f"{n} AVPLs (bounce 1-4, L log-uniform [1e-3,10], 30% antiradiance); AvplBvh rebuilt "
                         "per frame; light-cut radiance ("
-                        + (f"error bound {args.error_bound:g}, {'per 8x8 tile' if args.per_tile else 'per pixel'}"
+                        + (f"error bound {args.error_bound:g}, {'per 8x8 tile' if args.per_tile else 'per pixel'}, "
+                           f"kept nodes summed {args.eval_depth} levels down"
                            if eb else "solid angle < pi/10")
                         + "); disk antiradiance over per-cluster AVPL lists (exact)",
             "width": sc.width, "height": sc.height, "avpls": n, "parallelism": "single GPU",
             "phase_ms": {k: round(v, 4) for k, v in phase_ms.items()},
             "kernel_ms": {k: round(v, 4) for k, v in kernel_ms.items()},
             "cut": {"mode": args.cut, "error_bound": args.error_bound if eb else None,
+                    "eval_depth": args.eval_depth if eb else None,
                     "visited_nodes": int(st.visitedNodes), "evaluated_nodes": int(st.evaluatedNodes),
                     "stack_overflows": int(st.errorPixels), "budget_hits": int(st.budgetHits)},
             "antiradiance": {"pair_tests": int(st.antiEvaluatedNodes), "tile_filter_tests": int(st.antiVisitedNodes)},
@@ -251,7 +255,7 @@ BVH_KERNELS = {"k_bounds_partial": 1, "k_bounds_final": 1, "k_encode": 1, "k_his
                "k_refit_top<true>": 1}
 
 
-def profiled_traffic(dom: str, eb: bool, per_tile: bool):
+def profiled_traffic(dom: str, eb: bool, per_tile: bool, eval_depth: int = 0):
     """HBM bytes per launch of the dominant kernel on the 1080p x 1M frame,
     from the committed rocprofv3 PMC summary (FETCH_SIZE + WRITE_SIZE scaled
     by a plain copy's readings); None when the summary does not cover it."""
@@ -266,7 +270,10 @@ def profiled_traffic(dom: str, eb: bool, per_tile: bool):
     if dom == "antiradiance":
         names = {"k_anti_clustered<0, false>": 1}
     elif dom == "radiance":
-        names = {("k_cut_eb_tile" if per_tile else "k_cut_eb_px") if eb else "k_cut_sa": 1}
+        name = ("k_cut_eb_tile" if per_tile else "k_cut_eb_px") if eb else "k_cut_sa"
+        if eb and per_tile and f"{name}<{eval_depth}>" in table:
+            name = f"{name}<{eval_depth}>"
+        names = {name: 1}
     else:
         names = BVH_KERNELS
     if not all(k in table for k in names if names[k] == 1 and k != "k_bounds_final"):
@@ -341,7 +348,8 @@ def bench_batch(args) -> int:
     ctx = Context(0)
     b = Batch(ctx, lanes=args.lanes)
     base = default_params(useLightCuts=1, antiradianceModel=ffi.ARC_ANTIRAD_DISK,
-                          cutMode=ffi.ARC_CUT_ERROR_BOUND, errorBound=args.error_bound, cutPerTile=args.per_tile)
+                          cutMode=ffi.ARC_CUT_ERROR_BOUND, errorBound=args.error_bound, cutPerTile=args.per_tile,
+                          cutEvalDepth=args.eval_depth)
     views, alone_ms, pairs, tuned = [], [], 0.0, {}
     t0 = time.time()
     for k in range(args.views):

@@ -216,6 +216,8 @@ typedef struct arc_gather_params {
                                          estimate, cutImageFloor * image-mean estimate); 0 = per pixel */
     float cutConfidence;              /* ARC_CUT_LEARNED: a prediction p is taken when max(p, 1 - p) >=
                                          cutConfidence (default 0.9), else the error-bound criterion */
+    int cutEvalDepth;                 /* ARC_CUT_ERROR_BOUND: a kept node is summed as its descendants
+                                         this many levels down (0..4; leaves where the subtree ends) */
 } arc_gather_params;
 
 void arc_gather_params_default(arc_gather_params* p);

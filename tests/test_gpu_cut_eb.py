@@ -103,9 +103,32 @@ def test_rejects_specular_materials(gpu_ctx):
     g.close()
 
 
-@pytest.mark.parametrize("field,value", [("cutCriterion", 7), ("cutImageFloor", -1.0), ("cutBudget", 0)])
+@pytest.mark.parametrize("field,value", [("cutCriterion", 7), ("cutImageFloor", -1.0), ("cutBudget", 0),
+                                         ("cutEvalDepth", 5), ("cutEvalDepth", -1)])
 def test_invalid_parameters(gpu_ctx, rig, scene, field, value):
     g, av, _ = rig
     with pytest.raises(ffi.ArcError) as ei:
         g.run(av, scene.cam, scene.extent, eb(**{field: value}))
     assert ei.value.status == ffi.ARC_E_INVALID
+
+
+@pytest.mark.parametrize("depth", [1, 2, 3, 4])
+@pytest.mark.parametrize("per_tile", [0, 1])
+def test_descendant_sums(gpu_ctx, rig, scene, depth, per_tile):
+    """cutEvalDepth sums a kept node as its descendants `depth` levels down.
+    With no tolerance every kept node is a leaf (or contributes exactly 0),
+    so the image stays exact; at a loose bound the same cut is summed finer:
+    the node tests do not change, the estimates multiply (at most 2^depth
+    per kept node) and the error falls."""
+    g, av, r64 = rig
+    rad, st = run_rad(gpu_ctx, g, av, scene, eb(errorBound=0.0, cutImageFloor=0.0, cutPerTile=per_tile,
+                                                 cutEvalDepth=depth))
+    assert rel_l2(rad, r64) <= 1e-5
+    rad0, st0 = run_rad(gpu_ctx, g, av, scene, eb(errorBound=0.1, cutPerTile=per_tile))
+    radd, std = run_rad(gpu_ctx, g, av, scene, eb(errorBound=0.1, cutPerTile=per_tile, cutEvalDepth=depth))
+    e0, ed = rel_l2(rad0, r64), rel_l2(radd, r64)
+    print(f"[eb] depth {depth} tile={per_tile}: rel-L2 {e0:.3e} -> {ed:.3e}, "
+          f"estimates/px {st0.evaluatedNodes / scene.npix:.1f} -> {std.evaluatedNodes / scene.npix:.1f}", flush=True)
+    assert std.visitedNodes == st0.visitedNodes
+    assert st0.evaluatedNodes < std.evaluatedNodes <= st0.evaluatedNodes * 2 ** depth
+    assert ed < e0
