@@ -390,7 +390,8 @@ def bench_batch(args) -> int:
             "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
             "config": {"workload": f"{args.views} views, W,H uniform [256,1920]x[256,1080], AVPLs log-uniform "
                                    f"[16K,1M], Cornell walls + {args.tris} random triangles per view (seeded "
-                                   f"variants), error-bound cut {args.error_bound:g}, {args.lanes} lanes",
+                                   f"variants), error-bound cut {args.error_bound:g} with kept nodes summed {args.eval_depth} levels "
+                                   f"down, {args.lanes} lanes",
                        "views": args.views, "lanes": args.lanes, "pairs_per_batch": pairs,
                        "view_ms_alone_p50": round(float(np.percentile(alone, 50)), 4),
                        "view_ms_alone_p99": round(float(np.percentile(alone, 99)), 4),
