@@ -45,6 +45,7 @@ FLOP_TILE_FILTER = 35
 # position / normal, nodes, parents, ranges, node data and random numbers
 # (276), plus the error-bound cut's node moments and packed records (80 + 48)
 BYTES_BVH_LEAF, BYTES_MOMENTS_LEAF = 276, 128
+BYTES_SIMPLE_LEAF = 80  # SimpleBvh: codes, perm, sorted positions, nodes, parents, ranges
 PHASES = ("upload", "bvh", "radiance", "antiradiance", "combine", "total")
 
 
@@ -67,10 +68,11 @@ def parse():
     ap.add_argument("--per-tile", type=int, default=1, help="error-bound cut: one cut per 8x8 tile (1) or per pixel (0)")
     ap.add_argument("--eval-depth", type=int, default=3,
                     help="error-bound cut: a kept node is summed as its descendants this many levels down (0-2)")
-    ap.add_argument("--component", choices=("frame", "trace", "batch", "learned"), default="frame",
+    ap.add_argument("--component", choices=("frame", "trace", "batch", "learned", "lbvh", "exact"), default="frame",
                     help="frame: the gather frame (the headline metric); trace: the GPU AVPL shooter and "
                          "G-buffer ray caster on the same scene; batch: many views in one graph launch; "
-                         "learned: the learned cut against the error-bound cut (side measurements)")
+                         "learned: the learned cut against the error-bound cut; lbvh: the 1M-point LBVH build; "
+                         "exact: the 256x256 x 4096 exact gather (side measurements)")
     ap.add_argument("--paths", type=int, default=250_000, help="trace: paths shot per step")
     ap.add_argument("--views", type=int, default=64, help="batch: views per graph launch")
     ap.add_argument("--lanes", type=int, default=4, help="batch: streams the views are dealt over")
@@ -91,6 +93,10 @@ def main() -> int:
         return bench_batch(args)
     if args.component == "learned":
         return bench_learned(args)
+    if args.component == "lbvh":
+        return bench_lbvh(args)
+    if args.component == "exact":
+        return bench_exact(args)
     import torch
 
     from antiradiancecuts_amd import ffi
@@ -427,6 +433,115 @@ def bench_learned(args) -> int:
                                             "points": out["rows"], "equal_error": out["equal_error"]}}
     print(json.dumps(line), flush=True)
     net.close()
+    ctx.close()
+    return 0
+
+
+def _timed(fn, steps: int, warmup: int) -> float:
+    """ms per call on torch's current stream (the library's context follows it)."""
+    import torch
+
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    a.record()
+    for _ in range(steps):
+        fn()
+    b.record()
+    torch.cuda.synchronize()
+    return a.elapsed_time(b) / steps
+
+
+def bench_lbvh(args) -> int:
+    """SimpleBvh over 1,000,000 points (half uniform, half a 32-Gaussian
+    mixture, sigma 0.01): bounds, Morton codes, radix sort, Karras tree and
+    refit per step, inputs in HBM.  The CPU baseline is the oracle's build
+    of the same points (one thread)."""
+    from antiradiancecuts_amd import scene as S
+    from antiradiancecuts_amd.runtime import Bvh, Context, device_array
+
+    ctx = Context(0)
+    pos = S.points(1_000_000, seed=1)
+    dpos = device_array(pos)
+    b = Bvh(ctx)
+    ms = _timed(lambda: b.build_points(dpos), args.steps, args.warmup)
+    n = len(pos)
+    ach = BYTES_SIMPLE_LEAF * n / (ms * 1e-3) / 1e9
+    cpu = None
+    if args.cpu_sample > 0:
+        from oracle import pyoracle as orc
+
+        t0 = time.time()
+        orc.lbvh_build(pos)
+        t = time.time() - t0
+        cpu = {"value": n / t, "unit": "points/s", "cores": 1, "kind": "port", "ms_per_build": round(t * 1e3, 1),
+               "sample": "the oracle's full build of the same 1,000,000 points"}
+    line = {"metric": "LBVH build of 1M points (side measurement)", "value": n / (ms * 1e-3), "unit": "points/s",
+            "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms, 4),
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32+u32", "data": "synthetic",
+            "config": {"workload": "SimpleBvh, 1,000,000 points: half uniform, half 32 Gaussians (sigma 0.01)"},
+            "roofline": {"kernel": "SimpleBvh build (bounds, encode, sort, Karras, refit)", "bound": "hbm",
+                         "achieved": round(ach, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                         "frac": round(ach / PEAK_HBM_GBS, 4), "traffic": None, "ms": round(ms, 4)},
+            "cpu_baseline": cpu}
+    print(json.dumps(line), flush=True)
+    b.close()
+    ctx.close()
+    return 0
+
+
+def bench_exact(args) -> int:
+    """The exact gather (kernel_radiance_simple + kernel_antiradiance) on the
+    256x256 Cornell scene with 4096 AVPLs and 10-degree antiradiance cones:
+    every pixel sums every AVPL.  The CPU baseline is the oracle (FP32,
+    OpenMP) on the same frame."""
+    import torch
+
+    from antiradiancecuts_amd import ffi
+    from antiradiancecuts_amd import scene as S
+    from antiradiancecuts_amd.runtime import Context, DeviceAvpls, Gather, default_params, device_array, photon_radius
+
+    ctx = Context(0)
+    sc = S.cornell_256(seed=0)
+    g = Gather(ctx, sc.width, sc.height, sc.materials)
+    g.set_gbuffer(device_array(sc.pos4), device_array(sc.nrm4), device_array(sc.albedo4))
+    av = DeviceAvpls.from_scene(sc)
+    p = default_params(useLightCuts=0, antiradianceMode=ffi.ARC_ANTI_EXACT, antiradianceModel=ffi.ARC_ANTIRAD_CONE)
+    ms = _timed(lambda: g.run(av, sc.cam, sc.extent, p, ffi.ARC_TRUST_INPUT), args.steps, args.warmup)
+    g.run(av, sc.cam, sc.extent, p)
+    r64, a64 = g.exact_f64(av, sc.cam, sc.extent, p)
+    torch.cuda.synchronize()
+    ref = (r64 - a64).cpu().numpy()
+    res = g.result[:, :3].double().cpu().numpy()
+    err = float(np.linalg.norm(res - ref) / np.linalg.norm(ref))
+    pairs = float(sc.npix) * sc.num_avpls
+    flop = pairs * (FLOP_RAD_PAIR + FLOP_CONE_PAIR)
+    ach = flop / (ms * 1e-3) / 1e12
+    cpu = None
+    if args.cpu_sample > 0:
+        from oracle import pyoracle as orc
+
+        threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+        ref32 = orc.SceneRef(sc, photon_radius(p, sc.extent), model=ffi.ARC_ANTIRAD_CONE,
+                             cone_half_angle=float(p.coneHalfAngle))
+        t0 = time.time()
+        orc.gather_exact(ref32, 32)
+        t = time.time() - t0
+        cpu = {"value": pairs / t, "unit": "pixel-AVPL pairs/s", "cores": threads, "kind": "port",
+               "ms_per_frame": round(t * 1e3, 1), "sample": "the oracle's REF32 exact gather of the whole frame"}
+    line = {"metric": "exact gather pixel-AVPL pairs/s, 256x256 x 4096 (side measurement)",
+            "value": pairs / (ms * 1e-3), "unit": "pixel-AVPL pairs/s", "n_gpus": 1, "steps": args.steps,
+            "warmup": args.warmup, "ms_per_step": round(ms, 4), "higher_is_better": True, "scaling": "weak",
+            "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+            "config": {"workload": "cornell_256: 256x256 G-buffer, 4096 AVPLs (seed 0), exact radiance + exact "
+                                   "cone antiradiance, Kahan-compensated", "rel_l2_vs_f64": err},
+            "roofline": {"kernel": "k_exact (radiance and antiradiance pairs)", "bound": "valu",
+                         "achieved": round(ach, 3), "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+                         "frac": round(ach / PEAK_FP32_TFLOPS, 4), "traffic": None, "ms": round(ms, 4)},
+            "cpu_baseline": cpu}
+    print(json.dumps(line), flush=True)
+    g.close()
     ctx.close()
     return 0
 
