@@ -10,9 +10,11 @@
 //
 //   scene      triangles in the reference's Triangle layout (p0, p1, p2,
 //              normal = normalize(cross(p2 - p0, p1 - p0)), material), one
-//              64-byte record per triangle in LBVH order; the LBVH is the
-//              library's own (Morton codes of the triangle centroids, Karras
-//              tree), refitted here with the triangles' boxes;
+//              64-byte record per triangle in input order; the leaves are
+//              triangle pieces (large triangles split into slabs along their
+//              longest axis), and the LBVH is the library's own (Morton codes
+//              of the pieces' box centres, Karras tree), refitted here with
+//              the pieces' boxes;
 //   intersect  Triangle::intersect in FRONT_FACE mode (Triangle.h): Moller-
 //              Trumbore on unnormalised barycentrics with the reference's
 //              |det| < EPS = 1e-4 rejection; the closest hit with t > 0
@@ -43,8 +45,11 @@ using arc::f3;
 struct arc_trace_scene {
     arc_ctx* ctx = nullptr;
     int ntri = 0;
-    arc_bvh* bvh = nullptr;           // topology over the triangle centroids
-    arc::DevBuf tri;                  // 4 float4 per sorted triangle
+    int nleaf = 0;                    // leaves: triangle pieces (a large triangle is split)
+    arc_bvh* bvh = nullptr;           // topology over the pieces' box centres
+    arc::DevBuf tri;                  // 4 float4 per triangle, input order
+    arc::DevBuf leaf_tri;             // per leaf: its triangle
+    arc::DevBuf leaf_box;             // per leaf: box lo, hi (2 float4)
     arc::DevBuf nodes;                // refitted copy of the LBVH nodes (triangle boxes)
     arc::DevBuf counters;             // refit arrival counters
     arc::DevBuf mats;                 // material table, 256 entries
@@ -67,6 +72,7 @@ constexpr float kTwoPi = 6.28318530718f;          // TWO_PI (Defines.h:5)
 constexpr float kOneOverTwoPi = 0.15915494309f;   // ONE_OVER_TWO_PI (Defines.h:7)
 constexpr float kOneOverPi = 0.318309886f;
 constexpr int kThreads = 256;
+constexpr int kMaxPieces = 8;         // most leaves one triangle is split into
 constexpr int kTraceStack = 48;       // per-lane stack of the closest-hit traversal (LDS)
 constexpr int kScan = 1024;
 
@@ -96,9 +102,10 @@ struct Tri {  // 4 float4 per triangle
 };
 
 struct SceneDev {
-    const float4* tri;               // sorted triangles, 4 float4 each
+    const float4* tri;               // triangles in input order, 4 float4 each
     const arc_bvh_node* nodes;       // n - 1 internal nodes
-    int n;
+    const int* leaf_tri;             // leaf -> triangle
+    int n;                           // leaves
     const arc_material* mats;        // 256 entries
     const int* emissive;             // 256 entries
 };
@@ -140,17 +147,20 @@ __device__ inline float safe_inv(float w) {
     return 1.0f / copysignf(m, w);
 }
 
-// Closest front-facing hit with t > 0 (ties: the lower sorted slot).
-// stk: this lane's LDS stack, entry j at stk[j * kThreads].  Returns the
-// sorted slot or -1.
+// Closest front-facing hit with t > 0 (ties: the lower triangle index, as a
+// loop over the triangles in order finds it).  stk: this lane's LDS stack,
+// entry j at stk[j * kThreads].  Returns the triangle index or -1.  A
+// triangle split into several leaves is tested once per leaf its ray
+// reaches, with the same result each time.
 __device__ int closest_hit(const SceneDev& S, V o, V dir, float det_eps, float& t_best, int* stk) {
     t_best = INFINITY;
     int best = -1;
     if (S.n == 1) {
         float t;
-        if (tri_hit(load_tri(S.tri, 0), o, dir, det_eps, t) && t > 0.f) {
+        const int ti = S.leaf_tri[0];
+        if (tri_hit(load_tri(S.tri, ti), o, dir, det_eps, t) && t > 0.f) {
             t_best = t;
-            best = 0;
+            best = ti;
         }
         return best;
     }
@@ -165,12 +175,13 @@ __device__ int closest_hit(const SceneDev& S, V o, V dir, float det_eps, float& 
         int cnt = 0;
         for (int k = 0; k < 2; ++k) {
             const int c = ch[k];
-            if (c >= 0) {  // leaf: test the triangle at once
+            if (c >= 0) {  // leaf: test its triangle at once
                 float t;
-                if (tri_hit(load_tri(S.tri, c), o, dir, det_eps, t) && t > 0.f &&
-                    (t < t_best || (t == t_best && c < best))) {
+                const int ti = S.leaf_tri[c];
+                if (tri_hit(load_tri(S.tri, ti), o, dir, det_eps, t) && t > 0.f &&
+                    (t < t_best || (t == t_best && ti < best))) {
                     t_best = t;
-                    best = c;
+                    best = ti;
                 }
             } else {
                 const float te = box_enter(S.nodes[-(c + 1)], o, inv, t_best);
@@ -510,11 +521,11 @@ __global__ __launch_bounds__(kThreads) void k_gbuffer(SceneDev S, Camera cam, in
 // Packs the triangles in sorted order and their boxes.
 __global__ __launch_bounds__(kThreads) void k_pack_tris(const f3* __restrict__ p0, const f3* __restrict__ p1,
                                                         const f3* __restrict__ p2, const int* __restrict__ mat,
-                                                        const int* __restrict__ perm, const int* __restrict__ emis,
-                                                        int n, float4* __restrict__ tri) {
-    const int s = blockIdx.x * kThreads + threadIdx.x;
-    if (s >= n) return;
-    const int i = perm[s];
+                                                        const int* __restrict__ emis, int n,
+                                                        float4* __restrict__ tri) {
+    const int i = blockIdx.x * kThreads + threadIdx.x;
+    if (i >= n) return;
+    const int s = i;
     const V a = ld(p0[i]), b = ld(p1[i]), c = ld(p2[i]);
     V nn = cross(c - a, b - a);  // Triangle::calcNormal
     if (length(nn) > 0.f) nn = normalize(nn);
@@ -527,22 +538,12 @@ __global__ __launch_bounds__(kThreads) void k_pack_tris(const f3* __restrict__ p
     o[3] = make_float4(nn.x, nn.y, nn.z, (float)emis[m & 255]);
 }
 
-__device__ inline void tri_box(const float4* tri, int s, V& lo, V& hi) {
-    const float4* r = tri + 4 * (size_t)s;
-    const V a = mk(r[0].x, r[0].y, r[0].z);
-    const V c = a + mk(r[1].x, r[1].y, r[1].z);  // p2
-    const V b = a + mk(r[2].x, r[2].y, r[2].z);  // p1
-    lo = mk(fminf(a.x, fminf(b.x, c.x)), fminf(a.y, fminf(b.y, c.y)), fminf(a.z, fminf(b.z, c.z)));
-    hi = mk(fmaxf(a.x, fmaxf(b.x, c.x)), fmaxf(a.y, fmaxf(b.y, c.y)), fmaxf(a.z, fmaxf(b.z, c.z)));
-}
-
-// Bottom-up refit with triangle boxes: one thread per leaf climbs; the
-// second arrival at a node merges its children (agent-scope counters; a
-// one-time build per scene).  p0 + e is recomputed, so a leaf box can be a
-// rounding step larger than the triangle: boxes only cull.
+// Bottom-up refit with the leaves' boxes (each a conservative box of its
+// triangle piece): one thread per leaf climbs; the second arrival at a node
+// merges its children (agent-scope counters; a one-time build per scene).
 __global__ __launch_bounds__(kThreads) void k_refit_tris(arc_bvh_node* __restrict__ nodes,
                                                          const int* __restrict__ parents,
-                                                         const float4* __restrict__ tri, int n,
+                                                         const float4* __restrict__ leaf_box, int n,
                                                          int* __restrict__ arrivals) {
     const int s = blockIdx.x * kThreads + threadIdx.x;
     if (s >= n) return;
@@ -553,7 +554,9 @@ __global__ __launch_bounds__(kThreads) void k_refit_tris(arc_bvh_node* __restric
         const int ch[2] = {nodes[node].left, nodes[node].right};
         for (int k = 0; k < 2; ++k) {
             if (ch[k] >= 0) {
-                tri_box(tri, ch[k], lo[k], hi[k]);
+                const float4 a = leaf_box[2 * (size_t)ch[k]], b = leaf_box[2 * (size_t)ch[k] + 1];
+                lo[k] = mk(a.x, a.y, a.z);
+                hi[k] = mk(b.x, b.y, b.z);
             } else {
                 const arc_bvh_node& c = nodes[-(ch[k] + 1)];
                 lo[k] = mk(c.bbMin.x, c.bbMin.y, c.bbMin.z);
@@ -573,21 +576,21 @@ __global__ __launch_bounds__(kThreads) void k_zero_ints(int* __restrict__ p, int
 
 __global__ __launch_bounds__(kThreads) void k_intersect(SceneDev S, const f3* __restrict__ org,
                                                         const f3* __restrict__ dir, int n, float* __restrict__ t,
-                                                        int* __restrict__ tri_index, const int* __restrict__ perm) {
+                                                        int* __restrict__ tri_index) {
     __shared__ int stk[kTraceStack * kThreads];
     const int i = blockIdx.x * kThreads + threadIdx.x;
     if (i >= n) return;
     float th;
     const int s = scene_hit(S, ld(org[i]), ld(dir[i]), th, stk + threadIdx.x);
     t[i] = s < 0 ? -1.f : th;
-    tri_index[i] = s < 0 ? -1 : perm[s];
+    tri_index[i] = s;
 }
 
 int blocks(int n) { return (n + kThreads - 1) / kThreads; }
 
 SceneDev scene_dev(arc_trace_scene* sc) {
-    return SceneDev{sc->tri.as<float4>(), sc->nodes.as<arc_bvh_node>(), sc->ntri, sc->mats.as<arc_material>(),
-                    sc->emissive.as<int>()};
+    return SceneDev{sc->tri.as<float4>(), sc->nodes.as<arc_bvh_node>(), sc->leaf_tri.as<int>(), sc->nleaf,
+                    sc->mats.as<arc_material>(), sc->emissive.as<int>()};
 }
 
 // Depth of the deepest internal node (host, from the node array).
@@ -640,10 +643,96 @@ int arc_trace_scene_create(arc_ctx* ctx, const arc_float3* p0, const arc_float3*
         if (e == hipSuccess) e = hipMemcpy(hm.data(), material, sizeof(int) * ntri, hipMemcpyDeviceToHost);
         if (e != hipSuccess) return fail(arc::hip_fail(ctx, e, "arc_trace_scene_create", __FILE__, __LINE__));
     }
-    std::vector<arc_float3> cent(ntri);
-    for (int i = 0; i < ntri; ++i)
-        cent[i] = arc_float3{(h0[i].x + h1[i].x + h2[i].x) / 3.f, (h0[i].y + h1[i].y + h2[i].y) / 3.f,
-                             (h0[i].z + h1[i].z + h2[i].z) / 3.f};
+    // Leaves are triangle pieces: a triangle larger than half the median
+    // triangle is split along its box's longest axis into up to kMaxPieces
+    // slabs (early split clipping), each piece boxed by the triangle clipped
+    // to its slab.  Rays leaving a surface start inside the boxes of every
+    // large triangle around them and must open all of them; smaller boxes
+    // mean fewer.  The clipping runs in double and the boxes are rounded
+    // outward, so a piece's box always holds its part of the triangle.
+    std::vector<float> plo, phi;
+    std::vector<int> ptri;
+    {
+        std::vector<double> ext(ntri);
+        for (int i = 0; i < ntri; ++i) {
+            const double e0 = std::max({h0[i].x, h1[i].x, h2[i].x}) - std::min({h0[i].x, h1[i].x, h2[i].x});
+            const double e1 = std::max({h0[i].y, h1[i].y, h2[i].y}) - std::min({h0[i].y, h1[i].y, h2[i].y});
+            const double e2 = std::max({h0[i].z, h1[i].z, h2[i].z}) - std::min({h0[i].z, h1[i].z, h2[i].z});
+            ext[i] = std::max({e0, e1, e2});
+        }
+        std::vector<double> sorted = ext;
+        std::nth_element(sorted.begin(), sorted.begin() + ntri / 2, sorted.end());
+        const double unit = 0.5 * sorted[ntri / 2];
+        struct D3 { double v[3]; };
+        auto clip = [](const std::vector<D3>& in, int a, double c, bool keep_ge) {
+            std::vector<D3> out;
+            for (size_t k = 0; k < in.size(); ++k) {
+                const D3& p = in[k];
+                const D3& q = in[(k + 1) % in.size()];
+                const bool pin = keep_ge ? p.v[a] >= c : p.v[a] <= c;
+                const bool qin = keep_ge ? q.v[a] >= c : q.v[a] <= c;
+                if (pin) out.push_back(p);
+                if (pin != qin) {
+                    const double t = (c - p.v[a]) / (q.v[a] - p.v[a]);
+                    D3 r;
+                    for (int d = 0; d < 3; ++d) r.v[d] = p.v[d] + t * (q.v[d] - p.v[d]);
+                    r.v[a] = c;
+                    out.push_back(r);
+                }
+            }
+            return out;
+        };
+        auto push = [&](int i, const double lo[3], const double hi[3], double pad) {
+            for (int d = 0; d < 3; ++d) {
+                plo.push_back(std::nextafter((float)(lo[d] - pad), -INFINITY));
+                phi.push_back(std::nextafter((float)(hi[d] + pad), INFINITY));
+            }
+            ptri.push_back(i);
+        };
+        for (int i = 0; i < ntri; ++i) {
+            const D3 v[3] = {{{h0[i].x, h0[i].y, h0[i].z}}, {{h1[i].x, h1[i].y, h1[i].z}},
+                             {{h2[i].x, h2[i].y, h2[i].z}}};
+            double blo[3], bhi[3];
+            for (int d = 0; d < 3; ++d) {
+                blo[d] = std::min({v[0].v[d], v[1].v[d], v[2].v[d]});
+                bhi[d] = std::max({v[0].v[d], v[1].v[d], v[2].v[d]});
+            }
+            const double pad = 1e-6 * ext[i] + 1e-12;
+            const int k = unit > 0.0 ? std::min(kMaxPieces, std::max(1, (int)std::ceil(ext[i] / unit))) : 1;
+            if (k == 1) {
+                push(i, blo, bhi, pad);
+                continue;
+            }
+            int a = 0;
+            for (int d = 1; d < 3; ++d)
+                if (bhi[d] - blo[d] > bhi[a] - blo[a]) a = d;
+            for (int j = 0; j < k; ++j) {
+                const double lo_a = blo[a] + (bhi[a] - blo[a]) * j / k;
+                const double hi_a = j == k - 1 ? bhi[a] : blo[a] + (bhi[a] - blo[a]) * (j + 1) / k;
+                const std::vector<D3> poly = clip(clip({v[0], v[1], v[2]}, a, lo_a, true), a, hi_a, false);
+                double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+                for (const D3& q : poly)
+                    for (int d = 0; d < 3; ++d) {
+                        lo[d] = std::min(lo[d], q.v[d]);
+                        hi[d] = std::max(hi[d], q.v[d]);
+                    }
+                if (poly.empty())  // rounding only: keep the whole triangle box within the slab
+                    for (int d = 0; d < 3; ++d) {
+                        lo[d] = blo[d];
+                        hi[d] = bhi[d];
+                    }
+                lo[a] = lo_a;
+                hi[a] = hi_a;
+                push(i, lo, hi, pad);
+            }
+        }
+    }
+    const int np = (int)ptri.size();
+    sc->nleaf = np;
+    std::vector<arc_float3> cent(np);
+    for (int k = 0; k < np; ++k)
+        cent[k] = arc_float3{0.5f * (plo[3 * k] + phi[3 * k]), 0.5f * (plo[3 * k + 1] + phi[3 * k + 1]),
+                             0.5f * (plo[3 * k + 2] + phi[3 * k + 2])};
     std::vector<arc_material> table(256);
     std::vector<int> emis(256, 0);
     for (int k = 0; k < num_materials; ++k) {
@@ -654,8 +743,11 @@ int arc_trace_scene_create(arc_ctx* ctx, const arc_float3* p0, const arc_float3*
     int st = arc::ensure(ctx, sc->mats, sizeof(arc_material) * 256);
     if (st == ARC_OK) st = arc::ensure(ctx, sc->emissive, sizeof(int) * 256);
     if (st == ARC_OK) st = arc::ensure(ctx, sc->tri, sizeof(float4) * 4 * (size_t)ntri);
+    if (st == ARC_OK) st = arc::ensure(ctx, sc->leaf_tri, sizeof(int) * (size_t)np);
+    if (st == ARC_OK) st = arc::ensure(ctx, sc->leaf_box, sizeof(float4) * 2 * (size_t)np);
+    const size_t vc = sizeof(arc_float3) * (size_t)np;
     arc::DevBuf staging;
-    if (st == ARC_OK) st = arc::ensure(ctx, staging, 4 * v3 + sizeof(int) * 2 * (size_t)ntri);
+    if (st == ARC_OK) st = arc::ensure(ctx, staging, 3 * v3 + vc + sizeof(int) * (size_t)ntri);
     if (st != ARC_OK) {
         arc::release(staging);
         return fail(st);
@@ -665,11 +757,11 @@ int arc_trace_scene_create(arc_ctx* ctx, const arc_float3* p0, const arc_float3*
     f3* d1 = reinterpret_cast<f3*>(base + v3);
     f3* d2 = reinterpret_cast<f3*>(base + 2 * v3);
     f3* dc = reinterpret_cast<f3*>(base + 3 * v3);
-    int* dm = reinterpret_cast<int*>(base + 4 * v3);
+    int* dm = reinterpret_cast<int*>(base + 3 * v3 + vc);
     e = hipMemcpy(d0, h0.data(), v3, hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(d1, h1.data(), v3, hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(d2, h2.data(), v3, hipMemcpyHostToDevice);
-    if (e == hipSuccess) e = hipMemcpy(dc, cent.data(), v3, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(dc, cent.data(), vc, hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(dm, hm.data(), sizeof(int) * ntri, hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(sc->mats.p, table.data(), sizeof(arc_material) * 256, hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(sc->emissive.p, emis.data(), sizeof(int) * 256, hipMemcpyHostToDevice);
@@ -678,29 +770,42 @@ int arc_trace_scene_create(arc_ctx* ctx, const arc_float3* p0, const arc_float3*
         return fail(arc::hip_fail(ctx, e, "arc_trace_scene_create", __FILE__, __LINE__));
     }
     st = arc_bvh_create(ctx, 0, &sc->bvh);
-    if (st == ARC_OK && ntri > 1) st = arc::bvh_build_points(sc->bvh, dc, nullptr, ntri);
+    if (st == ARC_OK && np > 1) st = arc::bvh_build_points(sc->bvh, dc, nullptr, np);
     if (st != ARC_OK) {
         arc::release(staging);
         return fail(st);
     }
-    const int* perm = nullptr;
-    std::vector<int> one{0};
-    arc::DevBuf perm1;
-    if (ntri > 1) {
-        perm = sc->bvh->perm.as<int>();
-    } else {
-        st = arc::ensure(ctx, perm1, sizeof(int));
-        if (st == ARC_OK && hipMemcpy(perm1.p, one.data(), sizeof(int), hipMemcpyHostToDevice) != hipSuccess)
-            st = ARC_E_HIP;
-        perm = perm1.as<int>();
+    // leaves in the tree's order: their triangle and box
+    {
+        std::vector<int> perm(np, 0);
+        if (np > 1) {
+            e = hipMemcpy(perm.data(), sc->bvh->perm.p, sizeof(int) * (size_t)np, hipMemcpyDeviceToHost);
+            if (e != hipSuccess) {
+                arc::release(staging);
+                return fail(arc::hip_fail(ctx, e, "arc_trace_scene_create", __FILE__, __LINE__));
+            }
+        }
+        std::vector<int> lt(np);
+        std::vector<float4> lb(2 * (size_t)np);
+        for (int k = 0; k < np; ++k) {
+            const int r = perm[k];
+            lt[k] = ptri[r];
+            lb[2 * (size_t)k] = make_float4(plo[3 * r], plo[3 * r + 1], plo[3 * r + 2], 0.f);
+            lb[2 * (size_t)k + 1] = make_float4(phi[3 * r], phi[3 * r + 1], phi[3 * r + 2], 0.f);
+        }
+        e = hipMemcpy(sc->leaf_tri.p, lt.data(), sizeof(int) * (size_t)np, hipMemcpyHostToDevice);
+        if (e == hipSuccess) e = hipMemcpy(sc->leaf_box.p, lb.data(), sizeof(float4) * 2 * (size_t)np,
+                                           hipMemcpyHostToDevice);
+        if (e != hipSuccess) {
+            arc::release(staging);
+            return fail(arc::hip_fail(ctx, e, "arc_trace_scene_create", __FILE__, __LINE__));
+        }
     }
-    if (st == ARC_OK) {
-        hipLaunchKernelGGL(k_pack_tris, dim3(blocks(ntri)), dim3(kThreads), 0, ctx->stream, d0, d1, d2, dm, perm,
-                           sc->emissive.as<int>(), ntri, sc->tri.as<float4>());
-        if (hipGetLastError() != hipSuccess) st = ARC_E_HIP;
-    }
-    if (st == ARC_OK && ntri > 1) {
-        const int nn = ntri - 1;
+    hipLaunchKernelGGL(k_pack_tris, dim3(blocks(ntri)), dim3(kThreads), 0, ctx->stream, d0, d1, d2, dm,
+                       sc->emissive.as<int>(), ntri, sc->tri.as<float4>());
+    if (hipGetLastError() != hipSuccess) st = ARC_E_HIP;
+    if (st == ARC_OK && np > 1) {
+        const int nn = np - 1;
         st = arc::ensure(ctx, sc->nodes, sizeof(arc_bvh_node) * (size_t)nn);
         if (st == ARC_OK) st = arc::ensure(ctx, sc->counters, sizeof(int) * (size_t)nn);
         if (st == ARC_OK) {
@@ -711,9 +816,9 @@ int arc_trace_scene_create(arc_ctx* ctx, const arc_float3* p0, const arc_float3*
         if (st == ARC_OK) {
             hipLaunchKernelGGL(k_zero_ints, dim3(blocks(nn)), dim3(kThreads), 0, ctx->stream, sc->counters.as<int>(),
                                nn);
-            hipLaunchKernelGGL(k_refit_tris, dim3(blocks(ntri)), dim3(kThreads), 0, ctx->stream,
-                               sc->nodes.as<arc_bvh_node>(), sc->bvh->parents.as<int>(), sc->tri.as<float4>(), ntri,
-                               sc->counters.as<int>());
+            hipLaunchKernelGGL(k_refit_tris, dim3(blocks(np)), dim3(kThreads), 0, ctx->stream,
+                               sc->nodes.as<arc_bvh_node>(), sc->bvh->parents.as<int>(), sc->leaf_box.as<float4>(),
+                               np, sc->counters.as<int>());
             if (hipGetLastError() != hipSuccess) st = ARC_E_HIP;
         }
         if (st == ARC_OK) {
@@ -732,7 +837,6 @@ int arc_trace_scene_create(arc_ctx* ctx, const arc_float3* p0, const arc_float3*
         if (e != hipSuccess) st = arc::hip_fail(ctx, e, "arc_trace_scene_create", __FILE__, __LINE__);
     }
     arc::release(staging);
-    arc::release(perm1);
     if (st != ARC_OK) return fail(st);
     *out = sc;
     return ARC_OK;
@@ -743,6 +847,8 @@ void arc_trace_scene_destroy(arc_trace_scene* sc) {
     if (sc->ctx && sc->ctx->stream) (void)hipStreamSynchronize(sc->ctx->stream);
     if (sc->bvh) arc_bvh_destroy(sc->bvh);
     arc::release(sc->tri);
+    arc::release(sc->leaf_tri);
+    arc::release(sc->leaf_box);
     arc::release(sc->nodes);
     arc::release(sc->counters);
     arc::release(sc->mats);
@@ -761,20 +867,9 @@ int arc_trace_intersect(arc_trace_scene* sc, const arc_float3* org, const arc_fl
     if (!sc || n < 0 || (n > 0 && (!org || !dir || !t || !tri_index))) return ARC_E_INVALID;
     if (n == 0) return ARC_OK;
     arc_ctx* ctx = sc->ctx;
-    const int* perm = sc->ntri > 1 ? sc->bvh->perm.as<int>() : nullptr;
-    arc::DevBuf one;
-    if (!perm) {  // a single triangle: slot 0 is triangle 0
-        ARC_TRY(arc::ensure(ctx, one, sizeof(int)));
-        ARC_HIP(ctx, hipMemsetAsync(one.p, 0, sizeof(int), ctx->stream));
-        perm = one.as<int>();
-    }
     hipLaunchKernelGGL(k_intersect, dim3(blocks(n)), dim3(kThreads), 0, ctx->stream, scene_dev(sc),
-                       reinterpret_cast<const f3*>(org), reinterpret_cast<const f3*>(dir), n, t, tri_index, perm);
+                       reinterpret_cast<const f3*>(org), reinterpret_cast<const f3*>(dir), n, t, tri_index);
     ARC_LAUNCHED(ctx);
-    if (one.p) {
-        ARC_HIP(ctx, hipStreamSynchronize(ctx->stream));
-        arc::release(one);
-    }
     return ARC_OK;
 }
 
