@@ -643,7 +643,7 @@ int arc_trace_scene_create(arc_ctx* ctx, const arc_float3* p0, const arc_float3*
         if (e == hipSuccess) e = hipMemcpy(hm.data(), material, sizeof(int) * ntri, hipMemcpyDeviceToHost);
         if (e != hipSuccess) return fail(arc::hip_fail(ctx, e, "arc_trace_scene_create", __FILE__, __LINE__));
     }
-    // Leaves are triangle pieces: a triangle larger than half the median
+    // Leaves are triangle pieces: a triangle larger than the median
     // triangle is split along its box's longest axis into up to kMaxPieces
     // slabs (early split clipping), each piece boxed by the triangle clipped
     // to its slab.  Rays leaving a surface start inside the boxes of every
@@ -662,7 +662,7 @@ int arc_trace_scene_create(arc_ctx* ctx, const arc_float3* p0, const arc_float3*
         }
         std::vector<double> sorted = ext;
         std::nth_element(sorted.begin(), sorted.begin() + ntri / 2, sorted.end());
-        const double unit = 0.5 * sorted[ntri / 2];
+        const double unit = sorted[ntri / 2];
         struct D3 { double v[3]; };
         auto clip = [](const std::vector<D3>& in, int a, double c, bool keep_ge) {
             std::vector<D3> out;
