@@ -132,3 +132,26 @@ def test_descendant_sums(gpu_ctx, rig, scene, depth, per_tile):
     assert std.visitedNodes == st0.visitedNodes
     assert st0.evaluatedNodes < std.evaluatedNodes <= st0.evaluatedNodes * 2 ** depth
     assert ed < e0
+
+
+def test_kernel_spans_of_the_two_stream_frame(gpu_ctx, rig, scene):
+    """arc_gather_kernel_spans: in a cut frame with clustered antiradiance the
+    antiradiance kernel runs on the second stream, forked before the LBVH
+    rebuild, so it starts before the cut kernel; both spans lie inside the
+    frame, and the history durations are the spans' lengths."""
+    g, av, _ = rig
+    p = eb(errorBound=0.1, cutPerTile=1, cutEvalDepth=3, antiradianceMode=ffi.ARC_ANTI_CLUSTERED)
+    frames = 3
+    for _ in range(frames):
+        g.run(av, scene.cam, scene.extent, p, ffi.ARC_RUN_TIMELINE)
+    gpu_ctx.sync()
+    spans = np.zeros((frames, 4), np.float32)
+    hist = np.zeros((frames, 6), np.float32)
+    kern = np.zeros((frames, 2), np.float32)
+    assert g.lib.arc_gather_kernel_spans(g.h, frames, spans.ctypes.data) == frames
+    assert g.lib.arc_gather_profile_history(g.h, frames, hist.ctypes.data) == frames
+    assert g.lib.arc_gather_kernel_history(g.h, frames, kern.ctypes.data) == frames
+    for (r0, r1, a0, a1), h, k in zip(spans, hist, kern):
+        assert 0.0 <= r0 <= r1 <= h[5] + 1e-3 and 0.0 <= a0 <= a1 <= h[5] + 1e-3
+        assert a0 < r0  # forked before the rebuild
+        assert abs((r1 - r0) - k[0]) < 1e-2 and abs((a1 - a0) - k[1]) < 1e-2
