@@ -12,9 +12,9 @@
 //              normal = normalize(cross(p2 - p0, p1 - p0)), material), one
 //              64-byte record per triangle in input order; the leaves are
 //              triangle pieces (large triangles split into slabs along their
-//              longest axis), and the LBVH is the library's own (Morton codes
-//              of the pieces' box centres, Karras tree), refitted here with
-//              the pieces' boxes;
+//              longest axis), and the tree over them is a binned-SAH build
+//              on the host at scene creation (static geometry: built once,
+//              traversed by every path and pixel);
 //   intersect  Triangle::intersect in FRONT_FACE mode (Triangle.h): Moller-
 //              Trumbore on unnormalised barycentrics with the reference's
 //              |det| < EPS = 1e-4 rejection; the closest hit with t > 0
@@ -46,12 +46,10 @@ struct arc_trace_scene {
     arc_ctx* ctx = nullptr;
     int ntri = 0;
     int nleaf = 0;                    // leaves: triangle pieces (a large triangle is split)
-    arc_bvh* bvh = nullptr;           // topology over the pieces' box centres
     arc::DevBuf tri;                  // 4 float4 per triangle, input order
     arc::DevBuf leaf_tri;             // per leaf: its triangle
     arc::DevBuf leaf_box;             // per leaf: box lo, hi (2 float4)
-    arc::DevBuf nodes;                // refitted copy of the LBVH nodes (triangle boxes)
-    arc::DevBuf counters;             // refit arrival counters
+    arc::DevBuf nodes;                // the tree's internal nodes (BvhNode layout, boxes of the pieces)
     arc::DevBuf mats;                 // material table, 256 entries
     arc::DevBuf emissive;             // per material: emissive != 0
     int max_depth = 0;                // deepest internal node (root = 1)
@@ -538,36 +536,6 @@ __global__ __launch_bounds__(kThreads) void k_pack_tris(const f3* __restrict__ p
     o[3] = make_float4(nn.x, nn.y, nn.z, (float)emis[m & 255]);
 }
 
-// Bottom-up refit with the leaves' boxes (each a conservative box of its
-// triangle piece): one thread per leaf climbs; the second arrival at a node
-// merges its children (agent-scope counters; a one-time build per scene).
-__global__ __launch_bounds__(kThreads) void k_refit_tris(arc_bvh_node* __restrict__ nodes,
-                                                         const int* __restrict__ parents,
-                                                         const float4* __restrict__ leaf_box, int n,
-                                                         int* __restrict__ arrivals) {
-    const int s = blockIdx.x * kThreads + threadIdx.x;
-    if (s >= n) return;
-    int node = parents[s];
-    while (node >= 0) {
-        if (__hip_atomic_fetch_add(&arrivals[node], 1, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT) == 0) return;
-        V lo[2], hi[2];
-        const int ch[2] = {nodes[node].left, nodes[node].right};
-        for (int k = 0; k < 2; ++k) {
-            if (ch[k] >= 0) {
-                const float4 a = leaf_box[2 * (size_t)ch[k]], b = leaf_box[2 * (size_t)ch[k] + 1];
-                lo[k] = mk(a.x, a.y, a.z);
-                hi[k] = mk(b.x, b.y, b.z);
-            } else {
-                const arc_bvh_node& c = nodes[-(ch[k] + 1)];
-                lo[k] = mk(c.bbMin.x, c.bbMin.y, c.bbMin.z);
-                hi[k] = mk(c.bbMax.x, c.bbMax.y, c.bbMax.z);
-            }
-        }
-        nodes[node].bbMin = arc_float3{fminf(lo[0].x, lo[1].x), fminf(lo[0].y, lo[1].y), fminf(lo[0].z, lo[1].z)};
-        nodes[node].bbMax = arc_float3{fmaxf(hi[0].x, hi[1].x), fmaxf(hi[0].y, hi[1].y), fmaxf(hi[0].z, hi[1].z)};
-        node = parents[n + node];
-    }
-}
 
 __global__ __launch_bounds__(kThreads) void k_zero_ints(int* __restrict__ p, int n) {
     const int i = blockIdx.x * kThreads + threadIdx.x;
@@ -591,6 +559,140 @@ int blocks(int n) { return (n + kThreads - 1) / kThreads; }
 SceneDev scene_dev(arc_trace_scene* sc) {
     return SceneDev{sc->tri.as<float4>(), sc->nodes.as<arc_bvh_node>(), sc->leaf_tri.as<int>(), sc->nleaf,
                     sc->mats.as<arc_material>(), sc->emissive.as<int>()};
+}
+
+// Binned SAH build over the leaves' boxes (host, once per scene): 16 bins
+// of the centroids along the longest axis of their bounds, the split of
+// least area x count on either side; a run of equal centroids splits at its
+// middle.  Below depth kSahDepth the splits take the median instead, which
+// bounds the tree's depth by kSahDepth + log2(leaves) for the traversal's
+// stack.  Writes the leaf order (leaf slot -> box index) and the internal
+// nodes in the library's BvhNode layout (child >= 0: leaf slot; < 0: internal
+// -(k + 1)), with their boxes; node 0 is the root.
+constexpr int kSahBins = 16;
+constexpr int kSahDepth = 26;
+
+void sah_build(const std::vector<float>& lo, const std::vector<float>& hi, std::vector<int>& order,
+               std::vector<arc_bvh_node>& nodes) {
+    const int n = (int)(lo.size() / 3);
+    order.resize(n);
+    for (int i = 0; i < n; ++i) order[i] = i;
+    nodes.assign(n > 1 ? n - 1 : 0, arc_bvh_node{});
+    if (n < 2) return;
+    std::vector<float> cen(3 * (size_t)n);
+    for (size_t k = 0; k < cen.size(); ++k) cen[k] = 0.5f * (lo[k] + hi[k]);
+    auto area = [](const float* l, const float* h) {
+        const float x = h[0] - l[0], y = h[1] - l[1], z = h[2] - l[2];
+        return x * y + y * z + z * x;
+    };
+    struct Task { int b, e, node, depth; };
+    std::vector<Task> stack{{0, n, 0, 0}};
+    int next = 1;
+    while (!stack.empty()) {
+        const Task T = stack.back();
+        stack.pop_back();
+        float bl[3] = {INFINITY, INFINITY, INFINITY}, bh[3] = {-INFINITY, -INFINITY, -INFINITY};
+        float cl[3] = {INFINITY, INFINITY, INFINITY}, ch[3] = {-INFINITY, -INFINITY, -INFINITY};
+        for (int j = T.b; j < T.e; ++j) {
+            const int i = order[j];
+            for (int d = 0; d < 3; ++d) {
+                bl[d] = std::min(bl[d], lo[3 * i + d]);
+                bh[d] = std::max(bh[d], hi[3 * i + d]);
+                cl[d] = std::min(cl[d], cen[3 * i + d]);
+                ch[d] = std::max(ch[d], cen[3 * i + d]);
+            }
+        }
+        arc_bvh_node& nd = nodes[T.node];
+        nd.bbMin = arc_float3{bl[0], bl[1], bl[2]};
+        nd.bbMax = arc_float3{bh[0], bh[1], bh[2]};
+        nd.visited = 0;
+        int a = 0;
+        for (int d = 1; d < 3; ++d)
+            if (ch[d] - cl[d] > ch[a] - cl[a]) a = d;
+        const int count = T.e - T.b;
+        int mid = T.b + count / 2;
+        bool binned = false;
+        const float span = ch[a] - cl[a];
+        if (span > 0.f && T.depth < kSahDepth) {
+            int bc[kSahBins] = {};
+            float bbl[kSahBins][3], bbh[kSahBins][3];
+            for (int k = 0; k < kSahBins; ++k)
+                for (int d = 0; d < 3; ++d) {
+                    bbl[k][d] = INFINITY;
+                    bbh[k][d] = -INFINITY;
+                }
+            const float scale = kSahBins / span;
+            auto bin_of = [&](int i) {
+                return std::min(kSahBins - 1, (int)((cen[3 * i + a] - cl[a]) * scale));
+            };
+            for (int j = T.b; j < T.e; ++j) {
+                const int i = order[j], k = bin_of(i);
+                ++bc[k];
+                for (int d = 0; d < 3; ++d) {
+                    bbl[k][d] = std::min(bbl[k][d], lo[3 * i + d]);
+                    bbh[k][d] = std::max(bbh[k][d], hi[3 * i + d]);
+                }
+            }
+            // sweep: areas of the left prefixes and right suffixes
+            float ra[kSahBins];
+            int rc[kSahBins];
+            float sl[3] = {INFINITY, INFINITY, INFINITY}, sh[3] = {-INFINITY, -INFINITY, -INFINITY};
+            int cnt = 0;
+            for (int k = kSahBins - 1; k > 0; --k) {
+                cnt += bc[k];
+                for (int d = 0; d < 3; ++d) {
+                    sl[d] = std::min(sl[d], bbl[k][d]);
+                    sh[d] = std::max(sh[d], bbh[k][d]);
+                }
+                ra[k] = cnt ? area(sl, sh) : 0.f;
+                rc[k] = cnt;
+            }
+            float best = INFINITY;
+            int split = -1;
+            for (int d = 0; d < 3; ++d) {
+                sl[d] = INFINITY;
+                sh[d] = -INFINITY;
+            }
+            cnt = 0;
+            for (int k = 0; k < kSahBins - 1; ++k) {
+                cnt += bc[k];
+                for (int d = 0; d < 3; ++d) {
+                    sl[d] = std::min(sl[d], bbl[k][d]);
+                    sh[d] = std::max(sh[d], bbh[k][d]);
+                }
+                if (cnt == 0 || rc[k + 1] == 0) continue;
+                const float cost = (float)cnt * area(sl, sh) + (float)rc[k + 1] * ra[k + 1];
+                if (cost < best) {
+                    best = cost;
+                    split = k;
+                }
+            }
+            if (split >= 0) {  // both sides hold leaves by construction
+                auto it = std::partition(order.begin() + T.b, order.begin() + T.e,
+                                         [&](int i) { return bin_of(i) <= split; });
+                mid = (int)(it - order.begin());
+                binned = mid > T.b && mid < T.e;
+            }
+        }
+        if (!binned) {  // median split (deep subtrees, runs of equal centroids)
+            mid = T.b + count / 2;
+            std::nth_element(order.begin() + T.b, order.begin() + mid, order.begin() + T.e,
+                             [&](int x, int y) { return cen[3 * x + a] < cen[3 * y + a]; });
+        }
+        const int ranges[2][2] = {{T.b, mid}, {mid, T.e}};
+        int child[2];
+        for (int c = 0; c < 2; ++c) {
+            if (ranges[c][1] - ranges[c][0] == 1) {
+                child[c] = ranges[c][0];  // leaf slot
+            } else {
+                const int k = next++;
+                child[c] = -(k + 1);
+                stack.push_back({ranges[c][0], ranges[c][1], k, T.depth + 1});
+            }
+        }
+        nd.left = child[0];
+        nd.right = child[1];
+    }
 }
 
 // Depth of the deepest internal node (host, from the node array).
@@ -626,7 +728,7 @@ int arc_trace_scene_create(arc_ctx* ctx, const arc_float3* p0, const arc_float3*
         arc_trace_scene_destroy(sc);
         return st;
     };
-    // inputs on the device: centroids for the LBVH
+    // the triangles on the host (the tree is built there)
     const size_t v3 = sizeof(arc_float3) * (size_t)ntri;
     std::vector<arc_float3> h0(ntri), h1(ntri), h2(ntri);
     std::vector<int> hm(ntri);
@@ -729,10 +831,6 @@ int arc_trace_scene_create(arc_ctx* ctx, const arc_float3* p0, const arc_float3*
     }
     const int np = (int)ptri.size();
     sc->nleaf = np;
-    std::vector<arc_float3> cent(np);
-    for (int k = 0; k < np; ++k)
-        cent[k] = arc_float3{0.5f * (plo[3 * k] + phi[3 * k]), 0.5f * (plo[3 * k + 1] + phi[3 * k + 1]),
-                             0.5f * (plo[3 * k + 2] + phi[3 * k + 2])};
     std::vector<arc_material> table(256);
     std::vector<int> emis(256, 0);
     for (int k = 0; k < num_materials; ++k) {
@@ -745,9 +843,8 @@ int arc_trace_scene_create(arc_ctx* ctx, const arc_float3* p0, const arc_float3*
     if (st == ARC_OK) st = arc::ensure(ctx, sc->tri, sizeof(float4) * 4 * (size_t)ntri);
     if (st == ARC_OK) st = arc::ensure(ctx, sc->leaf_tri, sizeof(int) * (size_t)np);
     if (st == ARC_OK) st = arc::ensure(ctx, sc->leaf_box, sizeof(float4) * 2 * (size_t)np);
-    const size_t vc = sizeof(arc_float3) * (size_t)np;
     arc::DevBuf staging;
-    if (st == ARC_OK) st = arc::ensure(ctx, staging, 3 * v3 + vc + sizeof(int) * (size_t)ntri);
+    if (st == ARC_OK) st = arc::ensure(ctx, staging, 3 * v3 + sizeof(int) * (size_t)ntri);
     if (st != ARC_OK) {
         arc::release(staging);
         return fail(st);
@@ -756,12 +853,10 @@ int arc_trace_scene_create(arc_ctx* ctx, const arc_float3* p0, const arc_float3*
     f3* d0 = reinterpret_cast<f3*>(base);
     f3* d1 = reinterpret_cast<f3*>(base + v3);
     f3* d2 = reinterpret_cast<f3*>(base + 2 * v3);
-    f3* dc = reinterpret_cast<f3*>(base + 3 * v3);
-    int* dm = reinterpret_cast<int*>(base + 3 * v3 + vc);
+    int* dm = reinterpret_cast<int*>(base + 3 * v3);
     e = hipMemcpy(d0, h0.data(), v3, hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(d1, h1.data(), v3, hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(d2, h2.data(), v3, hipMemcpyHostToDevice);
-    if (e == hipSuccess) e = hipMemcpy(dc, cent.data(), vc, hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(dm, hm.data(), sizeof(int) * ntri, hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(sc->mats.p, table.data(), sizeof(arc_material) * 256, hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(sc->emissive.p, emis.data(), sizeof(int) * 256, hipMemcpyHostToDevice);
@@ -769,26 +864,15 @@ int arc_trace_scene_create(arc_ctx* ctx, const arc_float3* p0, const arc_float3*
         arc::release(staging);
         return fail(arc::hip_fail(ctx, e, "arc_trace_scene_create", __FILE__, __LINE__));
     }
-    st = arc_bvh_create(ctx, 0, &sc->bvh);
-    if (st == ARC_OK && np > 1) st = arc::bvh_build_points(sc->bvh, dc, nullptr, np);
-    if (st != ARC_OK) {
-        arc::release(staging);
-        return fail(st);
-    }
-    // leaves in the tree's order: their triangle and box
+    // the tree: binned SAH over the pieces' boxes, built once on the host
+    std::vector<int> order;
+    std::vector<arc_bvh_node> hn;
+    sah_build(plo, phi, order, hn);
     {
-        std::vector<int> perm(np, 0);
-        if (np > 1) {
-            e = hipMemcpy(perm.data(), sc->bvh->perm.p, sizeof(int) * (size_t)np, hipMemcpyDeviceToHost);
-            if (e != hipSuccess) {
-                arc::release(staging);
-                return fail(arc::hip_fail(ctx, e, "arc_trace_scene_create", __FILE__, __LINE__));
-            }
-        }
         std::vector<int> lt(np);
         std::vector<float4> lb(2 * (size_t)np);
         for (int k = 0; k < np; ++k) {
-            const int r = perm[k];
+            const int r = order[k];
             lt[k] = ptri[r];
             lb[2 * (size_t)k] = make_float4(plo[3 * r], plo[3 * r + 1], plo[3 * r + 2], 0.f);
             lb[2 * (size_t)k + 1] = make_float4(phi[3 * r], phi[3 * r + 1], phi[3 * r + 2], 0.f);
@@ -807,30 +891,14 @@ int arc_trace_scene_create(arc_ctx* ctx, const arc_float3* p0, const arc_float3*
     if (st == ARC_OK && np > 1) {
         const int nn = np - 1;
         st = arc::ensure(ctx, sc->nodes, sizeof(arc_bvh_node) * (size_t)nn);
-        if (st == ARC_OK) st = arc::ensure(ctx, sc->counters, sizeof(int) * (size_t)nn);
         if (st == ARC_OK) {
-            e = hipMemcpyAsync(sc->nodes.p, sc->bvh->nodes.p, sizeof(arc_bvh_node) * nn, hipMemcpyDeviceToDevice,
-                               ctx->stream);
-            if (e != hipSuccess) st = arc::hip_fail(ctx, e, "copy nodes", __FILE__, __LINE__);
+            e = hipMemcpy(sc->nodes.p, hn.data(), sizeof(arc_bvh_node) * (size_t)nn, hipMemcpyHostToDevice);
+            if (e != hipSuccess) st = arc::hip_fail(ctx, e, "arc_trace_scene_create", __FILE__, __LINE__);
         }
-        if (st == ARC_OK) {
-            hipLaunchKernelGGL(k_zero_ints, dim3(blocks(nn)), dim3(kThreads), 0, ctx->stream, sc->counters.as<int>(),
-                               nn);
-            hipLaunchKernelGGL(k_refit_tris, dim3(blocks(np)), dim3(kThreads), 0, ctx->stream,
-                               sc->nodes.as<arc_bvh_node>(), sc->bvh->parents.as<int>(), sc->leaf_box.as<float4>(),
-                               np, sc->counters.as<int>());
-            if (hipGetLastError() != hipSuccess) st = ARC_E_HIP;
-        }
-        if (st == ARC_OK) {
-            std::vector<arc_bvh_node> hn(nn);
-            e = hipMemcpyAsync(hn.data(), sc->nodes.p, sizeof(arc_bvh_node) * nn, hipMemcpyDeviceToHost, ctx->stream);
-            if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
-            if (e != hipSuccess) st = arc::hip_fail(ctx, e, "tree depth", __FILE__, __LINE__);
-            else sc->max_depth = tree_depth(hn);
-            // the traversal keeps at most one pending subtree per level
-            if (st == ARC_OK && sc->max_depth >= kTraceStack)
-                st = arc::fail(ctx, ARC_E_UNSUPPORTED, "triangle tree deeper than the traversal stack");
-        }
+        sc->max_depth = tree_depth(hn);
+        // the traversal keeps at most one pending subtree per level
+        if (st == ARC_OK && sc->max_depth >= kTraceStack)
+            st = arc::fail(ctx, ARC_E_UNSUPPORTED, "triangle tree deeper than the traversal stack");
     }
     if (st == ARC_OK) {
         e = hipStreamSynchronize(ctx->stream);
@@ -845,12 +913,10 @@ int arc_trace_scene_create(arc_ctx* ctx, const arc_float3* p0, const arc_float3*
 void arc_trace_scene_destroy(arc_trace_scene* sc) {
     if (!sc) return;
     if (sc->ctx && sc->ctx->stream) (void)hipStreamSynchronize(sc->ctx->stream);
-    if (sc->bvh) arc_bvh_destroy(sc->bvh);
     arc::release(sc->tri);
     arc::release(sc->leaf_tri);
     arc::release(sc->leaf_box);
     arc::release(sc->nodes);
-    arc::release(sc->counters);
     arc::release(sc->mats);
     arc::release(sc->emissive);
     arc::release(sc->state);
