@@ -84,12 +84,11 @@ struct arc_bvh {
     int consider_normals = 0;
     int n = 0;               // leaves of the current build (0: none)
     bool has_node_data = false;
-    bool has_moments = false;
+    bool has_cut_records = false;  // the last build was bvh_build_cut
     arc::DevBuf codes, perm, nodes, parents, ranges, positions, normals;
     arc::DevBuf in_pos, in_nrm;                 // staged host input
     arc::DevBuf nd_size, nd_mat, nd_rand, nd_pos, nd_nrm, nd_rad, nd_dir;
-    arc::DevBuf nd_radb;                        // per-slot sums of L x diffuse / pi (error-bound cut)
-    arc::DevBuf nd_mom, nd_nmom;                // per-slot weighted position / normal moments
+    arc::DevBuf nd_merge;                       // bvh_build_cut: kMrec float4 of sums per slot
     arc::DevBuf staged;                         // AVPLs as 64-byte records for the payload permute
     arc::DevBuf bounds;                         // 12 floats + validation flag
     arc::DevBuf partials;                       // reduction partials
@@ -105,9 +104,49 @@ struct AvplsDev {
     int n;
 };
 int bvh_build_points(arc_bvh* b, const f3* pos, const f3* nrm, int n);
-// moments: also sum the BRDF-weighted intensity and its position / normal
-// moments per node (the error-bound cut's node estimates), using mats.
-int bvh_build_avpls(arc_bvh* b, const AvplsDev& a, const float* rand, uint64_t seed, bool validate,
-                    const arc_material* mats = nullptr, bool moments = false);
+int bvh_build_avpls(arc_bvh* b, const AvplsDev& a, const float* rand, uint64_t seed, bool validate);
+
+// The light cuts' node records, kCrec float4 (one 64-byte line) per slot,
+// leaves then internal nodes:
+//   r0 = (centroid, max(I)), r1 = (m, (1 - |m|) / 4), r2 = (I, 1.2 (1 - |m|) / sqrt(count)),
+//   r3 = (s^2, s, count, -)
+// with I the sum of L x diffuse / pi over the node's AVPLs (the BRDF-weighted
+// intensity), w = (I.x + I.y + I.z) / 3 per AVPL, the centroid sum(w x) /
+// sum(w), m = sum(w n) / sum(w) over unit normals n, and s the half-diagonal
+// of the node's box (0 for a leaf).  A node with sum(w) = 0 contributes
+// nothing; its centroid is its AVPL's position (leaf) or its box centre and
+// its m the leaf's unit normal or 0.  Everything of the cut's tests that does
+// not depend on the pixel is here.
+constexpr int kCrec = 4;
+// Sums merged up the tree: (I, count), (sum w x, sum w), (sum w n, -).
+constexpr int kMrec = 3;
+
+__device__ inline void write_cut_record(float4* __restrict__ o, f3 I, float4 mo, f3 nm, int count, float s2, f3 fc,
+                                        f3 fn) {
+    f3 c, m;
+    if (mo.w > 0.f) {
+        const float iw = 1.0f / mo.w;
+        c = f3{mo.x * iw, mo.y * iw, mo.z * iw};
+        m = f3{nm.x * iw, nm.y * iw, nm.z * iw};
+    } else {
+        c = fc;
+        const float l = fn.x * fn.x + fn.y * fn.y + fn.z * fn.z;
+        const float iq = l > 0.f ? 1.0f / sqrtf(l) : 0.f;
+        m = f3{fn.x * iq, fn.y * iq, fn.z * iq};
+    }
+    const float ml = fminf(1.f, sqrtf(m.x * m.x + m.y * m.y + m.z * m.z));
+    const float spread = 1.2f * (1.f - ml) / sqrtf(fmaxf((float)count, 1.f));
+    o[0] = make_float4(c.x, c.y, c.z, fmaxf(I.x, fmaxf(I.y, I.z)));
+    o[1] = make_float4(m.x, m.y, m.z, 0.25f * (1.f - ml));
+    o[2] = make_float4(I.x, I.y, I.z, spread);
+    o[3] = make_float4(s2, sqrtf(s2), (float)count, 0.f);
+}
+
+// The LBVH of the light cuts: the same codes, sort and radix tree as
+// bvh_build_avpls, but the refit sums only what the cuts read (merge records
+// in nd_merge) and writes the node records to crec (kCrec x (2n - 1) float4),
+// not the reference's representative node data.  mats: the 256-entry
+// material table (Lambertian diffuse).
+int bvh_build_cut(arc_bvh* b, const AvplsDev& a, bool validate, const arc_material* mats, float4* crec);
 int random_numbers(arc_ctx* ctx, uint64_t seed, int count, float* out);
 }  // namespace arc

@@ -156,7 +156,7 @@ void arc_bvh_destroy(arc_bvh* b) {
     if (b->ctx) (void)hipStreamSynchronize(b->ctx->stream);
     arc::DevBuf* bufs[] = {&b->codes,   &b->perm,    &b->nodes,  &b->parents, &b->ranges,   &b->positions,
                            &b->normals, &b->in_pos,  &b->in_nrm, &b->nd_size, &b->nd_mat,   &b->nd_rand,
-                           &b->nd_pos,  &b->nd_nrm,  &b->nd_rad, &b->nd_dir,  &b->nd_radb, &b->nd_mom, &b->nd_nmom,  &b->staged, &b->bounds,
+                           &b->nd_pos,  &b->nd_nrm,  &b->nd_rad, &b->nd_dir,  &b->nd_merge, &b->staged, &b->bounds,
                            &b->partials, &b->ready, &b->sort.keys_alt, &b->sort.vals_alt, &b->sort.counts};
     for (arc::DevBuf* d : bufs) arc::release(*d);
     delete b;

@@ -372,6 +372,13 @@ void dispatch_exact(arc_gather* g, int model, const arc::AvplsDev& a, int chunk,
     }
 }
 
+// This frame's LBVH for the error-bound and learned cuts, with their node
+// records in g->crec.
+int build_cut_tree(arc_gather* g, const arc::AvplsDev& a, bool validate) {
+    if (a.n > 1) ARC_TRY(arc::ensure(g->ctx, g->crec, sizeof(float4) * arc::kCrec * (size_t)(2 * a.n - 1)));
+    return arc::bvh_build_cut(g->bvh, a, validate, g->mats.as<arc_material>(), g->crec.as<float4>());
+}
+
 // AVPL pointers on the device, staging host input when asked (AvplsGpu,
 // data_gpu.cu:5-52).
 int stage_avpls(arc_gather* g, const arc_avpls* in, int flags, arc::AvplsDev& out) {
@@ -529,6 +536,7 @@ void arc_gather_destroy(arc_gather* g) {
     arc::release(g->ccount);
     arc::release(g->clist);
     arc::release(g->crec);
+    arc::release(g->cest);
     arc_bvh_destroy(g->bvh);
     delete g;
 }
@@ -634,8 +642,10 @@ int arc_gather_run(arc_gather* g, const arc_avpls* avpls, arc_float3 camPos, flo
     }
     if (cut) {
         // AvplBvh(avpls, false) per frame (cudaGather.cu:858)
-        ARC_TRY(arc::bvh_build_avpls(g->bvh, a, nullptr, p->seed, (flags & ARC_TRUST_INPUT) == 0,
-                                     g->mats.as<arc_material>(), moments));
+        if (moments)
+            ARC_TRY(build_cut_tree(g, a, (flags & ARC_TRUST_INPUT) == 0));
+        else
+            ARC_TRY(arc::bvh_build_avpls(g->bvh, a, nullptr, p->seed, (flags & ARC_TRUST_INPUT) == 0));
         if (profile) ARC_HIP(ctx, hipEventRecord(ev[2], ctx->stream));
         arc_bvh* b = g->bvh;
         if (moments) {
@@ -708,6 +718,7 @@ int arc_gather_run(arc_gather* g, const arc_avpls* avpls, arc_float3 camPos, flo
             stats->errorPixels = (long long)c[C_ERRORS];
             stats->budgetHits = (long long)c[C_BUDGET];
             stats->fallbackNodes = (long long)c[C_FALLBACK];
+            stats->testedNodes = (long long)c[C_TESTED];
         }
         if (flags & ARC_RUN_PROFILE) {
             ARC_HIP(ctx, hipEventSynchronize(ev[5]));
@@ -798,8 +809,7 @@ int arc_gather_cut_samples(arc_gather* g, const arc_avpls* avpls, arc_float3 cam
     if (n < 2) return ARC_OK;
     arc::AvplsDev a{};
     ARC_TRY(stage_avpls(g, avpls, flags, a));
-    ARC_TRY(arc::bvh_build_avpls(g->bvh, a, nullptr, p->seed, (flags & ARC_TRUST_INPUT) == 0,
-                                 g->mats.as<arc_material>(), true));
+    ARC_TRY(build_cut_tree(g, a, (flags & ARC_TRUST_INPUT) == 0));
     ARC_TRY(arc::ensure(ctx, g->counters, sizeof(unsigned long long) * C_NUM));
     int* dcount = reinterpret_cast<int*>(g->counters.p);
     ARC_HIP(ctx, hipMemsetAsync(dcount, 0, sizeof(int), ctx->stream));

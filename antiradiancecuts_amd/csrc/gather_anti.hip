@@ -24,8 +24,10 @@
 //                  test against the box grown by the photon radius plus a
 //                  margin and the "pixel behind the AVPL's surface" bound
 //                  (disk), or the cone against the box's bounding sphere;
-//                  a ray meets ~10 of ~2000 clusters, so the descent tests
-//                  a few dozen boxes where a flat loop tests all of them.
+//                  a ray meets a few of ~2000 clusters (a grazing one
+//                  hundreds), so the descent tests a few dozen boxes where a
+//                  flat loop tests all of them; appends are buffered per
+//                  block in LDS and flushed by all of the block's threads.
 //                  The reference's C x N list buffer becomes C x kCap with an
 //                  overflow flag;
 //   k_anti_clustered
@@ -57,7 +59,6 @@ constexpr int kBlockTiles = kTiles / kParts;                   // tiles per bloc
 constexpr int kAntiThreads = 64 * kBlockTiles;                 // one wave per tile
 constexpr int kSub = 1536;       // tile sub-list entries in LDS (more: the tile walks the cluster list)
 constexpr int kTreeMaxLevels = 10;  // cluster quadtree levels: up to 512 x 512 clusters
-constexpr int kTreeStack = 32;      // >= 1 + 3 (levels - 1) entries of the tree descent
 
 __device__ inline bool has_anti(const arc::AvplsDev& a, int i) {
     if (!a.anti) return false;
@@ -232,44 +233,91 @@ __global__ __launch_bounds__(1024) void k_cluster_tree(float* __restrict__ tree,
 // Lists by tree descent: lane = antiradiance AVPL, depth-first through the
 // quadtree with the same conservative test at every level (a parent box
 // contains its children, so a rejected parent rejects every cluster below).
-// Each lane appends its own hits.
+// A popped node's children are loaded together and tested together, so a
+// lane waits on one round of loads per node and not one per child.  A ray
+// that sweeps the screen reaches hundreds of clusters; its appends go to a
+// block-wide LDS buffer (cluster << 8 | lane) instead of one global atomic
+// round trip each, and once every lane of the block has finished its
+// descent the block's threads share the global appends.  Entries past the
+// buffer's end are appended directly.  The descent stack (one column per
+// thread, 1 + 3 (levels - 1) entries) is dynamic LDS.
+constexpr int kListThreads = 256;
+constexpr int kPairBuf = 4096;  // buffered appends per block
+static_assert(kListThreads == 256, "an append packs the lane in 8 bits");
+
 template <int MODEL>
-__global__ __launch_bounds__(256) void k_cluster_lists_tree(arc::AvplsDev a, const int* __restrict__ idx,
-                                                            const int* __restrict__ n_anti_ptr,
-                                                            const float* __restrict__ tree, ClusterTree t,
-                                                            float grow, float cos_t, float sin_t,
-                                                            int* __restrict__ counts, int* __restrict__ lists) {
+__global__ __launch_bounds__(kListThreads) void k_cluster_lists_tree(arc::AvplsDev a, const int* __restrict__ idx,
+                                                                     const int* __restrict__ n_anti_ptr,
+                                                                     const float4* __restrict__ tree, ClusterTree t,
+                                                                     float grow, float cos_t, float sin_t,
+                                                                     int* __restrict__ counts, int* __restrict__ lists) {
     __shared__ int s_off[kTreeMaxLevels], s_w[kTreeMaxLevels], s_h[kTreeMaxLevels];
-    __shared__ int stk[kTreeStack][256];
-    if (threadIdx.x < kTreeMaxLevels) {
-        s_off[threadIdx.x] = t.off[threadIdx.x];
-        s_w[threadIdx.x] = t.w[threadIdx.x];
-        s_h[threadIdx.x] = t.h[threadIdx.x];
-    }
-    __syncthreads();
-    const int j = blockIdx.x * 256 + threadIdx.x;
-    if (j >= *n_anti_ptr) return;  // no barrier follows
-    const int i = idx[j];
-    const f3 p = a.pos[i], w = a.dir[i], na = a.nrm[i];
-    const float ix = safe_inv(w.x), iy = safe_inv(w.y), iz = safe_inv(w.z);
+    __shared__ int s_pair[kPairBuf];
+    __shared__ int s_n;
+    extern __shared__ int stk[];  // [depth][kListThreads]
     const int tid = threadIdx.x;
-    int sp = 0;
-    stk[sp++][tid] = (t.levels - 1) << 24;  // root: (level, y, x) = (top, 0, 0)
-    while (sp > 0) {
-        const int v = stk[--sp][tid];
-        const int l = v >> 24, y = (v >> 12) & 4095, x = v & 4095;
-        const float* b8 = tree + 8 * (size_t)(s_off[l] + y * s_w[l] + x);
-        const float b[6] = {b8[0], b8[1], b8[2], b8[4], b8[5], b8[6]};
-        if (!ray_box_hit<MODEL>(p, w, na, ix, iy, iz, b, grow, cos_t, sin_t)) continue;
-        if (l == 0) {
-            const int c = y * s_w[0] + x;
+    if (tid < kTreeMaxLevels) {
+        s_off[tid] = t.off[tid];
+        s_w[tid] = t.w[tid];
+        s_h[tid] = t.h[tid];
+    }
+    if (tid == 0) s_n = 0;
+    __syncthreads();
+    const int n_anti = *n_anti_ptr;
+    const int base = blockIdx.x * kListThreads;
+    const int j = base + tid;
+    auto append = [&](int c, int i) {
+        const int k = atomicAdd(&s_n, 1);
+        if (k < kPairBuf) {
+            s_pair[k] = (c << 8) | tid;
+        } else {
             const int slot = atomicAdd(&counts[c], 1);
             if (slot < kCap) lists[(size_t)c * kCap + slot] = i;
-            continue;
         }
-        const int cw = s_w[l - 1], ch = s_h[l - 1];
-        for (int cy = min(2 * y + 1, ch - 1); cy >= 2 * y; --cy)
-            for (int cx = min(2 * x + 1, cw - 1); cx >= 2 * x; --cx) stk[sp++][tid] = ((l - 1) << 24) | (cy << 12) | cx;
+    };
+    if (j < n_anti) {
+        const int i = idx[j];
+        const f3 p = a.pos[i], w = a.dir[i], na = a.nrm[i];
+        const float ix = safe_inv(w.x), iy = safe_inv(w.y), iz = safe_inv(w.z);
+        const int top = t.levels - 1;
+        const float4 r0 = tree[2 * s_off[top]], r1 = tree[2 * s_off[top] + 1];
+        const float rb[6] = {r0.x, r0.y, r0.z, r1.x, r1.y, r1.z};
+        if (ray_box_hit<MODEL>(p, w, na, ix, iy, iz, rb, grow, cos_t, sin_t)) {
+            if (top == 0) {
+                append(0, i);
+            } else {
+                int sp = 0;
+                stk[(sp++) * kListThreads + tid] = top << 24;  // (level, y, x) of a node whose box was hit
+                while (sp > 0) {
+                    const int v = stk[(--sp) * kListThreads + tid];
+                    const int l = (v >> 24) - 1, y = 2 * ((v >> 12) & 4095), x = 2 * (v & 4095);
+                    const int cw = s_w[l], ch = s_h[l], off = s_off[l];
+                    const bool ok[4] = {true, x + 1 < cw, y + 1 < ch, x + 1 < cw && y + 1 < ch};
+                    float4 lo[4], hi[4];
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) {  // a missing child loads the first one's box
+                        const int k = off + (ok[q] ? (y + (q >> 1)) * cw + x + (q & 1) : y * cw + x);
+                        lo[q] = tree[2 * k];
+                        hi[q] = tree[2 * k + 1];
+                    }
+#pragma unroll
+                    for (int q = 3; q >= 0; --q) {  // pushed in reverse: popped in row order
+                        const float b[6] = {lo[q].x, lo[q].y, lo[q].z, hi[q].x, hi[q].y, hi[q].z};
+                        if (!ok[q] || !ray_box_hit<MODEL>(p, w, na, ix, iy, iz, b, grow, cos_t, sin_t)) continue;
+                        const int cy = y + (q >> 1), cx = x + (q & 1);
+                        if (l == 0) append(cy * cw + cx, i);
+                        else stk[(sp++) * kListThreads + tid] = (l << 24) | (cy << 12) | cx;
+                    }
+                }
+            }
+        }
+    }
+    __syncthreads();
+    const int np = min(s_n, kPairBuf);
+    for (int k = tid; k < np; k += kListThreads) {
+        const int e = s_pair[k], c = e >> 8;
+        const int slot = atomicAdd(&counts[c], 1);
+        if (slot < kCap) lists[(size_t)c * kCap + slot] = idx[base + (e & 255)];
     }
 }
 
@@ -462,13 +510,15 @@ int anti_clustered(arc_gather* g, const AvplsDev& a, int model, float3 cam, cons
     const int* n_anti = offs + nb;
     if (tree.levels > 1)
         hipLaunchKernelGGL(k_cluster_tree, dim3(1), dim3(1024), 0, ctx->stream, g->cbox.as<float>(), tree);
-    const dim3 lgrid((n + 255) / 256);
+    const dim3 lgrid((n + kListThreads - 1) / kListThreads);
+    const size_t lstack = sizeof(int) * kListThreads * (size_t)(1 + 3 * (tree.levels - 1));
+    const float4* tbox = reinterpret_cast<const float4*>(g->cbox.as<float>());
     if (model == ARC_ANTIRAD_DISK)
-        hipLaunchKernelGGL(k_cluster_lists_tree<ARC_ANTIRAD_DISK>, lgrid, dim3(256), 0, ctx->stream, a, idx, n_anti,
-                           g->cbox.as<float>(), tree, grow, cos_t, sin_t, g->ccount.as<int>(), g->clist.as<int>());
+        hipLaunchKernelGGL(k_cluster_lists_tree<ARC_ANTIRAD_DISK>, lgrid, dim3(kListThreads), lstack, ctx->stream, a,
+                           idx, n_anti, tbox, tree, grow, cos_t, sin_t, g->ccount.as<int>(), g->clist.as<int>());
     else
-        hipLaunchKernelGGL(k_cluster_lists_tree<ARC_ANTIRAD_CONE>, lgrid, dim3(256), 0, ctx->stream, a, idx, n_anti,
-                           g->cbox.as<float>(), tree, grow, cos_t, sin_t, g->ccount.as<int>(), g->clist.as<int>());
+        hipLaunchKernelGGL(k_cluster_lists_tree<ARC_ANTIRAD_CONE>, lgrid, dim3(kListThreads), lstack, ctx->stream, a,
+                           idx, n_anti, tbox, tree, grow, cos_t, sin_t, g->ccount.as<int>(), g->clist.as<int>());
     ARC_LAUNCHED(ctx);
     if (mark) ARC_HIP(ctx, hipEventRecord(mark, ctx->stream));
     if (model == ARC_ANTIRAD_DISK) {

@@ -74,7 +74,7 @@ constexpr float kGuard = 1e-5f;
 
 enum Counter {
     C_VISITED = 0, C_EVALUATED = 1, C_ANTI_VISITED = 2, C_ANTI_EVALUATED = 3, C_ERRORS = 4, C_BUDGET = 5,
-    C_FALLBACK = 6, C_NUM = 8
+    C_FALLBACK = 6, C_TESTED = 7, C_NUM = 8
 };
 
 struct Kahan {

@@ -44,12 +44,14 @@
 //                 neighbouring pixels see the AVPLs alike (smooth surfaces far
 //                 from the AVPLs); with AVPLs around the pixels the tile cut
 //                 is the union of 64 different near fields.
+#include <algorithm>
 #include <cmath>
 
 #include "cutnet.hpp"
 #include "gather_common.hpp"
 
 using arc::f3;
+using arc::kCrec;
 using namespace arc::gm;
 using namespace arc::gd;
 namespace net = arc::net;
@@ -59,57 +61,13 @@ namespace {
 constexpr int kTileStack = 96;  // per-wave stack of the tile traversal
 constexpr int kPxStack = 48;    // per-lane stack of the pixel traversal (LDS)
 constexpr int kCutThreads = 256;
-constexpr int kCrec = 4;        // float4 per node record
 constexpr int kEstStack = 48;   // per-wave stack of the depth-d0 estimate cut (d0 <= 40)
 
 struct CutEB {
     const arc_bvh_node* nodes;  // n - 1 internal nodes
-    const float4* crec;         // kCrec float4 per slot, see k_pack_cut
+    const float4* crec;         // arc::kCrec float4 per slot (arc::write_cut_record)
     int n;
 };
-
-// Node records, one 64-byte line per slot (leaves, then internal nodes):
-//   r0 = (centroid, max(I)), r1 = (m, (1 - |m|) / 4), r2 = (I, 1.2 (1 - |m|) / sqrt(count)),
-//   r3 = (s^2, s, count, -) with s the half-diagonal of the node's box (0 for a leaf).
-// Everything of the tests that does not depend on the pixel is here.
-__global__ __launch_bounds__(kCutThreads) void k_pack_cut(int n, const f3* __restrict__ pos,
-                                                          const f3* __restrict__ nrm,
-                                                          const f3* __restrict__ radb,
-                                                          const float4* __restrict__ mom,
-                                                          const f3* __restrict__ nmom, const int* __restrict__ size,
-                                                          const arc_bvh_node* __restrict__ nodes,
-                                                          float4* __restrict__ crec) {
-    const int s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= 2 * n - 1) return;
-    const f3 I = radb[s];
-    const float4 mo = mom[s];
-    f3 c, m;
-    if (mo.w > 0.f) {
-        const float iw = 1.0f / mo.w;
-        c = f3{mo.x * iw, mo.y * iw, mo.z * iw};
-        const f3 nm = nmom[s];
-        m = f3{nm.x * iw, nm.y * iw, nm.z * iw};
-    } else {  // no intensity: the representative's data (the node contributes 0 anyway)
-        c = pos[s];
-        const f3 q = nrm[s];
-        const float l = q.x * q.x + q.y * q.y + q.z * q.z;
-        const float iq = l > 0.f ? 1.0f / sqrtf(l) : 0.f;
-        m = f3{q.x * iq, q.y * iq, q.z * iq};
-    }
-    const float ml = fminf(1.f, sqrtf(m.x * m.x + m.y * m.y + m.z * m.z));
-    float s2 = 0.f;
-    if (s >= n) {
-        const arc_bvh_node& b = nodes[s - n];
-        const float ex = b.bbMax.x - b.bbMin.x, ey = b.bbMax.y - b.bbMin.y, ez = b.bbMax.z - b.bbMin.z;
-        s2 = 0.25f * (ex * ex + ey * ey + ez * ez);
-    }
-    const float spread = 1.2f * (1.f - ml) / sqrtf(fmaxf((float)size[s], 1.f));
-    float4* o = crec + kCrec * (size_t)s;
-    o[0] = make_float4(c.x, c.y, c.z, fmaxf(I.x, fmaxf(I.y, I.z)));
-    o[1] = make_float4(m.x, m.y, m.z, 0.25f * (1.f - ml));
-    o[2] = make_float4(I.x, I.y, I.z, spread);
-    o[3] = make_float4(s2, sqrtf(s2), (float)size[s], 0.f);
-}
 
 // G~(p, node) (cudaGather.cu:45-57 with the node's moments); for a leaf the
 // exact G of the AVPL.
@@ -119,7 +77,10 @@ __device__ inline float geom_est(const PixelF& P, const float4& r0, const float4
     // coincident points (closer than 1e-15): no contribution, not 0 * inf
     const float inv = dd > 1e-30f ? __builtin_amdgcn_rsqf(dd) : 0.f;
     const float c1 = fminf(fmaxf((P.nx * dx + P.ny * dy + P.nz * dz) * inv, 0.f), 1.f);
-    const float c2 = fminf(r1.w + fmaxf(-(r1.x * dx + r1.y * dy + r1.z * dz) * inv, 0.f), 1.f);
+    // r1.w >= 0, so the sum is >= 0 and clamping it to [0, 1] is min(sum, 1):
+    // written as a clamp, the add carries it as an output modifier
+    const float c2 =
+        fminf(fmaxf(r1.w + fmaxf(-(r1.x * dx + r1.y * dy + r1.z * dz) * inv, 0.f), 0.f), 1.f);
     return c1 * c2 * (inv * inv);
 }
 
@@ -156,7 +117,9 @@ __device__ inline float node_error(const PixelF& P, const arc_bvh_node& b, const
     // 1 ulp) suffice
     const float near = fmaxf(__builtin_amdgcn_sqrtf(dc2) - r3.y, delta);
     const float inv_near = __builtin_amdgcn_rcpf(near);
-    const float cb = fminf(1.f, hmax * inv_near);
+    // clamped at 0 as well: an output modifier of the multiply (the result
+    // for hmax <= 0 is 0 below either way)
+    const float cb = fminf(fmaxf(hmax * inv_near, 0.f), 1.f);
     const float rel = r3.x * __builtin_amdgcn_rcpf(dc2) + r2.w;
     const float e = r0.w * cb * (inv_near * inv_near) * rel;
     // behind the pixel's tangent plane: exactly 0; within 2 s of the centroid: refine
@@ -273,6 +236,78 @@ __global__ __launch_bounds__(kCutThreads) void k_cut_estimate(CutEB t, const flo
     if (tid == 0) atomicAdd(sum, red[0] + red[1] + red[2] + red[3]);
 }
 
+// The same estimate with the depth-d0 cut listed per block.  The walk above
+// is a chain of dependent loads per wave (stack, node, children); here each
+// block first expands the frontier level by level (a node stays when it is
+// a leaf or at depth d0; wave ballots give each entry its place), stages the
+// cut's (r0, r1) records in LDS right to left, the order in which the walk
+// sums them, and then every lane streams them: both kernels give the same
+// sums.  The frontier holds at most 2^d0 <= kCutThreads entries.
+constexpr int kEstListDepth = 8;
+static_assert((1 << kEstListDepth) <= kCutThreads, "one frontier entry per thread");
+
+__global__ __launch_bounds__(kCutThreads) void k_cut_estimate_list(CutEB t, int d0, const float4* __restrict__ gpos,
+                                                                   const float4* __restrict__ gnrm,
+                                                                   const float4* __restrict__ galb,
+                                                                   const arc_material* __restrict__ mats, int W,
+                                                                   int H, float* __restrict__ est,
+                                                                   float* __restrict__ sum) {
+    extern __shared__ float4 cut[];  // (r0, r1) per cut node: 2 x min(2^d0, n) entries
+    __shared__ int fr[2][kCutThreads];
+    __shared__ int wsum[kCutThreads / 64];
+    __shared__ float red[kCutThreads / 64];
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const uint64_t below = (1ull << lane) - 1;
+    int cur = 0, m = 1;
+    if (tid == 0) fr[0][0] = -1;  // the root
+    __syncthreads();
+    for (int dep = 0; dep < d0; ++dep) {  // m <= 2^dep <= kCutThreads / 2
+        const bool act = tid < m;
+        const int v = act ? fr[cur][tid] : 0;
+        const bool inner = act && v < 0;
+        const uint64_t ba = __ballot(act), bi = __ballot(inner);
+        if (lane == 0) wsum[wave] = __popcll(ba) + __popcll(bi);
+        __syncthreads();
+        int pos = __popcll(ba & below) + __popcll(bi & below), next = 0;
+        for (int w = 0; w < kCutThreads / 64; ++w) {
+            pos += w < wave ? wsum[w] : 0;
+            next += wsum[w];
+        }
+        if (inner) {
+            const arc_bvh_node& b = t.nodes[-(v + 1)];
+            fr[1 - cur][pos] = b.left;
+            fr[1 - cur][pos + 1] = b.right;
+        } else if (act) {
+            fr[1 - cur][pos] = v;
+        }
+        __syncthreads();  // the new frontier is complete and wsum is read
+        cur = 1 - cur;
+        m = next;
+    }
+    for (int k = tid; k < m; k += kCutThreads) {
+        const int v = fr[cur][m - 1 - k];
+        const float4* r = t.crec + kCrec * (size_t)(v >= 0 ? v : t.n + (-(v + 1)));
+        cut[2 * k] = r[0];
+        cut[2 * k + 1] = r[1];
+    }
+    __syncthreads();
+    const int2 xy = tile_pixel(wave, lane, (W + 15) / 16, (H + 15) / 16);
+    const bool live = xy.x < W && xy.y < H;
+    const int q = live ? xy.y * W + xy.x : 0;
+    const PixelF P = load_pixel(gpos, gnrm, galb, mats, q);
+    float e = 0.f;
+    for (int k = 0; k < m; ++k) {
+        const float4 r0 = cut[2 * k], r1 = cut[2 * k + 1];
+        e += r0.w * geom_est(P, r0, r1);
+    }
+    if (live) est[q] = e;
+    float w = live ? e : 0.f;
+    for (int off = 32; off > 0; off >>= 1) w += __shfl_down(w, off);
+    if (lane == 0) red[wave] = w;
+    __syncthreads();
+    if (tid == 0) atomicAdd(sum, red[0] + red[1] + red[2] + red[3]);
+}
+
 // ------------------------------------------------------------ per pixel --
 __global__ __launch_bounds__(kCutThreads) void k_cut_eb_px(CutEB t, const float4* __restrict__ gpos,
                                                            const float4* __restrict__ gnrm,
@@ -285,7 +320,7 @@ __global__ __launch_bounds__(kCutThreads) void k_cut_eb_px(CutEB t, const float4
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const int2 xy = tile_pixel(wave, lane, (W + 15) / 16, (H + 15) / 16);
     const bool live = xy.x < W && xy.y < H;
-    unsigned long long nvis = 0, nev = 0, nerr = 0, nbud = 0;
+    unsigned long long nvis = 0, nev = 0, nerr = 0, nbud = 0, ntest = 0;
     if (live) {
         const int q = xy.y * W + xy.x;
         const PixelF P = load_pixel(gpos, gnrm, galb, mats, q);
@@ -306,6 +341,7 @@ __global__ __launch_bounds__(kCutThreads) void k_cut_eb_px(CutEB t, const float4
                 const float4* r = t.crec + kCrec * (size_t)slot;
                 const float crit = criterion ? node_error(P, b, r[0], r[2], r[3], delta)
                                              : node_bound(P, b, r[0].w, delta * delta);
+                ++ntest;
                 if (crit > thr) {
                     if ((int)nev >= budget) {
                         ++nbud;  // cut budget spent: keep the cluster
@@ -335,8 +371,10 @@ __global__ __launch_bounds__(kCutThreads) void k_cut_eb_px(CutEB t, const float4
             nev += __shfl_down(nev, off);
             nerr += __shfl_down(nerr, off);
             nbud += __shfl_down(nbud, off);
+            ntest += __shfl_down(ntest, off);
         }
         if (lane == 0) {
+            atomicAdd(&counters[C_TESTED], ntest);
             atomicAdd(&counters[C_VISITED], nvis);
             atomicAdd(&counters[C_EVALUATED], nev);
             atomicAdd(&counters[C_ERRORS], nerr);
@@ -369,7 +407,7 @@ __global__ __launch_bounds__(kCutThreads) void k_cut_eb_tile(CutEB t, const floa
     const float thr = live ? th.at(q) : INFINITY;
     const float delta2 = delta * delta;
     Acc acc;
-    unsigned nvis = 0, nev = 0, nerr = 0, nbud = 0;  // wave-uniform: scalar registers
+    unsigned nvis = 0, nev = 0, nerr = 0, nbud = 0, ntest = 0;  // wave-uniform: scalar registers
     int sp = 0;
     int node = -1;
     for (;;) {
@@ -381,6 +419,7 @@ __global__ __launch_bounds__(kCutThreads) void k_cut_eb_tile(CutEB t, const floa
             const float4* r = t.crec + kCrec * (size_t)slot;
             const float crit = criterion ? node_error(P, b, r[0], r[2], r[3], delta)
                                          : node_bound(P, b, r[0].w, delta2);
+            ++ntest;
             if (__ballot(crit > thr)) {
                 if ((int)nev >= budget) {
                     ++nbud;  // cut budget spent: keep the cluster
@@ -402,6 +441,7 @@ __global__ __launch_bounds__(kCutThreads) void k_cut_eb_tile(CutEB t, const floa
     acc.flush();
     const int alive = __popcll(__ballot(live));  // all lanes take part in the ballot
     if (counters && lane == 0) {
+        atomicAdd(&counters[C_TESTED], (unsigned long long)ntest * alive);
         atomicAdd(&counters[C_VISITED], (unsigned long long)nvis * alive);
         atomicAdd(&counters[C_EVALUATED], (unsigned long long)nev * alive);
         atomicAdd(&counters[C_ERRORS], (unsigned long long)nerr);
@@ -770,6 +810,7 @@ __global__ __launch_bounds__(kCutThreads) __attribute__((amdgpu_waves_per_eu(4))
         atomicAdd(&counters[C_ERRORS], (unsigned long long)nerr);
         atomicAdd(&counters[C_BUDGET], (unsigned long long)nbud);
         atomicAdd(&counters[C_FALLBACK], (unsigned long long)nfb);
+        atomicAdd(&counters[C_TESTED], (unsigned long long)nfb * alive);
     }
     if (live)
         out_rad[q] = make_float4(acc.k[0].s * P.br, acc.k[1].s * P.bg, acc.k[2].s * P.bb,
@@ -782,26 +823,30 @@ namespace arc {
 
 namespace {
 
-// Node records, per-pixel estimates of the depth-d0 cut and the thresholds.
+// Per-pixel estimates of the depth-d0 cut and the thresholds (the node
+// records come with the tree, bvh_build_cut).
 int prepare_cut(arc_gather* g, arc_bvh* b, const CutEBParams& cp, CutEB& t, Threshold& th) {
     arc_ctx* ctx = g->ctx;
     const int n = b->n;
     const int npix = g->width * g->height;
     if (n > (1 << 25)) return fail(ctx, ARC_E_UNSUPPORTED, "error-bound cut: more than 2^25 AVPLs");
-    if (!b->has_moments) return fail(ctx, ARC_E_STATE, "error-bound cut: LBVH built without node moments");
-    ARC_TRY(ensure(ctx, g->crec, sizeof(float4) * kCrec * (size_t)(2 * n - 1)));
+    if (!b->has_cut_records || !g->crec.p || g->crec.bytes < sizeof(float4) * kCrec * (size_t)(2 * n - 1))
+        return fail(ctx, ARC_E_STATE, "light cut: LBVH built without node records");
     ARC_TRY(ensure(ctx, g->cest, sizeof(float) * ((size_t)npix + 64)));
     float* est = g->cest.as<float>();
     float* sum = est + ((size_t)npix + 63) / 64 * 64;
-    hipLaunchKernelGGL(k_pack_cut, dim3((2 * n - 1 + kCutThreads - 1) / kCutThreads), dim3(kCutThreads), 0,
-                       ctx->stream, n, b->nd_pos.as<f3>(), b->nd_nrm.as<f3>(), b->nd_radb.as<f3>(),
-                       b->nd_mom.as<float4>(), b->nd_nmom.as<f3>(), b->nd_size.as<int>(), b->nodes.as<arc_bvh_node>(),
-                       g->crec.as<float4>());
     t = CutEB{b->nodes.as<arc_bvh_node>(), g->crec.as<float4>(), n};
     const dim3 grid(((g->width + 15) / 16) * ((g->height + 15) / 16));
     hipLaunchKernelGGL(k_cut_zero, dim3(1), dim3(64), 0, ctx->stream, sum);
-    hipLaunchKernelGGL(k_cut_estimate, grid, dim3(kCutThreads), 0, ctx->stream, t, g->gpos, g->gnrm, g->galb,
-                       g->mats.as<arc_material>(), g->width, g->height, cp.d0, est, sum);
+    if (cp.d0 <= kEstListDepth) {
+        const int bound = std::min(1 << cp.d0, n);  // the cut's nodes partition the n leaves
+        hipLaunchKernelGGL(k_cut_estimate_list, grid, dim3(kCutThreads), sizeof(float4) * 2 * bound, ctx->stream,
+                           t, cp.d0, g->gpos, g->gnrm, g->galb, g->mats.as<arc_material>(), g->width, g->height,
+                           est, sum);
+    } else {
+        hipLaunchKernelGGL(k_cut_estimate, grid, dim3(kCutThreads), 0, ctx->stream, t, g->gpos, g->gnrm, g->galb,
+                           g->mats.as<arc_material>(), g->width, g->height, cp.d0, est, sum);
+    }
     th = Threshold{est, sum, cp.eps, cp.image_floor, 1.0f / (float)npix};
     return ARC_OK;
 }

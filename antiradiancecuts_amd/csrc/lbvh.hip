@@ -26,6 +26,9 @@
 namespace {
 
 using arc::f3;
+using arc::kCrec;
+using arc::kMrec;
+using arc::write_cut_record;
 
 constexpr int kBlock = 256;
 constexpr int kReduceBlocks = 512;
@@ -169,9 +172,13 @@ struct NodeData {  // AvplBvhNodeDataParam over 2n-1 slots
     f3* nrm;
     f3* rad;
     f3* dir;
-    f3* radb;  // sums of L x diffuse / pi, the BRDF-weighted intensity (extension; may be null)
-    float4* mom;  // with radb: sums of w x position and w (w = mean of L x diffuse / pi)
-    f3* nmom;     // with radb: sums of w x unit normal
+};
+
+// bvh_build_cut's per-slot records: merge sums (kMrec float4) and the cut's
+// node records (kCrec float4).
+struct CutRecs {
+    float4* merge;
+    float4* crec;
 };
 
 // The payload of the sort gathered by the permutation: read at random, four
@@ -191,8 +198,7 @@ __global__ __launch_bounds__(kBlock) void k_stage_avpls(arc::AvplsDev a, float4*
 
 __global__ __launch_bounds__(kBlock) void k_permute_avpls(const int* __restrict__ perm, int n,
                                                           const float4* __restrict__ staged,
-                                                          f3* __restrict__ spos, f3* __restrict__ snrm, NodeData nd,
-                                                          const arc_material* __restrict__ mats) {
+                                                          f3* __restrict__ spos, f3* __restrict__ snrm, NodeData nd) {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= n) return;
     const float4* rec = staged + 4 * (size_t)perm[s];
@@ -207,23 +213,37 @@ __global__ __launch_bounds__(kBlock) void k_permute_avpls(const int* __restrict_
     nd.nrm[s] = q;
     nd.rad[s] = L;
     nd.dir[s] = w;
-    if (nd.radb) {
-        // L x f_r of the AVPL's Lambertian material (f_r = 1 for a light-source
-        // AVPL, w = 0, cudaGather.cu:89-90); the table holds 256 entries
-        f3 r = L;
-        if (mats && (w.x != 0.f || w.y != 0.f || w.z != 0.f)) {
-            const arc_material& m = mats[mat & 255];
-            r = f3{L.x * (0.318309886f * m.diffuse.x), L.y * (0.318309886f * m.diffuse.y),
-                   L.z * (0.318309886f * m.diffuse.z)};
-        }
-        nd.radb[s] = r;
-        const f3 x = p;
-        const float w8 = (r.x + r.y + r.z) * (1.0f / 3.0f);
-        const float ql = q.x * q.x + q.y * q.y + q.z * q.z;
-        const float iq = ql > 0.f ? 1.0f / sqrtf(ql) : 0.f;
-        nd.mom[s] = make_float4(w8 * x.x, w8 * x.y, w8 * x.z, w8);
-        nd.nmom[s] = f3{w8 * q.x * iq, w8 * q.y * iq, w8 * q.z * iq};
+}
+
+// The leaves of bvh_build_cut: position for the box refit, the merge sums
+// and the node record of each sorted AVPL.  L x f_r of the AVPL's Lambertian
+// material (f_r = 1 for a light-source AVPL, w = 0, cudaGather.cu:89-90); the
+// table holds 256 entries.
+__global__ __launch_bounds__(kBlock) void k_permute_cut(const int* __restrict__ perm, int n,
+                                                        const float4* __restrict__ staged, f3* __restrict__ spos,
+                                                        const arc_material* __restrict__ mats, CutRecs cr) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    const float4* rec = staged + 4 * (size_t)perm[s];
+    const float4 r0 = rec[0], r1 = rec[1], r2 = rec[2], r3 = rec[3];
+    const f3 p{r0.x, r0.y, r0.z}, q{r1.x, r1.y, r1.z}, L{r2.x, r2.y, r2.z};
+    spos[s] = p;
+    f3 r = L;
+    if (r3.x != 0.f || r3.y != 0.f || r3.z != 0.f) {
+        const arc_material& m = mats[__float_as_int(r0.w) & 255];
+        r = f3{L.x * (0.318309886f * m.diffuse.x), L.y * (0.318309886f * m.diffuse.y),
+               L.z * (0.318309886f * m.diffuse.z)};
     }
+    const float w8 = (r.x + r.y + r.z) * (1.0f / 3.0f);
+    const float ql = q.x * q.x + q.y * q.y + q.z * q.z;
+    const float iq = ql > 0.f ? 1.0f / sqrtf(ql) : 0.f;
+    const float4 mo = make_float4(w8 * p.x, w8 * p.y, w8 * p.z, w8);
+    const f3 nm{w8 * q.x * iq, w8 * q.y * iq, w8 * q.z * iq};
+    float4* mg = cr.merge + kMrec * (size_t)s;
+    mg[0] = make_float4(r.x, r.y, r.z, __int_as_float(1));
+    mg[1] = mo;
+    mg[2] = make_float4(nm.x, nm.y, nm.z, 0.f);
+    write_cut_record(cr.crec + kCrec * (size_t)s, r, mo, nm, 1, 0.f, p, q);
 }
 
 __global__ __launch_bounds__(kBlock) void k_random(uint64_t seed, int count, float* __restrict__ out) {
@@ -303,14 +323,29 @@ __device__ inline void cluster(int target, int left, int right, int n, const Nod
     nd.dir[tid] = nd.dir[repr];
     nd.pos[tid] = nd.pos[repr];
     nd.nrm[tid] = nd.nrm[repr];
-    if (nd.radb) {
-        const f3 al = nd.radb[lid], ar = nd.radb[rid];
-        nd.radb[tid] = f3{al.x + ar.x, al.y + ar.y, al.z + ar.z};
-        const float4 ml = nd.mom[lid], mr = nd.mom[rid];
-        nd.mom[tid] = make_float4(ml.x + mr.x, ml.y + mr.y, ml.z + mr.z, ml.w + mr.w);
-        const f3 nl = nd.nmom[lid], nr = nd.nmom[rid];
-        nd.nmom[tid] = f3{nl.x + nr.x, nl.y + nr.y, nl.z + nr.z};
-    }
+}
+
+// The cut's sums of an internal node from its children's (left + right, in
+// that order), and its node record; lo / hi: the node's box.
+__device__ inline void merge_cut(int target, int left, int right, int n, f3 lo, f3 hi, const CutRecs& cr) {
+    const size_t tid = (size_t)n + target;
+    const size_t lid = left < 0 ? (size_t)n - (left + 1) : (size_t)left;
+    const size_t rid = right < 0 ? (size_t)n - (right + 1) : (size_t)right;
+    const float4* a = cr.merge + kMrec * lid;
+    const float4* b = cr.merge + kMrec * rid;
+    const float4 a0 = a[0], a1 = a[1], a2 = a[2], b0 = b[0], b1 = b[1], b2 = b[2];
+    const f3 I{a0.x + b0.x, a0.y + b0.y, a0.z + b0.z};
+    const int count = __float_as_int(a0.w) + __float_as_int(b0.w);
+    const float4 mo = make_float4(a1.x + b1.x, a1.y + b1.y, a1.z + b1.z, a1.w + b1.w);
+    const f3 nm{a2.x + b2.x, a2.y + b2.y, a2.z + b2.z};
+    float4* mg = cr.merge + kMrec * tid;
+    mg[0] = make_float4(I.x, I.y, I.z, __int_as_float(count));
+    mg[1] = mo;
+    mg[2] = make_float4(nm.x, nm.y, nm.z, 0.f);
+    const float ex = hi.x - lo.x, ey = hi.y - lo.y, ez = hi.z - lo.z;
+    const float s2 = 0.25f * (ex * ex + ey * ey + ez * ez);
+    const f3 centre{0.5f * (lo.x + hi.x), 0.5f * (lo.y + hi.y), 0.5f * (lo.z + hi.z)};
+    write_cut_record(cr.crec + kCrec * tid, I, mo, nm, count, s2, centre, f3{0.f, 0.f, 0.f});
 }
 
 // kernel_innerNodes (bvh.cu:280-316).  One thread per leaf walks up; the
@@ -333,8 +368,12 @@ __device__ inline void cluster(int target, int left, int right, int n, const Nod
 // Every internal node ends with visited == 2, as the reference's counters do.
 constexpr int kRefitLeaves = 1024;
 
-template <bool kCluster>
-__device__ inline void merge_node(int parent, arc_bvh_node* nodes, const f3* spos, int n, const NodeData& nd) {
+// What a refit merges besides the boxes.
+enum RefitMode { kBoxes = 0, kReference = 1, kCut = 2 };
+
+template <int kMode>
+__device__ inline void merge_node(int parent, arc_bvh_node* nodes, const f3* spos, int n, const NodeData& nd,
+                                  const CutRecs& cr) {
     const int l = nodes[parent].left, r = nodes[parent].right;
     f3 lmin, lmax, rmin, rmax;
     if (l < 0) {
@@ -351,17 +390,21 @@ __device__ inline void merge_node(int parent, arc_bvh_node* nodes, const f3* spo
     } else {
         rmin = rmax = spos[r];
     }
-    nodes[parent].bbMin = arc_float3{fminf(lmin.x, rmin.x), fminf(lmin.y, rmin.y), fminf(lmin.z, rmin.z)};
-    nodes[parent].bbMax = arc_float3{fmaxf(lmax.x, rmax.x), fmaxf(lmax.y, rmax.y), fmaxf(lmax.z, rmax.z)};
-    if (kCluster) cluster(parent, l, r, n, nd);
+    const f3 lo{fminf(lmin.x, rmin.x), fminf(lmin.y, rmin.y), fminf(lmin.z, rmin.z)};
+    const f3 hi{fmaxf(lmax.x, rmax.x), fmaxf(lmax.y, rmax.y), fmaxf(lmax.z, rmax.z)};
+    nodes[parent].bbMin = arc_float3{lo.x, lo.y, lo.z};
+    nodes[parent].bbMax = arc_float3{hi.x, hi.y, hi.z};
+    if (kMode == kReference) cluster(parent, l, r, n, nd);
+    if (kMode == kCut) merge_cut(parent, l, r, n, lo, hi, cr);
 }
 
-template <bool kCluster>
+template <int kMode>
 __global__ __launch_bounds__(kRefitLeaves) void k_refit_local(arc_bvh_node* __restrict__ nodes,
                                                               const int* __restrict__ parents,
                                                               const int* __restrict__ ranges,
                                                               const f3* __restrict__ spos, int n, NodeData nd,
-                                                              int* __restrict__ ready, int* __restrict__ nready) {
+                                                              CutRecs cr, int* __restrict__ ready,
+                                                              int* __restrict__ nready) {
     __shared__ int arrivals[kRefitLeaves];
     arrivals[threadIdx.x] = 0;
     __syncthreads();
@@ -380,23 +423,23 @@ __global__ __launch_bounds__(kRefitLeaves) void k_refit_local(arc_bvh_node* __re
         }
         if (__hip_atomic_fetch_add(&arrivals[parent - lo], 1, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP) == 0)
             return;
-        merge_node<kCluster>(parent, nodes, spos, n, nd);
+        merge_node<kMode>(parent, nodes, spos, n, nd, cr);
         nodes[parent].visited = 2;
         parent = parents[n + parent];
     }
 }
 
-template <bool kCluster>
+template <int kMode>
 __global__ __launch_bounds__(kRefitLeaves) void k_refit_top(arc_bvh_node* __restrict__ nodes,
                                                             const int* __restrict__ parents,
                                                             const f3* __restrict__ spos, int n, NodeData nd,
-                                                            const int* __restrict__ ready,
+                                                            CutRecs cr, const int* __restrict__ ready,
                                                             const int* __restrict__ nready) {
     const int count = *nready;
     for (int j = threadIdx.x; j < count; j += blockDim.x) {
         int node = ready[j];
         while (true) {
-            merge_node<kCluster>(node, nodes, spos, n, nd);
+            merge_node<kMode>(node, nodes, spos, n, nd, cr);
             const int parent = parents[n + node];
             if (parent < 0) break;
             // 0: the sibling subtree is still pending and will climb on; 1: both done
@@ -441,27 +484,30 @@ int encode_and_sort(arc_bvh* b, const f3* pos, const f3* nrm, int n) {
     return arc::sort_pairs(ctx, b->sort, b->codes.as<uint32_t>(), b->perm.as<int>(), n, key_bits);
 }
 
-int build_tree(arc_bvh* b, int n, const NodeData* nd) {
+template <int kMode>
+void launch_refit(arc_bvh* b, int n, const NodeData& nd, const CutRecs& cr, int* ready, int* nready) {
+    const int rblocks = (n + kRefitLeaves - 1) / kRefitLeaves;
+    auto* nodes = b->nodes.as<arc_bvh_node>();
+    hipLaunchKernelGGL(k_refit_local<kMode>, dim3(rblocks), dim3(kRefitLeaves), 0, b->ctx->stream, nodes,
+                       b->parents.as<int>(), b->ranges.as<int>(), b->positions.as<f3>(), n, nd, cr, ready, nready);
+    hipLaunchKernelGGL(k_refit_top<kMode>, dim3(1), dim3(kRefitLeaves), 0, b->ctx->stream, nodes,
+                       b->parents.as<int>(), b->positions.as<f3>(), n, nd, cr, ready, nready);
+}
+
+// Karras tree and refit; nd (reference node data) or cr (cut records) or
+// neither (boxes only).
+int build_tree(arc_bvh* b, int n, const NodeData* nd, const CutRecs* cr) {
     arc_ctx* ctx = b->ctx;
     hipLaunchKernelGGL(k_karras, dim3(blocks_for(n - 1)), dim3(kBlock), 0, ctx->stream,
                        b->codes.as<uint32_t>(), n - 1, b->nodes.as<arc_bvh_node>(), b->parents.as<int>(),
                        b->ranges.as<int>(), reinterpret_cast<int*>(b->bounds.as<float>() + 13));
-    const int rblocks = (n + kRefitLeaves - 1) / kRefitLeaves;
     int* nready = reinterpret_cast<int*>(b->bounds.as<float>() + 13);  // zeroed by k_karras
     int* ready = b->ready.as<int>();
     const NodeData d = nd ? *nd : NodeData{};
-    auto* nodes = b->nodes.as<arc_bvh_node>();
-    if (nd) {
-        hipLaunchKernelGGL(k_refit_local<true>, dim3(rblocks), dim3(kRefitLeaves), 0, ctx->stream, nodes,
-                           b->parents.as<int>(), b->ranges.as<int>(), b->positions.as<f3>(), n, d, ready, nready);
-        hipLaunchKernelGGL(k_refit_top<true>, dim3(1), dim3(kRefitLeaves), 0, ctx->stream, nodes,
-                           b->parents.as<int>(), b->positions.as<f3>(), n, d, ready, nready);
-    } else {
-        hipLaunchKernelGGL(k_refit_local<false>, dim3(rblocks), dim3(kRefitLeaves), 0, ctx->stream, nodes,
-                           b->parents.as<int>(), b->ranges.as<int>(), b->positions.as<f3>(), n, d, ready, nready);
-        hipLaunchKernelGGL(k_refit_top<false>, dim3(1), dim3(kRefitLeaves), 0, ctx->stream, nodes,
-                           b->parents.as<int>(), b->positions.as<f3>(), n, d, ready, nready);
-    }
+    const CutRecs c = cr ? *cr : CutRecs{};
+    if (nd) launch_refit<kReference>(b, n, d, c, ready, nready);
+    else if (cr) launch_refit<kCut>(b, n, d, c, ready, nready);
+    else launch_refit<kBoxes>(b, n, d, c, ready, nready);
     ARC_LAUNCHED(ctx);
     return ARC_OK;
 }
@@ -488,15 +534,36 @@ int bvh_build_points(arc_bvh* b, const f3* pos, const f3* nrm, int n) {
     hipLaunchKernelGGL(k_permute_points, dim3(blocks_for(n)), dim3(kBlock), 0, b->ctx->stream,
                        b->perm.as<int>(), pos, nrm, n, b->positions.as<f3>(), b->normals.as<f3>());
     ARC_LAUNCHED(b->ctx);
-    ARC_TRY(build_tree(b, n, nullptr));
+    ARC_TRY(build_tree(b, n, nullptr, nullptr));
     b->n = n;
     b->has_node_data = false;
-    b->has_moments = false;
+    b->has_cut_records = false;
     return ARC_OK;
 }
 
-int bvh_build_avpls(arc_bvh* b, const AvplsDev& a, const float* rand, uint64_t seed, bool validate,
-                    const arc_material* mats, bool moments) {
+namespace {
+
+// Rejects bounce <= 0 (quirk Q2: such an AVPL would shift the leaves
+// against the node data).
+int check_bounces(arc_bvh* b, const AvplsDev& a) {
+    arc_ctx* ctx = b->ctx;
+    int* flag = reinterpret_cast<int*>(b->bounds.as<float>() + 12);
+    ARC_HIP(ctx, hipMemsetAsync(flag, 0, sizeof(int), ctx->stream));
+    hipLaunchKernelGGL(k_check_bounce, dim3(blocks_for(a.n)), dim3(kBlock), 0, ctx->stream, a.bounce, a.n, flag);
+    ARC_LAUNCHED(ctx);
+    int host_flag = 0;
+    ARC_HIP(ctx, hipMemcpyAsync(&host_flag, flag, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+    ARC_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    if (host_flag) {
+        b->n = 0;
+        return fail(ctx, ARC_E_INVALID, "AVPL with bounce <= 0: leaves would not match node data");
+    }
+    return ARC_OK;
+}
+
+}  // namespace
+
+int bvh_build_avpls(arc_bvh* b, const AvplsDev& a, const float* rand, uint64_t seed, bool validate) {
     arc_ctx* ctx = b->ctx;
     const int n = a.n;
     if (n <= 1) {
@@ -512,24 +579,7 @@ int bvh_build_avpls(arc_bvh* b, const AvplsDev& a, const float* rand, uint64_t s
     ARC_TRY(ensure(ctx, b->nd_nrm, sizeof(f3) * slots));
     ARC_TRY(ensure(ctx, b->nd_rad, sizeof(f3) * slots));
     ARC_TRY(ensure(ctx, b->nd_dir, sizeof(f3) * slots));
-    if (moments) {
-        ARC_TRY(ensure(ctx, b->nd_radb, sizeof(f3) * slots));
-        ARC_TRY(ensure(ctx, b->nd_mom, sizeof(float4) * slots));
-        ARC_TRY(ensure(ctx, b->nd_nmom, sizeof(f3) * slots));
-    }
-    if (validate) {
-        int* flag = reinterpret_cast<int*>(b->bounds.as<float>() + 12);
-        ARC_HIP(ctx, hipMemsetAsync(flag, 0, sizeof(int), ctx->stream));
-        hipLaunchKernelGGL(k_check_bounce, dim3(blocks_for(n)), dim3(kBlock), 0, ctx->stream, a.bounce, n, flag);
-        ARC_LAUNCHED(ctx);
-        int host_flag = 0;
-        ARC_HIP(ctx, hipMemcpyAsync(&host_flag, flag, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
-        ARC_HIP(ctx, hipStreamSynchronize(ctx->stream));
-        if (host_flag) {
-            b->n = 0;
-            return fail(ctx, ARC_E_INVALID, "AVPL with bounce <= 0: leaves would not match node data");
-        }
-    }
+    if (validate) ARC_TRY(check_bounces(b, a));
     // AvplBvh passes considerNormals = false from CudaGather (cudaGather.cu:858);
     // a Bvh created with normals considered keeps 6D codes here as well.
     ARC_TRY(encode_and_sort(b, a.pos, a.nrm, n));
@@ -540,18 +590,42 @@ int bvh_build_avpls(arc_bvh* b, const AvplsDev& a, const float* rand, uint64_t s
         ARC_TRY(random_numbers(ctx, seed, (int)slots, b->nd_rand.as<float>()));
     }
     NodeData nd{b->nd_size.as<int>(), b->nd_mat.as<int>(), b->nd_rand.as<float>(), b->nd_pos.as<f3>(),
-                b->nd_nrm.as<f3>(),   b->nd_rad.as<f3>(), b->nd_dir.as<f3>(),
-                moments ? b->nd_radb.as<f3>() : nullptr, moments ? b->nd_mom.as<float4>() : nullptr,
-                moments ? b->nd_nmom.as<f3>() : nullptr};
+                b->nd_nrm.as<f3>(),   b->nd_rad.as<f3>(), b->nd_dir.as<f3>()};
     ARC_TRY(ensure(ctx, b->staged, sizeof(float4) * 4 * (size_t)n));
     hipLaunchKernelGGL(k_stage_avpls, dim3(blocks_for(n)), dim3(kBlock), 0, ctx->stream, a, b->staged.as<float4>());
     hipLaunchKernelGGL(k_permute_avpls, dim3(blocks_for(n)), dim3(kBlock), 0, ctx->stream, b->perm.as<int>(), n,
-                       b->staged.as<float4>(), b->positions.as<f3>(), b->normals.as<f3>(), nd, mats);
+                       b->staged.as<float4>(), b->positions.as<f3>(), b->normals.as<f3>(), nd);
     ARC_LAUNCHED(ctx);
-    ARC_TRY(build_tree(b, n, &nd));
+    ARC_TRY(build_tree(b, n, &nd, nullptr));
     b->n = n;
     b->has_node_data = true;
-    b->has_moments = moments;
+    b->has_cut_records = false;
+    return ARC_OK;
+}
+
+int bvh_build_cut(arc_bvh* b, const AvplsDev& a, bool validate, const arc_material* mats, float4* crec) {
+    arc_ctx* ctx = b->ctx;
+    const int n = a.n;
+    if (n <= 1) {
+        b->n = 0;
+        return fail(ctx, ARC_E_TOO_FEW_POINTS, "not enough points for bvh construction");
+    }
+    if (!mats || !crec) return fail(ctx, ARC_E_INVALID, "cut records need the material table and a buffer");
+    ARC_TRY(ensure_tree(b, n));
+    const size_t slots = (size_t)(2 * n - 1);
+    ARC_TRY(ensure(ctx, b->nd_merge, sizeof(float4) * kMrec * slots));
+    if (validate) ARC_TRY(check_bounces(b, a));
+    ARC_TRY(encode_and_sort(b, a.pos, a.nrm, n));
+    const CutRecs cr{b->nd_merge.as<float4>(), crec};
+    ARC_TRY(ensure(ctx, b->staged, sizeof(float4) * 4 * (size_t)n));
+    hipLaunchKernelGGL(k_stage_avpls, dim3(blocks_for(n)), dim3(kBlock), 0, ctx->stream, a, b->staged.as<float4>());
+    hipLaunchKernelGGL(k_permute_cut, dim3(blocks_for(n)), dim3(kBlock), 0, ctx->stream, b->perm.as<int>(), n,
+                       b->staged.as<float4>(), b->positions.as<f3>(), mats, cr);
+    ARC_LAUNCHED(ctx);
+    ARC_TRY(build_tree(b, n, nullptr, &cr));
+    b->n = n;
+    b->has_node_data = false;
+    b->has_cut_records = true;
     return ARC_OK;
 }
 

@@ -95,7 +95,7 @@ class arc_gather_stats(C.Structure):
                 ("visitedNodes", C.c_longlong), ("evaluatedNodes", C.c_longlong),
                 ("antiVisitedNodes", C.c_longlong), ("antiEvaluatedNodes", C.c_longlong),
                 ("errorPixels", C.c_longlong), ("numLeafs", C.c_int), ("budgetHits", C.c_longlong),
-                ("fallbackNodes", C.c_longlong)]
+                ("fallbackNodes", C.c_longlong), ("testedNodes", C.c_longlong)]
 
 
 class arc_scene_desc(C.Structure):

@@ -34,7 +34,9 @@ PEAK_HBM_GBS = 8000.0
 # Algorithmic FLOP per unit (FMA = 2, transcendental = 1):
 #   radiance pair 31; disk antiradiance pair 37; cone antiradiance pair 33;
 #   solid-angle cut: visited inner node 32, evaluated node 31 + 31;
-#   error-bound cut: node test (criterion) 45, node estimate 37;
+#   error-bound cut: node test (criterion, counted per (pixel, inner node)
+#   test) 45, node estimate (per kept node or, with sums D levels down, per
+#   descendant summed) 37;
 #   clustered antiradiance: tile-filter ray-box test 35 per list entry and
 #   tile, plus the pair tests.
 FLOP_RAD_PAIR, FLOP_DISK_PAIR, FLOP_CONE_PAIR = 31, 37, 33
@@ -43,8 +45,10 @@ FLOP_EB_TEST, FLOP_EB_EST = 45, 37
 FLOP_TILE_FILTER = 35
 # AvplBvh rebuild, algorithmic HBM bytes per leaf: codes, perm, sorted
 # position / normal, nodes, parents, ranges, node data and random numbers
-# (276), plus the error-bound cut's node moments and packed records (80 + 48)
-BYTES_BVH_LEAF, BYTES_MOMENTS_LEAF = 276, 128
+# (276).  The light cuts' rebuild: AVPL input (52), codes, perm, sorted
+# position, nodes, parents, ranges (72), merge sums (48 per slot, 2 slots per
+# leaf) and node records (64 per slot): 348.
+BYTES_BVH_LEAF, BYTES_CUT_BVH_LEAF = 276, 348
 BYTES_SIMPLE_LEAF = 80  # SimpleBvh: codes, perm, sorted positions, nodes, parents, ranges
 PHASES = ("upload", "bvh", "radiance", "antiradiance", "combine", "total")
 
@@ -167,16 +171,16 @@ def main() -> int:
     # roofline of the dominant kernel: its algorithmic FLOPs (from the
     # kernel's own counters) over its HIP-event duration; the LBVH rebuild,
     # many small HBM-bound kernels, is taken as one unit.  The cut and the
-    # clustered antiradiance run concurrently on two streams, each stretched
-    # by the other, so they are taken together: their FLOPs over the union
-    # of their spans in the frame.
+    # clustered antiradiance run on two streams; when they overlap for a good
+    # part of their time, each stretched by the other, they are taken
+    # together: their FLOPs over the union of their spans in the frame.
     npix, n = sc.npix, sc.num_avpls
     spans = np.zeros((args.steps, 4), np.float32)
     got = g.lib.arc_gather_kernel_spans(g.h, args.steps, spans.ctypes.data)
     spans = spans[:max(got, 0)]
     flop_anti = float(st.antiEvaluatedNodes) * FLOP_DISK_PAIR + float(st.antiVisitedNodes) * FLOP_TILE_FILTER
     if eb:
-        flop_rad = float(st.visitedNodes + st.evaluatedNodes) * FLOP_EB_TEST + float(st.evaluatedNodes) * FLOP_EB_EST
+        flop_rad = float(st.testedNodes) * FLOP_EB_TEST + float(st.evaluatedNodes) * FLOP_EB_EST
         rad_name = "k_cut_eb_tile" if args.per_tile else "k_cut_eb_px"
     else:
         flop_rad = float(st.visitedNodes) * FLOP_VISIT + float(st.evaluatedNodes) * (FLOP_EVAL + FLOP_RAD_PAIR)
@@ -189,19 +193,32 @@ def main() -> int:
     t_bvh = phase_ms.get("bvh", 0.0)
     t_pair = float(np.mean(np.maximum(spans[:, 1], spans[:, 3]) - np.minimum(spans[:, 0], spans[:, 2]))) \
         if len(spans) else 0.0
-    if t_pair >= t_bvh and t_pair > 0.0:
+    t_overlap = float(np.mean(np.maximum(0.0, np.minimum(spans[:, 1], spans[:, 3]) -
+                                         np.maximum(spans[:, 0], spans[:, 2])))) if len(spans) else 0.0
+    t_rad, t_anti = kernel_ms.get("radiance", 0.0), kernel_ms.get("antiradiance", 0.0)
+    # the pair is one unit when each kernel spends a good part of its time
+    # beside the other; otherwise the longer of the two is the unit alone
+    paired = t_overlap > 0.25 * min(t_rad, t_anti)
+    per_kernel = {rad_name: {**rate(flop_rad, t_rad), "flop": flop_rad, "overlap_ms": round(t_overlap, 4)},
+                  "k_anti_clustered": {**rate(flop_anti, t_anti), "flop": flop_anti}}
+    if not paired and max(t_rad, t_anti) >= t_bvh and max(t_rad, t_anti) > 0.0:
+        dom = "radiance" if t_rad >= t_anti else "antiradiance"
+        name, flop, t_dom = (rad_name, flop_rad, t_rad) if dom == "radiance" else ("k_anti_clustered", flop_anti, t_anti)
+        ach = flop / (t_dom * 1e-3) / 1e12
+        roof = {"kernel": name, "bound": "valu", "achieved": round(ach, 3), "peak": PEAK_FP32_TFLOPS,
+                "unit": "TFLOP/s", "frac": round(ach / PEAK_FP32_TFLOPS, 4),
+                "traffic": profiled_traffic(dom, eb, bool(args.per_tile), args.eval_depth), "ms": round(t_dom, 4),
+                "kernels": per_kernel}
+    elif t_pair >= t_bvh and t_pair > 0.0:
         ach = (flop_rad + flop_anti) / (t_pair * 1e-3) / 1e12
         tr = [profiled_traffic(k, eb, bool(args.per_tile), args.eval_depth) for k in ("radiance", "antiradiance")]
         roof = {"kernel": f"{rad_name} + k_anti_clustered (concurrent on two streams)", "bound": "valu",
                 "achieved": round(ach, 3), "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
                 "frac": round(ach / PEAK_FP32_TFLOPS, 4),
                 "traffic": float(sum(tr)) if all(t is not None for t in tr) else None,
-                "ms": round(t_pair, 4),
-                "kernels": {rad_name: {**rate(flop_rad, kernel_ms.get("radiance", 0.0)), "flop": flop_rad},
-                            "k_anti_clustered": {**rate(flop_anti, kernel_ms.get("antiradiance", 0.0)),
-                                                 "flop": flop_anti}}}
+                "ms": round(t_pair, 4), "kernels": per_kernel}
     else:
-        bytes_ = float(BYTES_BVH_LEAF + (BYTES_MOMENTS_LEAF if eb else 0)) * n
+        bytes_ = float(BYTES_CUT_BVH_LEAF if eb else BYTES_BVH_LEAF) * n
         ach = bytes_ / (t_bvh * 1e-3) / 1e9
         roof = {"kernel": "AvplBvh rebuild (sort, Karras, refit)", "bound": "hbm", "achieved": round(ach, 1),
                 "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": round(ach / PEAK_HBM_GBS, 4),
@@ -239,8 +256,8 @@ def main() -> int:
             "kernel_ms": {k: round(v, 4) for k, v in kernel_ms.items()},
             "cut": {"mode": args.cut, "error_bound": args.error_bound if eb else None,
                     "eval_depth": args.eval_depth if eb else None,
-                    "visited_nodes": int(st.visitedNodes), "evaluated_nodes": int(st.evaluatedNodes),
-                    "stack_overflows": int(st.errorPixels), "budget_hits": int(st.budgetHits)},
+                    "visited_nodes": int(st.visitedNodes), "tested_nodes": int(st.testedNodes),
+                    "evaluated_nodes": int(st.evaluatedNodes), "stack_overflows": int(st.errorPixels), "budget_hits": int(st.budgetHits)},
             "antiradiance": {"pair_tests": int(st.antiEvaluatedNodes), "tile_filter_tests": int(st.antiVisitedNodes)},
             "scene_build_s": round(t_scene, 2),
         },
@@ -257,8 +274,8 @@ def main() -> int:
 TRAFFIC_PROFILE = ROOT / "profiles" / "frame_1080p_1M_traffic.txt"
 # LBVH kernels of one rebuild, with their dispatch counts (4 radix passes)
 BVH_KERNELS = {"k_bounds_partial": 1, "k_bounds_final": 1, "k_encode": 1, "k_hist": 4, "k_scan_digit": 4,
-               "k_scatter": 4, "k_stage_avpls": 1, "k_permute_avpls": 1, "k_random": 1, "k_karras": 1,
-               "k_refit_local<true>": 1, "k_refit_top<true>": 1}
+               "k_scatter": 4, "k_stage_avpls": 1, "k_permute_cut": 1, "k_karras": 1,
+               "k_refit_local<2>": 1, "k_refit_top<2>": 1}
 
 
 def profiled_traffic(dom: str, eb: bool, per_tile: bool, eval_depth: int = 0):

@@ -236,6 +236,8 @@ typedef struct arc_gather_stats {
                                          pixel's cutBudget was spent               */
     long long fallbackNodes;          /* ARC_CUT_LEARNED: (tile, node) decisions left to the
                                          error-bound criterion                      */
+    long long testedNodes;            /* error-bound and learned cuts: (pixel, inner node)
+                                         evaluations of the error-bound criterion   */
 } arc_gather_stats;
 
 #define ARC_RUN_PROFILE 1      /* time each phase with HIP events (synchronises)  */
