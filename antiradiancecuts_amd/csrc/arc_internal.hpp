@@ -85,6 +85,7 @@ struct arc_bvh {
     int n = 0;               // leaves of the current build (0: none)
     bool has_node_data = false;
     bool has_cut_records = false;  // the last build was bvh_build_cut
+    int refit_top_threads = 1024;  // width of the top refit (64..1024, a multiple of 64)
     arc::DevBuf codes, perm, nodes, parents, ranges, positions, normals;
     arc::DevBuf in_pos, in_nrm;                 // staged host input
     arc::DevBuf nd_size, nd_mat, nd_rand, nd_pos, nd_nrm, nd_rad, nd_dir;

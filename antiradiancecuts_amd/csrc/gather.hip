@@ -498,6 +498,8 @@ int arc_gather_create(arc_ctx* ctx, int width, int height, const arc_material* m
             if (e != hipSuccess) st = arc::hip_fail(ctx, e, "hipEventCreate", __FILE__, __LINE__);
         }
     if (st == ARC_OK) st = arc_bvh_create(ctx, 0, &g->bvh);
+    // the frame's tree is rebuilt beside the antiradiance on the second stream
+    if (st == ARC_OK) g->bvh->refit_top_threads = 256;
     if (st == ARC_OK) {
         hipError_t e = hipStreamCreateWithFlags(&g->aux, hipStreamNonBlocking);
         if (e == hipSuccess) e = hipEventCreateWithFlags(&g->fork, hipEventDisableTiming);

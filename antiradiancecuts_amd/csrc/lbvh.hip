@@ -364,7 +364,12 @@ __device__ inline void merge_cut(int target, int left, int right, int n, f3 lo, 
 //     appends the node to the ready list.
 //   k_refit_top: one workgroup, after the kernel boundary has published every
 //     local result, climbs from the ready nodes through the remaining
-//     crossing nodes with workgroup-scope counters on `visited`.
+//     crossing nodes with workgroup-scope counters on `visited`.  Its width
+//     is the tree's refit_top_threads: 1024 threads climb the most chains at
+//     once, but a 16-wave workgroup needs 16 free wave slots on one CU, and
+//     beside a kernel that fills the CUs (the gather's antiradiance on its
+//     second stream) it waited for them up to 0.8 ms; the gather's trees use
+//     four waves.
 // Every internal node ends with visited == 2, as the reference's counters do.
 constexpr int kRefitLeaves = 1024;
 
@@ -490,7 +495,7 @@ void launch_refit(arc_bvh* b, int n, const NodeData& nd, const CutRecs& cr, int*
     auto* nodes = b->nodes.as<arc_bvh_node>();
     hipLaunchKernelGGL(k_refit_local<kMode>, dim3(rblocks), dim3(kRefitLeaves), 0, b->ctx->stream, nodes,
                        b->parents.as<int>(), b->ranges.as<int>(), b->positions.as<f3>(), n, nd, cr, ready, nready);
-    hipLaunchKernelGGL(k_refit_top<kMode>, dim3(1), dim3(kRefitLeaves), 0, b->ctx->stream, nodes,
+    hipLaunchKernelGGL(k_refit_top<kMode>, dim3(1), dim3(b->refit_top_threads), 0, b->ctx->stream, nodes,
                        b->parents.as<int>(), b->positions.as<f3>(), n, nd, cr, ready, nready);
 }
 

@@ -53,7 +53,7 @@ BYTES_SIMPLE_LEAF = 80  # SimpleBvh: codes, perm, sorted positions, nodes, paren
 PHASES = ("upload", "bvh", "radiance", "antiradiance", "combine", "total")
 
 
-def parse():
+def parse(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
@@ -68,10 +68,11 @@ def parse():
                     help="CPU baseline on 1/k of the pixels (0 = skip)")
     ap.add_argument("--cut", choices=("error-bound", "solid-angle"), default="error-bound",
                     help="light cut of the radiance: the analytic error bound or kernel_bvh's solid angle")
-    ap.add_argument("--error-bound", type=float, default=0.1, help="relative error bound of the cut")
+    ap.add_argument("--error-bound", type=float, default=0.01,
+                    help="relative error bound of the cut (the workload's threshold is 1 %%)")
     ap.add_argument("--per-tile", type=int, default=1, help="error-bound cut: one cut per 8x8 tile (1) or per pixel (0)")
-    ap.add_argument("--eval-depth", type=int, default=3,
-                    help="error-bound cut: a kept node is summed as its descendants this many levels down (0-2)")
+    ap.add_argument("--eval-depth", type=int, default=1,
+                    help="error-bound cut: a kept node is summed as its descendants this many levels down (0-4)")
     ap.add_argument("--component", choices=("frame", "trace", "batch", "learned", "lbvh", "exact"), default="frame",
                     help="frame: the gather frame (the headline metric); trace: the GPU AVPL shooter and "
                          "G-buffer ray caster on the same scene; batch: many views in one graph launch; "
@@ -82,7 +83,7 @@ def parse():
     ap.add_argument("--lanes", type=int, default=4, help="batch: streams the views are dealt over")
     ap.add_argument("--batch-seed", type=int, default=7, help="batch: seed of the view sizes")
     ap.add_argument("--tune", type=int, default=1, help="batch: autotune each view's schedule (1) or not (0)")
-    return ap.parse_args()
+    return ap.parse_args(argv)
 
 
 def main() -> int:

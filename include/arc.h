@@ -298,7 +298,11 @@ int arc_gather_kernel_history(arc_gather* g, int frames, float* out);
  * return value as arc_gather_profile_history. */
 int arc_gather_kernel_spans(arc_gather* g, int frames, float* out);
 
-/* The AvplBvh of the last cut frame (borrowed; NULL before the first). */
+/* The AvplBvh of the last cut frame (borrowed; NULL before the first).  After
+ * a solid-angle cut frame it carries the reference's node data; after an
+ * error-bound or learned cut frame, whose tree sums only what those cuts
+ * read, it carries the topology and boxes, and arc_bvh_node_data returns no
+ * node data. */
 arc_bvh* arc_gather_get_bvh(arc_gather* g);
 
 /* The context a gather was created on. */

@@ -33,9 +33,10 @@ def test_bench_reads_the_traffic_profile():
 
     if not bench.TRAFFIC_PROFILE.exists():
         pytest.skip("no traffic profile committed")
-    anti = bench.profiled_traffic("antiradiance", True, True, 3)
-    cut = bench.profiled_traffic("radiance", True, True, 3)
-    bvh = bench.profiled_traffic("bvh", True, True, 3)
+    depth = bench.parse([]).eval_depth  # the benchmark's default setting
+    anti = bench.profiled_traffic("antiradiance", True, True, depth)
+    cut = bench.profiled_traffic("radiance", True, True, depth)
+    bvh = bench.profiled_traffic("bvh", True, True, depth)
     assert anti and cut and bvh
     assert bvh > cut  # the LBVH rebuild moves more bytes than the cut
 
