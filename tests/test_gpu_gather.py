@@ -188,3 +188,29 @@ def test_clustered_antiradiance_equals_exact(gpu_ctx, cornell, model, which):
     assert e < 1e-6
     np.testing.assert_allclose(clus, exact, rtol=1e-5, atol=1e-5 * np.abs(exact).max())
     g.close()
+
+
+def test_clustered_antiradiance_wide_disks(gpu_ctx, cornell):
+    """Disks half the scene wide: most rays reach most clusters, so a block's
+    appends overflow its LDS buffer into direct appends, and the sums still
+    equal the exact gather's."""
+    sc = cornell
+    print("[gather] clustered vs exact, wide disks", flush=True)
+    g = make_gather(gpu_ctx, sc)
+    av = DeviceAvpls.from_scene(sc)
+    base = dict(useLightCuts=0, antiradianceModel=ffi.ARC_ANTIRAD_DISK, photonRadiusScale=50.0)
+    g.run(av, sc.cam, sc.extent, default_params(antiradianceMode=ffi.ARC_ANTI_EXACT, **base))
+    gpu_ctx.sync()
+    exact = g.antiradiance[:, :3].double().cpu().numpy()
+    st = g.run(av, sc.cam, sc.extent, default_params(antiradianceMode=ffi.ARC_ANTI_CLUSTERED, **base),
+               ffi.ARC_RUN_COUNTERS)
+    gpu_ctx.sync()
+    clus = g.antiradiance[:, :3].double().cpu().numpy()
+    n_anti = int((np.abs(sc.avpl_anti).sum(axis=1) > 0).sum())
+    lists = st.antiVisitedNodes / 16  # each list entry is filtered against the 16 tiles of its cluster
+    print(f"[gather] wide disks: {lists / max(n_anti, 1):.1f} clusters per antiradiance AVPL, "
+          f"rel-L2 {rel_l2(clus, exact):.3e}", flush=True)
+    assert lists > 16 * n_anti  # more than 4096 appends per 256-AVPL block on average
+    assert np.abs(exact).max() > 0
+    np.testing.assert_allclose(clus, exact, rtol=1e-5, atol=1e-5 * np.abs(exact).max())
+    g.close()
