@@ -73,6 +73,8 @@ def parse(argv=None):
     ap.add_argument("--per-tile", type=int, default=1, help="error-bound cut: one cut per 8x8 tile (1) or per pixel (0)")
     ap.add_argument("--eval-depth", type=int, default=1,
                     help="error-bound cut: a kept node is summed as its descendants this many levels down (0-4)")
+    ap.add_argument("--estimate-depth", type=int, default=6,
+                    help="error-bound cut: depth of the fixed cut that estimates each pixel for its threshold")
     ap.add_argument("--component", choices=("frame", "trace", "batch", "learned", "lbvh", "exact"), default="frame",
                     help="frame: the gather frame (the headline metric); trace: the GPU AVPL shooter and "
                          "G-buffer ray caster on the same scene; batch: many views in one graph launch; "
@@ -118,7 +120,8 @@ def main() -> int:
     eb = args.cut == "error-bound"
     params = default_params(useLightCuts=1, antiradianceModel=ffi.ARC_ANTIRAD_DISK,
                             cutMode=ffi.ARC_CUT_ERROR_BOUND if eb else ffi.ARC_CUT_SOLID_ANGLE,
-                            errorBound=args.error_bound, cutPerTile=args.per_tile, cutEvalDepth=args.eval_depth)
+                            errorBound=args.error_bound, cutPerTile=args.per_tile, cutEvalDepth=args.eval_depth,
+                            cutEstimateDepth=args.estimate_depth)
     flags = ffi.ARC_RUN_TIMELINE | ffi.ARC_TRUST_INPUT
 
     for _ in range(args.warmup):
@@ -256,7 +259,7 @@ def main() -> int:
             "phase_ms": {k: round(v, 4) for k, v in phase_ms.items()},
             "kernel_ms": {k: round(v, 4) for k, v in kernel_ms.items()},
             "cut": {"mode": args.cut, "error_bound": args.error_bound if eb else None,
-                    "eval_depth": args.eval_depth if eb else None,
+                    "eval_depth": args.eval_depth if eb else None, "estimate_depth": args.estimate_depth if eb else None,
                     "visited_nodes": int(st.visitedNodes), "tested_nodes": int(st.testedNodes),
                     "evaluated_nodes": int(st.evaluatedNodes), "stack_overflows": int(st.errorPixels), "budget_hits": int(st.budgetHits)},
             "antiradiance": {"pair_tests": int(st.antiEvaluatedNodes), "tile_filter_tests": int(st.antiVisitedNodes)},
