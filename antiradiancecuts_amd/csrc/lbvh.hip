@@ -215,19 +215,21 @@ __global__ __launch_bounds__(kBlock) void k_permute_avpls(const int* __restrict_
     nd.dir[s] = w;
 }
 
-// The leaves of bvh_build_cut: position for the box refit, the merge sums
-// and the node record of each sorted AVPL.  L x f_r of the AVPL's Lambertian
+// The leaves of bvh_build_cut: position (for the box refit) and normal, the
+// merge sums and the node record of each sorted AVPL.  L x f_r of the AVPL's Lambertian
 // material (f_r = 1 for a light-source AVPL, w = 0, cudaGather.cu:89-90); the
 // table holds 256 entries.
 __global__ __launch_bounds__(kBlock) void k_permute_cut(const int* __restrict__ perm, int n,
                                                         const float4* __restrict__ staged, f3* __restrict__ spos,
-                                                        const arc_material* __restrict__ mats, CutRecs cr) {
+                                                        f3* __restrict__ snrm, const arc_material* __restrict__ mats,
+                                                        CutRecs cr) {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= n) return;
     const float4* rec = staged + 4 * (size_t)perm[s];
     const float4 r0 = rec[0], r1 = rec[1], r2 = rec[2], r3 = rec[3];
     const f3 p{r0.x, r0.y, r0.z}, q{r1.x, r1.y, r1.z}, L{r2.x, r2.y, r2.z};
     spos[s] = p;
+    snrm[s] = q;
     f3 r = L;
     if (r3.x != 0.f || r3.y != 0.f || r3.z != 0.f) {
         const arc_material& m = mats[__float_as_int(r0.w) & 255];
@@ -625,7 +627,7 @@ int bvh_build_cut(arc_bvh* b, const AvplsDev& a, bool validate, const arc_materi
     ARC_TRY(ensure(ctx, b->staged, sizeof(float4) * 4 * (size_t)n));
     hipLaunchKernelGGL(k_stage_avpls, dim3(blocks_for(n)), dim3(kBlock), 0, ctx->stream, a, b->staged.as<float4>());
     hipLaunchKernelGGL(k_permute_cut, dim3(blocks_for(n)), dim3(kBlock), 0, ctx->stream, b->perm.as<int>(), n,
-                       b->staged.as<float4>(), b->positions.as<f3>(), mats, cr);
+                       b->staged.as<float4>(), b->positions.as<f3>(), b->normals.as<f3>(), mats, cr);
     ARC_LAUNCHED(ctx);
     ARC_TRY(build_tree(b, n, nullptr, &cr));
     b->n = n;

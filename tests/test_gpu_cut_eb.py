@@ -15,7 +15,7 @@ import pytest
 
 from antiradiancecuts_amd import ffi
 from antiradiancecuts_amd import scene as S
-from antiradiancecuts_amd.runtime import DeviceAvpls, Gather, default_params, device_array
+from antiradiancecuts_amd.runtime import Bvh, DeviceAvpls, Gather, default_params, device_array
 
 pytestmark = pytest.mark.gpu
 
@@ -155,3 +155,23 @@ def test_kernel_spans_of_the_two_stream_frame(gpu_ctx, rig, scene):
         assert 0.0 <= r0 <= r1 <= h[5] + 1e-3 and 0.0 <= a0 <= a1 <= h[5] + 1e-3
         assert a0 < r0  # forked before the rebuild
         assert abs((r1 - r0) - k[0]) < 1e-2 and abs((a1 - a0) - k[1]) < 1e-2
+
+
+def test_frame_tree_is_the_avpl_bvh(gpu_ctx, rig, scene):
+    """The cut's rebuild sums only what the cut reads, but its tree is the
+    AvplBvh: the same codes, order, topology, leaf ranges, boxes and sorted
+    leaves as arc_bvh_build_avpls over the same AVPLs, without node data."""
+    g, av, _ = rig
+    run_rad(gpu_ctx, g, av, scene, eb(errorBound=0.01, cutEvalDepth=1))
+    frame = g.bvh_to_host()
+    own = Bvh(gpu_ctx)
+    own.build_avpls(av, trust=True)
+    gpu_ctx.sync()
+    ref = own.to_host()
+    own.close()
+    assert frame["n"] == ref["n"] == scene.num_avpls
+    for k in ("codes", "perm", "parents", "ranges", "positions", "normals"):
+        np.testing.assert_array_equal(frame[k], ref[k], err_msg=k)
+    for k in ("left", "right", "bbMin", "bbMax", "visited"):
+        np.testing.assert_array_equal(frame["nodes"][k], ref["nodes"][k], err_msg=k)
+    assert "nd_size" in ref and "nd_size" not in frame
