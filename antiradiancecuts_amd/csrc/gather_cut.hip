@@ -133,10 +133,21 @@ struct Acc {
     float s0 = 0.f, s1 = 0.f, s2 = 0.f;
     int run = 0;
     __device__ inline void add(const float4& I, float g) {
+        add_run(I, g);
+        if (run >= kRun) flush();
+    }
+    // add() without the flush test: a kept node's estimates then stay one
+    // straight run of code (a flush test between them costs register moves
+    // at the join); the caller flushes once per kept node (maybe_flush), so a
+    // plain run can exceed kRun by the node's 2^D - 1 extra terms
+    __device__ inline void add_run(const float4& I, float g) {
         s0 += I.x * g;
         s1 += I.y * g;
         s2 += I.z * g;
-        if (++run == kRun) flush();
+        ++run;
+    }
+    __device__ inline void maybe_flush() {
+        if (run >= kRun) flush();
     }
     __device__ inline void flush() {
         k[0].add(s0); k[1].add(s1); k[2].add(s2);
@@ -153,7 +164,7 @@ template <int D>
 __device__ inline int add_desc(const CutEB& t, const PixelF& P, int v, Acc& acc) {
     if (D == 0 || v >= 0) {
         const float4* r = t.crec + kCrec * (size_t)(v >= 0 ? v : t.n + (-(v + 1)));
-        acc.add(r[2], geom_est(P, r[0], r[1]));
+        acc.add_run(r[2], geom_est(P, r[0], r[1]));
         return 1;
     }
     const arc_bvh_node& b = t.nodes[-(v + 1)];
@@ -357,6 +368,7 @@ __global__ __launch_bounds__(kCutThreads) void k_cut_eb_px(CutEB t, const float4
             }
             (void)slot;
             nev += add_kept(t, P, node, eval_depth, acc);
+            acc.maybe_flush();
             if (sp == 0) break;
             node = S[(--sp) * kCutThreads];
         }
@@ -435,6 +447,7 @@ __global__ __launch_bounds__(kCutThreads) void k_cut_eb_tile(CutEB t, const floa
         }
         (void)slot;
         nev += add_desc<D>(t, P, node, acc);
+        acc.maybe_flush();
         if (sp == 0) break;
         node = uniform(S[--sp]);
     }
