@@ -132,14 +132,11 @@ struct Acc {
     Kahan k[3];
     float s0 = 0.f, s1 = 0.f, s2 = 0.f;
     int run = 0;
-    __device__ inline void add(const float4& I, float g) {
-        add_run(I, g);
-        if (run >= kRun) flush();
-    }
-    // add() without the flush test: a kept node's estimates then stay one
-    // straight run of code (a flush test between them costs register moves
-    // at the join); the caller flushes once per kept node (maybe_flush), so a
-    // plain run can exceed kRun by the node's 2^D - 1 extra terms
+    // one term of the plain run, without a flush test: a kept node's
+    // estimates then stay one straight run of code (a flush test between them
+    // costs register moves at the join); the caller flushes once per kept
+    // node, or per batch of kept nodes (maybe_flush), so a plain run can
+    // exceed kRun by that node's or batch's extra terms
     __device__ inline void add_run(const float4& I, float g) {
         s0 += I.x * g;
         s1 += I.y * g;
@@ -811,9 +808,10 @@ __global__ __launch_bounds__(kCutThreads) __attribute__((amdgpu_waves_per_eu(4))
             const int node = __builtin_amdgcn_readlane(mine, j);
             const int slot = node >= 0 ? node : n + (-(node + 1));
             const float4* r = t.crec + kCrec * (size_t)slot;
-            acc.add(r[2], geom_est(P, r[0], r[1]));
+            acc.add_run(r[2], geom_est(P, r[0], r[1]));
             ++nev;
         }
+        acc.maybe_flush();  // a batch keeps at most 16 nodes
     }
     acc.flush();
     const int alive = __popcll(__ballot(live));
