@@ -327,27 +327,45 @@ __device__ inline void cluster(int target, int left, int right, int n, const Nod
     nd.nrm[tid] = nd.nrm[repr];
 }
 
-// The cut's sums of an internal node from its children's (left + right, in
-// that order), and its node record; lo / hi: the node's box.
-__device__ inline void merge_cut(int target, int left, int right, int n, f3 lo, f3 hi, const CutRecs& cr) {
+// The cut's merge sums of an internal node from its children's, a (left) +
+// b (right) in that order: (I, count), (sum w x, sum w), (sum w n, -).
+__device__ inline void add_cut_sums(const float4 (&a)[kMrec], const float4 (&b)[kMrec], float4 (&m)[kMrec]) {
+    const int count = __float_as_int(a[0].w) + __float_as_int(b[0].w);
+    m[0] = make_float4(a[0].x + b[0].x, a[0].y + b[0].y, a[0].z + b[0].z, __int_as_float(count));
+    m[1] = make_float4(a[1].x + b[1].x, a[1].y + b[1].y, a[1].z + b[1].z, a[1].w + b[1].w);
+    m[2] = make_float4(a[2].x + b[2].x, a[2].y + b[2].y, a[2].z + b[2].z, 0.f);
+}
+
+// An internal node's cut record from its merge sums and its box lo / hi.
+__device__ inline void write_cut_node(int target, int n, f3 lo, f3 hi, const float4 (&m)[kMrec], const CutRecs& cr) {
     const size_t tid = (size_t)n + target;
-    const size_t lid = left < 0 ? (size_t)n - (left + 1) : (size_t)left;
-    const size_t rid = right < 0 ? (size_t)n - (right + 1) : (size_t)right;
-    const float4* a = cr.merge + kMrec * lid;
-    const float4* b = cr.merge + kMrec * rid;
-    const float4 a0 = a[0], a1 = a[1], a2 = a[2], b0 = b[0], b1 = b[1], b2 = b[2];
-    const f3 I{a0.x + b0.x, a0.y + b0.y, a0.z + b0.z};
-    const int count = __float_as_int(a0.w) + __float_as_int(b0.w);
-    const float4 mo = make_float4(a1.x + b1.x, a1.y + b1.y, a1.z + b1.z, a1.w + b1.w);
-    const f3 nm{a2.x + b2.x, a2.y + b2.y, a2.z + b2.z};
-    float4* mg = cr.merge + kMrec * tid;
-    mg[0] = make_float4(I.x, I.y, I.z, __int_as_float(count));
-    mg[1] = mo;
-    mg[2] = make_float4(nm.x, nm.y, nm.z, 0.f);
     const float ex = hi.x - lo.x, ey = hi.y - lo.y, ez = hi.z - lo.z;
     const float s2 = 0.25f * (ex * ex + ey * ey + ez * ez);
     const f3 centre{0.5f * (lo.x + hi.x), 0.5f * (lo.y + hi.y), 0.5f * (lo.z + hi.z)};
-    write_cut_record(cr.crec + kCrec * tid, I, mo, nm, count, s2, centre, f3{0.f, 0.f, 0.f});
+    write_cut_record(cr.crec + kCrec * tid, f3{m[0].x, m[0].y, m[0].z}, m[1], f3{m[2].x, m[2].y, m[2].z},
+                     __float_as_int(m[0].w), s2, centre, f3{0.f, 0.f, 0.f});
+}
+
+__device__ inline void store_merge(int target, int n, const float4 (&m)[kMrec], const CutRecs& cr) {
+    float4* mg = cr.merge + kMrec * ((size_t)n + target);
+#pragma unroll
+    for (int q = 0; q < kMrec; ++q) mg[q] = m[q];
+}
+
+// The cut's sums of an internal node from its children's merge records in
+// HBM, and its node record; lo / hi: the node's box.
+__device__ inline void merge_cut(int target, int left, int right, int n, f3 lo, f3 hi, const CutRecs& cr) {
+    const size_t lid = left < 0 ? (size_t)n - (left + 1) : (size_t)left;
+    const size_t rid = right < 0 ? (size_t)n - (right + 1) : (size_t)right;
+    float4 a[kMrec], b[kMrec], m[kMrec];
+#pragma unroll
+    for (int q = 0; q < kMrec; ++q) {
+        a[q] = cr.merge[kMrec * lid + q];
+        b[q] = cr.merge[kMrec * rid + q];
+    }
+    add_cut_sums(a, b, m);
+    store_merge(target, n, m, cr);
+    write_cut_node(target, n, lo, hi, m, cr);
 }
 
 // kernel_innerNodes (bvh.cu:280-316).  One thread per leaf walks up; the
@@ -405,6 +423,29 @@ __device__ inline void merge_node(int parent, arc_bvh_node* nodes, const f3* spo
     if (kMode == kCut) merge_cut(parent, l, r, n, lo, hi, cr);
 }
 
+// A child of a block-local node: a leaf (its point, and for the light cuts'
+// rebuild its merge record, from HBM) or a local node merged earlier in this
+// block (from LDS).
+template <int kMode>
+__device__ inline void local_child(int c, int lo, const f3* spos, const CutRecs& cr, const f3* sbmin,
+                                   const f3* sbmax, const float4* smerge, f3& bmin, f3& bmax, float4 (&m)[kMrec]) {
+    if (c >= 0) {
+        bmin = bmax = spos[c];
+        if (kMode == kCut) {
+#pragma unroll
+            for (int q = 0; q < kMrec; ++q) m[q] = cr.merge[kMrec * (size_t)c + q];
+        }
+    } else {
+        const int s = -(c + 1) - lo;
+        bmin = sbmin[s];
+        bmax = sbmax[s];
+        if (kMode == kCut) {
+#pragma unroll
+            for (int q = 0; q < kMrec; ++q) m[q] = smerge[kMrec * s + q];
+        }
+    }
+}
+
 template <int kMode>
 __global__ __launch_bounds__(kRefitLeaves) void k_refit_local(arc_bvh_node* __restrict__ nodes,
                                                               const int* __restrict__ parents,
@@ -412,7 +453,12 @@ __global__ __launch_bounds__(kRefitLeaves) void k_refit_local(arc_bvh_node* __re
                                                               const f3* __restrict__ spos, int n, NodeData nd,
                                                               CutRecs cr, int* __restrict__ ready,
                                                               int* __restrict__ nready) {
+    // The block's local nodes, by index - lo: arrival counters, boxes and, for
+    // the light cuts' rebuild, merge sums.  A local node's children are
+    // leaves or local nodes, so its merge reads HBM only for leaves.
     __shared__ int arrivals[kRefitLeaves];
+    __shared__ f3 sbmin[kRefitLeaves], sbmax[kRefitLeaves];
+    __shared__ float4 smerge[kMode == kCut ? kMrec * kRefitLeaves : 1];
     arrivals[threadIdx.x] = 0;
     __syncthreads();
     const int lo = blockIdx.x * kRefitLeaves;
@@ -420,8 +466,8 @@ __global__ __launch_bounds__(kRefitLeaves) void k_refit_local(arc_bvh_node* __re
     const int i = lo + threadIdx.x;
     if (i >= n) return;  // no barrier follows
     int parent = parents[i];
+    bool local = parent >= 0 && ranges[2 * parent] >= lo && ranges[2 * parent + 1] <= hi;
     while (parent >= 0) {
-        const bool local = ranges[2 * parent] >= lo && ranges[2 * parent + 1] <= hi;
         if (!local) {
             const int old = __hip_atomic_fetch_add(&nodes[parent].visited, 1, __ATOMIC_RELAXED,
                                                    __HIP_MEMORY_SCOPE_AGENT);
@@ -430,9 +476,34 @@ __global__ __launch_bounds__(kRefitLeaves) void k_refit_local(arc_bvh_node* __re
         }
         if (__hip_atomic_fetch_add(&arrivals[parent - lo], 1, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP) == 0)
             return;
-        merge_node<kMode>(parent, nodes, spos, n, nd, cr);
+        const int l = nodes[parent].left, r = nodes[parent].right;
+        f3 lmin, lmax, rmin, rmax;
+        float4 a[kMrec], b[kMrec];
+        local_child<kMode>(l, lo, spos, cr, sbmin, sbmax, smerge, lmin, lmax, a);
+        local_child<kMode>(r, lo, spos, cr, sbmin, sbmax, smerge, rmin, rmax, b);
+        const f3 blo{fminf(lmin.x, rmin.x), fminf(lmin.y, rmin.y), fminf(lmin.z, rmin.z)};
+        const f3 bhi{fmaxf(lmax.x, rmax.x), fmaxf(lmax.y, rmax.y), fmaxf(lmax.z, rmax.z)};
+        const int s = parent - lo;
+        sbmin[s] = blo;
+        sbmax[s] = bhi;
+        nodes[parent].bbMin = arc_float3{blo.x, blo.y, blo.z};
+        nodes[parent].bbMax = arc_float3{bhi.x, bhi.y, bhi.z};
+        if (kMode == kReference) cluster(parent, l, r, n, nd);
+        const int next = parents[n + parent];
+        const bool next_local = next >= 0 && ranges[2 * next] >= lo && ranges[2 * next + 1] <= hi;
+        if (kMode == kCut) {
+            float4 m[kMrec];
+            add_cut_sums(a, b, m);
+#pragma unroll
+            for (int q = 0; q < kMrec; ++q) smerge[kMrec * s + q] = m[q];
+            // HBM holds the merge sums the top refit reads: those of a node
+            // whose parent spans blocks (and the root's)
+            if (!next_local) store_merge(parent, n, m, cr);
+            write_cut_node(parent, n, blo, bhi, m, cr);
+        }
         nodes[parent].visited = 2;
-        parent = parents[n + parent];
+        parent = next;
+        local = next_local;
     }
 }
 
