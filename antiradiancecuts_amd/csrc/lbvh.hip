@@ -267,7 +267,7 @@ __global__ __launch_bounds__(kBlock) void k_karras(const uint32_t* __restrict__ 
                                                    int* __restrict__ parents, int* __restrict__ ranges,
                                                    int* __restrict__ nready) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i == 0) *nready = 0;  // ready-list length of the refit that follows
+    if (i == 0) nready[0] = nready[1] = 0;  // the refit's two ready-list lengths
     if (i >= num_nodes) return;
     const int root_slot = num_nodes + 1;
     // direction of the range: sign(0) = +1 (bvh.cu:170-173, 203-204)
@@ -382,9 +382,17 @@ __device__ inline void merge_cut(int target, int left, int right, int n, f3 lo, 
 //     node whose range crosses a block boundary the walk stops after a
 //     relaxed increment of the node's `visited`; the thread that makes it 2
 //     appends the node to the ready list.
+//   k_refit_mid: one workgroup per kMidLeaves-leaf span, after the kernel
+//     boundary, climbs from the ready nodes inside its span through the
+//     crossing nodes inside it, with workgroup-scope counters on `visited`;
+//     at a node that spans two of them it stops as k_refit_local does at a
+//     block boundary, and the second arrival appends the node to a second
+//     ready list.  One workgroup merging all ~10000 crossing nodes of 1M
+//     points is bound by its CU's rate of scattered memory requests (about
+//     20 per merge); the spans share them over tens of CUs.
 //   k_refit_top: one workgroup, after the kernel boundary has published every
-//     local result, climbs from the ready nodes through the remaining
-//     crossing nodes with workgroup-scope counters on `visited`.  Its width
+//     span's result, climbs from the second ready list through the remaining
+//     nodes with workgroup-scope counters on `visited`.  Its width
 //     is the tree's refit_top_threads: 1024 threads climb the most chains at
 //     once, but a 16-wave workgroup needs 16 free wave slots on one CU, and
 //     beside a kernel that fills the CUs (the gather's antiradiance on its
@@ -392,6 +400,7 @@ __device__ inline void merge_cut(int target, int left, int right, int n, f3 lo, 
 //     four waves.
 // Every internal node ends with visited == 2, as the reference's counters do.
 constexpr int kRefitLeaves = 1024;
+constexpr int kMidLeaves = 32 * kRefitLeaves, kMidThreads = 256;
 
 // What a refit merges besides the boxes.
 enum RefitMode { kBoxes = 0, kReference = 1, kCut = 2 };
@@ -508,6 +517,44 @@ __global__ __launch_bounds__(kRefitLeaves) void k_refit_local(arc_bvh_node* __re
 }
 
 template <int kMode>
+__global__ __launch_bounds__(kMidThreads) void k_refit_mid(arc_bvh_node* __restrict__ nodes,
+                                                           const int* __restrict__ parents,
+                                                           const int* __restrict__ ranges,
+                                                           const f3* __restrict__ spos, int n, NodeData nd,
+                                                           CutRecs cr, const int* __restrict__ ready,
+                                                           const int* __restrict__ nready, int* __restrict__ ready2,
+                                                           int* __restrict__ nready2) {
+    const int count = *nready;
+    const int lo = blockIdx.x * kMidLeaves;
+    const int hi = min(n, lo + kMidLeaves) - 1;
+    for (int j = threadIdx.x; j < count; j += kMidThreads) {
+        int node = ready[j];
+        const int first = ranges[2 * node], last = ranges[2 * node + 1];
+        if (first < lo || first > hi) continue;  // the span holding its first leaf takes it
+        if (last > hi) {                         // spans two spans: the top refit's
+            ready2[atomicAdd(nready2, 1)] = node;
+            continue;
+        }
+        while (true) {
+            merge_node<kMode>(node, nodes, spos, n, nd, cr);
+            const int parent = parents[n + node];
+            if (parent < 0) break;
+            if (ranges[2 * parent] >= lo && ranges[2 * parent + 1] <= hi) {
+                // 0: the sibling subtree is still pending and will climb on; 1: both done
+                if (__hip_atomic_fetch_add(&nodes[parent].visited, 1, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP) != 1)
+                    break;
+                node = parent;
+            } else {
+                const int old = __hip_atomic_fetch_add(&nodes[parent].visited, 1, __ATOMIC_RELAXED,
+                                                       __HIP_MEMORY_SCOPE_AGENT);
+                if (old == 1) ready2[atomicAdd(nready2, 1)] = parent;
+                break;
+            }
+        }
+    }
+}
+
+template <int kMode>
 __global__ __launch_bounds__(kRefitLeaves) void k_refit_top(arc_bvh_node* __restrict__ nodes,
                                                             const int* __restrict__ parents,
                                                             const f3* __restrict__ spos, int n, NodeData nd,
@@ -540,7 +587,7 @@ int ensure_tree(arc_bvh* b, int n) {
     ARC_TRY(arc::ensure(ctx, b->positions, sizeof(f3) * (size_t)n));
     ARC_TRY(arc::ensure(ctx, b->normals, sizeof(f3) * (size_t)n));
     ARC_TRY(arc::ensure(ctx, b->bounds, sizeof(float) * 16));
-    ARC_TRY(arc::ensure(ctx, b->ready, sizeof(int) * (size_t)n));
+    ARC_TRY(arc::ensure(ctx, b->ready, sizeof(int) * 2 * (size_t)n));  // two ready lists
     ARC_TRY(arc::ensure(ctx, b->partials, sizeof(float) * 12 * kReduceBlocks));
     return ARC_OK;
 }
@@ -568,8 +615,11 @@ void launch_refit(arc_bvh* b, int n, const NodeData& nd, const CutRecs& cr, int*
     auto* nodes = b->nodes.as<arc_bvh_node>();
     hipLaunchKernelGGL(k_refit_local<kMode>, dim3(rblocks), dim3(kRefitLeaves), 0, b->ctx->stream, nodes,
                        b->parents.as<int>(), b->ranges.as<int>(), b->positions.as<f3>(), n, nd, cr, ready, nready);
+    hipLaunchKernelGGL(k_refit_mid<kMode>, dim3((n + kMidLeaves - 1) / kMidLeaves), dim3(kMidThreads), 0,
+                       b->ctx->stream, nodes, b->parents.as<int>(), b->ranges.as<int>(), b->positions.as<f3>(), n, nd,
+                       cr, ready, nready, ready + n, nready + 1);
     hipLaunchKernelGGL(k_refit_top<kMode>, dim3(1), dim3(b->refit_top_threads), 0, b->ctx->stream, nodes,
-                       b->parents.as<int>(), b->positions.as<f3>(), n, nd, cr, ready, nready);
+                       b->parents.as<int>(), b->positions.as<f3>(), n, nd, cr, ready + n, nready + 1);
 }
 
 // Karras tree and refit; nd (reference node data) or cr (cut records) or
@@ -579,7 +629,7 @@ int build_tree(arc_bvh* b, int n, const NodeData* nd, const CutRecs* cr) {
     hipLaunchKernelGGL(k_karras, dim3(blocks_for(n - 1)), dim3(kBlock), 0, ctx->stream,
                        b->codes.as<uint32_t>(), n - 1, b->nodes.as<arc_bvh_node>(), b->parents.as<int>(),
                        b->ranges.as<int>(), reinterpret_cast<int*>(b->bounds.as<float>() + 13));
-    int* nready = reinterpret_cast<int*>(b->bounds.as<float>() + 13);  // zeroed by k_karras
+    int* nready = reinterpret_cast<int*>(b->bounds.as<float>() + 13);  // two counters, zeroed by k_karras
     int* ready = b->ready.as<int>();
     const NodeData d = nd ? *nd : NodeData{};
     const CutRecs c = cr ? *cr : CutRecs{};
