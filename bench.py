@@ -285,7 +285,7 @@ TRAFFIC_PROFILE = ROOT / "profiles" / "frame_1080p_1M_traffic.txt"
 # LBVH kernels of one rebuild, with their dispatch counts (4 radix passes)
 BVH_KERNELS = {"k_bounds_partial": 1, "k_bounds_final": 1, "k_encode": 1, "k_hist": 4, "k_scan_digit": 4,
                "k_scatter": 4, "k_stage_avpls": 1, "k_permute_cut": 1, "k_karras": 1,
-               "k_refit_local<2>": 1, "k_refit_top<2>": 1}
+               "k_refit_local<2>": 1, "k_refit_mid<2>": 1, "k_refit_top<2>": 1}
 
 
 def profiled_traffic(dom: str, eb: bool, per_tile: bool, eval_depth: int = 0):
