@@ -35,6 +35,7 @@ HIP_SOURCES = {
     "gather.hip": [],
     "gather_anti.hip": [],
     "gather_cut.hip": [],
+    "cut_moments.hip": [],
     "output.hip": [],
     "trace.hip": ["-ffp-contract=off"],
     "batch.hip": [],

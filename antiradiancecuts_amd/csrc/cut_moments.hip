@@ -1,0 +1,403 @@
+// The moment estimator of the error-bound light cut (ARC_ESTIMATOR_MOMENTS).
+//
+// The reference sums a kept cluster as its random representative times the
+// cluster's radiance (kernel_bvh, cudaGather.cu:416-420; cluster(),
+// bvh.cu:726-744).  The centroid estimator of gather_cut.hip replaces the
+// representative by the node's centroid; on scenes whose AVPLs carry
+// unrelated colours and normals both leave errors of several per cent,
+// because a node's clamped cosines and its intensity differ per colour
+// channel and the inverse-square falloff is convex over the node.  This
+// estimator keeps, per internal node:
+//
+//   per RGB channel c, over the node's AVPLs j (I_jc = L_jc * diffuse / pi):
+//     I_c = sum I_jc,  M_c = sum I_jc n_j,  T_c = sum I_jc n_j n_j^T
+//   so that the AVPL-side sum  sum_j I_jc max(0, -n_j . d^)  is taken to
+//   second order in spherical harmonics: max(0, t) ~ 3/32 + t/2 + 15/32 t^2
+//   (the l <= 2 Legendre terms of the clamped cosine), i.e.
+//     A_c(d^) = 3/32 I_c - 1/2 M_c . d^ + 15/32 d^T T_c d^;
+//   and, over w_j = (I_jr + I_jg + I_jb) / 3, the node's position mean c and
+//   covariance S, from which four points c + L v_k (L L^T = S, v_k the
+//   vertices (1,1,1), (1,-1,-1), (-1,1,-1), (-1,-1,1) of a tetrahedron, whose
+//   mean is 0 and mean outer product the identity) share the node's mean and
+//   covariance: the pixel-side factor max(0, n_p . d) / |d|^3 averaged over
+//   them is exact to second order in the node's extent.
+//
+// The estimate of a node is sum_c max(A_c(d^), 0) * K, with d^ the direction
+// from the pixel to c and K the four-point average.  Kept nodes of at most
+// cutExactCount AVPLs are summed exactly over their leaves instead (their
+// sorted leaves are contiguous, crec slots [first, first + count)).
+//
+// The sums come from FP64 prefix sums over the sorted leaves: a node's sum is
+// the difference of two prefix rows at its leaf range (the Karras ranges),
+// so no tree climb is needed and the covariance is formed in FP64.
+#include <algorithm>
+#include <cmath>
+
+#include "cut_common.hpp"
+
+using namespace arc::cut;
+
+namespace {
+
+constexpr int kMomTerms = 40;   // FP64 sums per leaf (see leaf_terms)
+constexpr int kScanBlock = 256;  // leaves per block of the prefix scan
+constexpr int kMom = 13;         // float4 per internal node record
+constexpr int kMomStack = 96;    // per-wave stack of the tile traversal
+
+// Terms of leaf i of the sorted order (crec leaf record: r0 = position,
+// r1 = unit normal, r2 = I):
+//   0 w, 1-3 w x, 4-9 w (xx, yy, zz, xy, xz, yz), 10-12 I_c,
+//   13 + 3c + k  I_c n_k,  22 + 6c + (xx, yy, zz, xy, xz, yz)  I_c n n^T.
+__device__ inline void leaf_terms(const float4* __restrict__ crec, int i, double* t) {
+    const float4 r0 = crec[kCrec * (size_t)i], r1 = crec[kCrec * (size_t)i + 1], r2 = crec[kCrec * (size_t)i + 2];
+    const double x = r0.x, y = r0.y, z = r0.z, nx = r1.x, ny = r1.y, nz = r1.z;
+    const double I[3] = {r2.x, r2.y, r2.z};
+    const double w = (I[0] + I[1] + I[2]) * (1.0 / 3.0);
+    t[0] = w;
+    t[1] = w * x; t[2] = w * y; t[3] = w * z;
+    t[4] = w * x * x; t[5] = w * y * y; t[6] = w * z * z;
+    t[7] = w * x * y; t[8] = w * x * z; t[9] = w * y * z;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        t[10 + c] = I[c];
+        t[13 + 3 * c] = I[c] * nx; t[14 + 3 * c] = I[c] * ny; t[15 + 3 * c] = I[c] * nz;
+        t[22 + 6 * c] = I[c] * nx * nx; t[23 + 6 * c] = I[c] * ny * ny; t[24 + 6 * c] = I[c] * nz * nz;
+        t[25 + 6 * c] = I[c] * nx * ny; t[26 + 6 * c] = I[c] * nx * nz; t[27 + 6 * c] = I[c] * ny * nz;
+    }
+}
+
+// Inclusive prefix sums of the terms within each block of kScanBlock leaves
+// (pre, leaf-major: 40 doubles per leaf) and the block totals (tot).
+__global__ __launch_bounds__(kScanBlock) void k_mom_scan(const float4* __restrict__ crec, int n,
+                                                         double* __restrict__ pre, double* __restrict__ tot) {
+    __shared__ double wt[kScanBlock / 64][kMomTerms];
+    const int i = blockIdx.x * kScanBlock + threadIdx.x;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    double v[kMomTerms];
+    if (i < n) {
+        leaf_terms(crec, i, v);
+    } else {
+#pragma unroll
+        for (int k = 0; k < kMomTerms; ++k) v[k] = 0.0;
+    }
+#pragma unroll
+    for (int k = 0; k < kMomTerms; ++k) {
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1) {
+            const double u = __shfl_up(v[k], off);
+            if (lane >= off) v[k] += u;
+        }
+        if (lane == 63) wt[wave][k] = v[k];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < kMomTerms; ++k) {
+        double add = 0.0;
+        for (int w = 0; w < wave; ++w) add += wt[w][k];
+        v[k] += add;
+    }
+    if (i < n) {
+        double* o = pre + (size_t)kMomTerms * i;
+#pragma unroll
+        for (int k = 0; k < kMomTerms; ++k) o[k] = v[k];
+    }
+    if (threadIdx.x == kScanBlock - 1) {
+        double* o = tot + (size_t)kMomTerms * blockIdx.x;
+#pragma unroll
+        for (int k = 0; k < kMomTerms; ++k) o[k] = v[k];
+    }
+}
+
+// Block totals -> exclusive block offsets, one workgroup per term.
+__global__ __launch_bounds__(kScanBlock) void k_mom_offsets(double* __restrict__ tot, int nblocks) {
+    __shared__ double wt[kScanBlock / 64];
+    __shared__ double carry_s;
+    const int term = blockIdx.x;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    double carry = 0.0;
+    for (int base = 0; base < nblocks; base += kScanBlock) {
+        const int j = base + threadIdx.x;
+        const double x = j < nblocks ? tot[(size_t)kMomTerms * j + term] : 0.0;
+        double v = x;
+        for (int off = 1; off < 64; off <<= 1) {
+            const double u = __shfl_up(v, off);
+            if (lane >= off) v += u;
+        }
+        if (lane == 63) wt[wave] = v;
+        __syncthreads();
+        double add = 0.0;
+        for (int w = 0; w < wave; ++w) add += wt[w];
+        v += add;
+        if (j < nblocks) tot[(size_t)kMomTerms * j + term] = carry + v - x;
+        if (threadIdx.x == kScanBlock - 1) carry_s = v;
+        __syncthreads();  // every lane has read wt and carry_s is written
+        carry += carry_s;
+        __syncthreads();  // carry_s read before the next chunk writes it
+    }
+}
+
+__device__ inline double prefix(const double* __restrict__ pre, const double* __restrict__ tot, int i, int k) {
+    return pre[(size_t)kMomTerms * i + k] + tot[(size_t)kMomTerms * (i / kScanBlock) + k];
+}
+
+// One record per internal node from the prefix rows at its leaf range.
+__global__ __launch_bounds__(kScanBlock) void k_mom_nodes(const int* __restrict__ ranges,
+                                                          const arc_bvh_node* __restrict__ nodes, int n,
+                                                          const double* __restrict__ pre,
+                                                          const double* __restrict__ tot, float4* __restrict__ mom) {
+    const int k = blockIdx.x * kScanBlock + threadIdx.x;
+    if (k >= n - 1) return;
+    const int first = ranges[2 * k], last = ranges[2 * k + 1];
+    double S[kMomTerms];
+#pragma unroll
+    for (int t = 0; t < kMomTerms; ++t)
+        S[t] = prefix(pre, tot, last, t) - (first > 0 ? prefix(pre, tot, first - 1, t) : 0.0);
+    const arc_bvh_node& b = nodes[k];
+    const double ex = (double)b.bbMax.x - b.bbMin.x, ey = (double)b.bbMax.y - b.bbMin.y,
+                 ez = (double)b.bbMax.z - b.bbMin.z;
+    const double s = 0.5 * sqrt(ex * ex + ey * ey + ez * ez);
+    double cx, cy, cz, sxx = 0, syy = 0, szz = 0, sxy = 0, sxz = 0, syz = 0;
+    if (S[0] > 0.0) {
+        const double iw = 1.0 / S[0];
+        cx = S[1] * iw; cy = S[2] * iw; cz = S[3] * iw;
+        sxx = fmax(S[4] * iw - cx * cx, 0.0);
+        syy = fmax(S[5] * iw - cy * cy, 0.0);
+        szz = fmax(S[6] * iw - cz * cz, 0.0);
+        sxy = S[7] * iw - cx * cy;
+        sxz = S[8] * iw - cx * cz;
+        syz = S[9] * iw - cy * cz;
+    } else {  // no intensity: the estimate is 0 wherever the point is
+        cx = 0.5 * ((double)b.bbMin.x + b.bbMax.x);
+        cy = 0.5 * ((double)b.bbMin.y + b.bbMax.y);
+        cz = 0.5 * ((double)b.bbMin.z + b.bbMax.z);
+    }
+    // Cholesky factor of the covariance; a pivot below tol (rounding of a
+    // flat or collinear node) ends its column
+    const double tol = 1e-12 * (sxx + syy + szz) + 1e-300;
+    double L00 = 0, L10 = 0, L20 = 0, L11 = 0, L21 = 0, L22 = 0;
+    if (sxx > tol) {
+        L00 = sqrt(sxx);
+        L10 = sxy / L00;
+        L20 = sxz / L00;
+    }
+    const double p11 = syy - L10 * L10;
+    if (p11 > tol) {
+        L11 = sqrt(p11);
+        L21 = (syz - L20 * L10) / L11;
+    }
+    L22 = sqrt(fmax(szz - L20 * L20 - L21 * L21, 0.0));
+    float4* o = mom + (size_t)kMom * k;
+    const double Ib = fmax(S[10], fmax(S[11], S[12]));
+    o[0] = make_float4((float)cx, (float)cy, (float)cz, (float)Ib);
+    const double vs[4][3] = {{1, 1, 1}, {1, -1, -1}, {-1, 1, -1}, {-1, -1, 1}};
+    for (int q = 0; q < 4; ++q) {
+        const double ox = L00 * vs[q][0];
+        const double oy = L10 * vs[q][0] + L11 * vs[q][1];
+        const double oz = L20 * vs[q][0] + L21 * vs[q][1] + L22 * vs[q][2];
+        const float wq = q == 0 ? (float)s : q == 1 ? __int_as_float(first) : q == 2 ? __int_as_float(last - first + 1)
+                                                                                       : 0.f;
+        o[1 + q] = make_float4((float)(cx + ox), (float)(cy + oy), (float)(cz + oz), wq);
+    }
+    // A_c(d^) = 3/32 I_c + (-1/2 M_c) . d^ + d^T (15/32 T_c) d^, the
+    // off-diagonal T terms doubled so that the form is a plain dot product
+    // with (dx^2, dy^2, dz^2, dx dy, dx dz, dy dz)
+    float T[18];
+    for (int c = 0; c < 3; ++c) {
+        o[5 + c] = make_float4((float)(-0.5 * S[13 + 3 * c]), (float)(-0.5 * S[14 + 3 * c]),
+                               (float)(-0.5 * S[15 + 3 * c]), (float)(0.09375 * S[10 + c]));
+        const double f = 0.46875;
+        T[6 * c + 0] = (float)(f * S[22 + 6 * c]);
+        T[6 * c + 1] = (float)(f * S[23 + 6 * c]);
+        T[6 * c + 2] = (float)(f * S[24 + 6 * c]);
+        T[6 * c + 3] = (float)(2.0 * f * S[25 + 6 * c]);
+        T[6 * c + 4] = (float)(2.0 * f * S[26 + 6 * c]);
+        T[6 * c + 5] = (float)(2.0 * f * S[27 + 6 * c]);
+    }
+    o[8] = make_float4(T[0], T[1], T[2], T[3]);
+    o[9] = make_float4(T[4], T[5], T[6], T[7]);
+    o[10] = make_float4(T[8], T[9], T[10], T[11]);
+    o[11] = make_float4(T[12], T[13], T[14], T[15]);
+    o[12] = make_float4(T[16], T[17], 0.f, 0.f);
+}
+
+struct CutMom {
+    const arc_bvh_node* nodes;  // n - 1 internal nodes (boxes)
+    const float4* crec;         // leaf records, slots [0, n) in sorted order
+    const float4* mom;          // kMom float4 per internal node
+    int n;
+};
+
+// max(0, n_p . d) / |d|^3 at point x: the pixel-side cosine over the squared
+// distance (cudaGather.cu:45-57, pixel factor).
+__device__ inline float pixel_g(const PixelF& P, const float4& x) {
+    const float dx = x.x - P.px, dy = x.y - P.py, dz = x.z - P.pz;
+    const float dd = dx * dx + dy * dy + dz * dz;
+    const float inv = dd > 1e-30f ? __builtin_amdgcn_rsqf(dd) : 0.f;
+    const float u = fmaxf(P.nx * dx + P.ny * dy + P.nz * dz, 0.f);
+    return u * (inv * inv * inv);
+}
+
+// The node's estimate for the pixel, per channel, added to the pixel's sums.
+__device__ inline void add_moments(const PixelF& P, const float4* __restrict__ r, Acc& acc) {
+    const float4 m0 = r[0];
+    const float K = 0.25f * (pixel_g(P, r[1]) + pixel_g(P, r[2]) + pixel_g(P, r[3]) + pixel_g(P, r[4]));
+    const float dx = m0.x - P.px, dy = m0.y - P.py, dz = m0.z - P.pz;
+    const float dd = dx * dx + dy * dy + dz * dz;
+    const float inv = dd > 1e-30f ? __builtin_amdgcn_rsqf(dd) : 0.f;
+    const float i2 = inv * inv;
+    const float xx = dx * dx, yy = dy * dy, zz = dz * dz, xy = dx * dy, xz = dx * dz, yz = dy * dz;
+    const float4 mr = r[5], mg = r[6], mb = r[7];
+    const float4 t0 = r[8], t1 = r[9], t2 = r[10], t3 = r[11], t4 = r[12];
+    const float qr = t0.x * xx + t0.y * yy + t0.z * zz + t0.w * xy + t1.x * xz + t1.y * yz;
+    const float qg = t1.z * xx + t1.w * yy + t2.x * zz + t2.y * xy + t2.z * xz + t2.w * yz;
+    const float qb = t3.x * xx + t3.y * yy + t3.z * zz + t3.w * xy + t4.x * xz + t4.y * yz;
+    const float ar = mr.w + inv * (mr.x * dx + mr.y * dy + mr.z * dz) + i2 * qr;
+    const float ag = mg.w + inv * (mg.x * dx + mg.y * dy + mg.z * dz) + i2 * qg;
+    const float ab = mb.w + inv * (mb.x * dx + mb.y * dy + mb.z * dz) + i2 * qb;
+    acc.s0 += fmaxf(ar, 0.f) * K;
+    acc.s1 += fmaxf(ag, 0.f) * K;
+    acc.s2 += fmaxf(ab, 0.f) * K;
+    ++acc.run;
+}
+
+// One depth-first cut per 8x8 tile (one wave), refined when any of the
+// tile's pixels asks for it (ballot), as k_cut_eb_tile; kept nodes are summed
+// with the moment estimate, or exactly when they hold at most exact_count
+// AVPLs.  criterion: ARC_CRITERION_BOUND (bound > threshold) or
+// ARC_CRITERION_EXPANSION (s > opening * d_c, or bound * (s / d_c)^2 >
+// threshold).
+__global__ __launch_bounds__(kCutThreads) void k_cut_mom_tile(CutMom t, const float4* __restrict__ gpos,
+                                                              const float4* __restrict__ gnrm,
+                                                              const float4* __restrict__ galb,
+                                                              const arc_material* __restrict__ mats, int W, int H,
+                                                              Threshold th, float delta, int budget, int criterion,
+                                                              float open2, int exact_count,
+                                                              float4* __restrict__ out_rad,
+                                                              unsigned long long* __restrict__ counters) {
+    __shared__ int stk[kCutThreads / 64][kMomStack];
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const int2 xy = tile_pixel(wave, lane, (W + 15) / 16, (H + 15) / 16);
+    const bool live = xy.x < W && xy.y < H;
+    const int q = live ? xy.y * W + xy.x : 0;
+    const PixelF P = load_pixel(gpos, gnrm, galb, mats, q);
+    int* S = stk[wave];
+    const float thr = live ? th.at(q) : INFINITY;
+    const float delta2 = delta * delta;
+    const bool expansion = criterion == ARC_CRITERION_EXPANSION;
+    Acc acc;
+    unsigned nvis = 0, nev = 0, nerr = 0, nbud = 0, ntest = 0, nexact = 0;  // wave-uniform
+    int sp = 0;
+    int node = -1;
+    for (;;) {
+        if (node < 0) {
+            const int k = -(node + 1);
+            const arc_bvh_node& b = t.nodes[k];
+            const float4* r = t.mom + kMom * (size_t)k;
+            const float4 m0 = r[0];
+            const float bound = node_bound(P, b, m0.w, delta2);
+            const int count = __float_as_int(r[3].w);
+            if (!__ballot(bound > 0.f)) {
+                // behind every pixel's tangent plane: contributes exactly 0
+            } else if (count <= exact_count) {
+                // small node: its AVPLs exactly (refining it would end in the same sum)
+                const int first = __float_as_int(r[2].w);
+                for (int i = first; i < first + count; ++i) {
+                    const float4* l = t.crec + kCrec * (size_t)i;
+                    acc.add_run(l[2], geom_est(P, l[0], l[1]));
+                    acc.maybe_flush();
+                }
+                nexact += count;
+            } else {
+                bool refine;
+                if (expansion) {
+                    const float dx = m0.x - P.px, dy = m0.y - P.py, dz = m0.z - P.pz;
+                    const float dc2 = dx * dx + dy * dy + dz * dz;
+                    const float s = r[1].w, s2 = s * s;
+                    refine = bound > 0.f && (s2 > open2 * dc2 || bound * s2 > thr * dc2);
+                } else {
+                    refine = bound > thr;
+                }
+                ++ntest;
+                if (__ballot(refine)) {
+                    if ((int)(nev + nexact) >= budget) {
+                        ++nbud;  // cut budget spent: keep the cluster
+                    } else if (sp < kMomStack) {
+                        S[sp++] = b.right;
+                        node = uniform(b.left);
+                        ++nvis;
+                        continue;
+                    } else {
+                        ++nerr;  // stack full: keep the cluster
+                    }
+                }
+                add_moments(P, r, acc);
+                ++nev;
+            }
+        } else {
+            const float4* l = t.crec + kCrec * (size_t)node;
+            acc.add_run(l[2], geom_est(P, l[0], l[1]));
+            ++nexact;
+        }
+        acc.maybe_flush();
+        if (sp == 0) break;
+        node = uniform(S[--sp]);
+    }
+    acc.flush();
+    const int alive = __popcll(__ballot(live));  // all lanes take part in the ballot
+    if (counters && lane == 0) {
+        atomicAdd(&counters[C_TESTED], (unsigned long long)ntest * alive);
+        atomicAdd(&counters[C_VISITED], (unsigned long long)nvis * alive);
+        atomicAdd(&counters[C_EVALUATED], (unsigned long long)(nev + nexact) * alive);
+        atomicAdd(&counters[C_EXACT], (unsigned long long)nexact * alive);
+        atomicAdd(&counters[C_ERRORS], (unsigned long long)nerr);
+        atomicAdd(&counters[C_BUDGET], (unsigned long long)nbud);
+    }
+    if (live)
+        out_rad[q] = make_float4(acc.k[0].s * P.br, acc.k[1].s * P.bg, acc.k[2].s * P.bb,
+                                 counters ? (float)(nev + nexact) : 1.f);
+}
+
+}  // namespace
+
+namespace arc {
+
+int cut_moments_build(arc_gather* g, arc_bvh* b) {
+    arc_ctx* ctx = g->ctx;
+    const int n = b->n;
+    if (n < 2) return ARC_OK;
+    if (!b->has_cut_records || !g->crec.p) return fail(ctx, ARC_E_STATE, "moment records need the cut tree");
+    const int nblocks = (n + kScanBlock - 1) / kScanBlock;
+    ARC_TRY(ensure(ctx, g->mom, sizeof(float4) * kMom * (size_t)(n - 1)));
+    ARC_TRY(ensure(ctx, g->mom_pre, sizeof(double) * kMomTerms * (size_t)n));
+    ARC_TRY(ensure(ctx, g->mom_tot, sizeof(double) * kMomTerms * (size_t)nblocks));
+    hipLaunchKernelGGL(k_mom_scan, dim3(nblocks), dim3(kScanBlock), 0, ctx->stream, g->crec.as<float4>(), n,
+                       g->mom_pre.as<double>(), g->mom_tot.as<double>());
+    hipLaunchKernelGGL(k_mom_offsets, dim3(kMomTerms), dim3(kScanBlock), 0, ctx->stream, g->mom_tot.as<double>(),
+                       nblocks);
+    hipLaunchKernelGGL(k_mom_nodes, dim3((n - 1 + kScanBlock - 1) / kScanBlock), dim3(kScanBlock), 0, ctx->stream,
+                       b->ranges.as<int>(), b->nodes.as<arc_bvh_node>(), n, g->mom_pre.as<double>(),
+                       g->mom_tot.as<double>(), g->mom.as<float4>());
+    ARC_LAUNCHED(ctx);
+    return ARC_OK;
+}
+
+int cut_moments(arc_gather* g, arc_bvh* b, const CutEBParams& cp, float4* rad, unsigned long long* counters,
+                hipEvent_t mark) {
+    arc_ctx* ctx = g->ctx;
+    const int n = b->n;
+    if (!g->mom.p || g->mom.bytes < sizeof(float4) * kMom * (size_t)(n - 1))
+        return fail(ctx, ARC_E_STATE, "moment estimator: node records not built");
+    CutEB t{};
+    Threshold th{};
+    ARC_TRY(prepare_cut(g, b, cp, t, th));
+    const CutMom tm{b->nodes.as<arc_bvh_node>(), g->crec.as<float4>(), g->mom.as<float4>(), n};
+    const dim3 grid(((g->width + 15) / 16) * ((g->height + 15) / 16));
+    if (mark) ARC_HIP(ctx, hipEventRecord(mark, ctx->stream));
+    hipLaunchKernelGGL(k_cut_mom_tile, grid, dim3(kCutThreads), 0, ctx->stream, tm, g->gpos, g->gnrm, g->galb,
+                       g->mats.as<arc_material>(), g->width, g->height, th, cp.delta, cp.budget, cp.criterion,
+                       cp.opening * cp.opening, cp.exact_count, rad, counters);
+    ARC_LAUNCHED(ctx);
+    return ARC_OK;
+}
+
+}  // namespace arc

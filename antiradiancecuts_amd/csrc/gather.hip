@@ -466,6 +466,9 @@ void arc_gather_params_default(arc_gather_params* p) {
     p->cutImageFloor = 1.0f;
     p->cutConfidence = 0.9f;
     p->cutEvalDepth = 0;
+    p->cutEstimator = ARC_ESTIMATOR_CENTROID;
+    p->cutExactCount = 16;
+    p->cutOpening = 0.3f;
 }
 
 int arc_gather_create(arc_ctx* ctx, int width, int height, const arc_material* materials, int num_materials,
@@ -538,6 +541,9 @@ void arc_gather_destroy(arc_gather* g) {
     arc::release(g->ccount);
     arc::release(g->clist);
     arc::release(g->crec);
+    arc::release(g->mom);
+    arc::release(g->mom_pre);
+    arc::release(g->mom_tot);
     arc::release(g->cest);
     arc_bvh_destroy(g->bvh);
     delete g;
@@ -621,8 +627,20 @@ int arc_gather_run(arc_gather* g, const arc_avpls* avpls, arc_float3 camPos, flo
     if (p->cutEstimateDepth < 0 || p->cutEstimateDepth > 40) return arc::fail(ctx, ARC_E_INVALID, "cutEstimateDepth");
     if (!(p->cutImageFloor >= 0.f)) return arc::fail(ctx, ARC_E_INVALID, "cutImageFloor");
     if (p->cutEvalDepth < 0 || p->cutEvalDepth > 4) return arc::fail(ctx, ARC_E_INVALID, "cutEvalDepth");
-    if (p->cutCriterion != ARC_CRITERION_BOUND && p->cutCriterion != ARC_CRITERION_ESTIMATE)
+    if (p->cutEstimator != ARC_ESTIMATOR_CENTROID && p->cutEstimator != ARC_ESTIMATOR_MOMENTS)
+        return arc::fail(ctx, ARC_E_INVALID, "cutEstimator");
+    const bool mom_est = p->useLightCuts && p->cutMode == ARC_CUT_ERROR_BOUND && p->cutEstimator == ARC_ESTIMATOR_MOMENTS;
+    if (p->cutCriterion != ARC_CRITERION_BOUND && p->cutCriterion != ARC_CRITERION_ESTIMATE &&
+        p->cutCriterion != ARC_CRITERION_EXPANSION)
         return arc::fail(ctx, ARC_E_INVALID, "cutCriterion");
+    if (mom_est && p->cutCriterion == ARC_CRITERION_ESTIMATE)
+        return arc::fail(ctx, ARC_E_INVALID, "the moment estimator takes ARC_CRITERION_BOUND or ARC_CRITERION_EXPANSION");
+    if (!mom_est && p->useLightCuts && p->cutMode == ARC_CUT_ERROR_BOUND && p->cutCriterion == ARC_CRITERION_EXPANSION)
+        return arc::fail(ctx, ARC_E_INVALID, "ARC_CRITERION_EXPANSION needs the moment estimator");
+    if (mom_est && (!p->cutPerTile || p->cutEvalDepth != 0))
+        return arc::fail(ctx, ARC_E_INVALID, "the moment estimator runs per 8x8 tile (cutPerTile 1, cutEvalDepth 0)");
+    if (p->cutExactCount < 0) return arc::fail(ctx, ARC_E_INVALID, "cutExactCount");
+    if (!(p->cutOpening > 0.f)) return arc::fail(ctx, ARC_E_INVALID, "cutOpening");
     if (p->cutBudget < 1) return arc::fail(ctx, ARC_E_INVALID, "cutBudget");
     // ---- radiance: light cut over this frame's AvplBvh (cudaGather.cu:854-923) or exact
     int rad_splits = -1, anti_splits = 0;
@@ -646,16 +664,19 @@ int arc_gather_run(arc_gather* g, const arc_avpls* avpls, arc_float3 camPos, flo
         // AvplBvh(avpls, false) per frame (cudaGather.cu:858)
         if (moments)
             ARC_TRY(build_cut_tree(g, a, (flags & ARC_TRUST_INPUT) == 0));
-        else
+        if (mom_est) ARC_TRY(arc::cut_moments_build(g, g->bvh));
+        if (!moments)
             ARC_TRY(arc::bvh_build_avpls(g->bvh, a, nullptr, p->seed, (flags & ARC_TRUST_INPUT) == 0));
         if (profile) ARC_HIP(ctx, hipEventRecord(ev[2], ctx->stream));
         arc_bvh* b = g->bvh;
         if (moments) {
             const arc::CutEBParams cp{p->errorBound, p->cutEstimateDepth, 1e-3f * sceneExtent, p->cutBudget,
                                       p->cutPerTile, p->cutCriterion, p->cutImageFloor, p->cutConfidence,
-                                      p->cutEvalDepth};
+                                      p->cutEvalDepth, p->cutExactCount, p->cutOpening};
             if (p->cutMode == ARC_CUT_LEARNED)
                 ARC_TRY(arc::cut_learned(g, b, cp, rad, counters, profile ? ev[6] : nullptr));
+            else if (mom_est)
+                ARC_TRY(arc::cut_moments(g, b, cp, rad, counters, profile ? ev[6] : nullptr));
             else
                 ARC_TRY(arc::cut_error_bound(g, b, cp, rad, counters, profile ? ev[6] : nullptr));
         } else {
@@ -721,6 +742,7 @@ int arc_gather_run(arc_gather* g, const arc_avpls* avpls, arc_float3 camPos, flo
             stats->budgetHits = (long long)c[C_BUDGET];
             stats->fallbackNodes = (long long)c[C_FALLBACK];
             stats->testedNodes = (long long)c[C_TESTED];
+            stats->exactLeaves = (long long)c[C_EXACT];
         }
         if (flags & ARC_RUN_PROFILE) {
             ARC_HIP(ctx, hipEventSynchronize(ev[5]));

@@ -30,6 +30,9 @@ struct arc_gather {
     arc::DevBuf anti_idx, cbox, ccount, clist;
     arc::DevBuf crec;  // error-bound cut: packed node records
     arc::DevBuf cest;  // error-bound cut: per-pixel estimates and their sum
+    // moment estimator: per-internal-node records, and the FP64 prefix sums
+    // over the sorted leaves (with per-block offsets) they are made from
+    arc::DevBuf mom, mom_pre, mom_tot;
     arc_cutnet* net = nullptr;  // learned cut: the predictor (borrowed)
     arc_bvh* bvh = nullptr;
     // per-frame phase events: frame slot, then 9 marks (start, upload done,
@@ -74,7 +77,7 @@ constexpr float kGuard = 1e-5f;
 
 enum Counter {
     C_VISITED = 0, C_EVALUATED = 1, C_ANTI_VISITED = 2, C_ANTI_EVALUATED = 3, C_ERRORS = 4, C_BUDGET = 5,
-    C_FALLBACK = 6, C_TESTED = 7, C_NUM = 8
+    C_FALLBACK = 6, C_TESTED = 7, C_EXACT = 8, C_NUM = 9
 };
 
 struct Kahan {
@@ -234,10 +237,19 @@ struct CutEBParams {
     float image_floor;  // cutImageFloor
     float confidence;   // cutConfidence (learned cut)
     int eval_depth;     // cutEvalDepth
+    int exact_count;    // cutExactCount (moment estimator)
+    float opening;      // cutOpening (ARC_CRITERION_EXPANSION)
 };
 // `mark` (may be null) is recorded right before the main traversal kernel.
 int cut_error_bound(arc_gather* g, arc_bvh* b, const CutEBParams& cp, float4* rad, unsigned long long* counters,
                     hipEvent_t mark);
+// The moment estimator's node records for this frame's cut tree (after
+// bvh_build_cut): FP64 prefix sums over the sorted leaves, then one record
+// per internal node.
+int cut_moments_build(arc_gather* g, arc_bvh* b);
+// The error-bound cut per 8x8 tile with the moment estimator.
+int cut_moments(arc_gather* g, arc_bvh* b, const CutEBParams& cp, float4* rad, unsigned long long* counters,
+                hipEvent_t mark);
 // The learned cut: per tile, refine / keep from g->net, the error-bound
 // criterion where the prediction is not confident.
 int cut_learned(arc_gather* g, arc_bvh* b, const CutEBParams& cp, float4* rad, unsigned long long* counters,

@@ -33,6 +33,9 @@ ARC_CUTNET_PARAMS = 6400
 ARC_CRITERION_BOUND = 0
 ARC_SHOOT_NO_EXPLICIT_DIRECT = 1
 ARC_CRITERION_ESTIMATE = 1
+ARC_CRITERION_EXPANSION = 2
+ARC_ESTIMATOR_CENTROID = 0
+ARC_ESTIMATOR_MOMENTS = 1
 ARC_ANTIRAD_DISK = 0
 ARC_ANTIRAD_CONE = 1
 ARC_ANTI_EXACT = 0
@@ -86,7 +89,8 @@ class arc_gather_params(C.Structure):
                 ("writeSentinel", C.c_int), ("stackSize", C.c_int), ("seed", C.c_uint64),
                 ("cutEstimateDepth", C.c_int), ("cutBudget", C.c_int), ("cutPerTile", C.c_int),
                 ("cutCriterion", C.c_int), ("cutImageFloor", C.c_float), ("cutConfidence", C.c_float),
-                ("cutEvalDepth", C.c_int)]
+                ("cutEvalDepth", C.c_int), ("cutEstimator", C.c_int), ("cutExactCount", C.c_int),
+                ("cutOpening", C.c_float)]
 
 
 class arc_gather_stats(C.Structure):
@@ -95,7 +99,8 @@ class arc_gather_stats(C.Structure):
                 ("visitedNodes", C.c_longlong), ("evaluatedNodes", C.c_longlong),
                 ("antiVisitedNodes", C.c_longlong), ("antiEvaluatedNodes", C.c_longlong),
                 ("errorPixels", C.c_longlong), ("numLeafs", C.c_int), ("budgetHits", C.c_longlong),
-                ("fallbackNodes", C.c_longlong), ("testedNodes", C.c_longlong)]
+                ("fallbackNodes", C.c_longlong), ("testedNodes", C.c_longlong),
+                ("exactLeaves", C.c_longlong)]
 
 
 class arc_scene_desc(C.Structure):

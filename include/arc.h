@@ -185,6 +185,17 @@ typedef struct arc_gather arc_gather;
 /* What the error-bound cut compares with errorBound * estimate. */
 #define ARC_CRITERION_BOUND 0     /* upper bound of the node's contribution (Lightcuts)       */
 #define ARC_CRITERION_ESTIMATE 1  /* contribution bound x expected relative error of the node */
+#define ARC_CRITERION_EXPANSION 2 /* contribution bound x (s / d)^2, and s <= cutOpening x d, with s the
+                                     node's half-diagonal and d the distance to its centroid          */
+
+/* How the error-bound cut sums a kept node. */
+#define ARC_ESTIMATOR_CENTROID 0  /* I x G at the node's intensity-weighted centroid, with the mean
+                                     normal's clamped-cosine heuristic (and cutEvalDepth sums)       */
+#define ARC_ESTIMATOR_MOMENTS 1   /* per RGB channel, the clamped cosine of the node's AVPL normals
+                                     to second order in spherical harmonics, times the pixel-side
+                                     G averaged over four points that match the node's position
+                                     mean and covariance; nodes of at most cutExactCount AVPLs are
+                                     summed exactly over their leaves (per 8x8 tile only)          */
 
 #define ARC_ANTIRAD_DISK 0      /* getAntiradiance, cudaGather.cu:109-144                 */
 #define ARC_ANTIRAD_CONE 1      /* GetAntiradiance, Shaders/Gather.frag:149-188 x BRDF    */
@@ -218,6 +229,10 @@ typedef struct arc_gather_params {
                                          cutConfidence (default 0.9), else the error-bound criterion */
     int cutEvalDepth;                 /* ARC_CUT_ERROR_BOUND: a kept node is summed as its descendants
                                          this many levels down (0..4; leaves where the subtree ends) */
+    int cutEstimator;                 /* ARC_CUT_ERROR_BOUND: ARC_ESTIMATOR_* (default CENTROID)   */
+    int cutExactCount;                /* ARC_ESTIMATOR_MOMENTS: a kept node of at most this many AVPLs
+                                         is summed exactly over its leaves (default 16)            */
+    float cutOpening;                 /* ARC_CRITERION_EXPANSION: largest s / d of a kept node (0.3) */
 } arc_gather_params;
 
 void arc_gather_params_default(arc_gather_params* p);
@@ -238,6 +253,8 @@ typedef struct arc_gather_stats {
                                          error-bound criterion                      */
     long long testedNodes;            /* error-bound and learned cuts: (pixel, inner node)
                                          evaluations of the error-bound criterion   */
+    long long exactLeaves;            /* ARC_ESTIMATOR_MOMENTS: (pixel, AVPL) pairs of kept nodes
+                                         summed exactly (also counted in evaluatedNodes)           */
 } arc_gather_stats;
 
 #define ARC_RUN_PROFILE 1      /* time each phase with HIP events (synchronises)  */

@@ -86,6 +86,56 @@ def test_oblique_pair_cosines(orc, prec):
 
 
 @pytest.mark.parametrize("prec", [32, 64])
+@pytest.mark.parametrize("off_deg", [0.0, 20.0, 45.0, 60.0])
+def test_phong_lobe_closed_form(orc, prec, off_deg):
+    """f_r's Phong lobe (cudaGather.cu:59-68) on the pixel side: the AVPL
+    sits 45 degrees off the pixel normal, so its mirror direction about the
+    normal is (1, 0, 1)/sqrt(2); the camera is rotated off_deg from it in the
+    plane of incidence, so c = max(0, reflect(-wi, n) . wo) = cos(off_deg)
+    for off_deg < 90 and 0 beyond (45 + off_deg >= 90 puts the camera at or
+    below the horizon: c = cos 60 = 0.5 and cos 45 only through the
+    formula).  f_r = rho_d/pi + 1/2 pi^-1 (e + 2) c^e rho_s: at c = 1 the
+    peak, at c = 0 the diffuse part alone."""
+    e, rs = 10.0, np.array([0.3, 0.2, 0.1])
+    mats = [[0] * 10, [0, 0, 0, 0.5, 0.6, 0.7, 0, 0, 0, 0],
+            [0, 0, 0, 0.5, 0.6, 0.7, rs[0], rs[1], rs[2], e]]
+    d = 0.5
+    a = np.array([-d, 0.0, d])
+    na = -a / np.linalg.norm(a)  # facing the pixel
+    th = math.radians(45.0 + off_deg)
+    cam = 10.0 * np.array([math.sin(th), 0.0, math.cos(th)])
+    sc = mk_scene([[0, 0, 0]], [[0, 0, 1]], [avpl(a, na, na, L=(1, 2, 3))], mats=mats, cam=cam)
+    sc.nrm4[:, 3] = 2  # the pixel's material: Phong; the AVPL's: Lambertian
+    rad, _ = orc.pair(orc.SceneRef(sc, 0.1), 0, 0, prec)
+    wo = f32(cam) / np.linalg.norm(f32(cam))  # the camera position as the oracle reads it
+    c = max(0.0, float(wo @ (np.array([1.0, 0.0, 1.0]) / math.sqrt(2.0))))
+    assert abs(c - math.cos(math.radians(off_deg))) < 1e-6
+    f_pix = RHO * INV_PI_F + (0.5 * INV_PI_F * (e + 2.0) * c ** e) * f32(rs)
+    G = (1.0 / math.sqrt(2.0)) / (2 * d * d)  # cos at the pixel 1/sqrt(2), at the AVPL 1
+    want = np.array([1.0, 2.0, 3.0]) * (RHO * INV_PI_F) * G * f_pix
+    np.testing.assert_allclose(rad, want, rtol=3e-6 if prec == 32 else 1e-9)
+
+
+@pytest.mark.parametrize("prec", [32, 64])
+def test_phong_lobe_behind_mirror_is_diffuse(orc, prec):
+    """c = reflect(-wi, n) . wo <= 0 (camera opposite the mirror direction):
+    only rho_d / pi remains (cudaGather.cu:63-66)."""
+    e, rs = 10.0, np.array([0.3, 0.2, 0.1])
+    mats = [[0] * 10, [0, 0, 0, 0.5, 0.6, 0.7, 0, 0, 0, 0],
+            [0, 0, 0, 0.5, 0.6, 0.7, rs[0], rs[1], rs[2], e]]
+    d = 0.5
+    a = np.array([-d, 0.0, d])
+    na = -a / np.linalg.norm(a)
+    for cam in ([-10.0, 0.0, 10.0], [-10.0, 0.0, 0.5], [0.0, 10.0, 0.1]):
+        sc = mk_scene([[0, 0, 0]], [[0, 0, 1]], [avpl(a, na, na)], mats=mats, cam=cam)
+        sc.nrm4[:, 3] = 2
+        rad, _ = orc.pair(orc.SceneRef(sc, 0.1), 0, 0, prec)
+        G = (1.0 / math.sqrt(2.0)) / (2 * d * d)
+        want = (RHO * INV_PI_F) * G * (RHO * INV_PI_F)
+        np.testing.assert_allclose(rad, want, rtol=3e-6 if prec == 32 else 1e-9)
+
+
+@pytest.mark.parametrize("prec", [32, 64])
 def test_back_facing_is_zero(orc, prec):
     sc = mk_scene([[0, 0, 0]], [[0, 0, 1]], [avpl([0, 0, 1], [0, 0, 1], [0, 0, -1])])
     rad, _ = orc.pair(orc.SceneRef(sc, 0.1), 0, 0, prec)

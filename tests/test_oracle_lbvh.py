@@ -49,6 +49,13 @@ def cases():
     flat[:, 2] = 0.5  # degenerate axis: 0/0 -> NaN -> quantised to 0
     yield "flat", flat
     yield "mixture", S.points(4000, seed=3)
+    # Q1 (bvh.cu:349-350): the max reduction starts at FLT_MIN, so an axis
+    # with no positive coordinate keeps max = FLT_MIN and normalises into
+    # [0, ~1] against it
+    neg = rng.random((1500, 3), dtype=np.float32)
+    neg[:, 1] -= np.float32(2.0)      # every y negative
+    neg[:, 2] = -neg[:, 2]            # every z <= 0
+    yield "nonpositive_axes", neg
 
 
 @pytest.mark.parametrize("name,pos", list(cases()), ids=[c[0] for c in cases()])
