@@ -375,6 +375,7 @@ void dispatch_exact(arc_gather* g, int model, const arc::AvplsDev& a, int chunk,
 // This frame's LBVH for the error-bound and learned cuts, with their node
 // records in g->crec.
 int build_cut_tree(arc_gather* g, const arc::AvplsDev& a, bool validate) {
+    g->mom_ready = false;
     if (a.n > 1) ARC_TRY(arc::ensure(g->ctx, g->crec, sizeof(float4) * arc::kCrec * (size_t)(2 * a.n - 1)));
     return arc::bvh_build_cut(g->bvh, a, validate, g->mats.as<arc_material>(), g->crec.as<float4>());
 }
@@ -544,6 +545,11 @@ void arc_gather_destroy(arc_gather* g) {
     arc::release(g->mom);
     arc::release(g->mom_pre);
     arc::release(g->mom_tot);
+    arc::release(g->porder);
+    arc::release(g->pkeys);
+    arc::release(g->psort.keys_alt);
+    arc::release(g->psort.vals_alt);
+    arc::release(g->psort.counts);
     arc::release(g->cest);
     arc_bvh_destroy(g->bvh);
     delete g;
@@ -554,7 +560,13 @@ int arc_gather_set_gbuffer(arc_gather* g, const float* pos4, const float* nrm4, 
     g->gpos = reinterpret_cast<const float4*>(pos4);
     g->gnrm = reinterpret_cast<const float4*>(nrm4);
     g->galb = reinterpret_cast<const float4*>(albedo4);
-    return ARC_OK;
+    return arc::pixel_order_build(g);
+}
+
+int arc_gather_rebuild_visible_points(arc_gather* g) {
+    if (!g) return ARC_E_INVALID;
+    if (!g->gpos || !g->gnrm) return arc::fail(g->ctx, ARC_E_STATE, "no G-buffer bound");
+    return arc::pixel_order_build(g);
 }
 
 int arc_gather_set_outputs(arc_gather* g, float* result4, float* radiance4, float* antiradiance4) {
@@ -566,6 +578,26 @@ int arc_gather_set_outputs(arc_gather* g, float* result4, float* radiance4, floa
 }
 
 arc_bvh* arc_gather_get_bvh(arc_gather* g) { return (g && g->bvh && g->bvh->n > 0) ? g->bvh : nullptr; }
+
+int arc_gather_copy_cut_records(arc_gather* g, int which, float* host, long long max_float4, long long* count) {
+    if (!g || !count || max_float4 < 0 || (max_float4 > 0 && !host)) return ARC_E_INVALID;
+    if (which != ARC_RECORDS_CENTROID && which != ARC_RECORDS_MOMENTS) return ARC_E_INVALID;
+    arc_ctx* ctx = g->ctx;
+    const int n = g->bvh ? g->bvh->n : 0;
+    *count = 0;
+    if (n < 2 || !g->bvh->has_cut_records) return arc::fail(ctx, ARC_E_STATE, "no error-bound cut frame has run");
+    const arc::DevBuf& buf = which == ARC_RECORDS_CENTROID ? g->crec : g->mom;
+    const long long avail = which == ARC_RECORDS_CENTROID ? (long long)arc::kCrec * (2LL * n - 1) : 13LL * (n - 1);
+    if (!buf.p || buf.bytes < sizeof(float4) * (size_t)avail || (which == ARC_RECORDS_MOMENTS && !g->mom_ready))
+        return arc::fail(ctx, ARC_E_STATE, "records of this kind were not built by the last frame");
+    *count = avail;
+    const long long m = std::min(avail, max_float4);
+    if (m > 0) {
+        ARC_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        ARC_HIP(ctx, hipMemcpy(host, buf.p, sizeof(float4) * (size_t)m, hipMemcpyDeviceToHost));
+    }
+    return ARC_OK;
+}
 
 int arc_gather_run(arc_gather* g, const arc_avpls* avpls, arc_float3 camPos, float sceneExtent,
                    const arc_gather_params* p, int flags, arc_gather_stats* stats) {

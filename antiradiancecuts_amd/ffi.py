@@ -36,6 +36,8 @@ ARC_CRITERION_ESTIMATE = 1
 ARC_CRITERION_EXPANSION = 2
 ARC_ESTIMATOR_CENTROID = 0
 ARC_ESTIMATOR_MOMENTS = 1
+ARC_RECORDS_CENTROID = 0
+ARC_RECORDS_MOMENTS = 1
 ARC_ANTIRAD_DISK = 0
 ARC_ANTIRAD_CONE = 1
 ARC_ANTI_EXACT = 0
@@ -156,6 +158,7 @@ ARC_SYMBOLS = [
     ("arc_gather_create", C.c_int, [vp, C.c_int, C.c_int, P(arc_material), C.c_int, P(vp)]),
     ("arc_gather_destroy", None, [vp]),
     ("arc_gather_set_gbuffer", C.c_int, [vp, vp, vp, vp]),
+    ("arc_gather_rebuild_visible_points", C.c_int, [vp]),
     ("arc_gather_set_outputs", C.c_int, [vp, vp, vp, vp]),
     ("arc_gather_run", C.c_int, [vp, P(arc_avpls), arc_float3, C.c_float, P(arc_gather_params), C.c_int,
                                  P(arc_gather_stats)]),
@@ -165,6 +168,7 @@ ARC_SYMBOLS = [
     ("arc_gather_kernel_history", C.c_int, [vp, C.c_int, vp]),
     ("arc_gather_kernel_spans", C.c_int, [vp, C.c_int, vp]),
     ("arc_gather_get_bvh", vp, [vp]),
+    ("arc_gather_copy_cut_records", C.c_int, [vp, C.c_int, vp, C.c_longlong, P(C.c_longlong)]),
     ("arc_gather_ctx", vp, [vp]),
     ("arc_cutnet_create", C.c_int, [vp, P(vp)]),
     ("arc_cutnet_destroy", None, [vp]),

@@ -257,6 +257,18 @@ class Gather:
         m = min(cnt.value, capacity)
         return X[:m], y[:m], cnt.value
 
+    def cut_records(self, which: int) -> np.ndarray:
+        """The last cut frame's node records (arc_gather_copy_cut_records):
+        (2n - 1, 4, 4) centroid records or (n - 1, 13, 4) moment records."""
+        cnt = C.c_longlong(0)
+        check(self.lib.arc_gather_copy_cut_records(self.h, which, None, 0, C.byref(cnt)),
+              "arc_gather_copy_cut_records", self.ctx.h)
+        out = np.zeros((cnt.value, 4), np.float32)
+        check(self.lib.arc_gather_copy_cut_records(self.h, which, out.ctypes.data, cnt.value, C.byref(cnt)),
+              "arc_gather_copy_cut_records", self.ctx.h)
+        per = 4 if which == ffi.ARC_RECORDS_CENTROID else 13
+        return out.reshape(-1, per, 4)
+
     def bvh_to_host(self) -> dict:
         h = self.lib.arc_gather_get_bvh(self.h)
         if not h:

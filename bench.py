@@ -47,6 +47,15 @@ FLOP_RAD_PAIR, FLOP_DISK_PAIR, FLOP_CONE_PAIR = 31, 37, 33
 FLOP_VISIT, FLOP_EVAL = 32, 31
 FLOP_EB_TEST, FLOP_EB_EST = 45, 37
 FLOP_TILE_FILTER = 35
+# moment estimator: Lightcuts-bound test per (pixel, inner node) 32 (the
+# box distance and tangent-plane height, 32 as a visited node); the
+# expansion criterion adds the centroid distance and two products, 45; a
+# moment estimate 161 (four pixel-side points 4 x 18 + 4, the direction to
+# the centroid 10 and its six products 6, per channel the normal moments'
+# form 20 and the clamp, the product and the add 3); a leaf summed exactly
+# is a radiance pair, 31; the cone filter of an antiradiance tile list 30.
+FLOP_MOM_TEST, FLOP_MOM_TEST_EXP, FLOP_MOM_EST = 32, 45, 161
+FLOP_CONE_FILTER = 30
 # AvplBvh rebuild, algorithmic HBM bytes per leaf: codes, perm, sorted
 # position / normal, nodes, parents, ranges, node data and random numbers
 # (276).  The light cuts' rebuild: AVPL input (52), codes, perm, sorted
@@ -68,15 +77,28 @@ def parse(argv=None):
     ap.add_argument("--tris", type=int, default=200_000)
     ap.add_argument("--error-stride", type=int, default=4,
                     help="FP64 error on every k-th pixel in x and y (0 = skip)")
-    ap.add_argument("--cpu-sample", type=int, default=256,
-                    help="CPU baseline on 1/k of the pixels (0 = skip)")
+    ap.add_argument("--cpu-sample", type=int, default=1024,
+                    help="CPU baseline on 1/k of the pixels, 1/(16 k) for its one-thread runs (0 = skip)")
     ap.add_argument("--cut", choices=("error-bound", "solid-angle"), default="error-bound",
                     help="light cut of the radiance: the analytic error bound or kernel_bvh's solid angle")
     ap.add_argument("--error-bound", type=float, default=0.01,
                     help="relative error bound of the cut (the workload's threshold is 1 %%)")
+    ap.add_argument("--estimator", choices=("moments", "centroid"), default="moments",
+                    help="error-bound cut: how a kept node is summed (second-order moments, or the centroid)")
+    ap.add_argument("--criterion", choices=("bound", "estimate", "expansion"), default="bound",
+                    help="error-bound cut: the Lightcuts contribution bound (the workload's analytic bound), the "
+                         "centroid estimator's error estimate, or the moment expansion's")
+    ap.add_argument("--image-floor", type=float, default=0.0,
+                    help="error-bound cut: threshold floor as a fraction of the image-mean estimate (0 = none)")
+    ap.add_argument("--exact-count", type=int, default=16,
+                    help="moment estimator: kept nodes of at most this many AVPLs are summed exactly")
+    ap.add_argument("--opening", type=float, default=0.3, help="expansion criterion: largest s / d of a kept node")
+    ap.add_argument("--antiradiance", choices=("cone", "disk"), default="cone",
+                    help="antiradiance model: 10-degree cones (Gather.frag, the workload's) or disks")
     ap.add_argument("--per-tile", type=int, default=1, help="error-bound cut: one cut per 8x8 tile (1) or per pixel (0)")
-    ap.add_argument("--eval-depth", type=int, default=1,
-                    help="error-bound cut: a kept node is summed as its descendants this many levels down (0-4)")
+    ap.add_argument("--eval-depth", type=int, default=0,
+                    help="centroid estimator: a kept node is summed as its descendants this many levels down (0-4)")
+    ap.add_argument("--cpu-runs", type=int, default=5, help="CPU baseline: timed runs (median reported)")
     ap.add_argument("--estimate-depth", type=int, default=6,
                     help="error-bound cut: depth of the fixed cut that estimates each pixel for its threshold")
     ap.add_argument("--component", choices=("frame", "trace", "batch", "learned", "lbvh", "exact"), default="frame",
@@ -122,10 +144,16 @@ def main() -> int:
     g.set_gbuffer(device_array(sc.pos4), device_array(sc.nrm4), device_array(sc.albedo4))
     av = DeviceAvpls.from_scene(sc)
     eb = args.cut == "error-bound"
-    params = default_params(useLightCuts=1, antiradianceModel=ffi.ARC_ANTIRAD_DISK,
+    mom = eb and args.estimator == "moments"
+    cone = args.antiradiance == "cone"
+    crit = {"bound": ffi.ARC_CRITERION_BOUND, "estimate": ffi.ARC_CRITERION_ESTIMATE,
+            "expansion": ffi.ARC_CRITERION_EXPANSION}[args.criterion]
+    params = default_params(useLightCuts=1, antiradianceModel=ffi.ARC_ANTIRAD_CONE if cone else ffi.ARC_ANTIRAD_DISK,
                             cutMode=ffi.ARC_CUT_ERROR_BOUND if eb else ffi.ARC_CUT_SOLID_ANGLE,
                             errorBound=args.error_bound, cutPerTile=args.per_tile, cutEvalDepth=args.eval_depth,
-                            cutEstimateDepth=args.estimate_depth)
+                            cutEstimateDepth=args.estimate_depth, cutCriterion=crit, cutImageFloor=args.image_floor,
+                            cutEstimator=ffi.ARC_ESTIMATOR_MOMENTS if mom else ffi.ARC_ESTIMATOR_CENTROID,
+                            cutExactCount=args.exact_count, cutOpening=args.opening)
     flags = ffi.ARC_RUN_TIMELINE | ffi.ARC_TRUST_INPUT
 
     for _ in range(args.warmup):
@@ -186,8 +214,14 @@ def main() -> int:
     spans = np.zeros((args.steps, 4), np.float32)
     got = g.lib.arc_gather_kernel_spans(g.h, args.steps, spans.ctypes.data)
     spans = spans[:max(got, 0)]
-    flop_anti = float(st.antiEvaluatedNodes) * FLOP_DISK_PAIR + float(st.antiVisitedNodes) * FLOP_TILE_FILTER
-    if eb:
+    flop_anti = float(st.antiEvaluatedNodes) * (FLOP_CONE_PAIR if cone else FLOP_DISK_PAIR) + \
+        float(st.antiVisitedNodes) * (FLOP_CONE_FILTER if cone else FLOP_TILE_FILTER)
+    if mom:
+        exact = float(st.exactLeaves)
+        flop_rad = float(st.testedNodes) * (FLOP_MOM_TEST_EXP if args.criterion == "expansion" else FLOP_MOM_TEST) \
+            + (float(st.evaluatedNodes) - exact) * FLOP_MOM_EST + exact * FLOP_RAD_PAIR
+        rad_name = "k_cut_mom_tile"
+    elif eb:
         flop_rad = float(st.testedNodes) * FLOP_EB_TEST + float(st.evaluatedNodes) * FLOP_EB_EST
         rad_name = "k_cut_eb_tile" if args.per_tile else "k_cut_eb_px"
     else:
@@ -216,11 +250,11 @@ def main() -> int:
         roof = {"kernel": name, "bound": COMPUTE_BOUND, "engine": "VALU", "achieved": round(ach, 3),
                 "peak": PEAK_FP32_TFLOPS,
                 "unit": "TFLOP/s", "frac": round(ach / PEAK_FP32_TFLOPS, 4),
-                "traffic": profiled_traffic(dom, eb, bool(args.per_tile), args.eval_depth), "ms": round(t_dom, 4),
+                "traffic": profiled_traffic(dom, eb, bool(args.per_tile), args.eval_depth, mom, cone), "ms": round(t_dom, 4),
                 "kernels": per_kernel}
     elif t_pair >= t_bvh and t_pair > 0.0:
         ach = (flop_rad + flop_anti) / (t_pair * 1e-3) / 1e12
-        tr = [profiled_traffic(k, eb, bool(args.per_tile), args.eval_depth) for k in ("radiance", "antiradiance")]
+        tr = [profiled_traffic(k, eb, bool(args.per_tile), args.eval_depth, mom, cone) for k in ("radiance", "antiradiance")]
         roof = {"kernel": f"{rad_name} + k_anti_clustered (concurrent on two streams)", "bound": COMPUTE_BOUND,
                 "engine": "VALU",
                 "achieved": round(ach, 3), "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
@@ -234,7 +268,7 @@ def main() -> int:
                 "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": round(ach / PEAK_HBM_GBS, 4),
                 "traffic": profiled_traffic("bvh", eb, bool(args.per_tile)), "ms": round(t_bvh, 4)}
 
-    cpu = cpu_baseline(sc, params, args.cpu_sample) if args.cpu_sample > 0 else None
+    cpu = cpu_baseline(sc, params, args.cpu_sample, args.cpu_runs) if args.cpu_sample > 0 else None
 
     value = npix * float(n) / (ms_per_step * 1e-3)
     line = {
@@ -257,21 +291,33 @@ def main() -> int:
             "workload": f"clutter_{sc.width}x{sc.height}: Cornell walls + {args.tris} random triangles, "
                         f"{n} AVPLs (bounce 1-4, L log-uniform [1e-3,10], 30% antiradiance); AvplBvh rebuilt "
                         "per frame; light-cut radiance ("
-                        + (f"error bound {args.error_bound:g}, {'per 8x8 tile' if args.per_tile else 'per pixel'}, "
-                           f"kept nodes summed {args.eval_depth} levels down"
+                        + (f"{args.criterion} criterion at {args.error_bound:g}"
+                           + (f", image floor {args.image_floor:g}" if args.image_floor > 0 else ", no image floor")
+                           + (f", second-order moment estimates, nodes of <= {args.exact_count} AVPLs exact, one cut "
+                              "per 64 pixels grouped in 3D" if mom else
+                              f", centroid estimates, {'per 8x8 tile' if args.per_tile else 'per pixel'}, kept nodes "
+                              f"summed {args.eval_depth} levels down")
                            if eb else "solid angle < pi/10")
-                        + "); disk antiradiance over per-cluster AVPL lists (exact)",
+                        + f"); {'10-degree cone' if cone else 'disk'} antiradiance over per-cluster AVPL lists (exact)",
             "width": sc.width, "height": sc.height, "avpls": n, "parallelism": "single GPU",
             "phase_ms": {k: round(v, 4) for k, v in phase_ms.items()},
             "kernel_ms": {k: round(v, 4) for k, v in kernel_ms.items()},
             "cut": {"mode": args.cut, "error_bound": args.error_bound if eb else None,
-                    "eval_depth": args.eval_depth if eb else None, "estimate_depth": args.estimate_depth if eb else None,
+                    "criterion": args.criterion if eb else None, "estimator": args.estimator if eb else None,
+                    "image_floor": args.image_floor if eb else None,
+                    "exact_count": args.exact_count if mom else None,
+                    "opening": args.opening if mom and args.criterion == "expansion" else None,
+                    "eval_depth": args.eval_depth if eb and not mom else None,
+                    "estimate_depth": args.estimate_depth if eb else None,
                     "visited_nodes": int(st.visitedNodes), "tested_nodes": int(st.testedNodes),
-                    "evaluated_nodes": int(st.evaluatedNodes), "stack_overflows": int(st.errorPixels), "budget_hits": int(st.budgetHits)},
+                    "evaluated_nodes": int(st.evaluatedNodes), "exact_leaves": int(st.exactLeaves),
+                    "stack_overflows": int(st.errorPixels), "budget_hits": int(st.budgetHits)},
+            "antiradiance_model": "cone 10 deg (Gather.frag:149-188)" if cone else "disk (cudaGather.cu:97-144)",
             "antiradiance": {"pair_tests": int(st.antiEvaluatedNodes), "tile_filter_tests": int(st.antiVisitedNodes)},
             "scene_build_s": round(t_scene, 2),
         },
         "error_vs_f64": err,
+        "accuracy_target": {"rel_l2": 1e-3, "met": bool(err is not None and err["rel_l2"] <= 1e-3)},
         "roofline": roof,
         "cpu_baseline": cpu,
     }
@@ -288,7 +334,7 @@ BVH_KERNELS = {"k_bounds_partial": 1, "k_bounds_final": 1, "k_encode": 1, "k_his
                "k_refit_local<2>": 1, "k_refit_mid<2>": 1, "k_refit_top<2>": 1}
 
 
-def profiled_traffic(dom: str, eb: bool, per_tile: bool, eval_depth: int = 0):
+def profiled_traffic(dom: str, eb: bool, per_tile: bool, eval_depth: int = 0, mom: bool = False, cone: bool = False):
     """HBM bytes per launch of the dominant kernel on the 1080p x 1M frame,
     from the committed rocprofv3 PMC summary (FETCH_SIZE + WRITE_SIZE scaled
     by a plain copy's readings); None when the summary does not cover it."""
@@ -301,9 +347,9 @@ def profiled_traffic(dom: str, eb: bool, per_tile: bool, eval_depth: int = 0):
         name, _, _, b = line.split("\t")
         table[name] = float(b)
     if dom == "antiradiance":
-        names = {"k_anti_clustered<0, false>": 1}
+        names = {f"k_anti_clustered<{1 if cone else 0}, false>": 1}
     elif dom == "radiance":
-        name = ("k_cut_eb_tile" if per_tile else "k_cut_eb_px") if eb else "k_cut_sa"
+        name = "k_cut_mom_tile" if mom else ("k_cut_eb_tile" if per_tile else "k_cut_eb_px") if eb else "k_cut_sa"
         if eb and per_tile and f"{name}<{eval_depth}>" in table:
             name = f"{name}<{eval_depth}>"
         names = {name: 1}
@@ -574,32 +620,62 @@ def bench_exact(args) -> int:
     return 0
 
 
-def cpu_baseline(sc, params, k: int):
-    """Oracle (C++ restatement, REF32) on this host: full AvplBvh build, then
-    the cut radiance and the exact antiradiance on 1/k of the pixels,
-    extrapolated to a frame."""
+def _cpu_model() -> str:
+    try:
+        for line in Path("/proc/cpuinfo").read_text().splitlines():
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(sc, params, k: int, runs: int = 5):
+    """The oracle (C++ restatement of the reference path, REF32) on this
+    host's cores: the full AvplBvh build (bvh.cu:746-808, one thread), then
+    kernel_bvh's solid-angle cut radiance (cudaGather.cu:316-444) and the
+    exact antiradiance of the frame's model on a pixel sample, extrapolated
+    to the frame.  The median of `runs` runs with all the host's worker
+    threads on every k-th pixel, and of `runs` runs with one thread on every
+    (16 k)-th pixel."""
     from oracle import pyoracle as orc
     from antiradiancecuts_amd.runtime import photon_radius
 
-    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    L = orc.lib()
+    avail = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    threads = int(os.environ.get("OMP_NUM_THREADS", avail))
     rand = np.random.default_rng(0).random(2 * sc.num_avpls - 1, dtype=np.float32)
-    t0 = time.time()
-    tree = orc.avpl_bvh_build(sc, rand)
-    t_bvh = time.time() - t0
-    pix = np.arange(0, sc.npix, k, dtype=np.int32)
-    ref = orc.SceneRef(sc, photon_radius(params, sc.extent))
-    t0 = time.time()
-    orc.gather_cut_solid_angle(ref, tree, params.clusterRefinementThreshold, 32, pix)
-    t_cut = time.time() - t0
-    t0 = time.time()
-    orc.gather_exact(ref, 32, pix, what=2)
-    t_anti = time.time() - t0
-    frame_s = t_bvh + (t_cut + t_anti) * sc.npix / len(pix)
-    return {"value": sc.npix * float(sc.num_avpls) / frame_s, "unit": "pixel-AVPL pairs/s", "cores": threads,
-            "kind": "port", "ms_per_frame": round(frame_s * 1e3, 1),
-            "sample": f"AvplBvh build of all {sc.num_avpls} AVPLs (1 thread) + cut radiance and exact "
-                      f"antiradiance on every {k}th pixel ({len(pix)} px, {threads} threads), extrapolated "
-                      f"to the {sc.npix}-pixel frame; measured {t_bvh + t_cut + t_anti:.1f} s"}
+    ref = orc.SceneRef(sc, photon_radius(params, sc.extent), model=params.antiradianceModel,
+                       cone_half_angle=params.coneHalfAngle)
+
+    def frame_s(pix, nthreads):
+        L.orc_set_threads(nthreads)
+        t0 = time.time()
+        tree = orc.avpl_bvh_build(sc, rand)
+        t_bvh = time.time() - t0
+        t0 = time.time()
+        orc.gather_cut_solid_angle(ref, tree, params.clusterRefinementThreshold, 32, pix)
+        orc.gather_exact(ref, 32, pix, what=2)
+        t_pix = time.time() - t0
+        return t_bvh + t_pix * sc.npix / len(pix), t_bvh + t_pix
+
+    t_all = time.time()
+    pix_t = np.arange(0, sc.npix, k, dtype=np.int32)
+    pix_1 = np.arange(0, sc.npix, 16 * k, dtype=np.int32)
+    multi = [frame_s(pix_t, threads) for _ in range(runs)]
+    single = [frame_s(pix_1, 1) for _ in range(runs)]
+    L.orc_set_threads(threads)
+    f_t = float(np.median([m[0] for m in multi]))
+    f_1 = float(np.median([m[0] for m in single]))
+    return {"value": sc.npix * float(sc.num_avpls) / f_t, "unit": "pixel-AVPL pairs/s", "cores": threads,
+            "kind": "port", "ms_per_frame": round(f_t * 1e3, 1),
+            "value_1_thread": sc.npix * float(sc.num_avpls) / f_1, "ms_per_frame_1_thread": round(f_1 * 1e3, 1),
+            "cpu_model": _cpu_model(), "nproc": avail, "runs": runs, "statistic": "median",
+            "sample": f"per run: AvplBvh build of all {sc.num_avpls} AVPLs (1 thread) + solid-angle cut radiance "
+                      f"and exact {'cone' if params.antiradianceModel == 1 else 'disk'} antiradiance on every "
+                      f"{k}th pixel ({len(pix_t)} px, {threads} threads) or every {16 * k}th pixel "
+                      f"({len(pix_1)} px, 1 thread), extrapolated to the {sc.npix}-pixel frame; "
+                      f"{runs} + {runs} runs took {time.time() - t_all:.1f} s"}
 
 
 if __name__ == "__main__":

@@ -276,6 +276,13 @@ void arc_gather_destroy(arc_gather* g);
  * albedo (float4, may be NULL) overrides the pixel material's diffuse colour. */
 int arc_gather_set_gbuffer(arc_gather* g, const float* pos4, const float* nrm4, const float* albedo4);
 
+/* CudaGather::rebuildVisiblePointsBvh (cudaGather.h:46): regroups the pixels
+ * of the bound G-buffer (sorted by the Morton code of their position; runs of
+ * 64 share a light cut, runs of 1024 an antiradiance list).  Done by
+ * arc_gather_set_gbuffer; call it again after changing the G-buffer in place.
+ * A stale grouping costs speed, not correctness. */
+int arc_gather_rebuild_visible_points(arc_gather* g);
+
 /* The output textures of the constructor, float4 each, any may be NULL:
  * result = radiance - antiradiance (kernel_combine, cudaGather.cu:782-803). */
 int arc_gather_set_outputs(arc_gather* g, float* result4, float* radiance4, float* antiradiance4);
@@ -321,6 +328,17 @@ int arc_gather_kernel_spans(arc_gather* g, int frames, float* out);
  * read, it carries the topology and boxes, and arc_bvh_node_data returns no
  * node data. */
 arc_bvh* arc_gather_get_bvh(arc_gather* g);
+
+/* Test accessor, no counterpart in the reference (the reference's node data
+ * is read through AvplBvh::getAvplBvhNodeDataParam, bvh.h:140): copies the
+ * error-bound cut's node records of the last cut frame to host memory.
+ * which = 0: the centroid records, 4 float4 per slot over 2n - 1 slots
+ * (leaves, then internal nodes); which = 1: the moment estimator's records,
+ * 13 float4 per internal node (ARC_ESTIMATOR_MOMENTS frames only).  *count
+ * receives the float4 available; at most max_float4 are copied.  Synchronous. */
+#define ARC_RECORDS_CENTROID 0
+#define ARC_RECORDS_MOMENTS 1
+int arc_gather_copy_cut_records(arc_gather* g, int which, float* host, long long max_float4, long long* count);
 
 /* The context a gather was created on. */
 arc_ctx* arc_gather_ctx(arc_gather* g);

@@ -15,6 +15,8 @@
 #include <numeric>
 #include <vector>
 
+#include <omp.h>
+
 namespace {
 
 // ===================================================================== LBVH
@@ -902,3 +904,8 @@ int orc_trace_shoot(int ntri, const float* p0, const float* p1, const float* p2,
 }
 
 }  // extern "C"
+
+int orc_set_threads(int n) {
+    if (n > 0) omp_set_num_threads(n);
+    return omp_get_max_threads();
+}

@@ -175,6 +175,10 @@ int orc_trace_shoot(int ntri, const float* p0, const float* p1, const float* p2,
                     int explicit_direct, int* counts, float* pos3, float* nrm3, float* dir3, float* rad3,
                     float* anti3, int* mat, int* bounce, int max_avpls);
 
+/* Worker threads of the oracle's parallel loops (OpenMP) from now on; returns
+ * the count in effect.  n <= 0 leaves it unchanged. */
+int orc_set_threads(int n);
+
 #ifdef __cplusplus
 }
 #endif

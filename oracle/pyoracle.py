@@ -72,6 +72,7 @@ def lib() -> C.CDLL:
                                       vp, vp, vp, i32]),
             "orc_image_normalize": (None, [vp, vp, i32, i32]),
             "orc_image_error": (f32, [vp, vp, i32, vp]),
+            "orc_set_threads": (i32, [i32]),
             "orc_gather_exact_parts": (i32, [C.POINTER(orc_scene), i32, vp, i32, vp, vp, i32]),
             "orc_gather_cut_solid_angle": (i32, [C.POINTER(orc_scene), i32, i32, vp, vp, vp, vp, vp, vp, f32,
                                                  vp, i32, vp, vp, C.POINTER(C.c_longlong),

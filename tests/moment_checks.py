@@ -1,0 +1,98 @@
+"""Host-side FP64 checks of the light cut's node records against sums over
+each node's leaf range (Karras ranges: node k covers sorted leaves
+ranges[k, 0] .. ranges[k, 1]).
+
+The leaves' own records (crec slots [0, n)) are the inputs: position r0.xyz,
+unit normal r1.xyz, BRDF-weighted intensity I = r2.xyz, w = mean(I).  Each
+node's sums come from numpy FP64 prefix sums; the GPU records must equal
+them to FP32 rounding.
+"""
+import numpy as np
+
+
+def leaf_inputs(crec: np.ndarray, n: int):
+    x = crec[:n, 0, :3].astype(np.float64)
+    nm = crec[:n, 1, :3].astype(np.float64)
+    I = crec[:n, 2, :3].astype(np.float64)
+    return x, nm, I
+
+
+def range_sums(values: np.ndarray, ranges: np.ndarray) -> np.ndarray:
+    """sum over each node's leaf range of the per-leaf rows of `values`."""
+    pre = np.concatenate([np.zeros((1,) + values.shape[1:]), np.cumsum(values, axis=0)], axis=0)
+    return pre[ranges[:, 1] + 1] - pre[ranges[:, 0]]
+
+
+def check_centroid_records(crec: np.ndarray, ranges: np.ndarray, n: int):
+    """The centroid records of the internal nodes (slots n + k): count, I,
+    centroid sum(w x)/sum(w) and mean normal sum(w n)/sum(w)."""
+    x, nm, I = leaf_inputs(crec, n)
+    w = I.mean(1, keepdims=True)
+    inner = crec[n:2 * n - 1]
+    count = ranges[:, 1] - ranges[:, 0] + 1
+    np.testing.assert_array_equal(inner[:, 3, 2].astype(np.int64), count)
+    Is = range_sums(I, ranges)
+    np.testing.assert_allclose(inner[:, 2, :3], Is, rtol=2e-5, atol=1e-6 * Is.max())
+    np.testing.assert_allclose(inner[:, 0, 3], Is.max(1), rtol=2e-5, atol=1e-6 * Is.max())
+    W = range_sums(w, ranges)[:, 0]
+    live = W > 1e-6 * W.max()
+    c = range_sums(w * x, ranges)[live] / W[live, None]
+    np.testing.assert_allclose(inner[live, 0, :3], c, rtol=0, atol=2e-5 * (1 + np.abs(c).max()))
+    m = range_sums(w * nm, ranges)[live] / W[live, None]
+    np.testing.assert_allclose(inner[live, 1, :3], m, rtol=0, atol=5e-5)
+    return int(live.sum())
+
+
+def moment_sums(crec: np.ndarray, ranges: np.ndarray, n: int):
+    x, nm, I = leaf_inputs(crec, n)
+    w = I.mean(1, keepdims=True)
+    W = range_sums(w, ranges)[:, 0]
+    Sx = range_sums(w * x, ranges)
+    Sxx = range_sums((w[:, :, None] * x[:, :, None] * x[:, None, :]).reshape(n, 9), ranges).reshape(-1, 3, 3)
+    Is = range_sums(I, ranges)
+    M = np.stack([range_sums(I[:, c:c + 1] * nm, ranges) for c in range(3)], 1)          # (k, c, 3)
+    nn = (nm[:, :, None] * nm[:, None, :]).reshape(n, 9)
+    T = np.stack([range_sums(I[:, c:c + 1] * nn, ranges).reshape(-1, 3, 3) for c in range(3)], 1)  # (k, c, 3, 3)
+    return W, Sx, Sxx, Is, M, T
+
+
+def check_moment_records(mom: np.ndarray, crec: np.ndarray, ranges: np.ndarray, n: int, nodes=None):
+    """The moment estimator's records (13 float4 per internal node)."""
+    W, Sx, Sxx, Is, M, T = moment_sums(crec, ranges, n)
+    k = len(ranges)
+    assert mom.shape == (k, 13, 4)
+    first = mom[:, 2, 3].copy().view(np.int32)
+    count = mom[:, 3, 3].copy().view(np.int32)
+    np.testing.assert_array_equal(first, ranges[:, 0])
+    np.testing.assert_array_equal(count, ranges[:, 1] - ranges[:, 0] + 1)
+    scale = Is.max()
+    np.testing.assert_allclose(mom[:, 0, 3], Is.max(1), rtol=1e-6, atol=1e-7 * scale)
+    # per channel: 3/32 I_c, -1/2 M_c, 15/32 T_c (off-diagonals doubled)
+    for c in range(3):
+        np.testing.assert_allclose(mom[:, 5 + c, 3], 0.09375 * Is[:, c], rtol=1e-6, atol=1e-7 * scale)
+        np.testing.assert_allclose(mom[:, 5 + c, :3], -0.5 * M[:, c], rtol=1e-6, atol=1e-7 * scale)
+    tf = mom[:, 8:13, :].reshape(k, 20)[:, :18].reshape(k, 3, 6)
+    for c in range(3):
+        Tc = 0.46875 * T[:, c]
+        want = np.stack([Tc[:, 0, 0], Tc[:, 1, 1], Tc[:, 2, 2], 2 * Tc[:, 0, 1], 2 * Tc[:, 0, 2], 2 * Tc[:, 1, 2]], 1)
+        np.testing.assert_allclose(tf[:, c], want, rtol=1e-6, atol=1e-7 * scale)
+    # position mean and covariance of the four points
+    live = W > 1e-6 * W.max()
+    c = Sx[live] / W[live, None]
+    S = Sxx[live] / W[live, None, None] - c[:, :, None] * c[:, None, :]
+    pts = mom[live, 1:5, :3].astype(np.float64)
+    np.testing.assert_allclose(mom[live, 0, :3], c, rtol=0, atol=1e-6 * (1 + np.abs(c).max()))
+    off = pts - c[:, None, :]
+    cmax = 1 + np.abs(c).max()
+    np.testing.assert_allclose(off.mean(1), 0, atol=1e-6 * cmax)
+    cov = np.einsum("kqi,kqj->kij", off, off) / 4
+    tr = np.trace(S, axis1=1, axis2=2)
+    # FP64 prefix differences carry about 2^-52 of the prefix total, over W
+    pref = 64 * 2.2e-16 * np.abs(Sxx).max() / W[live]
+    tol = 1e-4 * tr + 4e-7 * np.sqrt(np.maximum(tr, 0)) * cmax + pref + 1e-12
+    assert np.all(np.abs(cov - S) <= tol[:, None, None] + 1e-6 * np.abs(S)), \
+        float(np.max(np.abs(cov - S) - tol[:, None, None]))
+    if nodes is not None:  # the half-diagonal of the node's box
+        e = nodes["bbMax"].astype(np.float64) - nodes["bbMin"].astype(np.float64)
+        np.testing.assert_allclose(mom[:, 1, 3], 0.5 * np.sqrt((e * e).sum(1)), rtol=1e-6, atol=1e-12)
+    return int(live.sum())

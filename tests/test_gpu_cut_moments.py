@@ -1,0 +1,167 @@
+"""The moment estimator of the error-bound cut (ARC_ESTIMATOR_MOMENTS)
+through the C ABI.
+
+What fixes it without a reference implementation (the reference's cut sums a
+random representative, cudaGather.cu:416-420, bvh.cu:726-744):
+  * its node records equal FP64 sums over each node's leaf range (count,
+    first leaf, per-channel I, I n and I n n^T, the intensity-weighted
+    position mean, and four points whose mean and covariance are the node's);
+    the centroid records of the same tree are checked the same way, over
+    several 32K-leaf refit spans and at 1M AVPLs;
+  * errorBound = 0 refines every node that can contribute, and
+    cutExactCount >= n sums the root's AVPLs exactly: both equal the exact
+    FP64 radiance (relative L2 <= 1e-5);
+  * a node whose AVPLs share one position is summed exactly by the four
+    points, so the estimate equals sum_c max(sum_j I_jc f2(t_j), 0) G_p with
+    f2(t) = 3/32 + t/2 + 15/32 t^2 evaluated per AVPL (the addition theorem
+    that lets the node keep three moments instead of its AVPLs);
+  * on the 1080p x 1M frame the Lightcuts bound at 1 % stays within the
+    measured error, with cone antiradiance.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from antiradiancecuts_amd import ffi
+from antiradiancecuts_amd import scene as S
+from antiradiancecuts_amd.runtime import DeviceAvpls, Gather, default_params, device_array
+from tests import moment_checks as M
+
+pytestmark = pytest.mark.gpu
+
+
+def mom_params(**kw):
+    base = dict(useLightCuts=1, cutMode=ffi.ARC_CUT_ERROR_BOUND, cutPerTile=1,
+                cutEstimator=ffi.ARC_ESTIMATOR_MOMENTS, cutCriterion=ffi.ARC_CRITERION_BOUND,
+                cutImageFloor=0.0, errorBound=0.01, cutExactCount=16)
+    base.update(kw)
+    return default_params(**base)
+
+
+def rel_l2(a, b):
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+def make_rig(gpu_ctx, sc):
+    g = Gather(gpu_ctx, sc.width, sc.height, sc.materials)
+    g.set_gbuffer(device_array(sc.pos4), device_array(sc.nrm4), device_array(sc.albedo4))
+    return g, DeviceAvpls.from_scene(sc)
+
+
+@pytest.mark.parametrize("num_avpls", [100_000, 1_000_000])
+def test_node_records_are_range_sums(gpu_ctx, num_avpls):
+    sc = S.clutter_1080p(width=64, height=48, num_avpls=num_avpls)
+    print(f"[mom] records n={num_avpls}", flush=True)
+    g, av = make_rig(gpu_ctx, sc)
+    try:
+        g.run(av, sc.cam, sc.extent, mom_params())
+        gpu_ctx.sync()
+        tree = g.bvh_to_host()
+        crec = g.cut_records(ffi.ARC_RECORDS_CENTROID)
+        mom = g.cut_records(ffi.ARC_RECORDS_MOMENTS)
+    finally:
+        g.close()
+    n = tree["codes"].shape[0]
+    assert n == num_avpls
+    ranges = np.asarray(tree["ranges"]).reshape(-1, 2).astype(np.int64)
+    # the leaves' records hold the sorted AVPLs
+    perm = tree["perm"]
+    np.testing.assert_array_equal(crec[:n, 0, :3], sc.avpl_pos[perm])
+    diff = sc.materials[sc.avpl_mat[perm], 3:6].astype(np.float32)
+    want_I = sc.avpl_rad[perm] * (np.float32(0.318309886) * diff)
+    np.testing.assert_allclose(crec[:n, 2, :3], want_I, rtol=1e-6)
+    nrm = sc.avpl_nrm[perm].astype(np.float64)
+    np.testing.assert_allclose(crec[:n, 1, :3], nrm / np.linalg.norm(nrm, axis=1, keepdims=True), atol=1e-6)
+    live_c = M.check_centroid_records(crec, ranges, n)
+    live_m = M.check_moment_records(mom, crec, ranges, n, tree["nodes"])
+    assert live_c > 0.99 * (n - 1) and live_m > 0.99 * (n - 1)
+
+
+@pytest.fixture(scope="module")
+def small(gpu_ctx):
+    sc = S.clutter_1080p(width=96, height=64, num_avpls=20_000, num_tris=20_000)
+    g, av = make_rig(gpu_ctx, sc)
+    p = mom_params(antiradianceModel=ffi.ARC_ANTIRAD_CONE)
+    r64, a64 = g.exact_f64(av, sc.cam, sc.extent, p)
+    yield sc, g, av, r64.cpu().numpy(), a64.cpu().numpy()
+    g.close()
+
+
+def run(gpu_ctx, g, av, sc, p):
+    st = g.run(av, sc.cam, sc.extent, p, ffi.ARC_RUN_COUNTERS)
+    gpu_ctx.sync()
+    return g.radiance[:, :3].double().cpu().numpy(), g.result[:, :3].double().cpu().numpy(), st
+
+
+@pytest.mark.parametrize("criterion", [ffi.ARC_CRITERION_BOUND, ffi.ARC_CRITERION_EXPANSION])
+def test_zero_bound_is_exact(gpu_ctx, small, criterion):
+    sc, g, av, r64, a64 = small
+    print(f"[mom] zero bound criterion={criterion}", flush=True)
+    rad, res, st = run(gpu_ctx, g, av, sc, mom_params(errorBound=0.0, cutCriterion=criterion, cutExactCount=0,
+                                                      antiradianceModel=ffi.ARC_ANTIRAD_CONE))
+    assert st.errorPixels == 0 and st.budgetHits == 0
+    assert rel_l2(rad, r64) <= 1e-5
+    assert rel_l2(res, r64 - a64) <= 1e-5
+
+
+def test_exact_count_covers_everything(gpu_ctx, small):
+    sc, g, av, r64, a64 = small
+    print("[mom] root summed exactly", flush=True)
+    rad, _, st = run(gpu_ctx, g, av, sc, mom_params(errorBound=1e30, cutExactCount=sc.num_avpls,
+                                                    antiradianceModel=ffi.ARC_ANTIRAD_CONE))
+    assert st.testedNodes == 0  # the root is small enough: no test, every AVPL exactly
+    assert st.exactLeaves == sc.num_avpls * sc.npix
+    assert rel_l2(rad, r64) <= 1e-5
+
+
+def test_bound_tightens_error(gpu_ctx, small):
+    sc, g, av, r64, a64 = small
+    errs, work = [], []
+    for eb in (0.1, 0.01, 0.001):
+        rad, _, st = run(gpu_ctx, g, av, sc, mom_params(errorBound=eb, antiradianceModel=ffi.ARC_ANTIRAD_CONE))
+        errs.append(rel_l2(rad, r64))
+        work.append(st.evaluatedNodes)
+        print(f"[mom] bound {eb}: rel-L2 {errs[-1]:.2e}, evaluated {st.evaluatedNodes}", flush=True)
+    assert errs[0] > errs[1] > errs[2] and work[0] < work[1] < work[2]
+    assert errs[1] < 1e-2
+
+
+def test_colocated_node_is_the_sh2_sum(gpu_ctx):
+    """Three AVPLs at one point with different normals and colours, seen from
+    pixels far away: the root is kept (huge bound) and estimated; its four
+    points coincide with the AVPLs' position, so the estimate is
+    sum_c max(sum_j I_jc f2(-n_j . d^), 0) * max(n_p . d, 0) / |d|^3."""
+    rng = np.random.default_rng(4)
+    P = 64
+    pos = np.zeros((P, 4), np.float32)
+    pos[:, 0] = rng.uniform(-0.5, 0.5, P)
+    pos[:, 1] = rng.uniform(-0.5, 0.5, P)
+    pos[:, 3] = 1
+    nrm = np.zeros((P, 4), np.float32)
+    nrm[:, 2] = 1
+    nrm[:, 3] = 1
+    alb = np.zeros((P, 4), np.float32)
+    alb[:, :3] = 0.5
+    a = np.array([0.1, -0.2, 2.0], np.float32)
+    an = np.array([[0, 0, -1], [0.6, 0, -0.8], [0, 0.8, 0.6]], np.float32)
+    L = np.array([[1, 2, 3], [4, 0.5, 0.1], [0.3, 3, 1]], np.float32)
+    mats = np.array([[0] * 10, [0, 0, 0, 0.5, 0.6, 0.7, 0, 0, 0, 0]], np.float32)
+    sc = S.Scene(8, 8, pos, nrm, alb, mats, np.array([0, 0, 10], np.float32), 1.0, np.tile(a, (3, 1)), an,
+                 -an, L, np.zeros_like(L), np.ones(3, np.int32), np.ones(3, np.int32))
+    g, av = make_rig(gpu_ctx, sc)
+    try:
+        rad, _, st = run(gpu_ctx, g, av, sc, mom_params(errorBound=1e30, cutExactCount=0))
+    finally:
+        g.close()
+    assert st.evaluatedNodes == P and st.exactLeaves == 0
+    I = L.astype(np.float64) * (np.float32(0.318309886) * mats[1, 3:6]).astype(np.float64)
+    want = np.zeros((P, 3))
+    for q in range(P):
+        d = a.astype(np.float64) - pos[q, :3]
+        dl = np.linalg.norm(d)
+        t = -(an.astype(np.float64) @ d) / dl
+        f2 = 3 / 32 + t / 2 + 15 / 32 * t * t
+        A = np.maximum((I * f2[:, None]).sum(0), 0)
+        want[q] = A * max(d[2], 0) / dl ** 3 * (0.5 / math.pi)
+    np.testing.assert_allclose(rad, want, rtol=2e-5)
