@@ -45,43 +45,65 @@ constexpr int kMom = 13;         // float4 per internal node record
 constexpr int kMomStack = 96;    // per-wave stack of the tile traversal
 
 // Terms of leaf i of the sorted order (crec leaf record: r0 = position,
-// r1 = unit normal, r2 = I):
-//   0 w, 1-3 w x, 4-9 w (xx, yy, zz, xy, xz, yz), 10-12 I_c,
-//   13 + 3c + k  I_c n_k,  22 + 6c + (xx, yy, zz, xy, xz, yz)  I_c n n^T.
-__device__ inline void leaf_terms(const float4* __restrict__ crec, int i, double* t) {
-    const float4 r0 = crec[kCrec * (size_t)i], r1 = crec[kCrec * (size_t)i + 1], r2 = crec[kCrec * (size_t)i + 2];
-    const double x = r0.x, y = r0.y, z = r0.z, nx = r1.x, ny = r1.y, nz = r1.z;
-    const double I[3] = {r2.x, r2.y, r2.z};
-    const double w = (I[0] + I[1] + I[2]) * (1.0 / 3.0);
-    t[0] = w;
-    t[1] = w * x; t[2] = w * y; t[3] = w * z;
-    t[4] = w * x * x; t[5] = w * y * y; t[6] = w * z * z;
-    t[7] = w * x * y; t[8] = w * x * z; t[9] = w * y * z;
-#pragma unroll
-    for (int c = 0; c < 3; ++c) {
-        t[10 + c] = I[c];
-        t[13 + 3 * c] = I[c] * nx; t[14 + 3 * c] = I[c] * ny; t[15 + 3 * c] = I[c] * nz;
-        t[22 + 6 * c] = I[c] * nx * nx; t[23 + 6 * c] = I[c] * ny * ny; t[24 + 6 * c] = I[c] * nz * nz;
-        t[25 + 6 * c] = I[c] * nx * ny; t[26 + 6 * c] = I[c] * nx * nz; t[27 + 6 * c] = I[c] * ny * nz;
+// r1 = unit normal, r2 = I), in four groups of ten:
+//   group 0: w, w x, w y, w z, w (xx, yy, zz, xy, xz, yz)   (w = mean of I,
+//            x relative to the centre of the root's box: smaller prefix
+//            totals, less cancellation in a node's covariance)
+//   group 1 + c (channel c): I_c, I_c n (3), I_c (nx nx, ny ny, nz nz,
+//   nx ny, nx nz, ny nz).
+constexpr int kGroup = 10;
+constexpr int kGroups = kMomTerms / kGroup;
+static_assert(kGroups * kGroup == kMomTerms, "four groups of ten terms");
+
+__device__ inline void root_centre(const arc_bvh_node* __restrict__ nodes, double* ref) {
+    const arc_bvh_node& b = nodes[0];
+    ref[0] = 0.5 * ((double)b.bbMin.x + b.bbMax.x);
+    ref[1] = 0.5 * ((double)b.bbMin.y + b.bbMax.y);
+    ref[2] = 0.5 * ((double)b.bbMin.z + b.bbMax.z);
+}
+
+__device__ inline void leaf_terms(const float4* __restrict__ crec, int i, int group, const double* ref, double* t) {
+    const float4* r = crec + kCrec * (size_t)i;
+    if (group == 0) {
+        const float4 r0 = r[0], r2 = r[2];
+        const double x = r0.x - ref[0], y = r0.y - ref[1], z = r0.z - ref[2];
+        const double w = ((double)r2.x + (double)r2.y + (double)r2.z) * (1.0 / 3.0);
+        t[0] = w;
+        t[1] = w * x; t[2] = w * y; t[3] = w * z;
+        t[4] = w * x * x; t[5] = w * y * y; t[6] = w * z * z;
+        t[7] = w * x * y; t[8] = w * x * z; t[9] = w * y * z;
+    } else {
+        const float4 r1 = r[1], r2 = r[2];
+        const double nx = r1.x, ny = r1.y, nz = r1.z;
+        const double I = group == 1 ? r2.x : group == 2 ? r2.y : r2.z;
+        t[0] = I;
+        t[1] = I * nx; t[2] = I * ny; t[3] = I * nz;
+        t[4] = I * nx * nx; t[5] = I * ny * ny; t[6] = I * nz * nz;
+        t[7] = I * nx * ny; t[8] = I * nx * nz; t[9] = I * ny * nz;
     }
 }
 
-// Inclusive prefix sums of the terms within each block of kScanBlock leaves
-// (pre, leaf-major: 40 doubles per leaf) and the block totals (tot).
-__global__ __launch_bounds__(kScanBlock) void k_mom_scan(const float4* __restrict__ crec, int n,
+// Inclusive prefix sums of one group of terms (blockIdx.y) within each
+// block of kScanBlock leaves (pre, leaf-major: 40 doubles per leaf) and the
+// block totals (tot, 40 per block).
+__global__ __launch_bounds__(kScanBlock) void k_mom_scan(const float4* __restrict__ crec,
+                                                         const arc_bvh_node* __restrict__ nodes, int n,
                                                          double* __restrict__ pre, double* __restrict__ tot) {
-    __shared__ double wt[kScanBlock / 64][kMomTerms];
+    __shared__ double wt[kScanBlock / 64][kGroup];
     const int i = blockIdx.x * kScanBlock + threadIdx.x;
+    const int group = blockIdx.y;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    double v[kMomTerms];
+    double v[kGroup];
     if (i < n) {
-        leaf_terms(crec, i, v);
+        double ref[3];
+        root_centre(nodes, ref);
+        leaf_terms(crec, i, group, ref, v);
     } else {
 #pragma unroll
-        for (int k = 0; k < kMomTerms; ++k) v[k] = 0.0;
+        for (int k = 0; k < kGroup; ++k) v[k] = 0.0;
     }
 #pragma unroll
-    for (int k = 0; k < kMomTerms; ++k) {
+    for (int k = 0; k < kGroup; ++k) {
 #pragma unroll
         for (int off = 1; off < 64; off <<= 1) {
             const double u = __shfl_up(v[k], off);
@@ -91,20 +113,20 @@ __global__ __launch_bounds__(kScanBlock) void k_mom_scan(const float4* __restric
     }
     __syncthreads();
 #pragma unroll
-    for (int k = 0; k < kMomTerms; ++k) {
+    for (int k = 0; k < kGroup; ++k) {
         double add = 0.0;
         for (int w = 0; w < wave; ++w) add += wt[w][k];
         v[k] += add;
     }
     if (i < n) {
-        double* o = pre + (size_t)kMomTerms * i;
+        double* o = pre + (size_t)kMomTerms * i + kGroup * group;
 #pragma unroll
-        for (int k = 0; k < kMomTerms; ++k) o[k] = v[k];
+        for (int k = 0; k < kGroup; ++k) o[k] = v[k];
     }
     if (threadIdx.x == kScanBlock - 1) {
-        double* o = tot + (size_t)kMomTerms * blockIdx.x;
+        double* o = tot + (size_t)kMomTerms * blockIdx.x + kGroup * group;
 #pragma unroll
-        for (int k = 0; k < kMomTerms; ++k) o[k] = v[k];
+        for (int k = 0; k < kGroup; ++k) o[k] = v[k];
     }
 }
 
@@ -136,11 +158,23 @@ __global__ __launch_bounds__(kScanBlock) void k_mom_offsets(double* __restrict__
     }
 }
 
-__device__ inline double prefix(const double* __restrict__ pre, const double* __restrict__ tot, int i, int k) {
-    return pre[(size_t)kMomTerms * i + k] + tot[(size_t)kMomTerms * (i / kScanBlock) + k];
+// Sums of terms [k0, k0 + kGroup) over sorted leaves [first, last].
+__device__ inline void range_group(const double* __restrict__ pre, const double* __restrict__ tot, int first,
+                                   int last, int k0, double* S) {
+    const double* hi = pre + (size_t)kMomTerms * last + k0;
+    const double* th = tot + (size_t)kMomTerms * (last / kScanBlock) + k0;
+#pragma unroll
+    for (int k = 0; k < kGroup; ++k) S[k] = hi[k] + th[k];
+    if (first > 0) {
+        const double* lo = pre + (size_t)kMomTerms * (first - 1) + k0;
+        const double* tl = tot + (size_t)kMomTerms * ((first - 1) / kScanBlock) + k0;
+#pragma unroll
+        for (int k = 0; k < kGroup; ++k) S[k] -= lo[k] + tl[k];
+    }
 }
 
-// One record per internal node from the prefix rows at its leaf range.
+// One record per internal node from the prefix rows at its leaf range, one
+// group of terms at a time (the record's position part, then each channel).
 __global__ __launch_bounds__(kScanBlock) void k_mom_nodes(const int* __restrict__ ranges,
                                                           const arc_bvh_node* __restrict__ nodes, int n,
                                                           const double* __restrict__ pre,
@@ -148,76 +182,98 @@ __global__ __launch_bounds__(kScanBlock) void k_mom_nodes(const int* __restrict_
     const int k = blockIdx.x * kScanBlock + threadIdx.x;
     if (k >= n - 1) return;
     const int first = ranges[2 * k], last = ranges[2 * k + 1];
-    double S[kMomTerms];
-#pragma unroll
-    for (int t = 0; t < kMomTerms; ++t)
-        S[t] = prefix(pre, tot, last, t) - (first > 0 ? prefix(pre, tot, first - 1, t) : 0.0);
+    float4* o = mom + (size_t)kMom * k;
+    double S[kGroup];
+    range_group(pre, tot, first, last, 0, S);
     const arc_bvh_node& b = nodes[k];
     const double ex = (double)b.bbMax.x - b.bbMin.x, ey = (double)b.bbMax.y - b.bbMin.y,
                  ez = (double)b.bbMax.z - b.bbMin.z;
     const double s = 0.5 * sqrt(ex * ex + ey * ey + ez * ez);
     double cx, cy, cz, sxx = 0, syy = 0, szz = 0, sxy = 0, sxz = 0, syz = 0;
     if (S[0] > 0.0) {
+        double ref[3];
+        root_centre(nodes, ref);
         const double iw = 1.0 / S[0];
-        cx = S[1] * iw; cy = S[2] * iw; cz = S[3] * iw;
-        sxx = fmax(S[4] * iw - cx * cx, 0.0);
-        syy = fmax(S[5] * iw - cy * cy, 0.0);
-        szz = fmax(S[6] * iw - cz * cz, 0.0);
-        sxy = S[7] * iw - cx * cy;
-        sxz = S[8] * iw - cx * cz;
-        syz = S[9] * iw - cy * cz;
+        const double ux = S[1] * iw, uy = S[2] * iw, uz = S[3] * iw;  // relative to ref
+        sxx = fmax(S[4] * iw - ux * ux, 0.0);
+        syy = fmax(S[5] * iw - uy * uy, 0.0);
+        szz = fmax(S[6] * iw - uz * uz, 0.0);
+        sxy = S[7] * iw - ux * uy;
+        sxz = S[8] * iw - ux * uz;
+        syz = S[9] * iw - uy * uz;
+        cx = ref[0] + ux; cy = ref[1] + uy; cz = ref[2] + uz;
     } else {  // no intensity: the estimate is 0 wherever the point is
         cx = 0.5 * ((double)b.bbMin.x + b.bbMax.x);
         cy = 0.5 * ((double)b.bbMin.y + b.bbMax.y);
         cz = 0.5 * ((double)b.bbMin.z + b.bbMax.z);
     }
-    // Cholesky factor of the covariance; a pivot below tol (rounding of a
-    // flat or collinear node) ends its column
+    // Cholesky factor of the covariance, pivoted: the axis of largest
+    // variance first, so that the small and noisy entries of a flat or
+    // collinear node (FP64 prefix differences) never divide a large one; a
+    // pivot below tol ends its column
+    auto var = [&](int i) -> double { return i == 0 ? sxx : i == 1 ? syy : szz; };
+    int a0 = 0;
+    if (syy > var(a0)) a0 = 1;
+    if (szz > var(a0)) a0 = 2;
+    int a1 = a0 == 0 ? 1 : 0, a2 = 3 - a0 - a1;
+    if (var(a2) > var(a1)) {
+        const int t = a1;
+        a1 = a2;
+        a2 = t;
+    }
+    auto cov = [&](int i, int j) -> double {  // i != j
+        const int m = i + j;  // 1: xy, 2: xz, 3: yz
+        return m == 1 ? sxy : m == 2 ? sxz : syz;
+    };
     const double tol = 1e-12 * (sxx + syy + szz) + 1e-300;
     double L00 = 0, L10 = 0, L20 = 0, L11 = 0, L21 = 0, L22 = 0;
-    if (sxx > tol) {
-        L00 = sqrt(sxx);
-        L10 = sxy / L00;
-        L20 = sxz / L00;
+    if (var(a0) > tol) {
+        L00 = sqrt(var(a0));
+        L10 = cov(a1, a0) / L00;
+        L20 = cov(a2, a0) / L00;
+        const double p11 = var(a1) - L10 * L10;
+        if (p11 > tol) {
+            L11 = sqrt(p11);
+            L21 = (cov(a2, a1) - L20 * L10) / L11;
+        }
+        L22 = sqrt(fmax(var(a2) - L20 * L20 - L21 * L21, 0.0));
     }
-    const double p11 = syy - L10 * L10;
-    if (p11 > tol) {
-        L11 = sqrt(p11);
-        L21 = (syz - L20 * L10) / L11;
-    }
-    L22 = sqrt(fmax(szz - L20 * L20 - L21 * L21, 0.0));
-    float4* o = mom + (size_t)kMom * k;
-    const double Ib = fmax(S[10], fmax(S[11], S[12]));
-    o[0] = make_float4((float)cx, (float)cy, (float)cz, (float)Ib);
     const double vs[4][3] = {{1, 1, 1}, {1, -1, -1}, {-1, 1, -1}, {-1, -1, 1}};
+#pragma unroll
     for (int q = 0; q < 4; ++q) {
-        const double ox = L00 * vs[q][0];
-        const double oy = L10 * vs[q][0] + L11 * vs[q][1];
-        const double oz = L20 * vs[q][0] + L21 * vs[q][1] + L22 * vs[q][2];
+        // offsets along the pivot order (a0, a1, a2), put back on x, y, z
+        const double u0 = L00 * vs[q][0];
+        const double u1 = L10 * vs[q][0] + L11 * vs[q][1];
+        const double u2 = L20 * vs[q][0] + L21 * vs[q][1] + L22 * vs[q][2];
+        const double ox = a0 == 0 ? u0 : a1 == 0 ? u1 : u2;
+        const double oy = a0 == 1 ? u0 : a1 == 1 ? u1 : u2;
+        const double oz = a0 == 2 ? u0 : a1 == 2 ? u1 : u2;
         const float wq = q == 0 ? (float)s : q == 1 ? __int_as_float(first) : q == 2 ? __int_as_float(last - first + 1)
                                                                                        : 0.f;
         o[1 + q] = make_float4((float)(cx + ox), (float)(cy + oy), (float)(cz + oz), wq);
     }
-    // A_c(d^) = 3/32 I_c + (-1/2 M_c) . d^ + d^T (15/32 T_c) d^, the
-    // off-diagonal T terms doubled so that the form is a plain dot product
-    // with (dx^2, dy^2, dz^2, dx dy, dx dz, dy dz)
-    float T[18];
+    // per channel: A_c(d^) = 3/32 I_c + (-1/2 M_c) . d^ + d^T (15/32 T_c) d^,
+    // the off-diagonal T terms doubled so that the form is a plain dot
+    // product with (dx^2, dy^2, dz^2, dx dy, dx dz, dy dz)
+    float* tf = reinterpret_cast<float*>(o + 8);
+    double Ib = 0.0;
+#pragma unroll 1
     for (int c = 0; c < 3; ++c) {
-        o[5 + c] = make_float4((float)(-0.5 * S[13 + 3 * c]), (float)(-0.5 * S[14 + 3 * c]),
-                               (float)(-0.5 * S[15 + 3 * c]), (float)(0.09375 * S[10 + c]));
+        range_group(pre, tot, first, last, kGroup * (1 + c), S);
+        Ib = fmax(Ib, S[0]);
+        o[5 + c] = make_float4((float)(-0.5 * S[1]), (float)(-0.5 * S[2]), (float)(-0.5 * S[3]),
+                               (float)(0.09375 * S[0]));
         const double f = 0.46875;
-        T[6 * c + 0] = (float)(f * S[22 + 6 * c]);
-        T[6 * c + 1] = (float)(f * S[23 + 6 * c]);
-        T[6 * c + 2] = (float)(f * S[24 + 6 * c]);
-        T[6 * c + 3] = (float)(2.0 * f * S[25 + 6 * c]);
-        T[6 * c + 4] = (float)(2.0 * f * S[26 + 6 * c]);
-        T[6 * c + 5] = (float)(2.0 * f * S[27 + 6 * c]);
+        tf[6 * c + 0] = (float)(f * S[4]);
+        tf[6 * c + 1] = (float)(f * S[5]);
+        tf[6 * c + 2] = (float)(f * S[6]);
+        tf[6 * c + 3] = (float)(2.0 * f * S[7]);
+        tf[6 * c + 4] = (float)(2.0 * f * S[8]);
+        tf[6 * c + 5] = (float)(2.0 * f * S[9]);
     }
-    o[8] = make_float4(T[0], T[1], T[2], T[3]);
-    o[9] = make_float4(T[4], T[5], T[6], T[7]);
-    o[10] = make_float4(T[8], T[9], T[10], T[11]);
-    o[11] = make_float4(T[12], T[13], T[14], T[15]);
-    o[12] = make_float4(T[16], T[17], 0.f, 0.f);
+    tf[18] = 0.f;
+    tf[19] = 0.f;
+    o[0] = make_float4((float)cx, (float)cy, (float)cz, (float)Ib);
 }
 
 struct CutMom {
@@ -258,6 +314,32 @@ __device__ inline void add_moments(const PixelF& P, const float4* __restrict__ r
     acc.s1 += fmaxf(ag, 0.f) * K;
     acc.s2 += fmaxf(ab, 0.f) * K;
     ++acc.run;
+}
+
+// The AVPLs of sorted leaves [first, first + count) exactly, four leaves'
+// records loaded before their pairs are evaluated (one wait per four scalar
+// loads instead of one per leaf).
+__device__ inline void add_leaves(const float4* __restrict__ crec, int first, int count, const PixelF& P, Acc& acc) {
+    int i = first;
+    const int end = first + count;
+    for (; i + 4 <= end; i += 4) {
+        float4 a0[4], a1[4], a2[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const float4* l = crec + kCrec * (size_t)(i + k);
+            a0[k] = l[0];
+            a1[k] = l[1];
+            a2[k] = l[2];
+        }
+#pragma unroll
+        for (int k = 0; k < 4; ++k) acc.add_run(a2[k], geom_est(P, a0[k], a1[k]));
+        acc.maybe_flush();
+    }
+    for (; i < end; ++i) {
+        const float4* l = crec + kCrec * (size_t)i;
+        acc.add_run(l[2], geom_est(P, l[0], l[1]));
+    }
+    acc.maybe_flush();
 }
 
 // One depth-first cut per run of 64 pixels compact in 3D (one wave), refined
@@ -304,12 +386,7 @@ __global__ __launch_bounds__(kCutThreads) void k_cut_mom_tile(CutMom t, const fl
                 // behind every pixel's tangent plane: contributes exactly 0
             } else if (count <= exact_count) {
                 // small node: its AVPLs exactly (refining it would end in the same sum)
-                const int first = __float_as_int(r[2].w);
-                for (int i = first; i < first + count; ++i) {
-                    const float4* l = t.crec + kCrec * (size_t)i;
-                    acc.add_run(l[2], geom_est(P, l[0], l[1]));
-                    acc.maybe_flush();
-                }
+                add_leaves(t.crec, __float_as_int(r[2].w), count, P, acc);
                 nexact += count;
             } else {
                 bool refine;
@@ -374,7 +451,8 @@ int cut_moments_build(arc_gather* g, arc_bvh* b) {
     ARC_TRY(ensure(ctx, g->mom, sizeof(float4) * kMom * (size_t)(n - 1)));
     ARC_TRY(ensure(ctx, g->mom_pre, sizeof(double) * kMomTerms * (size_t)n));
     ARC_TRY(ensure(ctx, g->mom_tot, sizeof(double) * kMomTerms * (size_t)nblocks));
-    hipLaunchKernelGGL(k_mom_scan, dim3(nblocks), dim3(kScanBlock), 0, ctx->stream, g->crec.as<float4>(), n,
+    hipLaunchKernelGGL(k_mom_scan, dim3(nblocks, kGroups), dim3(kScanBlock), 0, ctx->stream, g->crec.as<float4>(),
+                       b->nodes.as<arc_bvh_node>(), n,
                        g->mom_pre.as<double>(), g->mom_tot.as<double>());
     hipLaunchKernelGGL(k_mom_offsets, dim3(kMomTerms), dim3(kScanBlock), 0, ctx->stream, g->mom_tot.as<double>(),
                        nblocks);

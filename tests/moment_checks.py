@@ -88,8 +88,11 @@ def check_moment_records(mom: np.ndarray, crec: np.ndarray, ranges: np.ndarray, 
     cov = np.einsum("kqi,kqj->kij", off, off) / 4
     tr = np.trace(S, axis1=1, axis2=2)
     # FP64 prefix differences carry about 2^-52 of the prefix total, over W
+    # (noise); a rank-deficient covariance (2 or 3 AVPLs, a flat node) turns
+    # noise e in its small directions into about sqrt(e tr) in its factor
     pref = 64 * 2.2e-16 * np.abs(Sxx).max() / W[live]
-    tol = 1e-4 * tr + 4e-7 * np.sqrt(np.maximum(tr, 0)) * cmax + pref + 1e-12
+    tr0 = np.maximum(tr, 0)
+    tol = 2e-3 * tr0 + 4e-7 * np.sqrt(tr0) * cmax + pref + 16 * np.sqrt(pref * tr0) + 1e-12
     assert np.all(np.abs(cov - S) <= tol[:, None, None] + 1e-6 * np.abs(S)), \
         float(np.max(np.abs(cov - S) - tol[:, None, None]))
     if nodes is not None:  # the half-diagonal of the node's box

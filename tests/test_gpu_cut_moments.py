@@ -67,7 +67,8 @@ def test_node_records_are_range_sums(gpu_ctx, num_avpls):
     ranges = np.asarray(tree["ranges"]).reshape(-1, 2).astype(np.int64)
     # the leaves' records hold the sorted AVPLs
     perm = tree["perm"]
-    np.testing.assert_array_equal(crec[:n, 0, :3], sc.avpl_pos[perm])
+    # a leaf's centroid is (w x) / w in FP32: its position within an ulp or two
+    np.testing.assert_allclose(crec[:n, 0, :3], sc.avpl_pos[perm], rtol=3e-7, atol=1e-7)
     diff = sc.materials[sc.avpl_mat[perm], 3:6].astype(np.float32)
     want_I = sc.avpl_rad[perm] * (np.float32(0.318309886) * diff)
     np.testing.assert_allclose(crec[:n, 2, :3], want_I, rtol=1e-6)

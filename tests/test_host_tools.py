@@ -33,9 +33,11 @@ def test_bench_reads_the_traffic_profile():
 
     if not bench.TRAFFIC_PROFILE.exists():
         pytest.skip("no traffic profile committed")
-    depth = bench.parse([]).eval_depth  # the benchmark's default setting
-    anti = bench.profiled_traffic("antiradiance", True, True, depth)
-    cut = bench.profiled_traffic("radiance", True, True, depth)
+    # the setting the committed traffic profile covers: the centroid
+    # estimator with sums one level down and disk antiradiance
+    depth = 1
+    anti = bench.profiled_traffic("antiradiance", True, True, depth, False, False)
+    cut = bench.profiled_traffic("radiance", True, True, depth, False, False)
     bvh = bench.profiled_traffic("bvh", True, True, depth)
     assert anti and cut and bvh
     assert bvh > cut  # the LBVH rebuild moves more bytes than the cut
