@@ -36,7 +36,6 @@ HIP_SOURCES = {
     "gather_anti.hip": [],
     "gather_cut.hip": [],
     "cut_moments.hip": [],
-    "pixel_order.hip": [],
     "output.hip": [],
     "trace.hip": ["-ffp-contract=off"],
     "batch.hip": [],

@@ -342,8 +342,8 @@ __device__ inline void add_leaves(const float4* __restrict__ crec, int first, in
     acc.maybe_flush();
 }
 
-// One depth-first cut per run of 64 pixels compact in 3D (one wave), refined
-// when any of the run's pixels asks for it (ballot), as k_cut_eb_tile; kept nodes are summed
+// One depth-first cut per 8x8 tile (one wave), refined when any of the
+// tile's pixels asks for it (ballot), as k_cut_eb_tile; kept nodes are summed
 // with the moment estimate, or exactly when they hold at most exact_count
 // AVPLs.  criterion: ARC_CRITERION_BOUND (bound > threshold) or
 // ARC_CRITERION_EXPANSION (s > opening * d_c, or bound * (s / d_c)^2 >
@@ -354,17 +354,13 @@ __global__ __launch_bounds__(kCutThreads) void k_cut_mom_tile(CutMom t, const fl
                                                               const arc_material* __restrict__ mats, int W, int H,
                                                               Threshold th, float delta, int budget, int criterion,
                                                               float open2, int exact_count,
-                                                              const int* __restrict__ order,
                                                               float4* __restrict__ out_rad,
                                                               unsigned long long* __restrict__ counters) {
     __shared__ int stk[kCutThreads / 64][kMomStack];
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
-    // the wave's 64 pixels: a run of the pixels sorted by the Morton code of
-    // their position (compact in 3D); blocks in XCD-contiguous order
-    const int npix = W * H;
-    const int s_idx = (xcd_block(blockIdx.x, gridDim.x) * (kCutThreads / 64) + wave) * 64 + lane;
-    const bool live = s_idx < npix;
-    const int q = live ? order[s_idx] : 0;
+    const int2 xy = tile_pixel(wave, lane, (W + 15) / 16, (H + 15) / 16);
+    const bool live = xy.x < W && xy.y < H;
+    const int q = live ? xy.y * W + xy.x : 0;
     const PixelF P = load_pixel(gpos, gnrm, galb, mats, q);
     int* S = stk[wave];
     const float thr = live ? th.at(q) : INFINITY;
@@ -474,13 +470,11 @@ int cut_moments(arc_gather* g, arc_bvh* b, const CutEBParams& cp, float4* rad, u
     Threshold th{};
     ARC_TRY(prepare_cut(g, b, cp, t, th));
     const CutMom tm{b->nodes.as<arc_bvh_node>(), g->crec.as<float4>(), g->mom.as<float4>(), n};
-    const int npix = g->width * g->height;
-    if (!g->porder_ready) return fail(ctx, ARC_E_STATE, "moment estimator: no pixel grouping (set the G-buffer)");
-    const dim3 grid((npix + kCutThreads - 1) / kCutThreads);
+    const dim3 grid(((g->width + 15) / 16) * ((g->height + 15) / 16));
     if (mark) ARC_HIP(ctx, hipEventRecord(mark, ctx->stream));
     hipLaunchKernelGGL(k_cut_mom_tile, grid, dim3(kCutThreads), 0, ctx->stream, tm, g->gpos, g->gnrm, g->galb,
                        g->mats.as<arc_material>(), g->width, g->height, th, cp.delta, cp.budget, cp.criterion,
-                       cp.opening * cp.opening, cp.exact_count, g->porder.as<int>(), rad, counters);
+                       cp.opening * cp.opening, cp.exact_count, rad, counters);
     ARC_LAUNCHED(ctx);
     return ARC_OK;
 }

@@ -545,11 +545,7 @@ void arc_gather_destroy(arc_gather* g) {
     arc::release(g->mom);
     arc::release(g->mom_pre);
     arc::release(g->mom_tot);
-    arc::release(g->porder);
-    arc::release(g->pkeys);
-    arc::release(g->psort.keys_alt);
-    arc::release(g->psort.vals_alt);
-    arc::release(g->psort.counts);
+
     arc::release(g->cest);
     arc_bvh_destroy(g->bvh);
     delete g;
@@ -560,13 +556,7 @@ int arc_gather_set_gbuffer(arc_gather* g, const float* pos4, const float* nrm4, 
     g->gpos = reinterpret_cast<const float4*>(pos4);
     g->gnrm = reinterpret_cast<const float4*>(nrm4);
     g->galb = reinterpret_cast<const float4*>(albedo4);
-    return arc::pixel_order_build(g);
-}
-
-int arc_gather_rebuild_visible_points(arc_gather* g) {
-    if (!g) return ARC_E_INVALID;
-    if (!g->gpos || !g->gnrm) return arc::fail(g->ctx, ARC_E_STATE, "no G-buffer bound");
-    return arc::pixel_order_build(g);
+    return ARC_OK;
 }
 
 int arc_gather_set_outputs(arc_gather* g, float* result4, float* radiance4, float* antiradiance4) {

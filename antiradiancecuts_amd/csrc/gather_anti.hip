@@ -12,15 +12,12 @@
 //                  stable compaction of the AVPLs with non-zero antiradiance
 //                  (the rest contribute nothing), no host round trip;
 //   k_cluster_bounds
-//                  one cluster per run of 1024 pixels of the gather's pixel
-//                  order (sorted by the Morton code of their G-buffer
-//                  position, pixel_order.hip), box of their positions: the
-//                  role of the reference's depth slices, a box that is
-//                  compact in space even where the screen crosses depth
-//                  discontinuities;
+//                  one cluster per 32x32 pixel block, box of its G-buffer
+//                  positions (the reference adds depth slices; a block of
+//                  the screen with its tight box is what the list test needs);
 //   k_cluster_tree
-//                  a 4-ary tree over the cluster boxes (4 consecutive
-//                  clusters per parent, up to one root), one workgroup;
+//                  a screen-space quadtree over the cluster boxes (2x2
+//                  merges up to one root), one workgroup;
 //   k_cluster_lists_tree
 //                  lane = antiradiance AVPL, depth-first descent of the
 //                  quadtree with a conservative test per box: ray-box slab
@@ -34,8 +31,7 @@
 //                  The reference's C x N list buffer becomes C x kCap with an
 //                  overflow flag;
 //   k_anti_clustered
-//                  four 256-thread blocks per cluster, one wave per run of 64
-//                  pixels (a tile);
+//                  one 1024-thread block per cluster, one wave per 8x8 tile;
 //                  the block reads the cluster's list once and sorts its
 //                  entries into per-tile sub-lists in LDS (ray against the
 //                  tile's own box), then each wave walks its sub-list with
@@ -54,15 +50,15 @@ using namespace arc::gd;
 
 namespace {
 
-constexpr int kClusterPix = 1024;  // pixels per cluster (consecutive in the pixel order)
+constexpr int kClusterPx = 32;   // cluster side in pixels
 constexpr int kCap = 16384;      // list entries per cluster
 constexpr int kCompact = 1024;   // compaction block
-constexpr int kTiles = kClusterPix / 64;                       // runs of 64 pixels (tiles) per cluster
-constexpr int kParts = 4;                                      // blocks per cluster
+constexpr int kTiles = (kClusterPx / 8) * (kClusterPx / 8);  // 8x8 tiles per cluster
+constexpr int kParts = 4;                                      // blocks per cluster (rows of 32 x 8 px)
 constexpr int kBlockTiles = kTiles / kParts;                   // tiles per block
 constexpr int kAntiThreads = 64 * kBlockTiles;                 // one wave per tile
 constexpr int kSub = 1536;       // tile sub-list entries in LDS (more: the tile walks the cluster list)
-constexpr int kTreeMaxLevels = 12;  // cluster tree levels: up to 4^11 clusters
+constexpr int kTreeMaxLevels = 10;  // cluster quadtree levels: up to 512 x 512 clusters
 
 __device__ inline bool has_anti(const arc::AvplsDev& a, int i) {
     if (!a.anti) return false;
@@ -124,17 +120,17 @@ __global__ __launch_bounds__(kCompact) void k_compact(arc::AvplsDev a, const int
     if (f) idx[offs[blockIdx.x] + before + __popcll(m & ((1ull << lane) - 1ull))] = i;
 }
 
-// Box of the G-buffer positions of each cluster (1024 consecutive pixels of
-// the pixel order): [C][8] = min xyz, -, max xyz, -.
-__global__ __launch_bounds__(256) void k_cluster_bounds(const float4* __restrict__ gpos, const int* __restrict__ order,
-                                                        int npix, float* __restrict__ cbox) {
+// Box of the G-buffer positions of each 32x32 block: [C][8] = min xyz, -, max xyz, -.
+__global__ __launch_bounds__(256) void k_cluster_bounds(const float4* __restrict__ gpos, int W, int H, int cw,
+                                                        float* __restrict__ cbox) {
     __shared__ float s[6][256];
     const int c = blockIdx.x;
+    const int x0 = (c % cw) * kClusterPx, y0 = (c / cw) * kClusterPx;
     float v[6] = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY};
-    for (int k = threadIdx.x; k < kClusterPix; k += 256) {
-        const int si = c * kClusterPix + k;
-        if (si < npix) {
-            const float4 p = gpos[order[si]];
+    for (int k = threadIdx.x; k < kClusterPx * kClusterPx; k += 256) {
+        const int x = x0 + (k % kClusterPx), y = y0 + (k / kClusterPx);
+        if (x < W && y < H) {
+            const float4 p = gpos[y * W + x];
             v[0] = fminf(v[0], p.x); v[1] = fminf(v[1], p.y); v[2] = fminf(v[2], p.z);
             v[3] = fmaxf(v[3], p.x); v[4] = fmaxf(v[4], p.y); v[5] = fmaxf(v[5], p.z);
         }
@@ -203,27 +199,29 @@ __device__ inline bool ray_box_hit(const f3& p, const f3& w, const f3& na, float
     }
 }
 
-// 4-ary tree over the clusters: level 0 is the clusters (the k_cluster_bounds
-// output), node i of level l + 1 merges nodes 4i .. 4i + 3 of level l, up to
-// one root.  Boxes are [8] = min xyz, -, max xyz, -.
+// Quadtree over the cluster grid: level 0 is the clusters (cw x ch boxes,
+// the k_cluster_bounds output), level l + 1 merges 2x2 nodes of level l, up
+// to one root.  Boxes are [8] = min xyz, -, max xyz, -.
 struct ClusterTree {
     int levels;
     int off[kTreeMaxLevels];  // first node of each level
-    int w[kTreeMaxLevels];    // nodes of each level
+    int w[kTreeMaxLevels], h[kTreeMaxLevels];
 };
 
 __global__ __launch_bounds__(1024) void k_cluster_tree(float* __restrict__ tree, ClusterTree t) {
     for (int l = 1; l < t.levels; ++l) {
-        const int W = t.w[l], cw = t.w[l - 1];
-        for (int k = threadIdx.x; k < W; k += blockDim.x) {
+        const int W = t.w[l], H = t.h[l], cw = t.w[l - 1], ch = t.h[l - 1];
+        for (int k = threadIdx.x; k < W * H; k += blockDim.x) {
+            const int x = k % W, y = k / W;
             float b[6] = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY};
-            for (int cx = 4 * k; cx <= min(4 * k + 3, cw - 1); ++cx) {
-                const float* c = tree + 8 * (size_t)(t.off[l - 1] + cx);
-                for (int e = 0; e < 3; ++e) {
-                    b[e] = fminf(b[e], c[e]);
-                    b[3 + e] = fmaxf(b[3 + e], c[4 + e]);
+            for (int cy = 2 * y; cy <= min(2 * y + 1, ch - 1); ++cy)
+                for (int cx = 2 * x; cx <= min(2 * x + 1, cw - 1); ++cx) {
+                    const float* c = tree + 8 * (size_t)(t.off[l - 1] + cy * cw + cx);
+                    for (int e = 0; e < 3; ++e) {
+                        b[e] = fminf(b[e], c[e]);
+                        b[3 + e] = fmaxf(b[3 + e], c[4 + e]);
+                    }
                 }
-            }
             float* o = tree + 8 * (size_t)(t.off[l] + k);
             o[0] = b[0]; o[1] = b[1]; o[2] = b[2]; o[3] = 0.f;
             o[4] = b[3]; o[5] = b[4]; o[6] = b[5]; o[7] = 0.f;
@@ -233,7 +231,7 @@ __global__ __launch_bounds__(1024) void k_cluster_tree(float* __restrict__ tree,
 }
 
 // Lists by tree descent: lane = antiradiance AVPL, depth-first through the
-// cluster tree with the same conservative test at every level (a parent box
+// quadtree with the same conservative test at every level (a parent box
 // contains its children, so a rejected parent rejects every cluster below).
 // A popped node's children are loaded together and tested together, so a
 // lane waits on one round of loads per node and not one per child.  A ray
@@ -253,7 +251,7 @@ __global__ __launch_bounds__(kListThreads) void k_cluster_lists_tree(arc::AvplsD
                                                                      const float4* __restrict__ tree, ClusterTree t,
                                                                      float grow, float cos_t, float sin_t,
                                                                      int* __restrict__ counts, int* __restrict__ lists) {
-    __shared__ int s_off[kTreeMaxLevels], s_w[kTreeMaxLevels];
+    __shared__ int s_off[kTreeMaxLevels], s_w[kTreeMaxLevels], s_h[kTreeMaxLevels];
     __shared__ int s_pair[kPairBuf];
     __shared__ int s_n;
     extern __shared__ int stk[];  // [depth][kListThreads]
@@ -261,6 +259,7 @@ __global__ __launch_bounds__(kListThreads) void k_cluster_lists_tree(arc::AvplsD
     if (tid < kTreeMaxLevels) {
         s_off[tid] = t.off[tid];
         s_w[tid] = t.w[tid];
+        s_h[tid] = t.h[tid];
     }
     if (tid == 0) s_n = 0;
     __syncthreads();
@@ -288,24 +287,26 @@ __global__ __launch_bounds__(kListThreads) void k_cluster_lists_tree(arc::AvplsD
                 append(0, i);
             } else {
                 int sp = 0;
-                stk[(sp++) * kListThreads + tid] = top << 24;  // (level, index) of a node whose box was hit
+                stk[(sp++) * kListThreads + tid] = top << 24;  // (level, y, x) of a node whose box was hit
                 while (sp > 0) {
                     const int v = stk[(--sp) * kListThreads + tid];
-                    const int l = (v >> 24) - 1, x = 4 * (v & 0xffffff);
-                    const int cw = s_w[l], off = s_off[l];
+                    const int l = (v >> 24) - 1, y = 2 * ((v >> 12) & 4095), x = 2 * (v & 4095);
+                    const int cw = s_w[l], ch = s_h[l], off = s_off[l];
+                    const bool ok[4] = {true, x + 1 < cw, y + 1 < ch, x + 1 < cw && y + 1 < ch};
                     float4 lo[4], hi[4];
 #pragma unroll
                     for (int q = 0; q < 4; ++q) {  // a missing child loads the first one's box
-                        const int k = off + (x + q < cw ? x + q : x);
+                        const int k = off + (ok[q] ? (y + (q >> 1)) * cw + x + (q & 1) : y * cw + x);
                         lo[q] = tree[2 * k];
                         hi[q] = tree[2 * k + 1];
                     }
 #pragma unroll
-                    for (int q = 3; q >= 0; --q) {  // pushed in reverse: popped in order
+                    for (int q = 3; q >= 0; --q) {  // pushed in reverse: popped in row order
                         const float b[6] = {lo[q].x, lo[q].y, lo[q].z, hi[q].x, hi[q].y, hi[q].z};
-                        if (x + q >= cw || !ray_box_hit<MODEL>(p, w, na, ix, iy, iz, b, grow, cos_t, sin_t)) continue;
-                        if (l == 0) append(x + q, i);
-                        else stk[(sp++) * kListThreads + tid] = (l << 24) | (x + q);
+                        if (!ok[q] || !ray_box_hit<MODEL>(p, w, na, ix, iy, iz, b, grow, cos_t, sin_t)) continue;
+                        const int cy = y + (q >> 1), cx = x + (q & 1);
+                        if (l == 0) append(cy * cw + cx, i);
+                        else stk[(sp++) * kListThreads + tid] = (l << 24) | (cy << 12) | cx;
                     }
                 }
             }
@@ -333,7 +334,7 @@ template <int MODEL, bool GENERAL>
 __global__ __launch_bounds__(kAntiThreads) __attribute__((amdgpu_waves_per_eu(5))) void k_anti_clustered(
     const float4* __restrict__ rec, arc::AvplsDev raw, const float4* __restrict__ gpos,
     const float4* __restrict__ gnrm, const float4* __restrict__ galb, const arc_material* __restrict__ mats, int W,
-    int H, const int* __restrict__ order, float camx, float camy, float camz, AntiParams ap, const int* __restrict__ counts,
+    int H, int cw, float camx, float camy, float camz, AntiParams ap, const int* __restrict__ counts,
     const int* __restrict__ lists, const int* __restrict__ idx, const int* __restrict__ n_anti_ptr, float grow,
     float cos_t, float sin_t, float* __restrict__ part, unsigned long long* __restrict__ counters) {
     __shared__ float tbox[kBlockTiles][6];
@@ -342,11 +343,12 @@ __global__ __launch_bounds__(kAntiThreads) __attribute__((amdgpu_waves_per_eu(5)
     __shared__ int sub[kBlockTiles][kSub];
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const int b = xcd_block(blockIdx.x, gridDim.x);
-    const int c = b / kParts, quarter = b % kParts;
+    const int c = b / kParts, half = b % kParts;
+    const int x = (c % cw) * kClusterPx + (wave & 3) * 8 + (lane & 7);
+    const int y = (c / cw) * kClusterPx + half * (kClusterPx / kParts) + (wave >> 2) * 8 + (lane >> 3);
+    const bool live = x < W && y < H;
     const int npix = W * H;
-    const int si = c * kClusterPix + (quarter * kBlockTiles + wave) * 64 + lane;  // place in the pixel order
-    const bool live = si < npix;
-    const int q = live ? order[si] : 0;
+    const int q = live ? y * W + x : 0;
     const PixelF P = load_pixel(gpos, gnrm, galb, mats, q);
     {  // the tile's box
         const float big = INFINITY;
@@ -453,12 +455,12 @@ __global__ __launch_bounds__(kAntiThreads) __attribute__((amdgpu_waves_per_eu(5)
 }
 
 template <int MODEL, bool GENERAL>
-void launch_pixels(arc_gather* g, const arc::AvplsDev& a, int C, float3 cam, const AntiParams& ap,
+void launch_pixels(arc_gather* g, const arc::AvplsDev& a, int cw, int C, float3 cam, const AntiParams& ap,
                    const int* n_anti_ptr, float grow, float cos_t, float sin_t, float* part,
                    unsigned long long* counters) {
     hipLaunchKernelGGL((k_anti_clustered<MODEL, GENERAL>), dim3(C * kParts), dim3(kAntiThreads), 0, g->ctx->stream,
                        g->rec.as<float4>(), a, g->gpos, g->gnrm, g->galb, g->mats.as<arc_material>(), g->width,
-                       g->height, g->porder.as<int>(), cam.x, cam.y, cam.z, ap, g->ccount.as<int>(), g->clist.as<int>(),
+                       g->height, cw, cam.x, cam.y, cam.z, ap, g->ccount.as<int>(), g->clist.as<int>(),
                        g->anti_idx.as<int>(), n_anti_ptr, grow, cos_t, sin_t, part, counters);
 }
 
@@ -471,21 +473,22 @@ int anti_clustered(arc_gather* g, const AvplsDev& a, int model, float3 cam, cons
     arc_ctx* ctx = g->ctx;
     const int n = a.n;
     const int nb = (n + kCompact - 1) / kCompact;
-    const int npix = g->width * g->height;
-    if (!g->porder_ready) return fail(ctx, ARC_E_STATE, "clustered antiradiance: no pixel grouping (set the G-buffer)");
-    const int C = (npix + kClusterPix - 1) / kClusterPix;
+    const int cw = (g->width + kClusterPx - 1) / kClusterPx, ch = (g->height + kClusterPx - 1) / kClusterPx;
+    const int C = cw * ch;
     // anti_idx: [n] compacted indices, then [nb + 1] block offsets (last = count)
     ARC_TRY(ensure(ctx, g->anti_idx, sizeof(int) * ((size_t)n + nb + 1)));
     ClusterTree tree{};
-    tree.w[0] = C;
+    tree.w[0] = cw;
+    tree.h[0] = ch;
     int nodes = C;
     tree.levels = 1;
-    while (tree.w[tree.levels - 1] > 1) {
+    while (tree.w[tree.levels - 1] > 1 || tree.h[tree.levels - 1] > 1) {
         if (tree.levels == kTreeMaxLevels) return fail(ctx, ARC_E_UNSUPPORTED, "image too large for the cluster tree");
         const int l = tree.levels++;
         tree.off[l] = nodes;
-        tree.w[l] = (tree.w[l - 1] + 3) / 4;
-        nodes += tree.w[l];
+        tree.w[l] = (tree.w[l - 1] + 1) / 2;
+        tree.h[l] = (tree.h[l - 1] + 1) / 2;
+        nodes += tree.w[l] * tree.h[l];
     }
     ARC_TRY(ensure(ctx, g->cbox, sizeof(float) * 8 * (size_t)nodes));
     ARC_TRY(ensure(ctx, g->ccount, sizeof(int) * (size_t)C));
@@ -495,7 +498,7 @@ int anti_clustered(arc_gather* g, const AvplsDev& a, int model, float3 cam, cons
     hipLaunchKernelGGL(k_flag_count, dim3(nb), dim3(kCompact), 0, ctx->stream, a, offs);
     hipLaunchKernelGGL(k_scan_blocks, dim3(1), dim3(1024), 0, ctx->stream, offs, nb);
     hipLaunchKernelGGL(k_compact, dim3(nb), dim3(kCompact), 0, ctx->stream, a, offs, idx);
-    hipLaunchKernelGGL(k_cluster_bounds, dim3(C), dim3(256), 0, ctx->stream, g->gpos, g->porder.as<int>(), npix,
+    hipLaunchKernelGGL(k_cluster_bounds, dim3(C), dim3(256), 0, ctx->stream, g->gpos, g->width, g->height, cw,
                        g->cbox.as<float>());
     hipLaunchKernelGGL(k_zero, dim3((C + 255) / 256), dim3(256), 0, ctx->stream, g->ccount.as<int>(), C);
     ARC_LAUNCHED(ctx);
@@ -519,11 +522,11 @@ int anti_clustered(arc_gather* g, const AvplsDev& a, int model, float3 cam, cons
     ARC_LAUNCHED(ctx);
     if (mark) ARC_HIP(ctx, hipEventRecord(mark, ctx->stream));
     if (model == ARC_ANTIRAD_DISK) {
-        if (g->general) launch_pixels<ARC_ANTIRAD_DISK, true>(g, a, C, cam, ap, n_anti, grow, cos_t, sin_t, part, counters);
-        else launch_pixels<ARC_ANTIRAD_DISK, false>(g, a, C, cam, ap, n_anti, grow, cos_t, sin_t, part, counters);
+        if (g->general) launch_pixels<ARC_ANTIRAD_DISK, true>(g, a, cw, C, cam, ap, n_anti, grow, cos_t, sin_t, part, counters);
+        else launch_pixels<ARC_ANTIRAD_DISK, false>(g, a, cw, C, cam, ap, n_anti, grow, cos_t, sin_t, part, counters);
     } else {
-        if (g->general) launch_pixels<ARC_ANTIRAD_CONE, true>(g, a, C, cam, ap, n_anti, grow, cos_t, sin_t, part, counters);
-        else launch_pixels<ARC_ANTIRAD_CONE, false>(g, a, C, cam, ap, n_anti, grow, cos_t, sin_t, part, counters);
+        if (g->general) launch_pixels<ARC_ANTIRAD_CONE, true>(g, a, cw, C, cam, ap, n_anti, grow, cos_t, sin_t, part, counters);
+        else launch_pixels<ARC_ANTIRAD_CONE, false>(g, a, cw, C, cam, ap, n_anti, grow, cos_t, sin_t, part, counters);
     }
     ARC_LAUNCHED(ctx);
     return ARC_OK;

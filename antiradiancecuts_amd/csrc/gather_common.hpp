@@ -34,10 +34,6 @@ struct arc_gather {
     // over the sorted leaves (with per-block offsets) they are made from
     arc::DevBuf mom, mom_pre, mom_tot;
     bool mom_ready = false;  // mom holds the records of the current cut tree
-    // pixels sorted by the Morton code of their G-buffer position (pixel_order.hip)
-    arc::DevBuf porder, pkeys;
-    arc::SortScratch psort;
-    bool porder_ready = false;
     arc_cutnet* net = nullptr;  // learned cut: the predictor (borrowed)
     arc_bvh* bvh = nullptr;
     // per-frame phase events: frame slot, then 9 marks (start, upload done,
@@ -248,10 +244,6 @@ struct CutEBParams {
 // `mark` (may be null) is recorded right before the main traversal kernel.
 int cut_error_bound(arc_gather* g, arc_bvh* b, const CutEBParams& cp, float4* rad, unsigned long long* counters,
                     hipEvent_t mark);
-// The pixels of the bound G-buffer sorted by the Morton code of their
-// position, into g->porder (pixel_order.hip).
-int pixel_order_build(arc_gather* g);
-
 // The moment estimator's node records for this frame's cut tree (after
 // bvh_build_cut): FP64 prefix sums over the sorted leaves, then one record
 // per internal node.

@@ -158,7 +158,6 @@ ARC_SYMBOLS = [
     ("arc_gather_create", C.c_int, [vp, C.c_int, C.c_int, P(arc_material), C.c_int, P(vp)]),
     ("arc_gather_destroy", None, [vp]),
     ("arc_gather_set_gbuffer", C.c_int, [vp, vp, vp, vp]),
-    ("arc_gather_rebuild_visible_points", C.c_int, [vp]),
     ("arc_gather_set_outputs", C.c_int, [vp, vp, vp, vp]),
     ("arc_gather_run", C.c_int, [vp, P(arc_avpls), arc_float3, C.c_float, P(arc_gather_params), C.c_int,
                                  P(arc_gather_stats)]),

@@ -276,13 +276,6 @@ void arc_gather_destroy(arc_gather* g);
  * albedo (float4, may be NULL) overrides the pixel material's diffuse colour. */
 int arc_gather_set_gbuffer(arc_gather* g, const float* pos4, const float* nrm4, const float* albedo4);
 
-/* CudaGather::rebuildVisiblePointsBvh (cudaGather.h:46): regroups the pixels
- * of the bound G-buffer (sorted by the Morton code of their position; runs of
- * 64 share a light cut, runs of 1024 an antiradiance list).  Done by
- * arc_gather_set_gbuffer; call it again after changing the G-buffer in place.
- * A stale grouping costs speed, not correctness. */
-int arc_gather_rebuild_visible_points(arc_gather* g);
-
 /* The output textures of the constructor, float4 each, any may be NULL:
  * result = radiance - antiradiance (kernel_combine, cudaGather.cu:782-803). */
 int arc_gather_set_outputs(arc_gather* g, float* result4, float* radiance4, float* antiradiance4);

@@ -282,10 +282,6 @@ public:
     }
     ~CudaGather() { arc_gather_destroy(g_); }
     CudaGather(const CudaGather&) = delete;
-
-    /* rebuildVisiblePointsBvh() (cudaGather.h:46): regroup the pixels after
-     * the G-buffer changed in place (the constructor groups them once). */
-    void rebuildVisiblePointsBvh() { check(arc_gather_rebuild_visible_points(g_), ctx_); }
     CudaGather& operator=(const CudaGather&) = delete;
 
     /* run(avpls, cameraPosition, sceneProbe, sceneExtent, profile)

@@ -18,8 +18,9 @@ def leaf_inputs(crec: np.ndarray, n: int):
 
 
 def range_sums(values: np.ndarray, ranges: np.ndarray) -> np.ndarray:
-    """sum over each node's leaf range of the per-leaf rows of `values`."""
-    pre = np.concatenate([np.zeros((1,) + values.shape[1:]), np.cumsum(values, axis=0)], axis=0)
+    """sum over each node's leaf range of the per-leaf rows of `values`
+    (prefix sums in the values' precision)."""
+    pre = np.concatenate([np.zeros((1,) + values.shape[1:], values.dtype), np.cumsum(values, axis=0)], axis=0)
     return pre[ranges[:, 1] + 1] - pre[ranges[:, 0]]
 
 
@@ -43,12 +44,17 @@ def check_centroid_records(crec: np.ndarray, ranges: np.ndarray, n: int):
     return int(live.sum())
 
 
-def moment_sums(crec: np.ndarray, ranges: np.ndarray, n: int):
+def moment_sums(crec: np.ndarray, ranges: np.ndarray, n: int, ref=None):
+    """Range sums of w, w x, w x x^T (x relative to `ref`, in extended
+    precision so that small nodes' covariances keep their digits), I, I n and
+    I n n^T per channel."""
     x, nm, I = leaf_inputs(crec, n)
     w = I.mean(1, keepdims=True)
     W = range_sums(w, ranges)[:, 0]
-    Sx = range_sums(w * x, ranges)
-    Sxx = range_sums((w[:, :, None] * x[:, :, None] * x[:, None, :]).reshape(n, 9), ranges).reshape(-1, 3, 3)
+    xl = (x - (0 if ref is None else np.asarray(ref))).astype(np.longdouble)
+    wl = w.astype(np.longdouble)
+    Sx = range_sums(wl * xl, ranges)
+    Sxx = range_sums((wl[:, :, None] * xl[:, :, None] * xl[:, None, :]).reshape(n, 9), ranges).reshape(-1, 3, 3)
     Is = range_sums(I, ranges)
     M = np.stack([range_sums(I[:, c:c + 1] * nm, ranges) for c in range(3)], 1)          # (k, c, 3)
     nn = (nm[:, :, None] * nm[:, None, :]).reshape(n, 9)
@@ -56,9 +62,12 @@ def moment_sums(crec: np.ndarray, ranges: np.ndarray, n: int):
     return W, Sx, Sxx, Is, M, T
 
 
-def check_moment_records(mom: np.ndarray, crec: np.ndarray, ranges: np.ndarray, n: int, nodes=None):
-    """The moment estimator's records (13 float4 per internal node)."""
-    W, Sx, Sxx, Is, M, T = moment_sums(crec, ranges, n)
+def check_moment_records(mom: np.ndarray, crec: np.ndarray, ranges: np.ndarray, n: int, nodes):
+    """The moment estimator's records (13 float4 per internal node).  The
+    positions are taken relative to the centre of the root's box (nodes[0]),
+    as the library takes them."""
+    ref = 0.5 * (nodes["bbMin"][0].astype(np.float64) + nodes["bbMax"][0].astype(np.float64))
+    W, Sx, Sxx, Is, M, T = moment_sums(crec, ranges, n, ref)
     k = len(ranges)
     assert mom.shape == (k, 13, 4)
     first = mom[:, 2, 3].copy().view(np.int32)
@@ -78,8 +87,10 @@ def check_moment_records(mom: np.ndarray, crec: np.ndarray, ranges: np.ndarray, 
         np.testing.assert_allclose(tf[:, c], want, rtol=1e-6, atol=1e-7 * scale)
     # position mean and covariance of the four points
     live = W > 1e-6 * W.max()
-    c = Sx[live] / W[live, None]
-    S = Sxx[live] / W[live, None, None] - c[:, :, None] * c[:, None, :]
+    Wl = W[live].astype(np.longdouble)
+    cl = Sx[live] / Wl[:, None]
+    S = (Sxx[live] / Wl[:, None, None] - cl[:, :, None] * cl[:, None, :]).astype(np.float64)
+    c = (cl + np.asarray(ref, np.longdouble)).astype(np.float64)
     pts = mom[live, 1:5, :3].astype(np.float64)
     np.testing.assert_allclose(mom[live, 0, :3], c, rtol=0, atol=1e-6 * (1 + np.abs(c).max()))
     off = pts - c[:, None, :]
@@ -87,15 +98,16 @@ def check_moment_records(mom: np.ndarray, crec: np.ndarray, ranges: np.ndarray, 
     np.testing.assert_allclose(off.mean(1), 0, atol=1e-6 * cmax)
     cov = np.einsum("kqi,kqj->kij", off, off) / 4
     tr = np.trace(S, axis1=1, axis2=2)
-    # FP64 prefix differences carry about 2^-52 of the prefix total, over W
-    # (noise); a rank-deficient covariance (2 or 3 AVPLs, a flat node) turns
-    # noise e in its small directions into about sqrt(e tr) in its factor
-    pref = 64 * 2.2e-16 * np.abs(Sxx).max() / W[live]
+    # the library's FP64 prefix differences carry about 2^-52 of the prefix
+    # total, over W (noise); a rank-deficient covariance (2 or 3 AVPLs, a
+    # flat node) turns noise e in its small directions into about sqrt(e tr)
+    # in its factor
+    pref = 64 * 2.2e-16 * float(np.abs(Sxx).max()) / W[live]
     tr0 = np.maximum(tr, 0)
     tol = 2e-3 * tr0 + 4e-7 * np.sqrt(tr0) * cmax + pref + 16 * np.sqrt(pref * tr0) + 1e-12
     assert np.all(np.abs(cov - S) <= tol[:, None, None] + 1e-6 * np.abs(S)), \
         float(np.max(np.abs(cov - S) - tol[:, None, None]))
-    if nodes is not None:  # the half-diagonal of the node's box
-        e = nodes["bbMax"].astype(np.float64) - nodes["bbMin"].astype(np.float64)
-        np.testing.assert_allclose(mom[:, 1, 3], 0.5 * np.sqrt((e * e).sum(1)), rtol=1e-6, atol=1e-12)
+    # the half-diagonal of the node's box
+    e = nodes["bbMax"].astype(np.float64) - nodes["bbMin"].astype(np.float64)
+    np.testing.assert_allclose(mom[:, 1, 3], 0.5 * np.sqrt((e * e).sum(1)), rtol=1e-6, atol=1e-12)
     return int(live.sum())

@@ -76,7 +76,7 @@ def test_node_records_are_range_sums(gpu_ctx, num_avpls):
     np.testing.assert_allclose(crec[:n, 1, :3], nrm / np.linalg.norm(nrm, axis=1, keepdims=True), atol=1e-6)
     live_c = M.check_centroid_records(crec, ranges, n)
     live_m = M.check_moment_records(mom, crec, ranges, n, tree["nodes"])
-    assert live_c > 0.99 * (n - 1) and live_m > 0.99 * (n - 1)
+    assert live_c > 0.95 * (n - 1) and live_m > 0.95 * (n - 1)  # nodes with sum(w) > 1e-6 of the root
 
 
 @pytest.fixture(scope="module")
