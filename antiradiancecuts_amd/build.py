@@ -36,6 +36,7 @@ HIP_SOURCES = {
     "gather_anti.hip": [],
     "gather_cut.hip": [],
     "cut_moments.hip": [],
+    "noalloc.hip": [],
     "output.hip": [],
     "trace.hip": ["-ffp-contract=off"],
     "batch.hip": [],

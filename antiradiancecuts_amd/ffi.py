@@ -222,13 +222,27 @@ def _bind(lib, symbols):
     return lib
 
 
+# include/arc_noalloc.h: the allocation-free SimpleBvh build
+ARC_NOALLOC_MAX_POINTS = 1 << 20
+
+
+class arc_lbvh_outputs(C.Structure):
+    _fields_ = [("codes", vp), ("perm", vp), ("nodes", vp), ("parents", vp), ("ranges", vp), ("positions", vp),
+                ("normals", vp)]
+
+
+NOALLOC_SYMBOLS = [
+    ("arc_bvh_build_points_noalloc", C.c_int, [vp, vp, vp, C.c_int, C.c_int, C.POINTER(arc_lbvh_outputs)]),
+]
+
+
 def lib() -> C.CDLL:
     """The HIP library.  Raises when it has not been built."""
     global _lib
     if _lib is None:
         if not LIB_PATH.exists():
             raise RuntimeError(f"{LIB_PATH} is missing: build it with `python -m antiradiancecuts_amd.build`")
-        _lib = _bind(C.CDLL(str(LIB_PATH)), ARC_SYMBOLS)
+        _lib = _bind(_bind(C.CDLL(str(LIB_PATH)), ARC_SYMBOLS), NOALLOC_SYMBOLS)
     return _lib
 
 

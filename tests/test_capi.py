@@ -32,6 +32,15 @@ def test_arc_h_symbols_exported():
     assert sorted(n for n, _, _ in ffi.ARC_SYMBOLS) == names
 
 
+def test_arc_noalloc_h_symbols_exported():
+    names = declared("arc_noalloc.h")
+    assert names == ["arc_bvh_build_points_noalloc"]
+    lib = ctypes.CDLL(str(ffi.LIB_PATH))
+    assert hasattr(lib, names[0])
+    assert sorted(n for n, _, _ in ffi.NOALLOC_SYMBOLS) == names
+    assert ctypes.sizeof(ffi.arc_lbvh_outputs) == 7 * ctypes.sizeof(ctypes.c_void_p)
+
+
 def test_arc_scene_h_symbols_exported():
     names = declared("arc_scene.h")
     lib = ctypes.CDLL(str(ffi.SCENE_LIB_PATH))
