@@ -41,7 +41,11 @@ namespace {
 
 constexpr int kMomTerms = 40;   // FP64 sums per leaf (see leaf_terms)
 constexpr int kScanBlock = 256;  // leaves per block of the prefix scan
-constexpr int kMom = 13;         // float4 per internal node record
+// float4 per internal node record:
+//   0 (c, max_c I_c)             1 (s, first leaf, leaf count, -)
+//   2-4 the four points, 12 floats   5-7 (-1/2 M_c, 3/32 I_c) per channel
+//   8-12 15/32 T_c per channel (xx, yy, zz, 2xy, 2xz, 2yz), 18 floats
+constexpr int kMom = 13;
 constexpr int kMomStack = 96;    // per-wave stack of the tile traversal
 
 // Terms of leaf i of the sorted order (crec leaf record: r0 = position,
@@ -225,7 +229,9 @@ __global__ __launch_bounds__(kScanBlock) void k_mom_nodes(const int* __restrict_
         const int m = i + j;  // 1: xy, 2: xz, 3: yz
         return m == 1 ? sxy : m == 2 ? sxz : syz;
     };
-    const double tol = 1e-12 * (sxx + syy + szz) + 1e-300;
+    // (a variance below 1e-6 of the trace is prefix-difference noise: with
+    // 1e6 leaves a prefix total carries about 1e-16 x 4e4 / W of it)
+    const double tol = 1e-6 * (sxx + syy + szz) + 1e-300;
     double L00 = 0, L10 = 0, L20 = 0, L11 = 0, L21 = 0, L22 = 0;
     if (var(a0) > tol) {
         L00 = sqrt(var(a0));
@@ -239,6 +245,7 @@ __global__ __launch_bounds__(kScanBlock) void k_mom_nodes(const int* __restrict_
         L22 = sqrt(fmax(var(a2) - L20 * L20 - L21 * L21, 0.0));
     }
     const double vs[4][3] = {{1, 1, 1}, {1, -1, -1}, {-1, 1, -1}, {-1, -1, 1}};
+    float pts[12];
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
         // offsets along the pivot order (a0, a1, a2), put back on x, y, z
@@ -248,10 +255,14 @@ __global__ __launch_bounds__(kScanBlock) void k_mom_nodes(const int* __restrict_
         const double ox = a0 == 0 ? u0 : a1 == 0 ? u1 : u2;
         const double oy = a0 == 1 ? u0 : a1 == 1 ? u1 : u2;
         const double oz = a0 == 2 ? u0 : a1 == 2 ? u1 : u2;
-        const float wq = q == 0 ? (float)s : q == 1 ? __int_as_float(first) : q == 2 ? __int_as_float(last - first + 1)
-                                                                                       : 0.f;
-        o[1 + q] = make_float4((float)(cx + ox), (float)(cy + oy), (float)(cz + oz), wq);
+        pts[3 * q] = (float)(cx + ox);
+        pts[3 * q + 1] = (float)(cy + oy);
+        pts[3 * q + 2] = (float)(cz + oz);
     }
+    o[1] = make_float4((float)s, __int_as_float(first), __int_as_float(last - first + 1), 0.f);
+    o[2] = make_float4(pts[0], pts[1], pts[2], pts[3]);
+    o[3] = make_float4(pts[4], pts[5], pts[6], pts[7]);
+    o[4] = make_float4(pts[8], pts[9], pts[10], pts[11]);
     // per channel: A_c(d^) = 3/32 I_c + (-1/2 M_c) . d^ + d^T (15/32 T_c) d^,
     // the off-diagonal T terms doubled so that the form is a plain dot
     // product with (dx^2, dy^2, dz^2, dx dy, dx dz, dy dz)
@@ -295,8 +306,11 @@ __device__ inline float pixel_g(const PixelF& P, const float4& x) {
 
 // The node's estimate for the pixel, per channel, added to the pixel's sums.
 __device__ inline void add_moments(const PixelF& P, const float4* __restrict__ r, Acc& acc) {
-    const float4 m0 = r[0];
-    const float K = 0.25f * (pixel_g(P, r[1]) + pixel_g(P, r[2]) + pixel_g(P, r[3]) + pixel_g(P, r[4]));
+    const float4 m0 = r[0], p01 = r[2], p12 = r[3], p23 = r[4];
+    const float K = 0.25f * (pixel_g(P, make_float4(p01.x, p01.y, p01.z, 0.f)) +
+                             pixel_g(P, make_float4(p01.w, p12.x, p12.y, 0.f)) +
+                             pixel_g(P, make_float4(p12.z, p12.w, p23.x, 0.f)) +
+                             pixel_g(P, make_float4(p23.y, p23.z, p23.w, 0.f)));
     const float dx = m0.x - P.px, dy = m0.y - P.py, dz = m0.z - P.pz;
     const float dd = dx * dx + dy * dy + dz * dz;
     const float inv = dd > 1e-30f ? __builtin_amdgcn_rsqf(dd) : 0.f;
@@ -348,6 +362,13 @@ __device__ inline void add_leaves(const float4* __restrict__ crec, int first, in
 // AVPLs.  criterion: ARC_CRITERION_BOUND (bound > threshold) or
 // ARC_CRITERION_EXPANSION (s > opening * d_c, or bound * (s / d_c)^2 >
 // threshold).
+//
+// The traversal only decides: kept nodes and exact leaf ranges are queued in
+// LDS (kKeep of each per wave) and summed in passes of their own when a queue
+// fills and at the end.  The traversal loop then carries no sums across its
+// branches, and each pass is a straight loop of one kind of work.
+constexpr int kKeep = 128;
+
 __global__ __launch_bounds__(kCutThreads) void k_cut_mom_tile(CutMom t, const float4* __restrict__ gpos,
                                                               const float4* __restrict__ gnrm,
                                                               const float4* __restrict__ galb,
@@ -357,44 +378,64 @@ __global__ __launch_bounds__(kCutThreads) void k_cut_mom_tile(CutMom t, const fl
                                                               float4* __restrict__ out_rad,
                                                               unsigned long long* __restrict__ counters) {
     __shared__ int stk[kCutThreads / 64][kMomStack];
+    __shared__ int qest[kCutThreads / 64][kKeep];       // internal nodes to estimate
+    __shared__ uint32_t qex[kCutThreads / 64][kKeep];   // exact ranges: first << 7 | (count - 1)
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const int2 xy = tile_pixel(wave, lane, (W + 15) / 16, (H + 15) / 16);
     const bool live = xy.x < W && xy.y < H;
     const int q = live ? xy.y * W + xy.x : 0;
     const PixelF P = load_pixel(gpos, gnrm, galb, mats, q);
     int* S = stk[wave];
+    int* E = qest[wave];
+    uint32_t* X = qex[wave];
     const float thr = live ? th.at(q) : INFINITY;
     const float delta2 = delta * delta;
     const bool expansion = criterion == ARC_CRITERION_EXPANSION;
     Acc acc;
     unsigned nvis = 0, nev = 0, nerr = 0, nbud = 0, ntest = 0, nexact = 0;  // wave-uniform
+    int ne = 0, nx = 0;                                                     // queued, wave-uniform
+    auto drain = [&]() {
+        for (int j = 0; j < ne; ++j) {
+            const int k = uniform(E[j]);
+            add_moments(P, t.mom + kMom * (size_t)k, acc);
+            acc.maybe_flush();
+        }
+        for (int j = 0; j < nx; ++j) {
+            const uint32_t e = (uint32_t)uniform((int)X[j]);
+            add_leaves(t.crec, (int)(e >> 7), (int)(e & 127u) + 1, P, acc);
+        }
+        ne = nx = 0;
+    };
     int sp = 0;
     int node = -1;
     for (;;) {
+        bool pop = true;  // take the next node from the stack (not a refined node's left child)
         if (node < 0) {
             const int k = -(node + 1);
             const arc_bvh_node& b = t.nodes[k];
             const float4* r = t.mom + kMom * (size_t)k;
-            const float4 m0 = r[0];
+            const float4 m0 = r[0], m1 = r[1];  // (c, max I), (s, first, count, -)
+            const float s = m1.x;
+            const int first = __float_as_int(m1.y), count = __float_as_int(m1.z);
             const float bound = node_bound(P, b, m0.w, delta2);
-            const int count = __float_as_int(r[3].w);
             if (!__ballot(bound > 0.f)) {
                 // behind every pixel's tangent plane: contributes exactly 0
             } else if (count <= exact_count) {
                 // small node: its AVPLs exactly (refining it would end in the same sum)
-                add_leaves(t.crec, __float_as_int(r[2].w), count, P, acc);
+                X[nx++] = ((uint32_t)first << 7) | (uint32_t)(count - 1);
                 nexact += count;
             } else {
                 bool refine;
                 if (expansion) {
                     const float dx = m0.x - P.px, dy = m0.y - P.py, dz = m0.z - P.pz;
                     const float dc2 = dx * dx + dy * dy + dz * dz;
-                    const float s = r[1].w, s2 = s * s;
+                    const float s2 = s * s;
                     refine = bound > 0.f && (s2 > open2 * dc2 || bound * s2 > thr * dc2);
                 } else {
                     refine = bound > thr;
                 }
                 ++ntest;
+                bool keep = true;
                 if (__ballot(refine)) {
                     if ((int)(nev + nexact) >= budget) {
                         ++nbud;  // cut budget spent: keep the cluster
@@ -402,23 +443,28 @@ __global__ __launch_bounds__(kCutThreads) void k_cut_mom_tile(CutMom t, const fl
                         S[sp++] = b.right;
                         node = uniform(b.left);
                         ++nvis;
-                        continue;
+                        keep = false;
+                        pop = false;
                     } else {
                         ++nerr;  // stack full: keep the cluster
                     }
                 }
-                add_moments(P, r, acc);
-                ++nev;
+                if (keep) {
+                    E[ne++] = k;
+                    ++nev;
+                }
             }
         } else {
-            const float4* l = t.crec + kCrec * (size_t)node;
-            acc.add_run(l[2], geom_est(P, l[0], l[1]));
+            X[nx++] = (uint32_t)node << 7;  // a leaf: a range of one
             ++nexact;
         }
-        acc.maybe_flush();
-        if (sp == 0) break;
-        node = uniform(S[--sp]);
+        if (ne == kKeep || nx == kKeep) drain();
+        if (pop) {
+            if (sp == 0) break;
+            node = uniform(S[--sp]);
+        }
     }
+    drain();
     acc.flush();
     const int alive = __popcll(__ballot(live));  // all lanes take part in the ballot
     if (counters && lane == 0) {

@@ -661,7 +661,8 @@ int arc_gather_run(arc_gather* g, const arc_avpls* avpls, arc_float3 camPos, flo
         return arc::fail(ctx, ARC_E_INVALID, "ARC_CRITERION_EXPANSION needs the moment estimator");
     if (mom_est && (!p->cutPerTile || p->cutEvalDepth != 0))
         return arc::fail(ctx, ARC_E_INVALID, "the moment estimator runs per 8x8 tile (cutPerTile 1, cutEvalDepth 0)");
-    if (p->cutExactCount < 0) return arc::fail(ctx, ARC_E_INVALID, "cutExactCount");
+    if (p->cutExactCount < 0 || (mom_est && p->cutExactCount > 128))
+        return arc::fail(ctx, ARC_E_INVALID, "cutExactCount out of [0, 128]");
     if (!(p->cutOpening > 0.f)) return arc::fail(ctx, ARC_E_INVALID, "cutOpening");
     if (p->cutBudget < 1) return arc::fail(ctx, ARC_E_INVALID, "cutBudget");
     // ---- radiance: light cut over this frame's AvplBvh (cudaGather.cu:854-923) or exact

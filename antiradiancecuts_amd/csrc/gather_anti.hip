@@ -31,13 +31,16 @@
 //                  The reference's C x N list buffer becomes C x kCap with an
 //                  overflow flag;
 //   k_anti_clustered
-//                  one 1024-thread block per cluster, one wave per 8x8 tile;
-//                  the block reads the cluster's list once and sorts its
+//                  four 256-thread blocks per cluster (its 32x8-pixel rows),
+//                  one wave per 8x8 tile; a block reads the cluster's list
+//                  once and sorts its
 //                  entries into per-tile sub-lists in LDS (ray against the
 //                  tile's own box), then each wave walks its sub-list with
 //                  wave-uniform (scalar) loads of the packed records and
-//                  evaluates the guarded pair test per pixel.  A cluster
-//                  whose list overflowed walks every antiradiance AVPL; the
+//                  evaluates the guarded pair test per pixel; the list is
+//                  filed and walked in chunks of the sub-lists' size, so no
+//                  sub-list overflows.  A cluster whose list overflowed
+//                  files every antiradiance AVPL the same way; the
 //                  summation order within a pixel follows the list order,
 //                  which the appends leave unspecified.
 #include <cmath>
@@ -57,7 +60,7 @@ constexpr int kTiles = (kClusterPx / 8) * (kClusterPx / 8);  // 8x8 tiles per cl
 constexpr int kParts = 4;                                      // blocks per cluster (rows of 32 x 8 px)
 constexpr int kBlockTiles = kTiles / kParts;                   // tiles per block
 constexpr int kAntiThreads = 64 * kBlockTiles;                 // one wave per tile
-constexpr int kSub = 1536;       // tile sub-list entries in LDS (more: the tile walks the cluster list)
+constexpr int kSub = 1536;       // tile sub-list entries in LDS: list entries filed per chunk
 constexpr int kTreeMaxLevels = 10;  // cluster quadtree levels: up to 512 x 512 clusters
 
 __device__ inline bool has_anti(const arc::AvplsDev& a, int i) {
@@ -326,7 +329,7 @@ __global__ __launch_bounds__(kListThreads) void k_cluster_lists_tree(arc::AvplsD
 // and tests its ray against the block's 4 tile boxes (the same conservative
 // test as the lists, on smaller boxes), appending hits to the tile's
 // sub-list in LDS.  Each wave then walks its tile's sub-list with
-// wave-uniform (scalar) loads of the packed records.  Small blocks keep the
+// wave-uniform (scalar) loads of the packed records.
 // LDS at 25 KB per block, so the occupancy is set by the registers (at least 5 waves
 // per SIMD) and the blocks share the CUs with the cut kernel that runs
 // beside them on the other stream.
@@ -370,26 +373,12 @@ __global__ __launch_bounds__(kAntiThreads) __attribute__((amdgpu_waves_per_eu(5)
     const bool overflow = listed > kCap;
     const int count = overflow ? *n_anti_ptr : listed;
     const int* list = overflow ? idx : lists + (size_t)c * kCap;
-    for (int j = tid; j < count; j += kAntiThreads) {
-        const int i = list[j];
-        const f3 p = raw.pos[i], w = raw.dir[i], na = raw.nrm[i];
-        const float ix = safe_inv(w.x), iy = safe_inv(w.y), iz = safe_inv(w.z);
-        for (int t = 0; t < kBlockTiles; ++t) {
-            if (tlive[t] && ray_box_hit<MODEL>(p, w, na, ix, iy, iz, tbox[t], grow, cos_t, sin_t)) {
-                const int slot = atomicAdd(&tcount[t], 1);
-                if (slot < kSub) sub[t][slot] = i;
-            }
-        }
-    }
-    __syncthreads();
-    const int nsub = tlive[wave] ? tcount[wave] : 0;
-    const bool use_sub = nsub <= kSub;  // else the tile walks the whole list
-    const int walk = !tlive[wave] ? 0 : use_sub ? nsub : count;
     // Entries are fetched 64 at a time, one per lane, and broadcast with
     // readlane.  Their records are read with scalar loads four entries at a
     // time: scalar loads complete out of order, so one wait covers a batch
     // and the four loads overlap.  Each chunk of 64 is one plain FP32 run,
-    // Kahan-added.
+    // Kahan-added.  (Staging each chunk's records in LDS instead measured
+    // slower: 3.8 -> 49 ms for the disk model on the 1080p x 1M frame.)
     static_assert(kRun == 64, "one compensated add per 64-entry chunk");
     auto pair = [&](int i, const float4& r0, const float4& r1, const float4& r2v, const float4& r4, float& s0,
                     float& s1, float& s2) {
@@ -414,35 +403,59 @@ __global__ __launch_bounds__(kAntiThreads) __attribute__((amdgpu_waves_per_eu(5)
             s0 += r4.x * m; s1 += r4.y * m; s2 += r4.z * m;
         }
     };
+    // The cluster's list is taken kSub entries at a time: the block files
+    // each chunk into the tiles' sub-lists (a chunk cannot overflow them),
+    // then each wave walks its tile's sub-list, so a tile never walks entries
+    // its own box test rejected.
     Kahan ka[3];
-    for (int base = 0; base < walk; base += 64) {
-        const int jj = base + lane;
-        // lanes past the end hold the chunk's first entry (a valid index)
-        const int mine = jj < walk ? (use_sub ? sub[wave][jj] : list[jj]) : (use_sub ? sub[wave][base] : list[base]);
-        const int cnt = __builtin_amdgcn_readfirstlane(min(64, walk - base));
-        float s0 = 0.f, s1 = 0.f, s2 = 0.f;
-        for (int u = 0; u < cnt; u += 4) {
-            int i[4];
-            float4 a0[4], a1[4], a2[4], a4[4];
-#pragma unroll
-            for (int k = 0; k < 4; ++k) {
-                i[k] = __builtin_amdgcn_readlane(mine, min(u + k, cnt - 1));
-                const float4* r = rec + (size_t)kRecF4 * i[k];
-                a0[k] = r[0];
-                a1[k] = r[1];
-                a2[k] = r[2];
-                a4[k] = r[4];
+    unsigned long long walked = 0;
+    for (int cbase = 0; cbase < count; cbase += kSub) {
+        const int cend = min(count, cbase + kSub);
+        if (lane == 0) tcount[wave] = 0;
+        __syncthreads();  // counts cleared; the previous chunk's sub-lists are read
+        for (int j = cbase + tid; j < cend; j += kAntiThreads) {
+            const int i = list[j];
+            const f3 p = raw.pos[i], w = raw.dir[i], na = raw.nrm[i];
+            const float ix = safe_inv(w.x), iy = safe_inv(w.y), iz = safe_inv(w.z);
+            for (int t = 0; t < kBlockTiles; ++t) {
+                if (tlive[t] && ray_box_hit<MODEL>(p, w, na, ix, iy, iz, tbox[t], grow, cos_t, sin_t)) {
+                    const int slot = atomicAdd(&tcount[t], 1);
+                    sub[t][slot] = i;  // slot < cend - cbase <= kSub
+                }
             }
-#pragma unroll
-            for (int k = 0; k < 4; ++k)
-                if (k == 0 || u + k < cnt) pair(i[k], a0[k], a1[k], a2[k], a4[k], s0, s1, s2);
         }
-        ka[0].add(s0); ka[1].add(s1); ka[2].add(s2);
+        __syncthreads();
+        const int walk = tlive[wave] ? tcount[wave] : 0;
+        walked += walk;
+        for (int base = 0; base < walk; base += 64) {
+            const int jj = base + lane;
+            // lanes past the end hold the chunk's first entry (a valid index)
+            const int mine = jj < walk ? sub[wave][jj] : sub[wave][base];
+            const int cnt = __builtin_amdgcn_readfirstlane(min(64, walk - base));
+            float s0 = 0.f, s1 = 0.f, s2 = 0.f;
+            for (int u = 0; u < cnt; u += 4) {
+                int i[4];
+                float4 a0[4], a1[4], a2[4], a4[4];
+#pragma unroll
+                for (int k = 0; k < 4; ++k) {
+                    i[k] = __builtin_amdgcn_readlane(mine, min(u + k, cnt - 1));
+                    const float4* r = rec + (size_t)kRecF4 * i[k];
+                    a0[k] = r[0];
+                    a1[k] = r[1];
+                    a2[k] = r[2];
+                    a4[k] = r[4];
+                }
+#pragma unroll
+                for (int k = 0; k < 4; ++k)
+                    if (k == 0 || u + k < cnt) pair(i[k], a0[k], a1[k], a2[k], a4[k], s0, s1, s2);
+            }
+            ka[0].add(s0); ka[1].add(s1); ka[2].add(s2);
+        }
     }
     const int alive = __popcll(__ballot(live));  // all lanes take part in the ballot
     if (counters && lane == 0) {
         if (wave == 0) atomicAdd(&counters[C_ANTI_VISITED], (unsigned long long)count * kBlockTiles);
-        atomicAdd(&counters[C_ANTI_EVALUATED], (unsigned long long)walk * alive);
+        atomicAdd(&counters[C_ANTI_EVALUATED], walked * alive);
     }
     if (!live) return;
     float* o = part + q;

@@ -231,7 +231,7 @@ typedef struct arc_gather_params {
                                          this many levels down (0..4; leaves where the subtree ends) */
     int cutEstimator;                 /* ARC_CUT_ERROR_BOUND: ARC_ESTIMATOR_* (default CENTROID)   */
     int cutExactCount;                /* ARC_ESTIMATOR_MOMENTS: a kept node of at most this many AVPLs
-                                         is summed exactly over its leaves (default 16)            */
+                                         (0..128) is summed exactly over its leaves (default 16)   */
     float cutOpening;                 /* ARC_CRITERION_EXPANSION: largest s / d of a kept node (0.3) */
 } arc_gather_params;
 

@@ -63,15 +63,17 @@ def moment_sums(crec: np.ndarray, ranges: np.ndarray, n: int, ref=None):
 
 
 def check_moment_records(mom: np.ndarray, crec: np.ndarray, ranges: np.ndarray, n: int, nodes):
-    """The moment estimator's records (13 float4 per internal node).  The
+    """The moment estimator's records (13 float4 per internal node: (c, max
+    I), (s, first, count, -), the four points' 12 floats, per channel
+    (-M/2, 3I/32), then 15T/32 per channel).  The
     positions are taken relative to the centre of the root's box (nodes[0]),
     as the library takes them."""
     ref = 0.5 * (nodes["bbMin"][0].astype(np.float64) + nodes["bbMax"][0].astype(np.float64))
     W, Sx, Sxx, Is, M, T = moment_sums(crec, ranges, n, ref)
     k = len(ranges)
     assert mom.shape == (k, 13, 4)
-    first = mom[:, 2, 3].copy().view(np.int32)
-    count = mom[:, 3, 3].copy().view(np.int32)
+    first = mom[:, 1, 1].copy().view(np.int32)
+    count = mom[:, 1, 2].copy().view(np.int32)
     np.testing.assert_array_equal(first, ranges[:, 0])
     np.testing.assert_array_equal(count, ranges[:, 1] - ranges[:, 0] + 1)
     scale = Is.max()
@@ -91,7 +93,7 @@ def check_moment_records(mom: np.ndarray, crec: np.ndarray, ranges: np.ndarray, 
     cl = Sx[live] / Wl[:, None]
     S = (Sxx[live] / Wl[:, None, None] - cl[:, :, None] * cl[:, None, :]).astype(np.float64)
     c = (cl + np.asarray(ref, np.longdouble)).astype(np.float64)
-    pts = mom[live, 1:5, :3].astype(np.float64)
+    pts = mom[live, 2:5, :].reshape(-1, 4, 3).astype(np.float64)
     np.testing.assert_allclose(mom[live, 0, :3], c, rtol=0, atol=1e-6 * (1 + np.abs(c).max()))
     off = pts - c[:, None, :]
     cmax = 1 + np.abs(c).max()
@@ -109,5 +111,5 @@ def check_moment_records(mom: np.ndarray, crec: np.ndarray, ranges: np.ndarray, 
         float(np.max(np.abs(cov - S) - tol[:, None, None]))
     # the half-diagonal of the node's box
     e = nodes["bbMax"].astype(np.float64) - nodes["bbMin"].astype(np.float64)
-    np.testing.assert_allclose(mom[:, 1, 3], 0.5 * np.sqrt((e * e).sum(1)), rtol=1e-6, atol=1e-12)
+    np.testing.assert_allclose(mom[:, 1, 0], 0.5 * np.sqrt((e * e).sum(1)), rtol=1e-6, atol=1e-12)
     return int(live.sum())

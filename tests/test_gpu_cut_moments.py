@@ -8,9 +8,9 @@ random representative, cudaGather.cu:416-420, bvh.cu:726-744):
     position mean, and four points whose mean and covariance are the node's);
     the centroid records of the same tree are checked the same way, over
     several 32K-leaf refit spans and at 1M AVPLs;
-  * errorBound = 0 refines every node that can contribute, and
-    cutExactCount >= n sums the root's AVPLs exactly: both equal the exact
-    FP64 radiance (relative L2 <= 1e-5);
+  * errorBound = 0 refines every node that can contribute down to nodes of
+    at most cutExactCount AVPLs, summed exactly: the exact FP64 radiance
+    (relative L2 <= 1e-5) for exact counts 0, 1, 16 and 128;
   * a node whose AVPLs share one position is summed exactly by the four
     points, so the estimate equals sum_c max(sum_j I_jc f2(t_j), 0) G_p with
     f2(t) = 3/32 + t/2 + 15/32 t^2 evaluated per AVPL (the addition theorem
@@ -76,7 +76,8 @@ def test_node_records_are_range_sums(gpu_ctx, num_avpls):
     np.testing.assert_allclose(crec[:n, 1, :3], nrm / np.linalg.norm(nrm, axis=1, keepdims=True), atol=1e-6)
     live_c = M.check_centroid_records(crec, ranges, n)
     live_m = M.check_moment_records(mom, crec, ranges, n, tree["nodes"])
-    assert live_c > 0.95 * (n - 1) and live_m > 0.95 * (n - 1)  # nodes with sum(w) > 1e-6 of the root
+    # nodes with sum(w) above 1e-6 of the root: 98 % at 100K AVPLs, 81 % at 1M
+    assert live_c > 0.75 * (n - 1) and live_m > 0.75 * (n - 1)
 
 
 @pytest.fixture(scope="module")
@@ -106,14 +107,18 @@ def test_zero_bound_is_exact(gpu_ctx, small, criterion):
     assert rel_l2(res, r64 - a64) <= 1e-5
 
 
-def test_exact_count_covers_everything(gpu_ctx, small):
+def test_exact_count_sums_small_nodes_exactly(gpu_ctx, small):
+    """errorBound = 0 refines down to nodes of at most cutExactCount AVPLs,
+    which are summed exactly: the exact radiance for every exact count."""
     sc, g, av, r64, a64 = small
-    print("[mom] root summed exactly", flush=True)
-    rad, _, st = run(gpu_ctx, g, av, sc, mom_params(errorBound=1e30, cutExactCount=sc.num_avpls,
-                                                    antiradianceModel=ffi.ARC_ANTIRAD_CONE))
-    assert st.testedNodes == 0  # the root is small enough: no test, every AVPL exactly
-    assert st.exactLeaves == sc.num_avpls * sc.npix
-    assert rel_l2(rad, r64) <= 1e-5
+    for ec in (1, 16, 128):
+        print(f"[mom] exact count {ec}", flush=True)
+        rad, _, st = run(gpu_ctx, g, av, sc, mom_params(errorBound=0.0, cutExactCount=ec,
+                                                        antiradianceModel=ffi.ARC_ANTIRAD_CONE))
+        assert st.exactLeaves > 0 and st.exactLeaves <= sc.num_avpls * sc.npix
+        assert rel_l2(rad, r64) <= 1e-5
+    with pytest.raises(Exception):
+        run(gpu_ctx, g, av, sc, mom_params(cutExactCount=129))
 
 
 def test_bound_tightens_error(gpu_ctx, small):
