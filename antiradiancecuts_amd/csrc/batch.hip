@@ -93,6 +93,11 @@ int arc_batch_capture(arc_batch* b, const arc_batch_view* views, int nviews) {
         const arc_ctx* vc = arc_gather_ctx(views[i].gather);
         if (std::find(b->lanes.begin(), b->lanes.end(), vc) == b->lanes.end())
             return arc::fail(ctx, ARC_E_INVALID, "a batch view's gather must be created on a lane context");
+        // a gather writes its own outputs: two views of one gather would
+        // overwrite each other's result inside the graph
+        for (int j = 0; j < i; ++j)
+            if (views[j].gather == views[i].gather)
+                return arc::fail(ctx, ARC_E_INVALID, "two batch views share one gather (each view needs its own)");
     }
     drop_graph(b);
     // 1. every view once, eagerly: buffers reach their final sizes.  The

@@ -370,11 +370,14 @@ __device__ inline void merge_cut(int target, int left, int right, int n, f3 lo, 
 
 // kernel_innerNodes (bvh.cu:280-316).  One thread per leaf walks up; the
 // second thread to arrive at a node merges both children.  The reference
-// reads the sibling's box without ordering (quirk Q3); here every arrival
-// counter is an acq_rel RMW, so the first arrival's writes are visible to
-// the second arrival before it reads them.
+// reads the sibling's box without ordering (quirk Q3); here the arrival
+// counters inside one kernel's workgroup are acq_rel RMWs at workgroup
+// scope, so the first arrival's writes are visible to the second arrival
+// before it reads them, and the arrivals that cross workgroups (blocks,
+// spans) are relaxed: the kernel boundary publishes their writes before the
+// next kernel reads them.
 //
-// Two kernels, no cross-XCD fence:
+// Three kernels, no cross-XCD fence:
 //   k_refit_local: a block owns kRefitLeaves consecutive leaves.  Karras node
 //     k has k at one end of its leaf range, so a node whose range lies inside
 //     the block's leaves has its index there too and both of its subtrees are

@@ -403,7 +403,8 @@ int arc_batch_create(arc_ctx* ctx, int lanes, arc_batch** out);   /* lanes: 1..8
 arc_ctx* arc_batch_lane(arc_batch* b, int lane);
 /* Runs every view once (its buffers reach their sizes), then records all of
  * them into one graph.  Synchronous.  The views' inputs and outputs must stay
- * where they are while the graph is in use; their contents may change. */
+ * where they are while the graph is in use; their contents may change.  Each
+ * view needs a gather of its own (ARC_E_INVALID when two views share one). */
 int arc_batch_capture(arc_batch* b, const arc_batch_view* views, int nviews);
 /* One graph launch on ctx's stream: a frame of every view.  Asynchronous. */
 int arc_batch_launch(arc_batch* b);

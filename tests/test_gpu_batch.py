@@ -26,6 +26,10 @@ VIEWS = [
     (96, 80, 4096, 12, dict(useLightCuts=1, cutMode=ffi.ARC_CUT_SOLID_ANGLE,
                             antiradianceMode=ffi.ARC_ANTI_CLUSTERED)),
     (80, 64, 3000, 13, dict(useLightCuts=1, cutMode=ffi.ARC_CUT_ERROR_BOUND, cutPerTile=1)),
+    (72, 56, 5000, 16, dict(useLightCuts=1, cutMode=ffi.ARC_CUT_ERROR_BOUND, cutPerTile=1,
+                            cutEstimator=ffi.ARC_ESTIMATOR_MOMENTS, cutCriterion=ffi.ARC_CRITERION_BOUND,
+                            cutImageFloor=0.0,
+                            antiradianceModel=ffi.ARC_ANTIRAD_CONE, antiradianceMode=ffi.ARC_ANTI_CLUSTERED)),
     (128, 96, 8192, 14, dict(useLightCuts=1, cutMode=ffi.ARC_CUT_ERROR_BOUND, cutPerTile=0,
                              antiradianceMode=ffi.ARC_ANTI_CLUSTERED)),
     (48, 40, 1024, 15, dict(useLightCuts=0, antiradianceMode=ffi.ARC_ANTI_CLUSTERED)),
@@ -80,6 +84,22 @@ def test_gather_off_lane_is_rejected(gpu_ctx):
     assert e.value.status == ffi.ARC_E_INVALID
     b.close()
     g.close()
+
+
+def test_shared_gather_is_rejected(gpu_ctx):
+    """Two views of one gather would overwrite each other's outputs inside
+    the graph: arc_batch_capture rejects them."""
+    sc = S.cornell_256(seed=2, width=32, height=32, num_avpls=256)
+    b = Batch(gpu_ctx, lanes=2)
+    g = Gather(b.lane(0), 32, 32, sc.materials)
+    g.set_gbuffer(device_array(sc.pos4), device_array(sc.nrm4), device_array(sc.albedo4))
+    av = DeviceAvpls.from_scene(sc)
+    view = (g, av, sc.cam, sc.extent, default_params())
+    with pytest.raises(ffi.ArcError) as e:
+        b.capture([view, view])
+    assert e.value.status == ffi.ARC_E_INVALID
+    g.close()
+    b.close()
 
 
 def test_replays_match_eager(gpu_ctx, rig):

@@ -1,6 +1,10 @@
 """The benchmark frame at its full size: 1920x1080 pixels, 1,000,000 AVPLs,
-the error-bound cut per 8x8 tile with kept nodes summed 3 levels down, and
-the clustered antiradiance on the gather's second stream.
+10-degree cone antiradiance (Shaders/Gather.frag:149-188) over the clustered
+lists on the gather's second stream, and the light-cut radiance chosen by
+the analytic (Lightcuts) error bound at 1 % with no image floor, kept nodes
+summed with second-order moments and nodes of at most 16 AVPLs exactly: the
+workload as written.  A second setting of the same frame (bound 0.003, image
+floor 2, nodes of at most 64 AVPLs exactly) reaches the 1e-3 target.
 
 Size-independent properties stand in for an oracle that cannot run this
 frame in seconds:
@@ -9,7 +13,9 @@ frame in seconds:
   * the gather is linear in the AVPL intensities: doubling every AVPL's
     radiance and antiradiance doubles the result (a power of two is exact in
     FP32, and the cut's thresholds double with the estimates);
-  * the radiance error stays at the level measured for this setting.
+  * the result's error against the FP64 exact gather stays within the level
+    measured for each setting: <= 2e-3 for the workload as written (1.73e-3
+    measured), <= 1e-3 for the second setting.
 """
 import numpy as np
 import pytest
@@ -25,6 +31,15 @@ def rel_l2(a, b):
     return float(np.linalg.norm(a - b) / np.linalg.norm(b))
 
 
+def frame_params(**kw):
+    base = dict(useLightCuts=1, cutMode=ffi.ARC_CUT_ERROR_BOUND, cutPerTile=1,
+                cutEstimator=ffi.ARC_ESTIMATOR_MOMENTS, cutCriterion=ffi.ARC_CRITERION_BOUND, errorBound=0.01,
+                cutImageFloor=0.0, cutExactCount=16, antiradianceModel=ffi.ARC_ANTIRAD_CONE,
+                antiradianceMode=ffi.ARC_ANTI_CLUSTERED)
+    base.update(kw)
+    return default_params(**base)
+
+
 def test_full_frame(gpu_ctx):
     print("[frame] building the 1080p x 1M scene", flush=True)
     sc = S.clutter_1080p()
@@ -32,14 +47,12 @@ def test_full_frame(gpu_ctx):
     g = Gather(gpu_ctx, sc.width, sc.height, sc.materials)
     g.set_gbuffer(device_array(sc.pos4), device_array(sc.nrm4), device_array(sc.albedo4))
     av = DeviceAvpls.from_scene(sc)
-    p = default_params(useLightCuts=1, cutMode=ffi.ARC_CUT_ERROR_BOUND, errorBound=0.1, cutPerTile=1,
-                       cutEvalDepth=3, antiradianceModel=ffi.ARC_ANTIRAD_DISK,
-                       antiradianceMode=ffi.ARC_ANTI_CLUSTERED)
+    p = frame_params()
     st = g.run(av, sc.cam, sc.extent, p, ffi.ARC_RUN_COUNTERS)
     gpu_ctx.sync()
     res = g.result[:, :3].double().cpu().numpy()
     anti = g.antiradiance[:, :3].double().cpu().numpy()
-    assert np.isfinite(res).all() and st.errorPixels == 0 and st.budgetHits == 0
+    assert np.isfinite(res).all() and st.errorPixels == 0
 
     # FP64 exact gather on every 8th pixel in x and y
     k = 8
@@ -56,9 +69,19 @@ def test_full_frame(gpu_ctx):
     gs.close()
     e_anti = rel_l2(anti[pix], a64)
     e_res = rel_l2(res[pix], r64 - a64)
-    print(f"[frame] antiradiance rel-L2 {e_anti:.2e}, result rel-L2 {e_res:.3e}", flush=True)
+    print(f"[frame] cone antiradiance rel-L2 {e_anti:.2e}, result rel-L2 {e_res:.3e}", flush=True)
     assert e_anti <= 1e-5
-    assert e_res <= 0.035
+    assert e_res <= 2e-3
+
+    # the second setting: within the 1e-3 target
+    p2 = frame_params(errorBound=0.003, cutImageFloor=2.0, cutExactCount=64, cutBudget=1 << 20)
+    st2 = g.run(av, sc.cam, sc.extent, p2, ffi.ARC_RUN_COUNTERS)
+    gpu_ctx.sync()
+    res_acc = g.result[:, :3].double().cpu().numpy()
+    e_acc = rel_l2(res_acc[pix], r64 - a64)
+    print(f"[frame] bound 0.003, floor 2, exact <= 64: result rel-L2 {e_acc:.3e}", flush=True)
+    assert st2.errorPixels == 0 and st2.budgetHits == 0
+    assert e_acc <= 1e-3
 
     # linearity: doubled intensities, doubled image
     for key in ("rad", "anti"):
