@@ -365,7 +365,8 @@ def main() -> int:
     return 0
 
 
-TRAFFIC_PROFILE = ROOT / "profiles" / "frame_1080p_1M_traffic.txt"
+# the PMC summary of the benchmark frame (moment estimates, cone antiradiance)
+TRAFFIC_PROFILE = ROOT / "profiles" / "frame_1080p_1M_cone_moments_traffic.txt"
 # LBVH kernels of one rebuild, with their dispatch counts (4 radix passes)
 BVH_KERNELS = {"k_bounds_partial": 1, "k_bounds_final": 1, "k_encode": 1, "k_hist": 4, "k_scan_digit": 4,
                "k_scatter": 4, "k_stage_avpls": 1, "k_permute_cut": 1, "k_karras": 1,

@@ -33,14 +33,15 @@ def test_bench_reads_the_traffic_profile():
 
     if not bench.TRAFFIC_PROFILE.exists():
         pytest.skip("no traffic profile committed")
-    # the setting the committed traffic profile covers: the centroid
-    # estimator with sums one level down and disk antiradiance
-    depth = 1
-    anti = bench.profiled_traffic("antiradiance", True, True, depth, False, False)
-    cut = bench.profiled_traffic("radiance", True, True, depth, False, False)
-    bvh = bench.profiled_traffic("bvh", True, True, depth)
+    # the setting the committed traffic profile covers: the moment
+    # estimator and cone antiradiance; other settings have no figure
+    anti = bench.profiled_traffic("antiradiance", True, True, 0, True, True)
+    cut = bench.profiled_traffic("radiance", True, True, 0, True, True)
+    bvh = bench.profiled_traffic("bvh", True, True, 0)
     assert anti and cut and bvh
-    assert bvh > cut  # the LBVH rebuild moves more bytes than the cut
+    assert cut > anti > bvh  # the cut's node reads dominate the frame's bytes
+    assert bench.profiled_traffic("radiance", True, True, 1, False, False) is None
+    assert bench.profiled_traffic("antiradiance", True, True, 0, True, False) is None
 
 
 def test_equal_error_interpolates_in_log_log():
