@@ -375,13 +375,29 @@ __global__ __launch_bounds__(kCutThreads) void k_cut_mom_tile(CutMom t, const fl
                                                               const arc_material* __restrict__ mats, int W, int H,
                                                               Threshold th, float delta, int budget, int criterion,
                                                               float open2, int exact_count,
+                                                              const int* __restrict__ order,
+                                                              float* __restrict__ tile_cost,
                                                               float4* __restrict__ out_rad,
                                                               unsigned long long* __restrict__ counters) {
     __shared__ int stk[kCutThreads / 64][kMomStack];
     __shared__ int qest[kCutThreads / 64][kKeep];       // internal nodes to estimate
     __shared__ uint32_t qex[kCutThreads / 64][kKeep];   // exact ranges: first << 7 | (count - 1)
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
-    const int2 xy = tile_pixel(wave, lane, (W + 15) / 16, (H + 15) / 16);
+    // the wave's 8x8 tile: the tile grid is (2 gx) x (2 gy) tiles of the
+    // launch's gx x gy blocks of 16x16 pixels; with an order, the launch's
+    // waves take the tiles in that order, else each block its four tiles
+    const int gx = (W + 15) / 16, gy = (H + 15) / 16, tw = 2 * gx;
+    int tx, ty;
+    if (order) {
+        const int ti = uniform(order[blockIdx.x * 4 + wave]);
+        tx = ti % tw;
+        ty = ti / tw;
+    } else {
+        const int b = xcd_block(blockIdx.x, gx * gy);
+        tx = (b % gx) * 2 + (wave & 1);
+        ty = (b / gx) * 2 + (wave >> 1);
+    }
+    const int2 xy = make_int2(tx * 8 + (lane & 7), ty * 8 + (lane >> 3));
     const bool live = xy.x < W && xy.y < H;
     const int q = live ? xy.y * W + xy.x : 0;
     const PixelF P = load_pixel(gpos, gnrm, galb, mats, q);
@@ -466,6 +482,11 @@ __global__ __launch_bounds__(kCutThreads) void k_cut_mom_tile(CutMom t, const fl
     }
     drain();
     acc.flush();
+    // the tile's cost in VALU instructions (about 30 per bound test, 130 per
+    // moment estimate, 26 per exact pair, and the pixel's own loads and
+    // stores): the next frame's launch order
+    if (lane == 0)
+        tile_cost[ty * tw + tx] = 300.f + 30.f * (float)ntest + 130.f * (float)nev + 26.f * (float)nexact;
     const int alive = __popcll(__ballot(live));  // all lanes take part in the ballot
     if (counters && lane == 0) {
         atomicAdd(&counters[C_TESTED], (unsigned long long)ntest * alive);
@@ -478,6 +499,39 @@ __global__ __launch_bounds__(kCutThreads) void k_cut_mom_tile(CutMom t, const fl
     if (live)
         out_rad[q] = make_float4(acc.k[0].s * P.br, acc.k[1].s * P.bg, acc.k[2].s * P.bb,
                                  counters ? (float)(nev + nexact) : 1.f);
+}
+
+// The launch order of the cut's tiles from their costs in the last frame,
+// heaviest first (longest-processing-time order): the tiles' costs spread
+// over two orders of magnitude, and a heavy tile started last runs alone at
+// the end of the launch.  A bucket sort on eight buckets per octave of cost,
+// in one workgroup; the order within a bucket is that of the LDS atomics.
+// A tile's result does not depend on the order.
+constexpr int kOrderBins = 256;
+constexpr int kOrderThreads = 1024;
+
+__device__ inline int order_bin(float c) {
+    const int b = (int)(__log2f(fmaxf(c, 1.f)) * 8.f);
+    return kOrderBins - 1 - min(max(b, 0), kOrderBins - 1);
+}
+
+__global__ __launch_bounds__(kOrderThreads) void k_tile_order(const float* __restrict__ cost, int n,
+                                                              int* __restrict__ order) {
+    __shared__ int hist[kOrderBins];
+    __shared__ int off[kOrderBins];
+    for (int i = threadIdx.x; i < kOrderBins; i += kOrderThreads) hist[i] = 0;
+    __syncthreads();
+    for (int i = threadIdx.x; i < n; i += kOrderThreads) atomicAdd(&hist[order_bin(cost[i])], 1);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int s = 0;
+        for (int b = 0; b < kOrderBins; ++b) {
+            off[b] = s;
+            s += hist[b];
+        }
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < n; i += kOrderThreads) order[atomicAdd(&off[order_bin(cost[i])], 1)] = i;
 }
 
 }  // namespace
@@ -516,12 +570,24 @@ int cut_moments(arc_gather* g, arc_bvh* b, const CutEBParams& cp, float4* rad, u
     Threshold th{};
     ARC_TRY(prepare_cut(g, b, cp, t, th));
     const CutMom tm{b->nodes.as<arc_bvh_node>(), g->crec.as<float4>(), g->mom.as<float4>(), n};
-    const dim3 grid(((g->width + 15) / 16) * ((g->height + 15) / 16));
+    const int blocks = ((g->width + 15) / 16) * ((g->height + 15) / 16);
+    const int ntiles = 4 * blocks;
+    if (g->tile_cost_n != ntiles || g->tile_cost.bytes < sizeof(float) * (size_t)ntiles) g->tile_cost_n = 0;
+    ARC_TRY(ensure(ctx, g->tile_cost, sizeof(float) * (size_t)ntiles));
+    ARC_TRY(ensure(ctx, g->tile_order, sizeof(int) * (size_t)ntiles));
+    // the last frame's tile costs, if they describe this image, order the launch
+    const int* order = nullptr;
+    if (g->tile_cost_n == ntiles) {
+        hipLaunchKernelGGL(k_tile_order, dim3(1), dim3(kOrderThreads), 0, ctx->stream, g->tile_cost.as<float>(), ntiles,
+                           g->tile_order.as<int>());
+        order = g->tile_order.as<int>();
+    }
     if (mark) ARC_HIP(ctx, hipEventRecord(mark, ctx->stream));
-    hipLaunchKernelGGL(k_cut_mom_tile, grid, dim3(kCutThreads), 0, ctx->stream, tm, g->gpos, g->gnrm, g->galb,
+    hipLaunchKernelGGL(k_cut_mom_tile, dim3(blocks), dim3(kCutThreads), 0, ctx->stream, tm, g->gpos, g->gnrm, g->galb,
                        g->mats.as<arc_material>(), g->width, g->height, th, cp.delta, cp.budget, cp.criterion,
-                       cp.opening * cp.opening, cp.exact_count, rad, counters);
+                       cp.opening * cp.opening, cp.exact_count, order, g->tile_cost.as<float>(), rad, counters);
     ARC_LAUNCHED(ctx);
+    g->tile_cost_n = ntiles;
     return ARC_OK;
 }
 

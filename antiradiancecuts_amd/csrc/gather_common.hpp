@@ -34,6 +34,11 @@ struct arc_gather {
     // over the sorted leaves (with per-block offsets) they are made from
     arc::DevBuf mom, mom_pre, mom_tot;
     bool mom_ready = false;  // mom holds the records of the current cut tree
+    // moment cut: each 8x8 tile's cost in the last frame (VALU instructions,
+    // estimated from its counts) and the launch order made from it, heaviest
+    // tile first; tile_cost_n is the tile count the costs describe (0: none)
+    arc::DevBuf tile_cost, tile_order;
+    int tile_cost_n = 0;
     arc_cutnet* net = nullptr;  // learned cut: the predictor (borrowed)
     arc_bvh* bvh = nullptr;
     // per-frame phase events: frame slot, then 9 marks (start, upload done,

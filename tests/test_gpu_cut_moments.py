@@ -133,6 +133,23 @@ def test_bound_tightens_error(gpu_ctx, small):
     assert errs[1] < 1e-2
 
 
+def test_tile_order_keeps_the_image(gpu_ctx):
+    """A frame's tiles launch in the order of the last frame's tile costs,
+    heaviest first; a tile's sums do not depend on when it runs, so the
+    second frame (ordered) equals the first (launch order) bit for bit."""
+    sc = S.clutter_1080p(width=200, height=120, num_avpls=60_000)
+    print("[mom] ordered launch", flush=True)
+    g, av = make_rig(gpu_ctx, sc)
+    try:
+        p = mom_params(antiradianceModel=ffi.ARC_ANTIRAD_CONE)
+        first, _, st1 = run(gpu_ctx, g, av, sc, p)
+        second, _, st2 = run(gpu_ctx, g, av, sc, p)
+        assert np.array_equal(first, second)
+        assert st1.evaluatedNodes == st2.evaluatedNodes and st1.testedNodes == st2.testedNodes
+    finally:
+        g.close()
+
+
 def test_colocated_node_is_the_sh2_sum(gpu_ctx):
     """Three AVPLs at one point with different normals and colours, seen from
     pixels far away: the root is kept (huge bound) and estimated; its four
