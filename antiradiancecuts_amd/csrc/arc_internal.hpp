@@ -63,6 +63,12 @@ static_assert(sizeof(arc_material) == 40, "MATERIAL layout");
 struct SortScratch {
     DevBuf keys_alt, vals_alt, counts;
 };
+constexpr int kSortTile = 2048;   // keys per sort block (one tile)
+constexpr int kSortRadix = 256;   // 8-bit digits
+// the digit-major count table: kSortRadix x tiles, then the kSortRadix digit totals
+constexpr size_t sort_counts_size(size_t n) {
+    return (size_t)kSortRadix * ((n + kSortTile - 1) / kSortTile) + kSortRadix;
+}
 int sort_pairs(arc_ctx* ctx, SortScratch& s, uint32_t* keys, int* vals, int n, int key_bits);
 
 // Counter-based uniform floats in [0,1): (mix64(seed, i) >> 40) * 2^-24.

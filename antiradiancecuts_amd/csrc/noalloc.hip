@@ -26,12 +26,11 @@ namespace {
 constexpr int kMaxPoints = ARC_NOALLOC_MAX_POINTS;
 constexpr size_t kAlign = 256;
 constexpr size_t align_up(size_t v) { return (v + kAlign - 1) / kAlign * kAlign; }
-// Sort tiles of 4096 keys, 256 digits (sort.hip); bounds partials of up to
-// 1024 blocks x 12 floats (lbvh.hip).
-constexpr size_t kSortTiles = (kMaxPoints + 4095) / 4096;
+// The sort's second key/value buffers and digit count table (sort.hip);
+// bounds partials of up to 1024 blocks x 12 floats (lbvh.hip).
 constexpr size_t kKeysAlt = align_up(sizeof(uint32_t) * kMaxPoints);
 constexpr size_t kValsAlt = align_up(sizeof(int) * kMaxPoints);
-constexpr size_t kCounts = align_up(sizeof(uint32_t) * (256 * kSortTiles + 256));
+constexpr size_t kCounts = align_up(sizeof(uint32_t) * arc::sort_counts_size(kMaxPoints));
 constexpr size_t kReady = align_up(sizeof(int) * 2 * (size_t)kMaxPoints);
 constexpr size_t kBounds = align_up(sizeof(float) * 16);
 constexpr size_t kPartials = align_up(sizeof(float) * 12 * 1024);
