@@ -24,7 +24,9 @@ def candidates(base: ffi.arc_gather_params):
     """The schedule variants of `base` the tuner tries."""
     out = []
     if base.useLightCuts and base.cutMode == ffi.ARC_CUT_ERROR_BOUND:
-        for per_tile, d0 in itertools.product((1, 0), (4, 6, 8)):
+        # the moment estimator runs per 8x8 tile only
+        tiles = (1,) if base.cutEstimator == ffi.ARC_ESTIMATOR_MOMENTS else (1, 0)
+        for per_tile, d0 in itertools.product(tiles, (4, 6, 8)):
             p = _copy(base)
             p.cutPerTile, p.cutEstimateDepth = per_tile, d0
             out.append(p)

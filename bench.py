@@ -118,6 +118,26 @@ def parse(argv=None):
     return ap.parse_args(argv)
 
 
+def frame_params(args):
+    """The gather parameters of the frame the options describe (the
+    workload as written by default)."""
+    from antiradiancecuts_amd import ffi
+    from antiradiancecuts_amd.runtime import default_params
+
+    eb = args.cut == "error-bound"
+    mom = eb and args.estimator == "moments"
+    crit = {"bound": ffi.ARC_CRITERION_BOUND, "estimate": ffi.ARC_CRITERION_ESTIMATE,
+            "expansion": ffi.ARC_CRITERION_EXPANSION}[args.criterion]
+    return default_params(useLightCuts=1,
+                          antiradianceModel=ffi.ARC_ANTIRAD_CONE if args.antiradiance == "cone" else ffi.ARC_ANTIRAD_DISK,
+                          antiradianceMode=ffi.ARC_ANTI_CLUSTERED,
+                          cutMode=ffi.ARC_CUT_ERROR_BOUND if eb else ffi.ARC_CUT_SOLID_ANGLE,
+                          errorBound=args.error_bound, cutPerTile=args.per_tile, cutEvalDepth=args.eval_depth,
+                          cutEstimateDepth=args.estimate_depth, cutCriterion=crit, cutImageFloor=args.image_floor,
+                          cutEstimator=ffi.ARC_ESTIMATOR_MOMENTS if mom else ffi.ARC_ESTIMATOR_CENTROID,
+                          cutExactCount=args.exact_count, cutOpening=args.opening)
+
+
 def main() -> int:
     args = parse()
     if args.gpus != 1:
@@ -150,14 +170,7 @@ def main() -> int:
     eb = args.cut == "error-bound"
     mom = eb and args.estimator == "moments"
     cone = args.antiradiance == "cone"
-    crit = {"bound": ffi.ARC_CRITERION_BOUND, "estimate": ffi.ARC_CRITERION_ESTIMATE,
-            "expansion": ffi.ARC_CRITERION_EXPANSION}[args.criterion]
-    params = default_params(useLightCuts=1, antiradianceModel=ffi.ARC_ANTIRAD_CONE if cone else ffi.ARC_ANTIRAD_DISK,
-                            cutMode=ffi.ARC_CUT_ERROR_BOUND if eb else ffi.ARC_CUT_SOLID_ANGLE,
-                            errorBound=args.error_bound, cutPerTile=args.per_tile, cutEvalDepth=args.eval_depth,
-                            cutEstimateDepth=args.estimate_depth, cutCriterion=crit, cutImageFloor=args.image_floor,
-                            cutEstimator=ffi.ARC_ESTIMATOR_MOMENTS if mom else ffi.ARC_ESTIMATOR_CENTROID,
-                            cutExactCount=args.exact_count, cutOpening=args.opening)
+    params = frame_params(args)
     flags = ffi.ARC_RUN_TIMELINE | ffi.ARC_TRUST_INPUT
 
     for _ in range(args.warmup):
@@ -465,9 +478,7 @@ def bench_batch(args) -> int:
     rng = np.random.default_rng(args.batch_seed)
     ctx = Context(0)
     b = Batch(ctx, lanes=args.lanes)
-    base = default_params(useLightCuts=1, antiradianceModel=ffi.ARC_ANTIRAD_DISK,
-                          cutMode=ffi.ARC_CUT_ERROR_BOUND, errorBound=args.error_bound, cutPerTile=args.per_tile,
-                          cutEvalDepth=args.eval_depth)
+    base = frame_params(args)
     views, alone_ms, pairs, tuned = [], [], 0.0, {}
     t0 = time.time()
     for k in range(args.views):
@@ -508,8 +519,9 @@ def bench_batch(args) -> int:
             "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
             "config": {"workload": f"{args.views} views, W,H uniform [256,1920]x[256,1080], AVPLs log-uniform "
                                    f"[16K,1M], Cornell walls + {args.tris} random triangles per view (seeded "
-                                   f"variants), error-bound cut {args.error_bound:g} with kept nodes summed {args.eval_depth} levels "
-                                   f"down, {args.lanes} lanes",
+                                   f"variants), {args.cut} cut {args.error_bound:g} ({args.criterion} criterion, "
+                                   f"{args.estimator} estimates, image floor {args.image_floor:g}, nodes of <= "
+                                   f"{args.exact_count} AVPLs exact), {args.antiradiance} antiradiance, {args.lanes} lanes",
                        "views": args.views, "lanes": args.lanes, "pairs_per_batch": pairs,
                        "view_ms_alone_p50": round(float(np.percentile(alone, 50)), 4),
                        "view_ms_alone_p99": round(float(np.percentile(alone, 99)), 4),
