@@ -674,7 +674,7 @@ int arc_gather_run(arc_gather* g, const arc_avpls* avpls, arc_float3 camPos, flo
         hipStream_t main_stream = ctx->stream;
         ctx->stream = g->aux;  // the antiradiance launches go to the second stream
         const int sa = arc::anti_clustered(g, a, p->antiradianceModel, cam, anti_params(radius, cos_half),
-                                           (double)p->coneHalfAngle, g->part.as<float>(), counters,
+                                           (double)p->coneHalfAngle, sceneExtent, g->part.as<float>(), counters,
                                            profile ? ev[7] : nullptr);
         hipError_t e = hipSuccess;
         if (sa == ARC_OK && profile) e = hipEventRecord(ev[8], g->aux);
@@ -733,7 +733,7 @@ int arc_gather_run(arc_gather* g, const arc_avpls* avpls, arc_float3 camPos, flo
         ARC_HIP(ctx, hipStreamWaitEvent(ctx->stream, g->join, 0));  // what the radiance did not hide
     } else if (clustered) {
         ARC_TRY(arc::anti_clustered(g, a, p->antiradianceModel, cam, anti_params(radius, cos_half),
-                                    (double)p->coneHalfAngle, g->part.as<float>(), counters,
+                                    (double)p->coneHalfAngle, sceneExtent, g->part.as<float>(), counters,
                                     profile ? ev[7] : nullptr));
         anti_splits = 1;
         if (profile) ARC_HIP(ctx, hipEventRecord(ev[8], ctx->stream));

@@ -341,6 +341,7 @@ __global__ __launch_bounds__(kAntiThreads) __attribute__((amdgpu_waves_per_eu(5)
     const int* __restrict__ lists, const int* __restrict__ idx, const int* __restrict__ n_anti_ptr, float grow,
     float cos_t, float sin_t, float* __restrict__ part, unsigned long long* __restrict__ counters) {
     __shared__ float tbox[kBlockTiles][6];
+    __shared__ float tface[kBlockTiles][8];  // cone: m xyz, h, s, box centre xyz
     __shared__ int tlive[kBlockTiles];
     __shared__ int tcount[kBlockTiles];
     __shared__ int sub[kBlockTiles][kSub];
@@ -366,6 +367,39 @@ __global__ __launch_bounds__(kAntiThreads) __attribute__((amdgpu_waves_per_eu(5)
             for (int k = 0; k < 6; ++k) tbox[wave][k] = tb[k];
             tlive[wave] = any ? 1 : 0;  // a tile wholly outside the image has nothing to sum
             tcount[wave] = 0;
+        }
+        if (MODEL == ARC_ANTIRAD_CONE) {
+            // the tile's normal cone: unit axis m (the normals' mean
+            // direction), s = max |n - m| and h = min m . p over its live
+            // pixels; a cone pair needs n . (a - p) > 0, and
+            // n . (a - p) <= m . a - h + s (|a - c| + R) for every pixel
+            float mx = live ? P.nx : 0.f, my = live ? P.ny : 0.f, mz = live ? P.nz : 0.f;
+            for (int off = 32; off > 0; off >>= 1) {
+                mx += __shfl_xor(mx, off);
+                my += __shfl_xor(my, off);
+                mz += __shfl_xor(mz, off);
+            }
+            const float ml = sqrtf(mx * mx + my * my + mz * mz);
+            if (ml > 1e-20f) {
+                mx /= ml; my /= ml; mz /= ml;
+            } else {
+                mx = 0.f; my = 0.f; mz = 1.f;  // any unit axis: s then covers every normal
+            }
+            const float ex = P.nx - mx, ey = P.ny - my, ez = P.nz - mz;
+            float s2 = live ? ex * ex + ey * ey + ez * ez : 0.f;
+            float h = live ? mx * P.px + my * P.py + mz * P.pz : INFINITY;
+            for (int off = 32; off > 0; off >>= 1) {
+                s2 = fmaxf(s2, __shfl_xor(s2, off));
+                h = fminf(h, __shfl_xor(h, off));
+            }
+            if (lane == 0) {
+                tface[wave][0] = mx; tface[wave][1] = my; tface[wave][2] = mz;
+                tface[wave][3] = h;
+                tface[wave][4] = sqrtf(s2) * 1.0001f + 1e-6f;
+                tface[wave][5] = 0.5f * (tb[0] + tb[3]);
+                tface[wave][6] = 0.5f * (tb[1] + tb[4]);
+                tface[wave][7] = 0.5f * (tb[2] + tb[5]);
+            }
         }
     }
     __syncthreads();
@@ -418,6 +452,15 @@ __global__ __launch_bounds__(kAntiThreads) __attribute__((amdgpu_waves_per_eu(5)
             const f3 p = raw.pos[i], w = raw.dir[i], na = raw.nrm[i];
             const float ix = safe_inv(w.x), iy = safe_inv(w.y), iz = safe_inv(w.z);
             for (int t = 0; t < kBlockTiles; ++t) {
+                if (MODEL == ARC_ANTIRAD_CONE && tlive[t]) {
+                    // behind every pixel's tangent plane: no pixel of the tile faces the AVPL
+                    const float* f = tface[t];
+                    const float* bb = tbox[t];
+                    const float qx = p.x - f[5], qy = p.y - f[6], qz = p.z - f[7];
+                    const float bx = bb[3] - bb[0], by = bb[4] - bb[1], bz = bb[5] - bb[2];
+                    const float reach = sqrtf(qx * qx + qy * qy + qz * qz) + 0.5f * sqrtf(bx * bx + by * by + bz * bz);
+                    if (f[0] * p.x + f[1] * p.y + f[2] * p.z - f[3] + f[4] * reach < -grow) continue;
+                }
                 if (tlive[t] && ray_box_hit<MODEL>(p, w, na, ix, iy, iz, tbox[t], grow, cos_t, sin_t)) {
                     const int slot = atomicAdd(&tcount[t], 1);
                     sub[t][slot] = i;  // slot < cend - cbase <= kSub
@@ -482,7 +525,7 @@ void launch_pixels(arc_gather* g, const arc::AvplsDev& a, int cw, int C, float3 
 namespace arc {
 
 int anti_clustered(arc_gather* g, const AvplsDev& a, int model, float3 cam, const AntiParams& ap,
-                   double half_angle, float* part, unsigned long long* counters, hipEvent_t mark) {
+                   double half_angle, float extent, float* part, unsigned long long* counters, hipEvent_t mark) {
     arc_ctx* ctx = g->ctx;
     const int n = a.n;
     const int nb = (n + kCompact - 1) / kCompact;
@@ -515,9 +558,13 @@ int anti_clustered(arc_gather* g, const AvplsDev& a, int model, float3 cam, cons
                        g->cbox.as<float>());
     hipLaunchKernelGGL(k_zero, dim3((C + 255) / 256), dim3(256), 0, ctx->stream, g->ccount.as<int>(), C);
     ARC_LAUNCHED(ctx);
-    // The list test must keep every pair the FP32 / FP64 pair test accepts: the
-    // box grows by the photon radius plus a relative and an absolute margin.
-    const float grow = ap.radius * 1.001f + 1e-5f;
+    // The list test must keep every pair the FP32 / FP64 pair test accepts.
+    // For disks the box grows by the photon radius plus a relative and an
+    // absolute margin (the line may pass the pixel that far away); the cone
+    // test has no radius, so its box grows only by a rounding margin, 2e-5 of
+    // the scene's extent (FP32 carries about 1e-7 of it).
+    const float grow = model == ARC_ANTIRAD_DISK ? ap.radius * 1.001f + 1e-5f
+                                                 : 2e-5f * fmaxf(extent, 1e-30f) + 1e-30f;
     const double th = std::min(M_PI, half_angle * 1.001 + 1e-4);
     const float cos_t = (float)std::cos(th), sin_t = (float)std::sin(th);
     const int* n_anti = offs + nb;

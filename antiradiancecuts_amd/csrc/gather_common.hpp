@@ -264,9 +264,10 @@ int cut_learned(arc_gather* g, arc_bvh* b, const CutEBParams& cp, float4* rad, u
 int cut_samples(arc_gather* g, arc_bvh* b, const CutEBParams& cp, float* X, float* y, int capacity, uint64_t seed,
                 float keep, int* count);
 
-// Clustered antiradiance (gather_anti.hip).
-// `mark` (may be null) is recorded right before the per-pixel kernel.
+// Clustered antiradiance (gather_anti.hip).  `extent` (the scene's extent)
+// scales the margins of the cone model's conservative tests; `mark` (may be
+// null) is recorded right before the per-pixel kernel.
 int anti_clustered(arc_gather* g, const AvplsDev& a, int model, float3 cam, const gd::AntiParams& ap,
-                   double half_angle, float* part, unsigned long long* counters, hipEvent_t mark);
+                   double half_angle, float extent, float* part, unsigned long long* counters, hipEvent_t mark);
 
 }  // namespace arc

@@ -190,6 +190,29 @@ def test_clustered_antiradiance_equals_exact(gpu_ctx, cornell, model, which):
     g.close()
 
 
+@pytest.mark.parametrize("half_angle", [math.pi / 4, 0.6 * math.pi])
+def test_clustered_antiradiance_wide_cones(gpu_ctx, half_angle):
+    """Cones of 45 and 108 degrees on a cluttered view (tiles that mix
+    foreground triangles and walls, normals spread over whole tiles): the
+    cluster and tile tests (the cone against a box's bounding sphere, and
+    the tile's normal cone against the AVPL's side of every pixel's tangent
+    plane) keep every pair the exact gather sums."""
+    sc = S.clutter_1080p(width=480, height=272, num_avpls=60_000)
+    print(f"[gather] clustered vs exact, cone half-angle {half_angle:.3f}", flush=True)
+    g = make_gather(gpu_ctx, sc)
+    av = DeviceAvpls.from_scene(sc)
+    base = dict(useLightCuts=0, antiradianceModel=ffi.ARC_ANTIRAD_CONE, coneHalfAngle=half_angle)
+    g.run(av, sc.cam, sc.extent, default_params(antiradianceMode=ffi.ARC_ANTI_EXACT, **base))
+    gpu_ctx.sync()
+    exact = g.antiradiance[:, :3].double().cpu().numpy()
+    g.run(av, sc.cam, sc.extent, default_params(antiradianceMode=ffi.ARC_ANTI_CLUSTERED, **base))
+    gpu_ctx.sync()
+    clus = g.antiradiance[:, :3].double().cpu().numpy()
+    assert np.abs(exact).max() > 0
+    np.testing.assert_allclose(clus, exact, rtol=1e-5, atol=1e-5 * np.abs(exact).max())
+    g.close()
+
+
 def test_clustered_antiradiance_wide_disks(gpu_ctx, cornell):
     """Disks half the scene wide: most rays reach most clusters, so a block's
     appends overflow its LDS buffer into direct appends, and the sums still
