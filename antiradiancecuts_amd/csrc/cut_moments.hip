@@ -330,30 +330,72 @@ __device__ inline void add_moments(const PixelF& P, const float4* __restrict__ r
     ++acc.run;
 }
 
-// The AVPLs of sorted leaves [first, first + count) exactly, four leaves'
-// records loaded before their pairs are evaluated (one wait per four scalar
-// loads instead of one per leaf).
-__device__ inline void add_leaves(const float4* __restrict__ crec, int first, int count, const PixelF& P, Acc& acc) {
-    int i = first;
-    const int end = first + count;
-    for (; i + 4 <= end; i += 4) {
-        float4 a0[4], a1[4], a2[4];
+// Inclusive sum over the wave's lanes.
+__device__ inline int wave_inclusive_sum(int v) {
+    const int lane = threadIdx.x & 63;
 #pragma unroll
-        for (int k = 0; k < 4; ++k) {
-            const float4* l = crec + kCrec * (size_t)(i + k);
-            a0[k] = l[0];
-            a1[k] = l[1];
-            a2[k] = l[2];
+    for (int off = 1; off < 64; off <<= 1) {
+        const int u = __shfl_up(v, off, 64);
+        if (lane >= off) v += u;
+    }
+    return v;
+}
+
+// The queued exact ranges' leaves, summed exactly.  A range is
+// (first << 7 | (count - 1)) over contiguous sorted leaves.  The leaves of
+// all ranges are taken 64 at a time: each lane finds its leaf (a binary
+// search over the ranges' prefix counts) and loads its record with vector
+// loads, the wave writes the 64 records to its LDS stage, and the pairs are
+// evaluated from there, every lane reading the same record (a broadcast).
+// The next 64 records are loaded while these are evaluated, so a record's
+// memory latency is paid once per 64 leaves, not once per leaf.
+__device__ inline void add_ranges(const float4* __restrict__ crec, const uint32_t* X, int nx, int* pfx,
+                                  float4 (*stage)[3], const PixelF& P, Acc& acc) {
+    const int lane = threadIdx.x & 63;
+    const int c0 = lane < nx ? (int)(X[lane] & 127u) + 1 : 0;
+    const int c1 = lane + 64 < nx ? (int)(X[lane + 64] & 127u) + 1 : 0;
+    const int s0 = wave_inclusive_sum(c0);
+    const int t0 = __builtin_amdgcn_readlane(s0, 63);
+    const int s1 = wave_inclusive_sum(c1) + t0;
+    const int total = __builtin_amdgcn_readlane(s1, 63);
+    if (lane < nx) pfx[lane] = s0 - c0;
+    if (lane + 64 < nx) pfx[lane + 64] = s1 - c1;
+    // the wave's LDS writes come before its reads of other lanes' slots
+    __builtin_amdgcn_wave_barrier();
+    auto fetch = [&](int base, float4& a0, float4& a1, float4& a2) {
+        const int g = base + lane;
+        if (g < total) {
+            int lo = 0, hi = nx - 1;  // the last range that starts at or before g
+            while (lo < hi) {
+                const int mid = (lo + hi + 1) >> 1;
+                if (pfx[mid] <= g) lo = mid;
+                else hi = mid - 1;
+            }
+            const float4* l = crec + kCrec * (size_t)((int)(X[lo] >> 7) + (g - pfx[lo]));
+            a0 = l[0];
+            a1 = l[1];
+            a2 = l[2];
         }
-#pragma unroll
-        for (int k = 0; k < 4; ++k) acc.add_run(a2[k], geom_est(P, a0[k], a1[k]));
+    };
+    float4 b0 = make_float4(0.f, 0.f, 0.f, 0.f), b1 = b0, b2 = b0;
+    fetch(0, b0, b1, b2);
+    for (int base = 0; base < total; base += 64) {
+        stage[lane][0] = b0;
+        stage[lane][1] = b1;
+        stage[lane][2] = b2;
+        __builtin_amdgcn_wave_barrier();
+        if (base + 64 < total) fetch(base + 64, b0, b1, b2);
+        const int cnt = min(64, total - base);
+        // not unrolled: unrolled, the loop holds several leaves' records in
+        // registers and the kernel loses waves per SIMD (measured slower)
+#pragma unroll 1
+        for (int j = 0; j < cnt; ++j) {
+            const float4 r0 = stage[j][0], r1 = stage[j][1], r2 = stage[j][2];
+            acc.add_run(r2, geom_est(P, r0, r1));
+        }
         acc.maybe_flush();
+        __builtin_amdgcn_wave_barrier();  // every lane's reads before the next chunk's writes
     }
-    for (; i < end; ++i) {
-        const float4* l = crec + kCrec * (size_t)i;
-        acc.add_run(l[2], geom_est(P, l[0], l[1]));
-    }
-    acc.maybe_flush();
 }
 
 // One depth-first cut per 8x8 tile (one wave), refined when any of the
@@ -369,7 +411,13 @@ __device__ inline void add_leaves(const float4* __restrict__ crec, int first, in
 // branches, and each pass is a straight loop of one kind of work.
 constexpr int kKeep = 128;
 
-__global__ __launch_bounds__(kCutThreads) void k_cut_mom_tile(CutMom t, const float4* __restrict__ gpos,
+// The tree's arrays are separate read-only, non-aliasing arguments: the
+// compiler then reads the wave-uniform node and moment records with scalar
+// loads.
+__global__ __launch_bounds__(kCutThreads) void k_cut_mom_tile(const arc_bvh_node* __restrict__ nodes,
+                                                              const float4* __restrict__ crec,
+                                                              const float4* __restrict__ mom,
+                                                              const float4* __restrict__ gpos,
                                                               const float4* __restrict__ gnrm,
                                                               const float4* __restrict__ galb,
                                                               const arc_material* __restrict__ mats, int W, int H,
@@ -382,6 +430,8 @@ __global__ __launch_bounds__(kCutThreads) void k_cut_mom_tile(CutMom t, const fl
     __shared__ int stk[kCutThreads / 64][kMomStack];
     __shared__ int qest[kCutThreads / 64][kKeep];       // internal nodes to estimate
     __shared__ uint32_t qex[kCutThreads / 64][kKeep];   // exact ranges: first << 7 | (count - 1)
+    __shared__ int qpfx[kCutThreads / 64][kKeep];       // their leaves' prefix counts
+    __shared__ float4 lstage[kCutThreads / 64][64][3];  // 64 leaf records
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     // the wave's 8x8 tile: the tile grid is (2 gx) x (2 gy) tiles of the
     // launch's gx x gy blocks of 16x16 pixels; with an order, the launch's
@@ -413,13 +463,10 @@ __global__ __launch_bounds__(kCutThreads) void k_cut_mom_tile(CutMom t, const fl
     auto drain = [&]() {
         for (int j = 0; j < ne; ++j) {
             const int k = uniform(E[j]);
-            add_moments(P, t.mom + kMom * (size_t)k, acc);
+            add_moments(P, mom + kMom * (size_t)k, acc);
             acc.maybe_flush();
         }
-        for (int j = 0; j < nx; ++j) {
-            const uint32_t e = (uint32_t)uniform((int)X[j]);
-            add_leaves(t.crec, (int)(e >> 7), (int)(e & 127u) + 1, P, acc);
-        }
+        if (nx > 0) add_ranges(crec, X, nx, qpfx[wave], lstage[wave], P, acc);
         ne = nx = 0;
     };
     int sp = 0;
@@ -428,8 +475,8 @@ __global__ __launch_bounds__(kCutThreads) void k_cut_mom_tile(CutMom t, const fl
         bool pop = true;  // take the next node from the stack (not a refined node's left child)
         if (node < 0) {
             const int k = -(node + 1);
-            const arc_bvh_node& b = t.nodes[k];
-            const float4* r = t.mom + kMom * (size_t)k;
+            const arc_bvh_node& b = nodes[k];
+            const float4* r = mom + kMom * (size_t)k;
             const float4 m0 = r[0], m1 = r[1];  // (c, max I), (s, first, count, -)
             const float s = m1.x;
             const int first = __float_as_int(m1.y), count = __float_as_int(m1.z);
@@ -583,7 +630,8 @@ int cut_moments(arc_gather* g, arc_bvh* b, const CutEBParams& cp, float4* rad, u
         order = g->tile_order.as<int>();
     }
     if (mark) ARC_HIP(ctx, hipEventRecord(mark, ctx->stream));
-    hipLaunchKernelGGL(k_cut_mom_tile, dim3(blocks), dim3(kCutThreads), 0, ctx->stream, tm, g->gpos, g->gnrm, g->galb,
+    hipLaunchKernelGGL(k_cut_mom_tile, dim3(blocks), dim3(kCutThreads), 0, ctx->stream, tm.nodes, tm.crec, tm.mom,
+                       g->gpos, g->gnrm, g->galb,
                        g->mats.as<arc_material>(), g->width, g->height, th, cp.delta, cp.budget, cp.criterion,
                        cp.opening * cp.opening, cp.exact_count, order, g->tile_cost.as<float>(), rad, counters);
     ARC_LAUNCHED(ctx);
