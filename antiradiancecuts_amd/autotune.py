@@ -1,4 +1,4 @@
-"""Per-view autotuner of the gather's schedule knobs.
+"""Autotuner of the gather's schedule knobs, per view and per batch.
 
 A view's frame time depends on knobs that leave its result (nearly) alone:
 the error-bound cut's traversal (one cut per 8x8 tile, or per pixel), the
@@ -54,3 +54,33 @@ def tune(gather, avpls, cam, extent, base: ffi.arc_gather_params | None = None, 
         if ms < best_ms:
             best, best_ms = p, ms
     return best, table
+
+
+def tune_batch(batch, views, reps: int = 3):
+    """The batch's schedule: the views' moment cuts as one launch over all
+    their tiles, or one launch per view.  Captures the views both ways, times
+    `reps` graph launches of each (HIP events on the batch's stream, median
+    after one warm-up launch) and leaves the batch captured with the faster.
+    Returns (one_launch, {mode: median ms})."""
+    import torch
+
+    table = {}
+    for one in (True, False):
+        batch.set_one_cut_launch(one)
+        batch.capture(views)
+        batch.launch()
+        times = []
+        for _ in range(reps):
+            start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            start.record()
+            batch.launch()
+            stop.record()
+            torch.cuda.synchronize()
+            times.append(start.elapsed_time(stop))
+        table["one cut launch" if one else "a cut per view"] = float(np.median(times))
+    best = table["one cut launch"] <= table["a cut per view"]
+    if not best:
+        return False, table
+    batch.set_one_cut_launch(True)
+    batch.capture(views)
+    return True, table

@@ -294,6 +294,34 @@ struct CutMom {
     int n;
 };
 
+// Wave-uniform record reads (the nodes' boxes and moment records, one
+// address for the whole wave) go through the constant address space in the
+// device pass: a uniform load from it is a scalar load (one request for the
+// wave, the values in scalar registers) whatever the compiler can prove
+// about aliasing stores.  The reads are of scalars, so no vector type's
+// constructor meets the address space.
+#if defined(__HIP_DEVICE_COMPILE__)
+#define ARC_UNIFORM_AS __attribute__((address_space(4)))
+#else
+#define ARC_UNIFORM_AS
+#endif
+using ufloat = const ARC_UNIFORM_AS float;
+using uint_ = const ARC_UNIFORM_AS int;
+__device__ inline ufloat* uptr(const void* p) { return (ufloat*)p; }
+__device__ inline float4 uload4(ufloat* p) { return make_float4(p[0], p[1], p[2], p[3]); }
+// A node's box and children (BvhNode layout: left, right, bbMin, bbMax, visited).
+__device__ inline arc_bvh_node uload_node(const arc_bvh_node* n) {
+    ufloat* f = uptr(n);
+    uint_* i = (uint_*)n;
+    arc_bvh_node b;
+    b.left = i[0];
+    b.right = i[1];
+    b.bbMin = arc_float3{f[2], f[3], f[4]};
+    b.bbMax = arc_float3{f[5], f[6], f[7]};
+    b.visited = 0;  // not read
+    return b;
+}
+
 // max(0, n_p . d) / |d|^3 at point x: the pixel-side cosine over the squared
 // distance (cudaGather.cu:45-57, pixel factor).
 __device__ inline float pixel_g(const PixelF& P, const float4& x) {
@@ -305,8 +333,15 @@ __device__ inline float pixel_g(const PixelF& P, const float4& x) {
 }
 
 // The node's estimate for the pixel, per channel, added to the pixel's sums.
+template <bool kUniformAs>
 __device__ inline void add_moments(const PixelF& P, const float4* __restrict__ r, Acc& acc) {
-    const float4 m0 = r[0], p01 = r[2], p12 = r[3], p23 = r[4];
+    // Records are read in three groups, each just before its use: fewer of
+    // them are live at once in scalar registers.
+    auto ld = [&](int i) -> float4 {
+        if constexpr (kUniformAs) return uload4(uptr(r + i));
+        else return r[i];
+    };
+    const float4 m0 = ld(0), p01 = ld(2), p12 = ld(3), p23 = ld(4);
     const float K = 0.25f * (pixel_g(P, make_float4(p01.x, p01.y, p01.z, 0.f)) +
                              pixel_g(P, make_float4(p01.w, p12.x, p12.y, 0.f)) +
                              pixel_g(P, make_float4(p12.z, p12.w, p23.x, 0.f)) +
@@ -316,8 +351,8 @@ __device__ inline void add_moments(const PixelF& P, const float4* __restrict__ r
     const float inv = dd > 1e-30f ? __builtin_amdgcn_rsqf(dd) : 0.f;
     const float i2 = inv * inv;
     const float xx = dx * dx, yy = dy * dy, zz = dz * dz, xy = dx * dy, xz = dx * dz, yz = dy * dz;
-    const float4 mr = r[5], mg = r[6], mb = r[7];
-    const float4 t0 = r[8], t1 = r[9], t2 = r[10], t3 = r[11], t4 = r[12];
+    const float4 mr = ld(5), mg = ld(6), mb = ld(7);
+    const float4 t0 = ld(8), t1 = ld(9), t2 = ld(10), t3 = ld(11), t4 = ld(12);
     const float qr = t0.x * xx + t0.y * yy + t0.z * zz + t0.w * xy + t1.x * xz + t1.y * yz;
     const float qg = t1.z * xx + t1.w * yy + t2.x * zz + t2.y * xy + t2.z * xz + t2.w * yz;
     const float qb = t3.x * xx + t3.y * yy + t3.z * zz + t3.w * xy + t4.x * xz + t4.y * yz;
@@ -328,6 +363,33 @@ __device__ inline void add_moments(const PixelF& P, const float4* __restrict__ r
     acc.s1 += fmaxf(ag, 0.f) * K;
     acc.s2 += fmaxf(ab, 0.f) * K;
     ++acc.run;
+}
+
+// G of one leaf (an AVPL with a unit normal n_a) for the pixel:
+// max(0, n . d) max(0, -n_a . d) / |d|^4 (cudaGather.cu:45-57), from one
+// reciprocal of |d|^2.  With unit normals the cosines need no clamp at 1;
+// coincident points (|d|^2 below 1e-18) give 0, as both cosines vanish there.
+__device__ inline float leaf_g(const PixelF& P, const float4& r0, const float4& r1) {
+    const float dx = r0.x - P.px, dy = r0.y - P.py, dz = r0.z - P.pz;
+    const float dd = fmaxf(dx * dx + dy * dy + dz * dz, 1e-18f);
+    const float u1 = fmaxf(P.nx * dx + P.ny * dy + P.nz * dz, 0.f);
+    const float u2 = fmaxf(-(r1.x * dx + r1.y * dy + r1.z * dz), 0.f);
+    const float r = __builtin_amdgcn_rcpf(dd);
+    return (u1 * u2) * (r * r);
+}
+
+// The Lightcuts bound of a node for the pixel (node_bound) with one
+// transcendental: 1 / dmin^2 as the square of 1 / dmin.
+__device__ inline float node_bound1(const PixelF& P, const arc_bvh_node& b, float ilum, float delta2) {
+    const float ex = fmaxf(fmaxf(b.bbMin.x - P.px, P.px - b.bbMax.x), 0.f);
+    const float ey = fmaxf(fmaxf(b.bbMin.y - P.py, P.py - b.bbMax.y), 0.f);
+    const float ez = fmaxf(fmaxf(b.bbMin.z - P.pz, P.pz - b.bbMax.z), 0.f);
+    const float dmin2 = fmaxf(ex * ex + ey * ey + ez * ez, delta2);
+    const float hmax = P.nx * (P.nx > 0.f ? b.bbMax.x - P.px : b.bbMin.x - P.px) +
+                       P.ny * (P.ny > 0.f ? b.bbMax.y - P.py : b.bbMin.y - P.py) +
+                       P.nz * (P.nz > 0.f ? b.bbMax.z - P.pz : b.bbMin.z - P.pz);
+    const float ri = __builtin_amdgcn_rsqf(dmin2);
+    return hmax > 0.f ? ilum * fminf(1.f, hmax * ri) * (ri * ri) : 0.f;
 }
 
 // Inclusive sum over the wave's lanes.
@@ -391,7 +453,7 @@ __device__ inline void add_ranges(const float4* __restrict__ crec, const uint32_
 #pragma unroll 1
         for (int j = 0; j < cnt; ++j) {
             const float4 r0 = stage[j][0], r1 = stage[j][1], r2 = stage[j][2];
-            acc.add_run(r2, geom_est(P, r0, r1));
+            acc.add_run(r2, leaf_g(P, r0, r1));
         }
         acc.maybe_flush();
         __builtin_amdgcn_wave_barrier();  // every lane's reads before the next chunk's writes
@@ -411,49 +473,41 @@ __device__ inline void add_ranges(const float4* __restrict__ crec, const uint32_
 // branches, and each pass is a straight loop of one kind of work.
 constexpr int kKeep = 128;
 
-// The tree's arrays are separate read-only, non-aliasing arguments: the
-// compiler then reads the wave-uniform node and moment records with scalar
-// loads.
-__global__ __launch_bounds__(kCutThreads) void k_cut_mom_tile(const arc_bvh_node* __restrict__ nodes,
-                                                              const float4* __restrict__ crec,
-                                                              const float4* __restrict__ mom,
-                                                              const float4* __restrict__ gpos,
-                                                              const float4* __restrict__ gnrm,
-                                                              const float4* __restrict__ galb,
-                                                              const arc_material* __restrict__ mats, int W, int H,
-                                                              Threshold th, float delta, int budget, int criterion,
-                                                              float open2, int exact_count,
-                                                              const int* __restrict__ order,
-                                                              float* __restrict__ tile_cost,
-                                                              float4* __restrict__ out_rad,
-                                                              unsigned long long* __restrict__ counters) {
-    __shared__ int stk[kCutThreads / 64][kMomStack];
-    __shared__ int qest[kCutThreads / 64][kKeep];       // internal nodes to estimate
-    __shared__ uint32_t qex[kCutThreads / 64][kKeep];   // exact ranges: first << 7 | (count - 1)
-    __shared__ int qpfx[kCutThreads / 64][kKeep];       // their leaves' prefix counts
-    __shared__ float4 lstage[kCutThreads / 64][64][3];  // 64 leaf records
-    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
-    // the wave's 8x8 tile: the tile grid is (2 gx) x (2 gy) tiles of the
-    // launch's gx x gy blocks of 16x16 pixels; with an order, the launch's
-    // waves take the tiles in that order, else each block its four tiles
-    const int gx = (W + 15) / 16, gy = (H + 15) / 16, tw = 2 * gx;
-    int tx, ty;
-    if (order) {
-        const int ti = uniform(order[blockIdx.x * 4 + wave]);
-        tx = ti % tw;
-        ty = ti / tw;
-    } else {
-        const int b = xcd_block(blockIdx.x, gx * gy);
-        tx = (b % gx) * 2 + (wave & 1);
-        ty = (b / gx) * 2 + (wave >> 1);
-    }
+// The cut of one 8x8 tile (tx, ty) of a view, by one wave, with the wave's
+// LDS: stack S, the two queues E and X, the exact ranges' prefix counts and
+// the leaf stage.  The tree's arrays are read-only, non-aliasing arguments:
+// the compiler then reads the wave-uniform node and moment records with
+// scalar loads.
+// The traversal's node and record reads: through the constant address space
+// (kUniformAs, a copy of the node) or plainly (a reference into the array,
+// so fields are read where they are used).
+template <bool kUniformAs>
+struct NodeRead;
+template <>
+struct NodeRead<false> {
+    __device__ static const arc_bvh_node& node(const arc_bvh_node* __restrict__ n, int k) { return n[k]; }
+    __device__ static float4 rec(const float4* __restrict__ m, int k, int i) { return m[kMom * (size_t)k + i]; }
+};
+template <>
+struct NodeRead<true> {
+    __device__ static arc_bvh_node node(const arc_bvh_node* n, int k) { return uload_node(n + k); }
+    __device__ static float4 rec(const float4* m, int k, int i) { return uload4(uptr(m + kMom * (size_t)k + i)); }
+};
+
+template <bool kUniformAs>
+__device__ __forceinline__ void cut_tile(const arc_bvh_node* __restrict__ nodes, const float4* __restrict__ crec,
+                                         const float4* __restrict__ mom, const float4* __restrict__ gpos,
+                                         const float4* __restrict__ gnrm, const float4* __restrict__ galb,
+                                         const arc_material* __restrict__ mats, int W, int H, const Threshold& th,
+                                         float delta, int budget, int criterion, float open2, int exact_count,
+                                         float* __restrict__ tile_cost, float4* __restrict__ out_rad,
+                                         unsigned long long* __restrict__ counters, int tx, int ty, int tw, int* S,
+                                         int* E, uint32_t* X, int* pfx, float4 (*stage)[3]) {
+    const int lane = threadIdx.x & 63;
     const int2 xy = make_int2(tx * 8 + (lane & 7), ty * 8 + (lane >> 3));
     const bool live = xy.x < W && xy.y < H;
     const int q = live ? xy.y * W + xy.x : 0;
     const PixelF P = load_pixel(gpos, gnrm, galb, mats, q);
-    int* S = stk[wave];
-    int* E = qest[wave];
-    uint32_t* X = qex[wave];
     const float thr = live ? th.at(q) : INFINITY;
     const float delta2 = delta * delta;
     const bool expansion = criterion == ARC_CRITERION_EXPANSION;
@@ -463,10 +517,10 @@ __global__ __launch_bounds__(kCutThreads) void k_cut_mom_tile(const arc_bvh_node
     auto drain = [&]() {
         for (int j = 0; j < ne; ++j) {
             const int k = uniform(E[j]);
-            add_moments(P, mom + kMom * (size_t)k, acc);
+            add_moments<kUniformAs>(P, mom + kMom * (size_t)k, acc);
             acc.maybe_flush();
         }
-        if (nx > 0) add_ranges(crec, X, nx, qpfx[wave], lstage[wave], P, acc);
+        if (nx > 0) add_ranges(crec, X, nx, pfx, stage, P, acc);
         ne = nx = 0;
     };
     int sp = 0;
@@ -475,12 +529,12 @@ __global__ __launch_bounds__(kCutThreads) void k_cut_mom_tile(const arc_bvh_node
         bool pop = true;  // take the next node from the stack (not a refined node's left child)
         if (node < 0) {
             const int k = -(node + 1);
-            const arc_bvh_node& b = nodes[k];
-            const float4* r = mom + kMom * (size_t)k;
-            const float4 m0 = r[0], m1 = r[1];  // (c, max I), (s, first, count, -)
+            const auto& b = NodeRead<kUniformAs>::node(nodes, k);
+            const float4 m0 = NodeRead<kUniformAs>::rec(mom, k, 0);  // (c, max I)
+            const float4 m1 = NodeRead<kUniformAs>::rec(mom, k, 1);  // (s, first, count, -)
             const float s = m1.x;
             const int first = __float_as_int(m1.y), count = __float_as_int(m1.z);
-            const float bound = node_bound(P, b, m0.w, delta2);
+            const float bound = node_bound1(P, b, m0.w, delta2);
             if (!__ballot(bound > 0.f)) {
                 // behind every pixel's tangent plane: contributes exactly 0
             } else if (count <= exact_count) {
@@ -548,6 +602,79 @@ __global__ __launch_bounds__(kCutThreads) void k_cut_mom_tile(const arc_bvh_node
                                  counters ? (float)(nev + nexact) : 1.f);
 }
 
+// One launch per view: the view's tiles, in `order` when given (heaviest
+// first), else each block its four tiles of a 16x16-pixel block.
+__global__ __launch_bounds__(kCutThreads) void k_cut_mom_tile(const arc_bvh_node* __restrict__ nodes,
+                                                              const float4* __restrict__ crec,
+                                                              const float4* __restrict__ mom,
+                                                              const float4* __restrict__ gpos,
+                                                              const float4* __restrict__ gnrm,
+                                                              const float4* __restrict__ galb,
+                                                              const arc_material* __restrict__ mats, int W, int H,
+                                                              Threshold th, float delta, int budget, int criterion,
+                                                              float open2, int exact_count,
+                                                              const int* __restrict__ order,
+                                                              float* __restrict__ tile_cost,
+                                                              float4* __restrict__ out_rad,
+                                                              unsigned long long* __restrict__ counters) {
+    __shared__ int stk[kCutThreads / 64][kMomStack];
+    __shared__ int qest[kCutThreads / 64][kKeep];       // internal nodes to estimate
+    __shared__ uint32_t qex[kCutThreads / 64][kKeep];   // exact ranges: first << 7 | (count - 1)
+    __shared__ int qpfx[kCutThreads / 64][kKeep];       // their leaves' prefix counts
+    __shared__ float4 lstage[kCutThreads / 64][64][3];  // 64 leaf records
+    const int wave = threadIdx.x >> 6;
+    // the wave's 8x8 tile: the tile grid is (2 gx) x (2 gy) tiles of the
+    // launch's gx x gy blocks of 16x16 pixels
+    const int gx = (W + 15) / 16, gy = (H + 15) / 16, tw = 2 * gx;
+    int tx, ty;
+    if (order) {
+        const int ti = uniform(order[blockIdx.x * 4 + wave]);
+        tx = ti % tw;
+        ty = ti / tw;
+    } else {
+        const int b = xcd_block(blockIdx.x, gx * gy);
+        tx = (b % gx) * 2 + (wave & 1);
+        ty = (b / gx) * 2 + (wave >> 1);
+    }
+    cut_tile<false>(nodes, crec, mom, gpos, gnrm, galb, mats, W, H, th, delta, budget, criterion, open2, exact_count,
+             tile_cost, out_rad, counters, tx, ty, tw, stk[wave], qest[wave], qex[wave], qpfx[wave], lstage[wave]);
+}
+
+// One launch over the tiles of many views: the launch's tiles are the views'
+// tile lists one after the other (view v's from views[v].tile_base), taken
+// in `order` when given, so the heaviest tiles of all views start first and
+// no view's tail runs alone.
+__global__ __launch_bounds__(kCutThreads) void k_cut_mom_views(const arc::CutView* __restrict__ views, int nviews,
+                                                               const int* __restrict__ order, int nitems) {
+    __shared__ int stk[kCutThreads / 64][kMomStack];
+    __shared__ int qest[kCutThreads / 64][kKeep];
+    __shared__ uint32_t qex[kCutThreads / 64][kKeep];
+    __shared__ int qpfx[kCutThreads / 64][kKeep];
+    __shared__ float4 lstage[kCutThreads / 64][64][3];
+    const int wave = threadIdx.x >> 6;
+    const int slot = uniform(blockIdx.x * 4 + wave);  // one per wave (the compiler cannot see that)
+    if (slot >= nitems) return;  // the whole wave: the cut has no block barrier
+    const int item = order ? uniform(order[slot]) : slot;
+    int lo = 0, hi = nviews - 1;  // the last view whose tiles start at or before item
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (views[mid].tile_base <= item) lo = mid;
+        else hi = mid - 1;
+    }
+    const arc::CutView& v = views[lo];
+    const int t = item - v.tile_base;
+    const int tw = 2 * ((v.W + 15) / 16);
+    Threshold th;
+    th.est = v.th_est;
+    th.sum = v.th_sum;
+    th.eps = v.th_eps;
+    th.floor_frac = v.th_floor;
+    th.inv_npix = v.th_inv_npix;
+    cut_tile<true>(v.nodes, v.crec, v.mom, v.gpos, v.gnrm, v.galb, v.mats, v.W, v.H, th, v.delta, v.budget, v.criterion,
+             v.open2, v.exact_count, v.tile_cost, v.out_rad, v.counters, t % tw, t / tw, tw, stk[wave], qest[wave],
+             qex[wave], qpfx[wave], lstage[wave]);
+}
+
 // The launch order of the cut's tiles from their costs in the last frame,
 // heaviest first (longest-processing-time order): the tiles' costs spread
 // over two orders of magnitude, and a heavy tile started last runs alone at
@@ -608,7 +735,7 @@ int cut_moments_build(arc_gather* g, arc_bvh* b) {
 }
 
 int cut_moments(arc_gather* g, arc_bvh* b, const CutEBParams& cp, float4* rad, unsigned long long* counters,
-                hipEvent_t mark) {
+                hipEvent_t mark, CutView* defer) {
     arc_ctx* ctx = g->ctx;
     const int n = b->n;
     if (!g->mom_ready || !g->mom.p || g->mom.bytes < sizeof(float4) * kMom * (size_t)(n - 1))
@@ -619,6 +746,13 @@ int cut_moments(arc_gather* g, arc_bvh* b, const CutEBParams& cp, float4* rad, u
     const CutMom tm{b->nodes.as<arc_bvh_node>(), g->crec.as<float4>(), g->mom.as<float4>(), n};
     const int blocks = ((g->width + 15) / 16) * ((g->height + 15) / 16);
     const int ntiles = 4 * blocks;
+    if (defer) {
+        *defer = CutView{tm.nodes, tm.crec, tm.mom, g->gpos, g->gnrm, g->galb, g->mats.as<arc_material>(),
+                         th.est, th.sum, th.eps, th.floor_frac, th.inv_npix, g->width, g->height, cp.delta,
+                         cp.budget, cp.criterion, cp.opening * cp.opening, cp.exact_count, nullptr, rad, counters,
+                         0, ntiles};
+        return ARC_OK;
+    }
     if (g->tile_cost_n != ntiles || g->tile_cost.bytes < sizeof(float) * (size_t)ntiles) g->tile_cost_n = 0;
     ARC_TRY(ensure(ctx, g->tile_cost, sizeof(float) * (size_t)ntiles));
     ARC_TRY(ensure(ctx, g->tile_order, sizeof(int) * (size_t)ntiles));
@@ -636,6 +770,16 @@ int cut_moments(arc_gather* g, arc_bvh* b, const CutEBParams& cp, float4* rad, u
                        cp.opening * cp.opening, cp.exact_count, order, g->tile_cost.as<float>(), rad, counters);
     ARC_LAUNCHED(ctx);
     g->tile_cost_n = ntiles;
+    return ARC_OK;
+}
+
+int cut_moments_views(arc_ctx* ctx, hipStream_t stream, const CutView* views, int nviews, int nitems,
+                      float* costs, int* order, bool ordered) {
+    if (nviews < 1 || nitems < 1) return ARC_OK;
+    if (ordered) hipLaunchKernelGGL(k_tile_order, dim3(1), dim3(kOrderThreads), 0, stream, costs, nitems, order);
+    hipLaunchKernelGGL(k_cut_mom_views, dim3((nitems + 3) / 4), dim3(kCutThreads), 0, stream, views, nviews,
+                       ordered ? order : nullptr, nitems);
+    ARC_LAUNCHED(ctx);
     return ARC_OK;
 }
 

@@ -653,6 +653,9 @@ int arc_gather_run(arc_gather* g, const arc_avpls* avpls, arc_float3 camPos, flo
     if (p->cutEstimator != ARC_ESTIMATOR_CENTROID && p->cutEstimator != ARC_ESTIMATOR_MOMENTS)
         return arc::fail(ctx, ARC_E_INVALID, "cutEstimator");
     const bool mom_est = p->useLightCuts && p->cutMode == ARC_CUT_ERROR_BOUND && p->cutEstimator == ARC_ESTIMATOR_MOMENTS;
+    if (g->defer_cut && (!mom_est || !cut || profile || stats || count))
+        return arc::fail(ctx, ARC_E_INVALID, "a frame whose cut joins a batched launch must be a moment-cut frame "
+                                             "without profiling or statistics");
     if (p->cutCriterion != ARC_CRITERION_BOUND && p->cutCriterion != ARC_CRITERION_ESTIMATE &&
         p->cutCriterion != ARC_CRITERION_EXPANSION)
         return arc::fail(ctx, ARC_E_INVALID, "cutCriterion");
@@ -700,7 +703,8 @@ int arc_gather_run(arc_gather* g, const arc_avpls* avpls, arc_float3 camPos, flo
             if (p->cutMode == ARC_CUT_LEARNED)
                 ARC_TRY(arc::cut_learned(g, b, cp, rad, counters, profile ? ev[6] : nullptr));
             else if (mom_est)
-                ARC_TRY(arc::cut_moments(g, b, cp, rad, counters, profile ? ev[6] : nullptr));
+                ARC_TRY(arc::cut_moments(g, b, cp, rad, counters, profile ? ev[6] : nullptr,
+                                         g->defer_cut ? &g->def_view : nullptr));
             else
                 ARC_TRY(arc::cut_error_bound(g, b, cp, rad, counters, profile ? ev[6] : nullptr));
         } else {
@@ -747,6 +751,15 @@ int arc_gather_run(arc_gather* g, const arc_avpls* avpls, arc_float3 camPos, flo
         ARC_HIP(ctx, hipEventRecord(ev[7], ctx->stream));  // summed with the radiance: no pass of its own
         ARC_HIP(ctx, hipEventRecord(ev[8], ctx->stream));
     }
+    if (g->defer_cut) {
+        // the cut of this frame is launched with other views' (arc_batch);
+        // arc::gather_finish combines once it has run
+        g->def_rad_splits = rad_splits;
+        g->def_anti_splits = anti_splits;
+        g->def_rad = rad;
+        g->deferred = true;
+        return ARC_OK;
+    }
     if (profile) ARC_HIP(ctx, hipEventRecord(ev[4], ctx->stream));
     hipLaunchKernelGGL(k_finalize, dim3(blocks(npix)), dim3(kThreads), 0, ctx->stream, g->part.as<float>(),
                        rad_splits, anti_splits, npix, rad, rad, anti, res);
@@ -786,6 +799,25 @@ int arc_gather_run(arc_gather* g, const arc_avpls* avpls, arc_float3 camPos, flo
     }
     return ARC_OK;
 }
+
+}  // extern "C"
+
+namespace arc {
+
+int gather_finish(arc_gather* g) {
+    if (!g || !g->deferred) return ARC_E_STATE;
+    arc_ctx* ctx = g->ctx;
+    const int npix = g->width * g->height;
+    hipLaunchKernelGGL(k_finalize, dim3(blocks(npix)), dim3(kThreads), 0, ctx->stream, g->part.as<float>(),
+                       g->def_rad_splits, g->def_anti_splits, npix, g->def_rad, g->def_rad, g->out_anti, g->out_res);
+    ARC_LAUNCHED(ctx);
+    g->deferred = false;
+    return ARC_OK;
+}
+
+}  // namespace arc
+
+extern "C" {
 
 int arc_gather_profile_history(arc_gather* g, int frames, float* out) {
     if (!g || frames < 0 || (frames > 0 && !out)) return ARC_E_INVALID;

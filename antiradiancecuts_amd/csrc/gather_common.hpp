@@ -10,6 +10,34 @@
 #include "arc_internal.hpp"
 #include "gather_math.hpp"
 
+namespace arc {
+// One view's moment cut, as a launch over many views reads it: the tree's
+// arrays, the G-buffer, the pixel thresholds (estimate, image sum and the
+// factors of Threshold), the cut's parameters, the outputs, and where the
+// view's tiles start in the launch's list of tiles (and their cost slots).
+struct CutView {
+    const arc_bvh_node* nodes;
+    const float4* crec;
+    const float4* mom;
+    const float4* gpos;
+    const float4* gnrm;
+    const float4* galb;
+    const arc_material* mats;
+    const float* th_est;
+    const float* th_sum;
+    float th_eps, th_floor, th_inv_npix;
+    int W, H;
+    float delta;
+    int budget, criterion;
+    float open2;
+    int exact_count;
+    float* tile_cost;
+    float4* out_rad;
+    unsigned long long* counters;
+    int tile_base, ntiles;
+};
+}  // namespace arc
+
 struct arc_gather {
     arc_ctx* ctx = nullptr;
     int width = 0, height = 0;
@@ -52,6 +80,14 @@ struct arc_gather {
     hipStream_t aux = nullptr;
     hipEvent_t fork = nullptr, join = nullptr;
     int ring_next = 0, ring_count = 0;
+    // one launch of the moment cut over many views (arc_batch): with
+    // defer_cut, a frame stops where its cut kernel would be launched,
+    // leaving the cut's descriptor and the combine's arguments here for
+    // arc::gather_finish
+    bool defer_cut = false, deferred = false;
+    arc::CutView def_view{};
+    int def_rad_splits = 0, def_anti_splits = 0;
+    float4* def_rad = nullptr;
 };
 
 
@@ -253,9 +289,18 @@ int cut_error_bound(arc_gather* g, arc_bvh* b, const CutEBParams& cp, float4* ra
 // bvh_build_cut): FP64 prefix sums over the sorted leaves, then one record
 // per internal node.
 int cut_moments_build(arc_gather* g, arc_bvh* b);
-// The error-bound cut per 8x8 tile with the moment estimator.
+// The error-bound cut per 8x8 tile with the moment estimator.  With `defer`,
+// everything up to the cut kernel is enqueued and the view's descriptor is
+// written there instead of launching it (cut_moments_views launches it).
 int cut_moments(arc_gather* g, arc_bvh* b, const CutEBParams& cp, float4* rad, unsigned long long* counters,
-                hipEvent_t mark);
+                hipEvent_t mark, CutView* defer = nullptr);
+// One launch of the moment cut over the tiles of many views (device array
+// `views`, nitems tiles in all), heaviest tile first by the costs of the
+// last launch when `ordered`; costs and order hold nitems each.
+int cut_moments_views(arc_ctx* ctx, hipStream_t stream, const CutView* views, int nviews, int nitems,
+                      float* costs, int* order, bool ordered);
+// The combine of a frame run with defer_cut, once its cut has run.
+int gather_finish(arc_gather* g);
 // The learned cut: per tile, refine / keep from g->net, the error-bound
 // criterion where the prediction is not confident.
 int cut_learned(arc_gather* g, arc_bvh* b, const CutEBParams& cp, float4* rad, unsigned long long* counters,

@@ -182,6 +182,7 @@ ARC_SYMBOLS = [
     ("arc_batch_create", C.c_int, [vp, C.c_int, P(vp)]),
     ("arc_batch_lane", vp, [vp, C.c_int]),
     ("arc_batch_capture", C.c_int, [vp, vp, C.c_int]),
+    ("arc_batch_set_one_cut_launch", C.c_int, [vp, C.c_int]),
     ("arc_batch_launch", C.c_int, [vp]),
     ("arc_batch_destroy", None, [vp]),
     ("arc_trace_scene_create", C.c_int, [vp, vp, vp, vp, vp, C.c_int, vp, C.c_int, C.c_int, P(vp)]),

@@ -397,6 +397,12 @@ class Batch:
         self._views = (arr, views)  # keep the inputs alive
         check(self.lib.arc_batch_capture(self.h, arr, len(views)), "arc_batch_capture", self.ctx.h)
 
+    def set_one_cut_launch(self, on: bool) -> None:
+        """All views' moment cuts in one launch, or one per view (the
+        default); read by the next capture."""
+        check(self.lib.arc_batch_set_one_cut_launch(self.h, int(bool(on))), "arc_batch_set_one_cut_launch",
+              self.ctx.h)
+
     def launch(self) -> None:
         check(self.lib.arc_batch_launch(self.h), "arc_batch_launch", self.ctx.h)
 

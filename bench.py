@@ -463,12 +463,14 @@ def bench_batch(args) -> int:
     """Batched views in one HIP graph: --views per-view seeded variants of the
     frame workload (Cornell walls + --tris random triangles), each with a
     width and height drawn uniformly from [256, 1920] x [256, 1080] (rounded
-    to 8) and an AVPL count log-uniform over [16K, 1M].  Each view's
-    schedule (error-bound cut per tile or per pixel, estimate-cut depth) is
-    picked by the autotuner on the view itself; then all views are captured
-    into one graph over --lanes streams and a step is one hipGraphLaunch of
-    the whole batch.  Per-view ms are the same views run alone (HIP events,
-    median of 3)."""
+    to 8) and an AVPL count log-uniform over [16K, 1M], at the frame's
+    setting.  Each view's schedule (the estimate-cut depth; for the centroid
+    estimator also the cut per tile or per pixel) is picked by the autotuner
+    on the view itself, and the batch's (its views' moment cuts as one launch
+    over all their tiles, or one launch per view) on the batch; all views are
+    captured into one graph over --lanes streams and a step is one
+    hipGraphLaunch of the whole batch.  Per-view ms are the same views run
+    alone (HIP events, median of 3)."""
     import torch
 
     from antiradiancecuts_amd import autotune, ffi
@@ -501,7 +503,9 @@ def bench_batch(args) -> int:
         views.append((g, av, sc.cam, sc.extent, p))
         print(f"[batch] view {k}: {w}x{h}, {n} AVPLs, alone {alone_ms[-1]:.3f} ms", file=sys.stderr, flush=True)
     t_setup = time.time() - t0
-    b.capture(views)
+    one_launch, batch_table = autotune.tune_batch(b, views) if args.tune else (True, None)
+    if not args.tune:
+        b.capture(views)
     for _ in range(args.warmup):
         b.launch()
     torch.cuda.synchronize()
@@ -526,7 +530,9 @@ def bench_batch(args) -> int:
                        "view_ms_alone_p50": round(float(np.percentile(alone, 50)), 4),
                        "view_ms_alone_p99": round(float(np.percentile(alone, 99)), 4),
                        "sum_view_ms_alone": round(float(alone.sum()), 3),
-                       "tuned_schedules": tuned if args.tune else None, "setup_s": round(t_setup, 1)}}
+                       "tuned_schedules": tuned if args.tune else None,
+                       "one_cut_launch": one_launch, "batch_schedule_ms": batch_table,
+                       "setup_s": round(t_setup, 1)}}
     print(json.dumps(line), flush=True)
     ctx.sync()
     for g, *_ in views:

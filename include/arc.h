@@ -406,6 +406,12 @@ arc_ctx* arc_batch_lane(arc_batch* b, int lane);
  * where they are while the graph is in use; their contents may change.  Each
  * view needs a gather of its own (ARC_E_INVALID when two views share one). */
 int arc_batch_capture(arc_batch* b, const arc_batch_view* views, int nviews);
+/* When every view's radiance is the moment cut (ARC_ESTIMATOR_MOMENTS), the
+ * captured graph runs the views' cuts as one launch over all views' tiles,
+ * heaviest first (on = 1), or one launch per view (on = 0, the default).  No
+ * counterpart in the reference, which renders one view per run; read at the
+ * next arc_batch_capture. */
+int arc_batch_set_one_cut_launch(arc_batch* b, int on);
 /* One graph launch on ctx's stream: a frame of every view.  Asynchronous. */
 int arc_batch_launch(arc_batch* b);
 void arc_batch_destroy(arc_batch* b);

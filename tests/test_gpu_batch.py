@@ -102,6 +102,47 @@ def test_shared_gather_is_rejected(gpu_ctx):
     b.close()
 
 
+def test_one_cut_launch_matches_eager(gpu_ctx):
+    """Views whose radiance is the moment cut: the graph runs their cuts as
+    one launch over all their tiles (heaviest first from the second replay
+    on), and each replay gives every view's eager result."""
+    shapes = [(72, 56, 5000, 21), (200, 120, 60_000, 22), (96, 64, 20_000, 23), (40, 40, 3000, 24),
+              (160, 96, 30_000, 25)]
+    p = default_params(useLightCuts=1, cutMode=ffi.ARC_CUT_ERROR_BOUND, cutPerTile=1,
+                       cutEstimator=ffi.ARC_ESTIMATOR_MOMENTS, cutCriterion=ffi.ARC_CRITERION_BOUND,
+                       cutImageFloor=0.0, antiradianceModel=ffi.ARC_ANTIRAD_CONE,
+                       antiradianceMode=ffi.ARC_ANTI_CLUSTERED)
+    b = Batch(gpu_ctx, lanes=2)
+    views, ref = [], []
+    try:
+        for k, (w, h, n, seed) in enumerate(shapes):
+            sc = S.clutter_1080p(seed=seed, width=w, height=h, num_avpls=n, num_tris=20_000)
+            lane = b.lane(k)
+            g = Gather(lane, w, h, sc.materials)
+            g.set_gbuffer(device_array(sc.pos4), device_array(sc.nrm4), device_array(sc.albedo4))
+            av = DeviceAvpls.from_scene(sc)
+            g.run(av, sc.cam, sc.extent, p)
+            lane.sync()  # the frame ran on the lane's stream
+            ref.append(g.result[:, :3].double().cpu().numpy())
+            views.append((g, av, sc.cam, sc.extent, p))
+        print(f"[batch] one cut launch over {len(views)} views", flush=True)
+        b.set_one_cut_launch(True)
+        b.capture(views)
+        for rep in range(3):
+            for g, *_ in views:
+                g.result.fill_(-1.0)
+            b.launch()
+            gpu_ctx.sync()
+            for k, (g, *_rest) in enumerate(views):
+                e = rel_l2(g.result[:, :3].double().cpu().numpy(), ref[k])
+                print(f"[batch] one cut launch, replay {rep} view {k}: rel-L2 {e:.2e}", flush=True)
+                assert e <= 1e-5
+    finally:
+        for g, *_ in views:
+            g.close()
+        b.close()
+
+
 def test_replays_match_eager(gpu_ctx, rig):
     import torch
 

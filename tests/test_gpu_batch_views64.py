@@ -6,11 +6,13 @@ AVPL count log-uniform over [16K, 1M], all at the workload's setting
 exact, 10-degree cone antiradiance over clustered lists), dealt over four
 lanes and captured into one graph.
 
-Every replay must give each view's eager result: the cut's tiles sum the
-same terms in the same order wherever they run; the clustered antiradiance
-sums its lists in an order the appends leave open, so images agree to 1e-5
-relative L2 rather than bit for bit.  The views' frame times alone and the
-graph launch's time are printed (p50 / p99 over the views).
+Every replay must give each view's eager result, both when the views' cuts
+run as one launch over all their tiles and when each view's cut is a launch
+of its own (the default): the cut's tiles sum the same terms in the same
+order wherever they run; the clustered antiradiance sums its lists in an
+order the appends leave open, so images agree to 1e-5 relative L2 rather
+than bit for bit.  The views' frame times alone and each graph launch's time
+are printed (p50 / p99 over the views).
 """
 import math
 import time
@@ -77,24 +79,10 @@ def test_64_views_replay_matches_eager(gpu_ctx):
             views.append((g, av, sc.cam, sc.extent, p))
             print(f"[views64] view {k}: {w}x{h}, {n} AVPLs, alone {alone[-1]:.3f} ms", flush=True)
         print(f"[views64] setup {time.time() - t0:.1f} s", flush=True)
-        b.capture(views)
-        for rep in range(2):
-            for g, *_ in views:
-                g.result.fill_(-1.0)
-            start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-            start.record()
-            b.launch()
-            stop.record()
-            gpu_ctx.sync()
-            worst = 0.0
-            for k, (g, *_rest) in enumerate(views):
-                e = rel_l2(g.result[:, :3].double().cpu().numpy(), ref[k])
-                worst = max(worst, e)
-                assert e <= 1e-5, (k, e)
-            pairs = float((sizes * counts).sum())
-            ms = start.elapsed_time(stop)
-            print(f"[views64] replay {rep}: {ms:.1f} ms for {NVIEWS} views ({pairs / (ms * 1e-3):.3e} pairs/s), "
-                  f"worst rel-L2 {worst:.2e}", flush=True)
+        for one in (True, False):
+            b.set_one_cut_launch(one)
+            b.capture(views)
+            replay_and_check(gpu_ctx, b, views, ref, sizes, counts, "one cut launch" if one else "a cut per view")
         a = np.array(alone)
         print(f"[views64] views alone: p50 {np.percentile(a, 50):.2f} ms, p99 {np.percentile(a, 99):.2f} ms, "
               f"sum {a.sum():.1f} ms", flush=True)
@@ -102,3 +90,25 @@ def test_64_views_replay_matches_eager(gpu_ctx):
         for g, *_ in views:
             g.close()
         b.close()
+
+
+def replay_and_check(gpu_ctx, b, views, ref, sizes, counts, what):
+    import torch
+
+    for rep in range(2):
+        for g, *_ in views:
+            g.result.fill_(-1.0)
+        start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        start.record()
+        b.launch()
+        stop.record()
+        gpu_ctx.sync()
+        worst = 0.0
+        for k, (g, *_rest) in enumerate(views):
+            e = rel_l2(g.result[:, :3].double().cpu().numpy(), ref[k])
+            worst = max(worst, e)
+            assert e <= 1e-5, (k, e)
+        pairs = float((sizes * counts).sum())
+        ms = start.elapsed_time(stop)
+        print(f"[views64] {what}, replay {rep}: {ms:.1f} ms for {NVIEWS} views "
+              f"({pairs / (ms * 1e-3):.3e} pairs/s), worst rel-L2 {worst:.2e}", flush=True)

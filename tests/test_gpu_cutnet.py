@@ -206,3 +206,53 @@ def test_committed_network_runs_the_learned_cut(gpu_ctx):
     assert e < 0.2
     g.close()
     net.close()
+
+
+def test_committed_network_on_the_full_frame(gpu_ctx):
+    """The shipped weights on the 1920x1080 x 1M-AVPL frame (the inputs the
+    network's training scenes were drawn like): the learned cut runs, decides
+    a share of the nodes itself and falls back to the criterion for the rest,
+    and its radiance stays within a loose bound of the FP64 exact gather on
+    every 8th pixel; error and time beside the error-bound cut at the same
+    errorBound are printed (the equal-error comparison is bench.py's)."""
+    import time
+
+    from antiradiancecuts_amd import learned_cut as L
+
+    sc = S.clutter_1080p()
+    g = Gather(gpu_ctx, sc.width, sc.height, sc.materials)
+    g.set_gbuffer(device_array(sc.pos4), device_array(sc.nrm4), device_array(sc.albedo4))
+    av = DeviceAvpls.from_scene(sc)
+    k = 8
+    ys, xs = np.arange(0, sc.height, k), np.arange(0, sc.width, k)
+    pix = (ys[:, None] * sc.width + xs[None, :]).ravel()
+    sub = S.Scene(**{**sc.__dict__, "width": len(xs), "height": len(ys), "pos4": sc.pos4[pix].copy(),
+                     "nrm4": sc.nrm4[pix].copy(), "albedo4": sc.albedo4[pix].copy()})
+    gs = Gather(gpu_ctx, sub.width, sub.height, sub.materials)
+    gs.set_gbuffer(device_array(sub.pos4), device_array(sub.nrm4), device_array(sub.albedo4))
+    r64, _ = gs.exact_f64(av, sc.cam, sc.extent, default_params())
+    gpu_ctx.sync()
+    r64 = r64.cpu().numpy()
+    gs.close()
+    net = L.load(gpu_ctx)
+    g.set_cutnet(net)
+    out = {}
+    for name, p in (("learned", learned(errorBound=0.01)),
+                    ("error-bound", default_params(useLightCuts=1, cutMode=ffi.ARC_CUT_ERROR_BOUND, errorBound=0.01,
+                                                   cutPerTile=1))):
+        g.run(av, sc.cam, sc.extent, p)
+        gpu_ctx.sync()
+        t0 = time.time()
+        st = g.run(av, sc.cam, sc.extent, p, ffi.ARC_RUN_COUNTERS)
+        gpu_ctx.sync()
+        ms = (time.time() - t0) * 1e3
+        e = rel_l2(g.radiance[:, :3].double().cpu().numpy()[pix], r64)
+        out[name] = (e, st)
+        print(f"[learned] full frame, {name} cut at 0.01: radiance rel-L2 {e:.3e}, evaluated/px "
+              f"{st.evaluatedNodes / sc.npix:.1f}, about {ms:.1f} ms with counters", flush=True)
+    e, st = out["learned"]
+    decisions = (st.visitedNodes + st.evaluatedNodes) / 64.0
+    assert 0 < st.fallbackNodes < decisions
+    assert e < 0.2
+    g.close()
+    net.close()
