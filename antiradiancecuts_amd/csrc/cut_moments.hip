@@ -31,6 +31,7 @@
 // the difference of two prefix rows at its leaf range (the Karras ranges),
 // so no tree climb is needed and the covariance is formed in FP64.
 #include <algorithm>
+#include <climits>
 #include <cmath>
 
 #include "cut_common.hpp"
@@ -473,6 +474,20 @@ __device__ inline void add_ranges(const float4* __restrict__ crec, const uint32_
 // branches, and each pass is a straight loop of one kind of work.
 constexpr int kKeep = 128;
 
+// Kept nodes handed to the matrix-core pass (k_mom_est) instead of being
+// estimated in the traversal: a frame's pool of chunks of kKeep node
+// indices, taken with one atomic per chunk, each chunk linked to the
+// previous chunk of its tile.  A tile whose chunk does not fit estimates it
+// in place, so the pool's size changes only the speed.
+struct EstPool {
+    uint32_t* entries;  // cap chunks of kKeep internal-node indices
+    int2* meta;         // per chunk: (entries, the tile's previous chunk or -1)
+    int* taken;         // chunks taken this frame (may pass cap)
+    int cap;
+    int* head;          // per tile: its last chunk, -1 for none
+    float* queued;      // per tile: the nodes it queued (the order of k_mom_est)
+};
+
 // The cut of one 8x8 tile (tx, ty) of a view, by one wave, with the wave's
 // LDS: stack S, the two queues E and X, the exact ranges' prefix counts and
 // the leaf stage.  The tree's arrays are read-only, non-aliasing arguments:
@@ -494,7 +509,7 @@ struct NodeRead<true> {
     __device__ static float4 rec(const float4* m, int k, int i) { return uload4(uptr(m + kMom * (size_t)k + i)); }
 };
 
-template <bool kUniformAs>
+template <bool kUniformAs, bool kPool>
 __device__ __forceinline__ void cut_tile(const arc_bvh_node* __restrict__ nodes, const float4* __restrict__ crec,
                                          const float4* __restrict__ mom, const float4* __restrict__ gpos,
                                          const float4* __restrict__ gnrm, const float4* __restrict__ galb,
@@ -502,7 +517,8 @@ __device__ __forceinline__ void cut_tile(const arc_bvh_node* __restrict__ nodes,
                                          float delta, int budget, int criterion, float open2, int exact_count,
                                          float* __restrict__ tile_cost, float4* __restrict__ out_rad,
                                          unsigned long long* __restrict__ counters, int tx, int ty, int tw, int* S,
-                                         int* E, uint32_t* X, int* pfx, float4 (*stage)[3]) {
+                                         int* E, uint32_t* X, int* pfx, float4 (*stage)[3],
+                                         const EstPool* __restrict__ pool) {
     const int lane = threadIdx.x & 63;
     const int2 xy = make_int2(tx * 8 + (lane & 7), ty * 8 + (lane >> 3));
     const bool live = xy.x < W && xy.y < H;
@@ -514,14 +530,36 @@ __device__ __forceinline__ void cut_tile(const arc_bvh_node* __restrict__ nodes,
     Acc acc;
     unsigned nvis = 0, nev = 0, nerr = 0, nbud = 0, ntest = 0, nexact = 0;  // wave-uniform
     int ne = 0, nx = 0;                                                     // queued, wave-uniform
-    auto drain = [&]() {
+    int head = -1;                                                          // the tile's last pool chunk
+    unsigned npool = 0;                                                     // nodes handed to the pool
+    // the queued kept nodes: to the pool when there is room, else estimated here
+    auto flush_est = [&]() {
+        if (ne == 0) return;
+        if constexpr (kPool) {
+            int c = 0;
+            if (lane == 0) c = atomicAdd(pool->taken, 1);
+            c = uniform(c);  // lane 0's
+            if (c < pool->cap) {
+                uint32_t* dst = pool->entries + (size_t)c * kKeep;
+                if (lane < ne) dst[lane] = (uint32_t)E[lane];
+                if (lane + 64 < ne) dst[lane + 64] = (uint32_t)E[lane + 64];
+                if (lane == 0) pool->meta[c] = make_int2(ne, head);
+                head = c;
+                npool += ne;
+                ne = 0;
+                return;
+            }
+        }
         for (int j = 0; j < ne; ++j) {
             const int k = uniform(E[j]);
             add_moments<kUniformAs>(P, mom + kMom * (size_t)k, acc);
             acc.maybe_flush();
         }
+        ne = 0;
+    };
+    auto drain_exact = [&]() {
         if (nx > 0) add_ranges(crec, X, nx, pfx, stage, P, acc);
-        ne = nx = 0;
+        nx = 0;
     };
     int sp = 0;
     int node = -1;
@@ -575,19 +613,27 @@ __device__ __forceinline__ void cut_tile(const arc_bvh_node* __restrict__ nodes,
             X[nx++] = (uint32_t)node << 7;  // a leaf: a range of one
             ++nexact;
         }
-        if (ne == kKeep || nx == kKeep) drain();
+        if (ne == kKeep) flush_est();
+        if (nx == kKeep) drain_exact();
         if (pop) {
             if (sp == 0) break;
             node = uniform(S[--sp]);
         }
     }
-    drain();
+    flush_est();
+    drain_exact();
     acc.flush();
     // the tile's cost in VALU instructions (about 30 per bound test, 130 per
-    // moment estimate, 26 per exact pair, and the pixel's own loads and
-    // stores): the next frame's launch order
-    if (lane == 0)
-        tile_cost[ty * tw + tx] = 300.f + 30.f * (float)ntest + 130.f * (float)nev + 26.f * (float)nexact;
+    // moment estimate made here, 26 per exact pair, and the pixel's own loads
+    // and stores): the next frame's launch order
+    if (lane == 0) {
+        tile_cost[ty * tw + tx] = 300.f + 30.f * (float)ntest + 130.f * (float)(nev - npool) +
+                                  26.f * (float)nexact + 2.f * (float)npool;
+        if constexpr (kPool) {
+            pool->head[ty * tw + tx] = head;
+            pool->queued[ty * tw + tx] = (float)npool;
+        }
+    }
     const int alive = __popcll(__ballot(live));  // all lanes take part in the ballot
     if (counters && lane == 0) {
         atomicAdd(&counters[C_TESTED], (unsigned long long)ntest * alive);
@@ -604,6 +650,7 @@ __device__ __forceinline__ void cut_tile(const arc_bvh_node* __restrict__ nodes,
 
 // One launch per view: the view's tiles, in `order` when given (heaviest
 // first), else each block its four tiles of a 16x16-pixel block.
+template <bool kPool>
 __global__ __launch_bounds__(kCutThreads) void k_cut_mom_tile(const arc_bvh_node* __restrict__ nodes,
                                                               const float4* __restrict__ crec,
                                                               const float4* __restrict__ mom,
@@ -616,7 +663,8 @@ __global__ __launch_bounds__(kCutThreads) void k_cut_mom_tile(const arc_bvh_node
                                                               const int* __restrict__ order,
                                                               float* __restrict__ tile_cost,
                                                               float4* __restrict__ out_rad,
-                                                              unsigned long long* __restrict__ counters) {
+                                                              unsigned long long* __restrict__ counters,
+                                                              const EstPool* __restrict__ pool) {
     __shared__ int stk[kCutThreads / 64][kMomStack];
     __shared__ int qest[kCutThreads / 64][kKeep];       // internal nodes to estimate
     __shared__ uint32_t qex[kCutThreads / 64][kKeep];   // exact ranges: first << 7 | (count - 1)
@@ -636,8 +684,9 @@ __global__ __launch_bounds__(kCutThreads) void k_cut_mom_tile(const arc_bvh_node
         tx = (b % gx) * 2 + (wave & 1);
         ty = (b / gx) * 2 + (wave >> 1);
     }
-    cut_tile<false>(nodes, crec, mom, gpos, gnrm, galb, mats, W, H, th, delta, budget, criterion, open2, exact_count,
-             tile_cost, out_rad, counters, tx, ty, tw, stk[wave], qest[wave], qex[wave], qpfx[wave], lstage[wave]);
+    cut_tile<false, kPool>(nodes, crec, mom, gpos, gnrm, galb, mats, W, H, th, delta, budget, criterion, open2,
+                           exact_count, tile_cost, out_rad, counters, tx, ty, tw, stk[wave], qest[wave], qex[wave],
+                           qpfx[wave], lstage[wave], pool);
 }
 
 // One launch over the tiles of many views: the launch's tiles are the views'
@@ -670,9 +719,228 @@ __global__ __launch_bounds__(kCutThreads) void k_cut_mom_views(const arc::CutVie
     th.eps = v.th_eps;
     th.floor_frac = v.th_floor;
     th.inv_npix = v.th_inv_npix;
-    cut_tile<true>(v.nodes, v.crec, v.mom, v.gpos, v.gnrm, v.galb, v.mats, v.W, v.H, th, v.delta, v.budget, v.criterion,
-             v.open2, v.exact_count, v.tile_cost, v.out_rad, v.counters, t % tw, t / tw, tw, stk[wave], qest[wave],
-             qex[wave], qpfx[wave], lstage[wave]);
+    cut_tile<true, false>(v.nodes, v.crec, v.mom, v.gpos, v.gnrm, v.galb, v.mats, v.W, v.H, th, v.delta, v.budget,
+                          v.criterion, v.open2, v.exact_count, v.tile_cost, v.out_rad, v.counters, t % tw, t / tw, tw,
+                          stk[wave], qest[wave], qex[wave], qpfx[wave], lstage[wave], nullptr);
+}
+
+// ---- kept nodes on the matrix cores ---------------------------------------
+//
+// A kept node's estimate for a pixel p (add_moments) is made of dot products
+// of pixel features with node coefficients.  With d = c - p:
+//   |d|^2              = |c|^2 - 2 c . p + |p|^2
+//   n_p . (q_j - p)    = (q_j, -1) . (n_p, n_p . p)          (the four points)
+//   |q_j - p|^2        = |q_j|^2 - 2 q_j . p + |p|^2
+//   (-1/2 M_c) . d     = (mr_c . c, -mr_c) . (1, p)
+//   d^T T'_c d         = a quadratic polynomial in p: 10 coefficients,
+// where T'_c is 15/32 T_c plus 3/32 I_c on its diagonal, which folds the
+// constant term in (3/32 I_c = 3/32 I_c |d|^2 / |d|^2).  Per channel
+//   A_c / 4 = ((-1/2 M_c) . d / |d| + d^T T'_c d / |d|^2) / 4,
+// and the node adds max(A_c / 4, 0) * sum_j max(n_p . (q_j - p), 0) / |q_j - p|^3.
+// For 16 nodes x 16 pixels each product is one 16x16x4 FP32 MFMA (the
+// quadratic form three, K = 12): 21 per block.  What stays on the VALU is
+// the five reciprocal square roots, the clamps and the sums.
+//
+// Coordinates are relative to the tile's first pixel.  The rounding of an
+// expanded square |c|^2 - 2 c . p + |p|^2 is relative to |c|^2 + |p|^2,
+// which stays within a small factor of |d|^2 for a kept node (its bound is
+// small: the node is far against its own size and the tile's).
+//
+// Lane l of v_mfma_f32_16x16x4f32 holds A[l % 16][l / 16], B[l / 16][l % 16]
+// and D[4 (l / 16) + r][l % 16], r = 0..3.  A row is a node (16 per group),
+// a column a pixel (16 per block, four blocks per tile), so lane l of the
+// result holds pixel l % 16 of the block for nodes 4 (l / 16) .. + 3: its sums
+// run over a quarter of the nodes and meet across lanes once at the end.
+typedef float mf4 __attribute__((ext_vector_type(4)));
+
+__device__ inline mf4 mfma4(float a, float b, mf4 c) { return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0); }
+
+constexpr int kEstRows = 21;  // MFMA A operands per node
+constexpr int kEstFeat = 17;  // pixel features (16 in four sets of 4, and |p|^2)
+
+// Row k of the A operands of node record r (kMom float4), relative to o.
+__device__ inline void est_rows(const float4* __restrict__ r, float ox, float oy, float oz, int k,
+                                float (&a)[kEstRows]) {
+    const float4 m0 = r[0];
+    const float cx = m0.x - ox, cy = m0.y - oy, cz = m0.z - oz;
+    // component k - 1 of a vector (rows 1..3)
+    auto comp = [k](float x, float y, float z) { return k == 1 ? x : (k == 2 ? y : z); };
+    a[0] = k == 0 ? cx * cx + cy * cy + cz * cz : -2.f * comp(cx, cy, cz);
+    const float4 p01 = r[2], p12 = r[3], p23 = r[4];
+    const float q[12] = {p01.x, p01.y, p01.z, p01.w, p12.x, p12.y, p12.z, p12.w, p23.x, p23.y, p23.z, p23.w};
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const float qx = q[3 * j] - ox, qy = q[3 * j + 1] - oy, qz = q[3 * j + 2] - oz;
+        a[1 + 2 * j] = k == 0 ? qx : (k == 1 ? qy : (k == 2 ? qz : -1.f));
+        a[2 + 2 * j] = k == 0 ? qx * qx + qy * qy + qz * qz : -2.f * comp(qx, qy, qz);
+    }
+#pragma unroll
+    for (int ch = 0; ch < 3; ++ch) {
+        const float4 mr = r[5 + ch];
+        const float4 ta = r[8 + (6 * ch) / 4], tb = r[8 + (6 * ch + 4) / 4];
+        // the six floats 6 ch .. 6 ch + 5 of records 8..12: (xx, yy, zz, 2xy, 2xz, 2yz)
+        const float tt[8] = {ta.x, ta.y, ta.z, ta.w, tb.x, tb.y, tb.z, tb.w};
+        const int o6 = (6 * ch) % 4;
+        const float al = mr.w;
+        const float txx = tt[o6] + al, tyy = tt[o6 + 1] + al, tzz = tt[o6 + 2] + al;
+        const float txy = tt[o6 + 3], txz = tt[o6 + 4], tyz = tt[o6 + 5];
+        a[9 + ch] = 0.25f * (k == 0 ? mr.x * cx + mr.y * cy + mr.z * cz : -comp(mr.x, mr.y, mr.z));
+        const float lx = -(2.f * txx * cx + txy * cy + txz * cz);
+        const float ly = -(2.f * tyy * cy + txy * cx + tyz * cz);
+        const float lz = -(2.f * tzz * cz + txz * cx + tyz * cy);
+        const float c0 = txx * cx * cx + tyy * cy * cy + tzz * cz * cz + txy * cx * cy + txz * cx * cz + tyz * cy * cz;
+        a[12 + 3 * ch] = 0.25f * (k == 0 ? c0 : comp(lx, ly, lz));
+        a[13 + 3 * ch] = 0.25f * (k == 0 ? txx : (k == 1 ? tyy : (k == 2 ? tzz : txy)));
+        a[14 + 3 * ch] = 0.25f * (k == 0 ? txz : (k == 1 ? tyz : 0.f));
+    }
+}
+
+// The pooled kept nodes of each tile, one wave per tile, added to the
+// traversal's sums in out_rad.  Tiles in `order` when given.
+__global__ __launch_bounds__(kCutThreads) void k_mom_est(const float4* __restrict__ mom,
+                                                         const uint32_t* __restrict__ entries,
+                                                         const int2* __restrict__ meta,
+                                                         const int* __restrict__ head,
+                                                         const float4* __restrict__ gpos,
+                                                         const float4* __restrict__ gnrm,
+                                                         const float4* __restrict__ galb,
+                                                         const arc_material* __restrict__ mats, int W, int H,
+                                                         const int* __restrict__ order,
+                                                         float4* __restrict__ out_rad) {
+    __shared__ float feat[kCutThreads / 64][kEstFeat][64];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int gx = (W + 15) / 16, gy = (H + 15) / 16, tw = 2 * gx;
+    int tx, ty;
+    if (order) {
+        const int ti = uniform(order[blockIdx.x * 4 + wave]);
+        tx = ti % tw;
+        ty = ti / tw;
+    } else {
+        const int b = xcd_block(blockIdx.x, gx * gy);
+        tx = (b % gx) * 2 + (wave & 1);
+        ty = (b / gx) * 2 + (wave >> 1);
+    }
+    int c = uniform(head[ty * tw + tx]);
+    if (c < 0) return;  // the whole wave: nothing pooled (no block barrier below)
+    const int2 xy = make_int2(tx * 8 + (lane & 7), ty * 8 + (lane >> 3));
+    const bool live = xy.x < W && xy.y < H;
+    const int q = live ? xy.y * W + xy.x : 0;
+    const PixelF P = load_pixel(gpos, gnrm, galb, mats, q);
+    // the tile's origin: its first pixel (lane 0's)
+    const float o_x = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(P.px)));
+    const float o_y = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(P.py)));
+    const float o_z = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(P.pz)));
+    {
+        const float px = P.px - o_x, py = P.py - o_y, pz = P.pz - o_z;
+        float (*f)[64] = feat[wave];
+        f[0][lane] = 1.f;
+        f[1][lane] = px;
+        f[2][lane] = py;
+        f[3][lane] = pz;
+        f[4][lane] = P.nx;
+        f[5][lane] = P.ny;
+        f[6][lane] = P.nz;
+        f[7][lane] = P.nx * px + P.ny * py + P.nz * pz;
+        f[8][lane] = px * px;
+        f[9][lane] = py * py;
+        f[10][lane] = pz * pz;
+        f[11][lane] = px * py;
+        f[12][lane] = px * pz;
+        f[13][lane] = py * pz;
+        f[14][lane] = 0.f;
+        f[15][lane] = 0.f;
+        f[16][lane] = px * px + py * py + pz * pz;
+    }
+    __builtin_amdgcn_wave_barrier();  // the wave's feature writes before its reads of other lanes'
+    const int kr = lane >> 4, col = lane & 15;
+    const mf4 zero = {0.f, 0.f, 0.f, 0.f};
+    float tot[4][3];
+#pragma unroll
+    for (int b = 0; b < 4; ++b) tot[b][0] = tot[b][1] = tot[b][2] = 0.f;
+    while (c >= 0) {
+        const int2 m = meta[c];
+        const int cnt = uniform(m.x), prev = uniform(m.y);
+        for (int g0 = 0; g0 < cnt; g0 += 16) {
+            float a[kEstRows];
+            if (g0 + col < cnt) {
+                const int node = (int)entries[(size_t)c * kKeep + g0 + col];
+                est_rows(mom + kMom * (size_t)node, o_x, o_y, o_z, kr, a);
+            } else {
+#pragma unroll
+                for (int i = 0; i < kEstRows; ++i) a[i] = 0.f;  // no node: every product 0
+            }
+#pragma unroll
+            for (int b = 0; b < 4; ++b) {
+                const int px = 16 * b + col;
+                const float (*f)[64] = feat[wave];
+                const float b1 = f[kr][px], b2 = f[4 + kr][px], b4 = f[8 + kr][px], b5 = f[12 + kr][px];
+                const float p2 = f[16][px];
+                mf4 K = zero;
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    const mf4 cs = mfma4(a[1 + 2 * j], b2, zero);
+                    const mf4 dd = mfma4(a[2 + 2 * j], b1, zero);
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) {
+                        const float rr = __builtin_amdgcn_rsqf(fmaxf(dd[r] + p2, 1e-24f));
+                        K[r] = fmaf(fmaxf(cs[r], 0.f), rr * rr * rr, K[r]);
+                    }
+                }
+                const mf4 dc = mfma4(a[0], b1, zero);
+                mf4 inv, i2;
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    inv[r] = __builtin_amdgcn_rsqf(fmaxf(dc[r] + p2, 1e-30f));
+                    i2[r] = inv[r] * inv[r];
+                }
+#pragma unroll
+                for (int ch = 0; ch < 3; ++ch) {
+                    const mf4 n1 = mfma4(a[9 + ch], b1, zero);
+                    mf4 n2 = mfma4(a[12 + 3 * ch], b1, zero);
+                    n2 = mfma4(a[13 + 3 * ch], b4, n2);
+                    n2 = mfma4(a[14 + 3 * ch], b5, n2);
+#pragma unroll
+                    for (int r = 0; r < 4; ++r)
+                        tot[b][ch] = fmaf(fmaxf(fmaf(n1[r], inv[r], n2[r] * i2[r]), 0.f), K[r], tot[b][ch]);
+                }
+                // one block's products at a time: the blocks' results are not
+                // held together
+                __builtin_amdgcn_sched_barrier(0);
+            }
+        }
+        c = prev;
+    }
+    // pixel 16 b + col's sums are spread over lanes col, col + 16, col + 32,
+    // col + 48; lane l keeps block l / 16, its own pixel
+    float out[3] = {0.f, 0.f, 0.f};
+#pragma unroll
+    for (int b = 0; b < 4; ++b)
+#pragma unroll
+        for (int ch = 0; ch < 3; ++ch) {
+            float v = tot[b][ch];
+            v += __shfl_xor(v, 16);
+            v += __shfl_xor(v, 32);
+            if (kr == b) out[ch] = v;
+        }
+    if (live) {
+        float4 o = out_rad[q];
+        o.x += out[0] * P.br;
+        o.y += out[1] * P.bg;
+        o.z += out[2] * P.bb;
+        out_rad[q] = o;
+    }
+}
+
+// A frame's pool: no chunk taken, no tile's chain, and the descriptor the
+// traversal reads (in device memory, so that it is read only where a chunk
+// is handed over).
+__global__ void k_pool_reset(EstPool p, EstPool* __restrict__ desc, int ntiles) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < ntiles) p.head[i] = -1;
+    if (i == 0) {
+        *p.taken = 0;
+        *desc = p;
+    }
 }
 
 // The launch order of the cut's tiles from their costs in the last frame,
@@ -763,13 +1031,46 @@ int cut_moments(arc_gather* g, arc_bvh* b, const CutEBParams& cp, float4* rad, u
                            g->tile_order.as<int>());
         order = g->tile_order.as<int>();
     }
+    // the kept nodes' pool for the matrix-core pass
+    const size_t cap = (size_t)ntiles * (size_t)std::max(g->est_chunks, 0);
+    EstPool pool{};
+    EstPool* desc = nullptr;
+    if (cap > 0) {
+        if (cap > (size_t)INT_MAX) return fail(ctx, ARC_E_INVALID, "estimate pool too large");
+        ARC_TRY(ensure(ctx, g->est_entries, sizeof(uint32_t) * kKeep * cap));
+        // + the chunk counter and the descriptor
+        ARC_TRY(ensure(ctx, g->est_meta, sizeof(int2) * (cap + 2) + sizeof(EstPool)));
+        ARC_TRY(ensure(ctx, g->est_head, sizeof(int) * (size_t)ntiles));
+        ARC_TRY(ensure(ctx, g->est_queued, sizeof(float) * (size_t)ntiles));
+        ARC_TRY(ensure(ctx, g->est_order, sizeof(int) * (size_t)ntiles));
+        int2* meta = g->est_meta.as<int2>();
+        pool = EstPool{g->est_entries.as<uint32_t>(), meta, reinterpret_cast<int*>(meta + cap), (int)cap,
+                       g->est_head.as<int>(), g->est_queued.as<float>()};
+        desc = reinterpret_cast<EstPool*>(meta + cap + 2);
+        hipLaunchKernelGGL(k_pool_reset, dim3((ntiles + 255) / 256), dim3(256), 0, ctx->stream, pool, desc, ntiles);
+    }
     if (mark) ARC_HIP(ctx, hipEventRecord(mark, ctx->stream));
-    hipLaunchKernelGGL(k_cut_mom_tile, dim3(blocks), dim3(kCutThreads), 0, ctx->stream, tm.nodes, tm.crec, tm.mom,
-                       g->gpos, g->gnrm, g->galb,
-                       g->mats.as<arc_material>(), g->width, g->height, th, cp.delta, cp.budget, cp.criterion,
-                       cp.opening * cp.opening, cp.exact_count, order, g->tile_cost.as<float>(), rad, counters);
+    if (cap > 0)
+        hipLaunchKernelGGL(k_cut_mom_tile<true>, dim3(blocks), dim3(kCutThreads), 0, ctx->stream, tm.nodes, tm.crec,
+                           tm.mom, g->gpos, g->gnrm, g->galb, g->mats.as<arc_material>(), g->width, g->height, th,
+                           cp.delta, cp.budget, cp.criterion, cp.opening * cp.opening, cp.exact_count, order,
+                           g->tile_cost.as<float>(), rad, counters, desc);
+    else
+        hipLaunchKernelGGL(k_cut_mom_tile<false>, dim3(blocks), dim3(kCutThreads), 0, ctx->stream, tm.nodes, tm.crec,
+                           tm.mom, g->gpos, g->gnrm, g->galb, g->mats.as<arc_material>(), g->width, g->height, th,
+                           cp.delta, cp.budget, cp.criterion, cp.opening * cp.opening, cp.exact_count, order,
+                           g->tile_cost.as<float>(), rad, counters, desc);
     ARC_LAUNCHED(ctx);
     g->tile_cost_n = ntiles;
+    if (cap > 0) {
+        // this frame's pooled counts order the estimate pass, heaviest tile first
+        hipLaunchKernelGGL(k_tile_order, dim3(1), dim3(kOrderThreads), 0, ctx->stream, pool.queued, ntiles,
+                           g->est_order.as<int>());
+        hipLaunchKernelGGL(k_mom_est, dim3(blocks), dim3(kCutThreads), 0, ctx->stream, tm.mom, pool.entries, pool.meta,
+                           pool.head, g->gpos, g->gnrm, g->galb, g->mats.as<arc_material>(), g->width, g->height,
+                           g->est_order.as<int>(), rad);
+        ARC_LAUNCHED(ctx);
+    }
     return ARC_OK;
 }
 

@@ -547,6 +547,11 @@ void arc_gather_destroy(arc_gather* g) {
     arc::release(g->mom_tot);
     arc::release(g->tile_cost);
     arc::release(g->tile_order);
+    arc::release(g->est_entries);
+    arc::release(g->est_meta);
+    arc::release(g->est_head);
+    arc::release(g->est_queued);
+    arc::release(g->est_order);
     arc::release(g->cest);
     arc_bvh_destroy(g->bvh);
     delete g;
@@ -870,6 +875,14 @@ int arc_gather_set_cutnet(arc_gather* g, arc_cutnet* net) {
     if (net && net->ctx->device != g->ctx->device)
         return arc::fail(g->ctx, ARC_E_INVALID, "cutnet on another device");
     g->net = net;
+    return ARC_OK;
+}
+
+int arc_gather_set_estimate_pool(arc_gather* g, int chunks_per_tile) {
+    if (!g) return ARC_E_INVALID;
+    if (chunks_per_tile < 0 || chunks_per_tile > 4096)
+        return arc::fail(g->ctx, ARC_E_INVALID, "estimate pool: 0..4096 chunks per tile");
+    g->est_chunks = chunks_per_tile;
     return ARC_OK;
 }
 

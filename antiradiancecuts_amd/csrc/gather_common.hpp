@@ -67,6 +67,12 @@ struct arc_gather {
     // tile first; tile_cost_n is the tile count the costs describe (0: none)
     arc::DevBuf tile_cost, tile_order;
     int tile_cost_n = 0;
+    // moment cut: the kept nodes handed from the traversal to the
+    // matrix-core pass, est_chunks chunks of 128 per tile (0, the default:
+    // estimated in the traversal, measured faster), each tile's chain, its
+    // queued count and the pass's order
+    int est_chunks = 0;
+    arc::DevBuf est_entries, est_meta, est_head, est_queued, est_order;
     arc_cutnet* net = nullptr;  // learned cut: the predictor (borrowed)
     arc_bvh* bvh = nullptr;
     // per-frame phase events: frame slot, then 9 marks (start, upload done,

@@ -242,6 +242,12 @@ class Gather:
               self.ctx.h)
         self._net = net
 
+    def set_estimate_pool(self, chunks_per_tile: int) -> None:
+        """The moment cut's pool of kept nodes for the matrix-core pass, in
+        chunks of 128 per tile (0: every node estimated in the traversal)."""
+        check(self.lib.arc_gather_set_estimate_pool(self.h, int(chunks_per_tile)), "arc_gather_set_estimate_pool",
+              self.ctx.h)
+
     def cut_samples(self, avpls: DeviceAvpls, cam, extent: float, params, capacity: int, seed: int = 1,
                     keep: float = 1.0, flags: int = 0):
         """Training rows of this frame (arc_gather_cut_samples): (X, y, rows met)."""

@@ -371,6 +371,17 @@ int arc_cutnet_train(arc_cutnet* net, const float* X, const float* y, const floa
                      int batch, float learning_rate, uint64_t seed, float* loss);
 /* The network the gather's ARC_CUT_LEARNED frames use (borrowed; NULL detaches). */
 int arc_gather_set_cutnet(arc_gather* g, arc_cutnet* net);
+/* No counterpart in the reference (a schedule setting of this library's
+ * moment cut).  With chunks_per_tile > 0 the moment cut's traversal hands its
+ * kept nodes to a second pass that estimates them on the matrix cores,
+ * through a pool of chunks_per_tile chunks of 128 nodes per 8x8 tile (device
+ * memory 512 bytes per chunk).  A tile that finds the pool full estimates its
+ * nodes in the traversal, so the setting changes the speed and the memory,
+ * and the image only by rounding.  0 (the default) estimates every kept node
+ * in the traversal, where the estimates fill the issue slots that the
+ * traversal's dependent loads leave idle.  Read by the next frame.
+ * ARC_E_INVALID for a negative count or more than 4096. */
+int arc_gather_set_estimate_pool(arc_gather* g, int chunks_per_tile);
 /* Training rows of one frame: builds the frame's AvplBvh and thresholds as
  * arc_gather_run does with params (cutMode is ignored), then walks each tile's
  * cut with exact labels (a node is refined exactly when its label is

@@ -150,6 +150,39 @@ def test_tile_order_keeps_the_image(gpu_ctx):
         g.close()
 
 
+@pytest.mark.parametrize("w,h,n", [(200, 120, 60_000), (96, 64, 300_000)])
+def test_estimate_pool_matches_traversal_estimates(gpu_ctx, w, h, n):
+    """Kept nodes estimated on the matrix cores (the default pool), in the
+    traversal (pool 0) and mixed (one chunk per tile: the pool fills and the
+    later tiles estimate in place) give one cut and one image up to
+    rounding; the matrix-core frame repeats bit for bit."""
+    sc = S.clutter_1080p(width=w, height=h, num_avpls=n)
+    print(f"[mom] estimate pool {w}x{h} n={n}", flush=True)
+    g, av = make_rig(gpu_ctx, sc)
+    try:
+        p = mom_params(antiradianceModel=ffi.ARC_ANTIRAD_CONE)
+        g.set_estimate_pool(0)
+        ref, _, st0 = run(gpu_ctx, g, av, sc, p)
+        assert np.isfinite(ref).all() and ref.max() > 0
+        for pool in (32, 1):
+            g.set_estimate_pool(pool)
+            rad, _, st = run(gpu_ctx, g, av, sc, p)
+            e = rel_l2(rad, ref)
+            worst = float(np.max(np.abs(rad - ref)) / np.sqrt(np.mean(ref ** 2)))
+            print(f"[mom] pool {pool}: rel-L2 {e:.2e} vs estimates in the traversal, max {worst:.2e} x rms",
+                  flush=True)
+            assert st.evaluatedNodes == st0.evaluatedNodes and st.testedNodes == st0.testedNodes
+            assert e <= 1e-5 and worst <= 1e-4
+        g.set_estimate_pool(32)
+        first, _, _ = run(gpu_ctx, g, av, sc, p)
+        second, _, _ = run(gpu_ctx, g, av, sc, p)
+        assert np.array_equal(first, second)
+        with pytest.raises(ffi.ArcError):
+            g.set_estimate_pool(-1)
+    finally:
+        g.close()
+
+
 def test_colocated_node_is_the_sh2_sum(gpu_ctx):
     """Three AVPLs at one point with different normals and colours, seen from
     pixels far away: the root is kept (huge bound) and estimated; its four
