@@ -100,6 +100,7 @@ struct arc_bvh {
     arc::DevBuf bounds;                         // 12 floats + validation flag
     arc::DevBuf partials;                       // reduction partials
     arc::DevBuf ready;                          // refit: nodes spanning blocks, ready to merge
+    arc::DevBuf boxes;                          // points refit: chunk and block boxes
     arc::SortScratch sort;
 };
 

@@ -157,7 +157,8 @@ void arc_bvh_destroy(arc_bvh* b) {
     arc::DevBuf* bufs[] = {&b->codes,   &b->perm,    &b->nodes,  &b->parents, &b->ranges,   &b->positions,
                            &b->normals, &b->in_pos,  &b->in_nrm, &b->nd_size, &b->nd_mat,   &b->nd_rand,
                            &b->nd_pos,  &b->nd_nrm,  &b->nd_rad, &b->nd_dir,  &b->nd_merge, &b->staged, &b->bounds,
-                           &b->partials, &b->ready, &b->sort.keys_alt, &b->sort.vals_alt, &b->sort.counts};
+                           &b->partials, &b->ready, &b->boxes, &b->sort.keys_alt, &b->sort.vals_alt,
+                           &b->sort.counts};
     for (arc::DevBuf* d : bufs) arc::release(*d);
     delete b;
 }

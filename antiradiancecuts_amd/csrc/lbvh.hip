@@ -579,6 +579,186 @@ __global__ __launch_bounds__(kRefitLeaves) void k_refit_top(arc_bvh_node* __rest
 }
 
 inline int blocks_for(int n) { return (n + kBlock - 1) / kBlock; }
+// The boxes-only refit (SimpleBvh) as range queries.  A node's box is the
+// min / max over the sorted leaves of its Karras range [first, last], and
+// min / max are exact in any order, so the boxes equal those of the
+// bottom-up merge bit for bit, without a climb or an arrival counter:
+//   k_refit_points_local: one workgroup per kRefitLeaves leaves builds a
+//     segment tree of their points in LDS, writes the box of each kChunk
+//     leaves and of the whole block, and answers the nodes whose range lies
+//     in the block (node k is one end of its range, so it lives there too);
+//     the others go to a list.
+//   k_refit_points_crossing: one wave per listed node: the leaves and chunks
+//     at both ends of its range and the whole blocks between, read by the
+//     lanes side by side and reduced across the wave.
+constexpr int kChunk = 32;                             // leaves per chunk box
+constexpr int kChunksPerBlock = kRefitLeaves / kChunk;  // 32
+
+__device__ inline void grow(f3& lo, f3& hi, const f3& a, const f3& b) {
+    lo = f3{fminf(lo.x, a.x), fminf(lo.y, a.y), fminf(lo.z, a.z)};
+    hi = f3{fmaxf(hi.x, b.x), fmaxf(hi.y, b.y), fmaxf(hi.z, b.z)};
+}
+
+__global__ __launch_bounds__(kRefitLeaves) void k_refit_points_local(arc_bvh_node* __restrict__ nodes,
+                                                                     const int* __restrict__ ranges,
+                                                                     const f3* __restrict__ spos, int n,
+                                                                     f3* __restrict__ chunk_box,
+                                                                     f3* __restrict__ block_box,
+                                                                     int* __restrict__ crossing,
+                                                                     int* __restrict__ ncross) {
+    // segment tree: node 1 the block, nodes kRefitLeaves + t the leaves
+    __shared__ f3 tlo[2 * kRefitLeaves], thi[2 * kRefitLeaves];
+    __shared__ int xlist[kRefitLeaves];  // the block's nodes that cross blocks
+    __shared__ int xcount, xbase;
+    const int t = threadIdx.x;
+    const int lo = blockIdx.x * kRefitLeaves;
+    const int hi = min(n, lo + kRefitLeaves) - 1;
+    const int i = lo + t;
+    constexpr float kInf = __builtin_inff();
+    // internal nodes are 0 .. n - 2: this thread's node's range, read while the tree is built
+    const bool inner = i < n - 1;
+    const int first = inner ? ranges[2 * i] : 0, last = inner ? ranges[2 * i + 1] : 0;
+    if (t == 0) xcount = 0;
+    if (i < n) {
+        const f3 p = spos[i];
+        tlo[kRefitLeaves + t] = p;
+        thi[kRefitLeaves + t] = p;
+    } else {  // beyond the last leaf: neutral
+        tlo[kRefitLeaves + t] = f3{kInf, kInf, kInf};
+        thi[kRefitLeaves + t] = f3{-kInf, -kInf, -kInf};
+    }
+    for (int w = kRefitLeaves / 2; w >= 1; w >>= 1) {
+        __syncthreads();
+        if (t < w) {
+            const int k = w + t;
+            f3 a = tlo[2 * k], b = thi[2 * k];
+            grow(a, b, tlo[2 * k + 1], thi[2 * k + 1]);
+            tlo[k] = a;
+            thi[k] = b;
+        }
+    }
+    __syncthreads();
+    if (t < kChunksPerBlock && lo + t * kChunk < n) {
+        const int c = blockIdx.x * kChunksPerBlock + t;
+        chunk_box[2 * c] = tlo[kChunksPerBlock + t];
+        chunk_box[2 * c + 1] = thi[kChunksPerBlock + t];
+    }
+    if (t == 0) {
+        block_box[2 * blockIdx.x] = tlo[1];
+        block_box[2 * blockIdx.x + 1] = thi[1];
+    }
+    // the crossing nodes: listed in LDS, then appended with one atomic per block
+    const bool cross = inner && (first < lo || last > hi);
+    if (cross) xlist[atomicAdd(&xcount, 1)] = i;
+    __syncthreads();
+    if (t == 0) xbase = xcount > 0 ? atomicAdd(ncross, xcount) : 0;
+    __syncthreads();
+    if (t < xcount) crossing[xbase + t] = xlist[t];
+    if (!inner || cross) return;  // no barrier follows
+    f3 blo{kInf, kInf, kInf}, bhi{-kInf, -kInf, -kInf};
+    int l = first - lo + kRefitLeaves, r = last - lo + kRefitLeaves + 1;  // [l, r)
+    while (l < r) {
+        if (l & 1) {
+            grow(blo, bhi, tlo[l], thi[l]);
+            ++l;
+        }
+        if (r & 1) {
+            --r;
+            grow(blo, bhi, tlo[r], thi[r]);
+        }
+        l >>= 1;
+        r >>= 1;
+    }
+    nodes[i].bbMin = arc_float3{blo.x, blo.y, blo.z};
+    nodes[i].bbMax = arc_float3{bhi.x, bhi.y, bhi.z};
+    nodes[i].visited = 2;
+}
+
+constexpr int kCrossThreads = 256;
+
+__global__ __launch_bounds__(kCrossThreads) void k_refit_points_crossing(arc_bvh_node* __restrict__ nodes,
+                                                                         const int* __restrict__ ranges,
+                                                                         const f3* __restrict__ spos,
+                                                                         const f3* __restrict__ chunk_box,
+                                                                         const f3* __restrict__ block_box,
+                                                                         const int* __restrict__ crossing,
+                                                                         const int* __restrict__ ncross) {
+    const int lane = threadIdx.x & 63;
+    const int nwaves = gridDim.x * (kCrossThreads / 64);
+    const int count = *ncross;
+    constexpr float kInf = __builtin_inff();
+    for (int w = blockIdx.x * (kCrossThreads / 64) + (threadIdx.x >> 6); w < count; w += nwaves) {
+        const int k = crossing[w];
+        const int first = ranges[2 * k], last = ranges[2 * k + 1];
+        // [first, last] spans blocks bf < bl, each full but the last: the
+        // leaves to the end of first's chunk, the rest of block bf's chunks,
+        // the blocks between, block bl's chunks before last's chunk, and the
+        // leaves of last's chunk up to last
+        const int bf = first / kRefitLeaves, bl = last / kRefitLeaves;
+        const int cf = first / kChunk, cl = last / kChunk;
+        const int nA = (cf + 1) * kChunk - first;
+        const int c1 = cf + 1, nB = (bf + 1) * kChunksPerBlock - c1;
+        const int b1 = bf + 1, nC = bl - b1;
+        const int c3 = bl * kChunksPerBlock, nD = cl - c3;
+        const int a4 = cl * kChunk, nE = last - a4 + 1;
+        const int total = nA + nB + nC + nD + nE;
+        f3 blo{kInf, kInf, kInf}, bhi{-kInf, -kInf, -kInf};
+        for (int e = lane; e < total; e += 64) {
+            f3 a, b;
+            if (e < nA) {
+                a = b = spos[first + e];
+            } else if (e < nA + nB) {
+                const int c = c1 + (e - nA);
+                a = chunk_box[2 * c];
+                b = chunk_box[2 * c + 1];
+            } else if (e < nA + nB + nC) {
+                const int q = b1 + (e - nA - nB);
+                a = block_box[2 * q];
+                b = block_box[2 * q + 1];
+            } else if (e < nA + nB + nC + nD) {
+                const int c = c3 + (e - nA - nB - nC);
+                a = chunk_box[2 * c];
+                b = chunk_box[2 * c + 1];
+            } else {
+                a = b = spos[a4 + (e - nA - nB - nC - nD)];
+            }
+            grow(blo, bhi, a, b);
+        }
+#pragma unroll
+        for (int off = 32; off >= 1; off >>= 1) {
+            const f3 olo{__shfl_xor(blo.x, off), __shfl_xor(blo.y, off), __shfl_xor(blo.z, off)};
+            const f3 ohi{__shfl_xor(bhi.x, off), __shfl_xor(bhi.y, off), __shfl_xor(bhi.z, off)};
+            grow(blo, bhi, olo, ohi);
+        }
+        if (lane == 0) {
+            nodes[k].bbMin = arc_float3{blo.x, blo.y, blo.z};
+            nodes[k].bbMax = arc_float3{bhi.x, bhi.y, bhi.z};
+            nodes[k].visited = 2;
+        }
+    }
+}
+
+// Chunk and block boxes of the points refit: 2 f3 per kChunk leaves and per
+// kRefitLeaves leaves.
+inline size_t point_boxes_bytes(int n) {
+    const size_t chunks = ((size_t)n + kChunk - 1) / kChunk, blocks = ((size_t)n + kRefitLeaves - 1) / kRefitLeaves;
+    return sizeof(f3) * 2 * (chunks + blocks);
+}
+
+void launch_refit_points(arc_bvh* b, int n, int* crossing, int* ncross) {
+    const int rblocks = (n + kRefitLeaves - 1) / kRefitLeaves;
+    f3* chunk_box = b->boxes.as<f3>();
+    f3* block_box = chunk_box + 2 * (((size_t)n + kChunk - 1) / kChunk);
+    hipLaunchKernelGGL(k_refit_points_local, dim3(rblocks), dim3(kRefitLeaves), 0, b->ctx->stream,
+                       b->nodes.as<arc_bvh_node>(), b->ranges.as<int>(), b->positions.as<f3>(), n, chunk_box,
+                       block_box, crossing, ncross);
+    // about ten nodes per block cross blocks; a wave takes one at a time
+    const int cblocks = std::max(1, std::min(1024, 2 * rblocks));
+    hipLaunchKernelGGL(k_refit_points_crossing, dim3(cblocks), dim3(kCrossThreads), 0, b->ctx->stream,
+                       b->nodes.as<arc_bvh_node>(), b->ranges.as<int>(), b->positions.as<f3>(), chunk_box,
+                       block_box, crossing, ncross);
+}
+
 
 int ensure_tree(arc_bvh* b, int n) {
     arc_ctx* ctx = b->ctx;
@@ -592,6 +772,7 @@ int ensure_tree(arc_bvh* b, int n) {
     ARC_TRY(arc::ensure(ctx, b->bounds, sizeof(float) * 16));
     ARC_TRY(arc::ensure(ctx, b->ready, sizeof(int) * 2 * (size_t)n));  // two ready lists
     ARC_TRY(arc::ensure(ctx, b->partials, sizeof(float) * 12 * kReduceBlocks));
+    ARC_TRY(arc::ensure(ctx, b->boxes, point_boxes_bytes(n)));
     return ARC_OK;
 }
 
@@ -638,7 +819,7 @@ int build_tree(arc_bvh* b, int n, const NodeData* nd, const CutRecs* cr) {
     const CutRecs c = cr ? *cr : CutRecs{};
     if (nd) launch_refit<kReference>(b, n, d, c, ready, nready);
     else if (cr) launch_refit<kCut>(b, n, d, c, ready, nready);
-    else launch_refit<kBoxes>(b, n, d, c, ready, nready);
+    else launch_refit_points(b, n, ready, nready);
     ARC_LAUNCHED(ctx);
     return ARC_OK;
 }

@@ -2,8 +2,8 @@
 //
 // arc_bvh_build_points (Bvh::create for SimpleBvh, bvh.cu:325-454) keeps its
 // outputs and its scratch (the radix sort's second key/value buffers and
-// digit counts, the bounds partials, the refit's ready lists) in buffers it
-// allocates and grows.  This variant writes into buffers the caller owns and
+// digit counts, the bounds partials, the refit's node list and its chunk and
+// block boxes) in buffers it allocates and grows.  This variant writes into buffers the caller owns and
 // keeps its scratch in `scratch`, a fixed device array of this file: it calls
 // no hipMalloc, hipFree or allocating library routine, so callers that cannot
 // allocate (code under HIP graph capture, tight serving loops, harnesses that
@@ -14,8 +14,9 @@
 // array's device address is looked up once per device, on the first call
 // there, and kept.  Every call on a GPU shares that GPU's scratch: calls that
 // overlap on one GPU race.  Nothing in it has to start at zero: every kernel
-// writes what it reads (the sort's counts and totals, the bounds, the ready
-// lists and their counters, which the tree kernel clears) before reading it.
+// writes what it reads (the sort's counts and totals, the bounds, the node
+// list and its counter, which the tree kernel clears, the chunk and block
+// boxes) before reading it.
 #include <mutex>
 
 #include "arc_internal.hpp"
@@ -34,7 +35,9 @@ constexpr size_t kCounts = align_up(sizeof(uint32_t) * arc::sort_counts_size(kMa
 constexpr size_t kReady = align_up(sizeof(int) * 2 * (size_t)kMaxPoints);
 constexpr size_t kBounds = align_up(sizeof(float) * 16);
 constexpr size_t kPartials = align_up(sizeof(float) * 12 * 1024);
-constexpr size_t kScratchBytes = kKeysAlt + kValsAlt + kCounts + kReady + kBounds + kPartials;
+// the points refit's chunk (32 leaves) and block (1024 leaves) boxes, 24 bytes each
+constexpr size_t kBoxes = align_up(24 * ((size_t)kMaxPoints / 32 + (size_t)kMaxPoints / 1024 + 2));
+constexpr size_t kScratchBytes = kKeysAlt + kValsAlt + kCounts + kReady + kBounds + kPartials + kBoxes;
 
 static __device__ __align__(256) unsigned char scratch[kScratchBytes];
 
@@ -110,6 +113,8 @@ int arc_bvh_build_points_noalloc(arc_ctx* ctx, const arc_float3* positions, cons
     b.bounds = view(s + off, kBounds);
     off += kBounds;
     b.partials = view(s + off, kPartials);
+    off += kPartials;
+    b.boxes = view(s + off, kBoxes);
     const int st = arc::bvh_build_points(&b, reinterpret_cast<const arc::f3*>(positions),
                                          reinterpret_cast<const arc::f3*>(normals), n);
     // the views belong to the caller and to `scratch`: never released here

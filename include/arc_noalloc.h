@@ -6,7 +6,7 @@
  * under HIP graph capture, tight serving loops, harnesses that forbid
  * hipMalloc.  It calls no hipMalloc, hipFree or allocating library routine.
  * The caller passes the inputs (device pointers) and owns the outputs; the
- * build's scratch is a fixed device array of the library, about 16.4 MB per
+ * build's scratch is a fixed device array of the library, about 18.2 MB per
  * GPU, held by each GPU the library runs on until the process ends.
  *
  * Limits: 2 <= n <= ARC_NOALLOC_MAX_POINTS, larger n returns ARC_E_INVALID.
