@@ -254,36 +254,48 @@ __global__ __launch_bounds__(kBlock) void k_random(uint64_t seed, int count, flo
 }
 
 // ---------------------------------------------------------- Karras ----
-// delta (bvh.cu:124-135): -1 outside [0, num_nodes]; on equal codes the
-// common prefix continues into the slot indices.
-__device__ inline int delta(int i, int j, const uint32_t* __restrict__ m, int num_nodes) {
-    if (j < 0 || j > num_nodes) return -1;
-    const uint32_t x = m[i] ^ m[j];
-    return x ? __clz(x) : 32 + __clz((uint32_t)(i ^ j));
-}
-
 __global__ __launch_bounds__(kBlock) void k_karras(const uint32_t* __restrict__ m, int num_nodes,
                                                    arc_bvh_node* __restrict__ nodes,
                                                    int* __restrict__ parents, int* __restrict__ ranges,
                                                    int* __restrict__ nready) {
+    // The searches of most nodes stay within 128 slots of the node (the first
+    // l_max): the block's codes and 256 on each side are read once into LDS,
+    // and a probe outside that window reads global memory.
+    constexpr int kWin = 3 * kBlock;
+    __shared__ uint32_t win[kWin];
+    const int wbase = (int)(blockIdx.x * blockDim.x) - kBlock;
+    for (int k = threadIdx.x; k < kWin; k += blockDim.x) {
+        const int j = wbase + k;
+        win[k] = (j >= 0 && j <= num_nodes) ? m[j] : 0u;
+    }
+    __syncthreads();
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i == 0) nready[0] = nready[1] = 0;  // the refit's two ready-list lengths
-    if (i >= num_nodes) return;
+    if (i >= num_nodes) return;             // no barrier follows
+    const uint32_t mi = win[i - wbase];
+    // delta(i, j) (bvh.cu:124-135): -1 outside [0, num_nodes]; on equal codes
+    // the common prefix continues into the slot indices
+    auto dl = [&](int j) -> int {
+        if (j < 0 || j > num_nodes) return -1;
+        const int k = j - wbase;
+        const uint32_t x = mi ^ ((k >= 0 && k < kWin) ? win[k] : m[j]);
+        return x ? __clz(x) : 32 + __clz((uint32_t)(i ^ j));
+    };
     const int root_slot = num_nodes + 1;
     // direction of the range: sign(0) = +1 (bvh.cu:170-173, 203-204)
-    const int d = (delta(i, i + 1, m, num_nodes) - delta(i, i - 1, m, num_nodes)) < 0 ? -1 : 1;
-    const int dmin = delta(i, i - d, m, num_nodes);
+    const int d = (dl(i + 1) - dl(i - 1)) < 0 ? -1 : 1;
+    const int dmin = dl(i - d);
     int lmax = 128;  // grows by 4x (bvh.cu:208-211)
-    while (delta(i, i + lmax * d, m, num_nodes) > dmin) lmax *= 4;
+    while (dl(i + lmax * d) > dmin) lmax *= 4;
     int l = 0;
     for (int t = lmax / 2; t > 0; t /= 2)
-        if (delta(i, i + (l + t) * d, m, num_nodes) > dmin) l += t;
+        if (dl(i + (l + t) * d) > dmin) l += t;
     const int j = i + l * d;
-    const int dnode = delta(i, j, m, num_nodes);
+    const int dnode = dl(j);
     int s = 0;
     for (int k = 2;; k *= 2) {  // t = ceil(l / k) down to 1 (bvh.cu:226-238)
         const int t = (l + k - 1) / k;
-        if (delta(i, i + (s + t) * d, m, num_nodes) > dnode) s += t;
+        if (dl(i + (s + t) * d) > dnode) s += t;
         if (t <= 1) break;
     }
     const int split = i + s * d + min(d, 0);
