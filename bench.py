@@ -402,7 +402,9 @@ def profiled_traffic(dom: str, eb: bool, per_tile: bool, eval_depth: int = 0, mo
         names = {f"k_anti_clustered<{1 if cone else 0}, false>": 1}
     elif dom == "radiance":
         name = "k_cut_mom_tile" if mom else ("k_cut_eb_tile" if per_tile else "k_cut_eb_px") if eb else "k_cut_sa"
-        if eb and per_tile and f"{name}<{eval_depth}>" in table:
+        if mom and f"{name}<false>" in table:  # the traversal that estimates its kept nodes itself
+            name = f"{name}<false>"
+        elif eb and per_tile and f"{name}<{eval_depth}>" in table:
             name = f"{name}<{eval_depth}>"
         names = {name: 1}
     else:
