@@ -47,7 +47,7 @@ constexpr int kScanBlock = 256;  // leaves per block of the prefix scan
 //   2-4 the four points, 12 floats   5-7 (-1/2 M_c, 3/32 I_c) per channel
 //   8-12 15/32 T_c per channel (xx, yy, zz, 2xy, 2xz, 2yz), 18 floats
 constexpr int kMom = 13;
-constexpr int kMomStack = 96;    // per-wave stack of the tile traversal
+constexpr int kMomStack = 96;    // per-wave stack of the tile traversal (default; 64 and 128 also built)
 
 // Terms of leaf i of the sorted order (crec leaf record: r0 = position,
 // r1 = unit normal, r2 = I), in four groups of ten:
@@ -509,7 +509,7 @@ struct NodeRead<true> {
     __device__ static float4 rec(const float4* m, int k, int i) { return uload4(uptr(m + kMom * (size_t)k + i)); }
 };
 
-template <bool kUniformAs, bool kPool>
+template <bool kUniformAs, bool kPool, int kStack>
 __device__ __forceinline__ void cut_tile(const arc_bvh_node* __restrict__ nodes, const float4* __restrict__ crec,
                                          const float4* __restrict__ mom, const float4* __restrict__ gpos,
                                          const float4* __restrict__ gnrm, const float4* __restrict__ galb,
@@ -594,7 +594,7 @@ __device__ __forceinline__ void cut_tile(const arc_bvh_node* __restrict__ nodes,
                 if (__ballot(refine)) {
                     if ((int)(nev + nexact) >= budget) {
                         ++nbud;  // cut budget spent: keep the cluster
-                    } else if (sp < kMomStack) {
+                    } else if (sp < kStack) {
                         S[sp++] = b.right;
                         node = uniform(b.left);
                         ++nvis;
@@ -650,7 +650,7 @@ __device__ __forceinline__ void cut_tile(const arc_bvh_node* __restrict__ nodes,
 
 // One launch per view: the view's tiles, in `order` when given (heaviest
 // first), else each block its four tiles of a 16x16-pixel block.
-template <bool kPool>
+template <bool kPool, int kStack>
 __global__ __launch_bounds__(kCutThreads) void k_cut_mom_tile(const arc_bvh_node* __restrict__ nodes,
                                                               const float4* __restrict__ crec,
                                                               const float4* __restrict__ mom,
@@ -665,7 +665,7 @@ __global__ __launch_bounds__(kCutThreads) void k_cut_mom_tile(const arc_bvh_node
                                                               float4* __restrict__ out_rad,
                                                               unsigned long long* __restrict__ counters,
                                                               const EstPool* __restrict__ pool) {
-    __shared__ int stk[kCutThreads / 64][kMomStack];
+    __shared__ int stk[kCutThreads / 64][kStack];
     __shared__ int qest[kCutThreads / 64][kKeep];       // internal nodes to estimate
     __shared__ uint32_t qex[kCutThreads / 64][kKeep];   // exact ranges: first << 7 | (count - 1)
     __shared__ int qpfx[kCutThreads / 64][kKeep];       // their leaves' prefix counts
@@ -684,7 +684,7 @@ __global__ __launch_bounds__(kCutThreads) void k_cut_mom_tile(const arc_bvh_node
         tx = (b % gx) * 2 + (wave & 1);
         ty = (b / gx) * 2 + (wave >> 1);
     }
-    cut_tile<false, kPool>(nodes, crec, mom, gpos, gnrm, galb, mats, W, H, th, delta, budget, criterion, open2,
+    cut_tile<false, kPool, kStack>(nodes, crec, mom, gpos, gnrm, galb, mats, W, H, th, delta, budget, criterion, open2,
                            exact_count, tile_cost, out_rad, counters, tx, ty, tw, stk[wave], qest[wave], qex[wave],
                            qpfx[wave], lstage[wave], pool);
 }
@@ -719,7 +719,7 @@ __global__ __launch_bounds__(kCutThreads) void k_cut_mom_views(const arc::CutVie
     th.eps = v.th_eps;
     th.floor_frac = v.th_floor;
     th.inv_npix = v.th_inv_npix;
-    cut_tile<true, false>(v.nodes, v.crec, v.mom, v.gpos, v.gnrm, v.galb, v.mats, v.W, v.H, th, v.delta, v.budget,
+    cut_tile<true, false, kMomStack>(v.nodes, v.crec, v.mom, v.gpos, v.gnrm, v.galb, v.mats, v.W, v.H, th, v.delta, v.budget,
                           v.criterion, v.open2, v.exact_count, v.tile_cost, v.out_rad, v.counters, t % tw, t / tw, tw,
                           stk[wave], qest[wave], qex[wave], qpfx[wave], lstage[wave], nullptr);
 }
@@ -1050,16 +1050,21 @@ int cut_moments(arc_gather* g, arc_bvh* b, const CutEBParams& cp, float4* rad, u
         hipLaunchKernelGGL(k_pool_reset, dim3((ntiles + 255) / 256), dim3(256), 0, ctx->stream, pool, desc, ntiles);
     }
     if (mark) ARC_HIP(ctx, hipEventRecord(mark, ctx->stream));
-    if (cap > 0)
-        hipLaunchKernelGGL(k_cut_mom_tile<true>, dim3(blocks), dim3(kCutThreads), 0, ctx->stream, tm.nodes, tm.crec,
-                           tm.mom, g->gpos, g->gnrm, g->galb, g->mats.as<arc_material>(), g->width, g->height, th,
-                           cp.delta, cp.budget, cp.criterion, cp.opening * cp.opening, cp.exact_count, order,
-                           g->tile_cost.as<float>(), rad, counters, desc);
-    else
-        hipLaunchKernelGGL(k_cut_mom_tile<false>, dim3(blocks), dim3(kCutThreads), 0, ctx->stream, tm.nodes, tm.crec,
-                           tm.mom, g->gpos, g->gnrm, g->galb, g->mats.as<arc_material>(), g->width, g->height, th,
-                           cp.delta, cp.budget, cp.criterion, cp.opening * cp.opening, cp.exact_count, order,
-                           g->tile_cost.as<float>(), rad, counters, desc);
+    auto launch = [&](auto kernel) {
+        hipLaunchKernelGGL(kernel, dim3(blocks), dim3(kCutThreads), 0, ctx->stream, tm.nodes, tm.crec, tm.mom, g->gpos,
+                           g->gnrm, g->galb, g->mats.as<arc_material>(), g->width, g->height, th, cp.delta, cp.budget,
+                           cp.criterion, cp.opening * cp.opening, cp.exact_count, order, g->tile_cost.as<float>(), rad,
+                           counters, desc);
+    };
+    // the traversal's stack depth (its deepest cut): a launch-shape setting
+    switch ((cap > 0 ? 1000 : 0) + g->cut_stack) {
+        case 64: launch(k_cut_mom_tile<false, 64>); break;
+        case 128: launch(k_cut_mom_tile<false, 128>); break;
+        case 1064: launch(k_cut_mom_tile<true, 64>); break;
+        case 1096: launch(k_cut_mom_tile<true, 96>); break;
+        case 1128: launch(k_cut_mom_tile<true, 128>); break;
+        default: launch(k_cut_mom_tile<false, 96>); break;
+    }
     ARC_LAUNCHED(ctx);
     g->tile_cost_n = ntiles;
     if (cap > 0) {

@@ -886,6 +886,17 @@ int arc_gather_set_estimate_pool(arc_gather* g, int chunks_per_tile) {
     return ARC_OK;
 }
 
+int arc_gather_set_launch_shape(arc_gather* g, int cut_stack, int anti_chunk) {
+    if (!g) return ARC_E_INVALID;
+    if (cut_stack != 64 && cut_stack != 96 && cut_stack != 128)
+        return arc::fail(g->ctx, ARC_E_INVALID, "launch shape: cut_stack is 64, 96 or 128");
+    if (anti_chunk != 768 && anti_chunk != 1536 && anti_chunk != 3072)
+        return arc::fail(g->ctx, ARC_E_INVALID, "launch shape: anti_chunk is 768, 1536 or 3072");
+    g->cut_stack = cut_stack;
+    g->anti_chunk = anti_chunk;
+    return ARC_OK;
+}
+
 int arc_gather_cut_samples(arc_gather* g, const arc_avpls* avpls, arc_float3 camPos, float sceneExtent,
                            const arc_gather_params* p, int flags, float* X, float* y, int capacity, uint64_t seed,
                            float keep, int* count) {

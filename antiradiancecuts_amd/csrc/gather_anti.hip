@@ -60,7 +60,8 @@ constexpr int kTiles = (kClusterPx / 8) * (kClusterPx / 8);  // 8x8 tiles per cl
 constexpr int kParts = 4;                                      // blocks per cluster (rows of 32 x 8 px)
 constexpr int kBlockTiles = kTiles / kParts;                   // tiles per block
 constexpr int kAntiThreads = 64 * kBlockTiles;                 // one wave per tile
-constexpr int kSub = 1536;       // tile sub-list entries in LDS: list entries filed per chunk
+constexpr int kSub = 1536;       // tile sub-list entries in LDS: list entries filed per chunk (default;
+                                 // 768 and 3072 also built)
 constexpr int kTreeMaxLevels = 10;  // cluster quadtree levels: up to 512 x 512 clusters
 
 __device__ inline bool has_anti(const arc::AvplsDev& a, int i) {
@@ -333,7 +334,7 @@ __global__ __launch_bounds__(kListThreads) void k_cluster_lists_tree(arc::AvplsD
 // LDS at 25 KB per block, so the occupancy is set by the registers (at least 5 waves
 // per SIMD) and the blocks share the CUs with the cut kernel that runs
 // beside them on the other stream.
-template <int MODEL, bool GENERAL>
+template <int MODEL, bool GENERAL, int kChunk>
 __global__ __launch_bounds__(kAntiThreads) __attribute__((amdgpu_waves_per_eu(5))) void k_anti_clustered(
     const float4* __restrict__ rec, arc::AvplsDev raw, const float4* __restrict__ gpos,
     const float4* __restrict__ gnrm, const float4* __restrict__ galb, const arc_material* __restrict__ mats, int W,
@@ -344,7 +345,7 @@ __global__ __launch_bounds__(kAntiThreads) __attribute__((amdgpu_waves_per_eu(5)
     __shared__ float tface[kBlockTiles][8];  // cone: m xyz, h, s, box centre xyz
     __shared__ int tlive[kBlockTiles];
     __shared__ int tcount[kBlockTiles];
-    __shared__ int sub[kBlockTiles][kSub];
+    __shared__ int sub[kBlockTiles][kChunk];
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const int b = xcd_block(blockIdx.x, gridDim.x);
     const int c = b / kParts, half = b % kParts;
@@ -437,14 +438,14 @@ __global__ __launch_bounds__(kAntiThreads) __attribute__((amdgpu_waves_per_eu(5)
             s0 += r4.x * m; s1 += r4.y * m; s2 += r4.z * m;
         }
     };
-    // The cluster's list is taken kSub entries at a time: the block files
+    // The cluster's list is taken kChunk entries at a time: the block files
     // each chunk into the tiles' sub-lists (a chunk cannot overflow them),
     // then each wave walks its tile's sub-list, so a tile never walks entries
     // its own box test rejected.
     Kahan ka[3];
     unsigned long long walked = 0;
-    for (int cbase = 0; cbase < count; cbase += kSub) {
-        const int cend = min(count, cbase + kSub);
+    for (int cbase = 0; cbase < count; cbase += kChunk) {
+        const int cend = min(count, cbase + kChunk);
         if (lane == 0) tcount[wave] = 0;
         __syncthreads();  // counts cleared; the previous chunk's sub-lists are read
         for (int j = cbase + tid; j < cend; j += kAntiThreads) {
@@ -463,7 +464,7 @@ __global__ __launch_bounds__(kAntiThreads) __attribute__((amdgpu_waves_per_eu(5)
                 }
                 if (tlive[t] && ray_box_hit<MODEL>(p, w, na, ix, iy, iz, tbox[t], grow, cos_t, sin_t)) {
                     const int slot = atomicAdd(&tcount[t], 1);
-                    sub[t][slot] = i;  // slot < cend - cbase <= kSub
+                    sub[t][slot] = i;  // slot < cend - cbase <= kChunk
                 }
             }
         }
@@ -514,10 +515,16 @@ template <int MODEL, bool GENERAL>
 void launch_pixels(arc_gather* g, const arc::AvplsDev& a, int cw, int C, float3 cam, const AntiParams& ap,
                    const int* n_anti_ptr, float grow, float cos_t, float sin_t, float* part,
                    unsigned long long* counters) {
-    hipLaunchKernelGGL((k_anti_clustered<MODEL, GENERAL>), dim3(C * kParts), dim3(kAntiThreads), 0, g->ctx->stream,
-                       g->rec.as<float4>(), a, g->gpos, g->gnrm, g->galb, g->mats.as<arc_material>(), g->width,
-                       g->height, cw, cam.x, cam.y, cam.z, ap, g->ccount.as<int>(), g->clist.as<int>(),
-                       g->anti_idx.as<int>(), n_anti_ptr, grow, cos_t, sin_t, part, counters);
+    auto launch = [&](auto kernel) {
+        hipLaunchKernelGGL(kernel, dim3(C * kParts), dim3(kAntiThreads), 0, g->ctx->stream, g->rec.as<float4>(), a,
+                           g->gpos, g->gnrm, g->galb, g->mats.as<arc_material>(), g->width, g->height, cw, cam.x,
+                           cam.y, cam.z, ap, g->ccount.as<int>(), g->clist.as<int>(), g->anti_idx.as<int>(),
+                           n_anti_ptr, grow, cos_t, sin_t, part, counters);
+    };
+    // list entries filed per pass: a launch-shape setting (LDS against passes)
+    if (g->anti_chunk == 768) launch(k_anti_clustered<MODEL, GENERAL, 768>);
+    else if (g->anti_chunk == 3072) launch(k_anti_clustered<MODEL, GENERAL, 3072>);
+    else launch(k_anti_clustered<MODEL, GENERAL, kSub>);
 }
 
 }  // namespace

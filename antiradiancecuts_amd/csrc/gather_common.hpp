@@ -72,6 +72,10 @@ struct arc_gather {
     // estimated in the traversal, measured faster), each tile's chain, its
     // queued count and the pass's order
     int est_chunks = 0;
+    // launch shapes the results do not depend on: the moment cut's traversal
+    // stack (64, 96 or 128 nodes) and the clustered antiradiance's list
+    // entries filed per pass (768, 1536 or 3072)
+    int cut_stack = 96, anti_chunk = 1536;
     arc::DevBuf est_entries, est_meta, est_head, est_queued, est_order;
     arc_cutnet* net = nullptr;  // learned cut: the predictor (borrowed)
     arc_bvh* bvh = nullptr;

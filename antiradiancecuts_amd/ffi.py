@@ -178,6 +178,7 @@ ARC_SYMBOLS = [
     ("arc_cutnet_train", C.c_int, [vp, vp, vp, vp, C.c_int, C.c_int, C.c_int, C.c_float, C.c_uint64, vp]),
     ("arc_gather_set_cutnet", C.c_int, [vp, vp]),
     ("arc_gather_set_estimate_pool", C.c_int, [vp, C.c_int]),
+    ("arc_gather_set_launch_shape", C.c_int, [vp, C.c_int, C.c_int]),
     ("arc_gather_cut_samples", C.c_int, [vp, P(arc_avpls), arc_float3, C.c_float, P(arc_gather_params), C.c_int,
                                          vp, vp, C.c_int, C.c_uint64, C.c_float, P(C.c_int)]),
     ("arc_batch_create", C.c_int, [vp, C.c_int, P(vp)]),

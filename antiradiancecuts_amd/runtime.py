@@ -248,6 +248,13 @@ class Gather:
         check(self.lib.arc_gather_set_estimate_pool(self.h, int(chunks_per_tile)), "arc_gather_set_estimate_pool",
               self.ctx.h)
 
+    def set_launch_shape(self, cut_stack: int = 96, anti_chunk: int = 1536) -> None:
+        """The moment cut's traversal stack (64, 96, 128 nodes) and the
+        clustered antiradiance's list entries filed per pass (768, 1536,
+        3072); neither changes the image beyond rounding."""
+        check(self.lib.arc_gather_set_launch_shape(self.h, int(cut_stack), int(anti_chunk)),
+              "arc_gather_set_launch_shape", self.ctx.h)
+
     def cut_samples(self, avpls: DeviceAvpls, cam, extent: float, params, capacity: int, seed: int = 1,
                     keep: float = 1.0, flags: int = 0):
         """Training rows of this frame (arc_gather_cut_samples): (X, y, rows met)."""

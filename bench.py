@@ -16,6 +16,7 @@ this host's cores on a bounded sample of the same frame).
 from __future__ import annotations
 
 import argparse
+import dataclasses
 import json
 import math
 import os
@@ -114,7 +115,11 @@ def parse(argv=None):
     ap.add_argument("--views", type=int, default=64, help="batch: views per graph launch")
     ap.add_argument("--lanes", type=int, default=4, help="batch: streams the views are dealt over")
     ap.add_argument("--batch-seed", type=int, default=7, help="batch: seed of the view sizes")
-    ap.add_argument("--tune", type=int, default=1, help="batch: autotune each view's schedule (1) or not (0)")
+    ap.add_argument("--tune", type=int, default=1,
+                    help="autotune the frame's (or each batched view's) schedule before timing (1) or not (0)")
+    ap.add_argument("--schedule", default="",
+                    help="cut_stack,anti_chunk,estimate_pool: a fixed schedule instead of the tuner's "
+                         "(e.g. 96,1536,32 estimates the kept nodes on the matrix cores)")
     return ap.parse_args(argv)
 
 
@@ -172,6 +177,18 @@ def main() -> int:
     cone = args.antiradiance == "cone"
     params = frame_params(args)
     flags = ffi.ARC_RUN_TIMELINE | ffi.ARC_TRUST_INPUT
+    # the schedule: launch shapes and estimate pool, fixed or tuned on the frame (untimed)
+    from antiradiancecuts_amd import autotune
+    tune_table = None
+    if args.schedule:
+        stack, chunk, pool = (int(v) for v in args.schedule.split(","))
+        sched = autotune.Schedule(stack, chunk, pool)
+        sched.apply(g)
+    elif args.tune:
+        params, sched, tune_table = autotune.tune(g, av, sc.cam, sc.extent, params)
+    else:
+        sched = autotune.Schedule()
+        sched.apply(g)
 
     for _ in range(args.warmup):
         g.run(av, sc.cam, sc.extent, params, flags)
@@ -352,6 +369,9 @@ def main() -> int:
             "width": sc.width, "height": sc.height, "avpls": n, "parallelism": "single GPU",
             "phase_ms": {k: round(v, 4) for k, v in phase_ms.items()},
             "kernel_ms": {k: round(v, 4) for k, v in kernel_ms.items()},
+            "schedule": {**dataclasses.asdict(sched), "tuned": tune_table is not None,
+                         "tried_ms": {c: (round(ms, 3) if ms is not None else None) for c, ms in tune_table}
+                         if tune_table else None},
             "cut": {"mode": args.cut, "error_bound": args.error_bound if eb else None,
                     "criterion": args.criterion if eb else None, "estimator": args.estimator if eb else None,
                     "image_floor": args.image_floor if eb else None,
@@ -496,8 +516,9 @@ def bench_batch(args) -> int:
         torch.cuda.synchronize()
         p = base
         if args.tune:
-            p, table = autotune.tune(g, av, sc.cam, sc.extent, base)
-            key = f"tile={p.cutPerTile},d0={p.cutEstimateDepth}"
+            p, sched, table = autotune.tune(g, av, sc.cam, sc.extent, base)
+            key = (f"tile={p.cutPerTile},d0={p.cutEstimateDepth},stack={sched.cut_stack},"
+                   f"chunk={sched.anti_chunk},pool={sched.estimate_pool}")
             tuned[key] = tuned.get(key, 0) + 1
         ms = [g.run(av, sc.cam, sc.extent, p, ffi.ARC_RUN_PROFILE | ffi.ARC_TRUST_INPUT).msTotal for _ in range(3)]
         alone_ms.append(float(np.median(ms)))

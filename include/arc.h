@@ -382,6 +382,16 @@ int arc_gather_set_cutnet(arc_gather* g, arc_cutnet* net);
  * traversal's dependent loads leave idle.  Read by the next frame.
  * ARC_E_INVALID for a negative count or more than 4096. */
 int arc_gather_set_estimate_pool(arc_gather* g, int chunks_per_tile);
+/* No counterpart in the reference (launch shapes of this library's kernels,
+ * searched by its autotuner).  cut_stack: the moment cut's traversal stack,
+ * in nodes, per 8x8 tile (64, 96 or 128; default 96): the deepest cut a tile
+ * can refine to; a tile that needs more keeps the node it cannot refine and
+ * is counted in errorPixels.  anti_chunk: the clustered antiradiance's list
+ * entries filed into the tiles' LDS sub-lists per pass (768, 1536 or 3072;
+ * default 1536): more per pass is fewer passes and more LDS per workgroup.
+ * Neither changes the image beyond rounding while errorPixels stays 0.  Read
+ * by the next frame.  ARC_E_INVALID for other values. */
+int arc_gather_set_launch_shape(arc_gather* g, int cut_stack, int anti_chunk);
 /* Training rows of one frame: builds the frame's AvplBvh and thresholds as
  * arc_gather_run does with params (cutMode is ignored), then walks each tile's
  * cut with exact labels (a node is refined exactly when its label is

@@ -183,6 +183,45 @@ def test_estimate_pool_matches_traversal_estimates(gpu_ctx, w, h, n):
         g.close()
 
 
+def test_launch_shapes_keep_the_image(gpu_ctx):
+    """The traversal stack (64, 96, 128 nodes) and the antiradiance's list
+    entries per pass (768, 1536, 3072) change the launch, not the image: the
+    same cut bit for bit while no tile overflows its stack, the antiradiance
+    to rounding (its entries are summed in the order of the LDS appends).
+    The tuner keeps a schedule without error pixels and the same image."""
+    from antiradiancecuts_amd import autotune
+
+    sc = S.clutter_1080p(width=200, height=120, num_avpls=60_000)
+    print("[mom] launch shapes", flush=True)
+    g, av = make_rig(gpu_ctx, sc)
+    try:
+        p = mom_params(antiradianceModel=ffi.ARC_ANTIRAD_CONE, antiradianceMode=ffi.ARC_ANTI_CLUSTERED)
+        rad0, res0, st0 = run(gpu_ctx, g, av, sc, p)
+        assert st0.errorPixels == 0
+        for stack in (64, 128):
+            g.set_launch_shape(stack, 1536)
+            rad, _, st = run(gpu_ctx, g, av, sc, p)
+            print(f"[mom] stack {stack}: error pixels {st.errorPixels}", flush=True)
+            if st.errorPixels == 0:
+                assert np.array_equal(rad, rad0) and st.evaluatedNodes == st0.evaluatedNodes
+        for chunk in (768, 3072):
+            g.set_launch_shape(96, chunk)
+            rad, res, st = run(gpu_ctx, g, av, sc, p)
+            e = rel_l2(res, res0)
+            print(f"[mom] antiradiance chunk {chunk}: result rel-L2 {e:.2e}", flush=True)
+            assert np.array_equal(rad, rad0) and e <= 1e-6
+        for bad in ((32, 1536), (96, 1000)):
+            with pytest.raises(ffi.ArcError):
+                g.set_launch_shape(*bad)
+        g.set_launch_shape(96, 1536)
+        best, sched, table = autotune.tune(g, av, sc.cam, sc.extent, p, reps=2)
+        print(f"[mom] tuned {sched} from {table}", flush=True)
+        rad, res, st = run(gpu_ctx, g, av, sc, best)
+        assert st.errorPixels == 0 and rel_l2(res, res0) <= 1e-5
+    finally:
+        g.close()
+
+
 def test_colocated_node_is_the_sh2_sum(gpu_ctx):
     """Three AVPLs at one point with different normals and colours, seen from
     pixels far away: the root is kept (huge bound) and estimated; its four

@@ -26,6 +26,16 @@ def test_tuner_candidates_cover_the_schedule_knobs():
     assert all(math.isclose(c.errorBound, 0.07, rel_tol=1e-6) and c.cutEvalDepth == 3 for c in cands)
     assert base.cutEstimateDepth == 6  # copies, not the base itself
     assert len(autotune.candidates(params(useLightCuts=0))) == 1
+    # the moment cut: one parameter set, three launch-shape knobs
+    mom = params(useLightCuts=1, cutMode=ffi.ARC_CUT_ERROR_BOUND, cutEstimator=ffi.ARC_ESTIMATOR_MOMENTS,
+                 antiradianceMode=ffi.ARC_ANTI_CLUSTERED)
+    assert len(autotune.candidates(mom)) == 1
+    axes = dict(autotune.schedule_axes(mom))
+    assert axes == {"cut_stack": (96, 64, 128), "estimate_pool": (0, 32), "anti_chunk": (1536, 768, 3072)}
+    # each axis starts at the library's default
+    d = autotune.Schedule()
+    assert all(getattr(d, k) == v[0] for k, v in axes.items())
+    assert autotune.schedule_axes(params(useLightCuts=0, antiradianceMode=ffi.ARC_ANTI_EXACT)) == []
 
 
 def test_bench_reads_the_traffic_profile():
