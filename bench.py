@@ -419,7 +419,9 @@ def profiled_traffic(dom: str, eb: bool, per_tile: bool, eval_depth: int = 0, mo
         name, _, _, b = line.split("\t")
         table[name] = float(b)
     if dom == "antiradiance":
-        names = {f"k_anti_clustered<{1 if cone else 0}, false>": 1}
+        # the kernel's name, with or without its list-block template argument
+        stem = f"k_anti_clustered<{1 if cone else 0}, false"
+        names = {next((k for k in table if k.startswith(stem)), stem + ">"): 1}
     elif dom == "radiance":
         name = "k_cut_mom_tile" if mom else ("k_cut_eb_tile" if per_tile else "k_cut_eb_px") if eb else "k_cut_sa"
         if mom and f"{name}<false>" in table:  # the traversal that estimates its kept nodes itself
