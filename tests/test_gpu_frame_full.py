@@ -13,6 +13,9 @@ frame in seconds:
   * the gather is linear in the AVPL intensities: doubling every AVPL's
     radiance and antiradiance doubles the result (a power of two is exact in
     FP32, and the cut's thresholds double with the estimates);
+  * the schedule settings (kept nodes on the matrix cores, traversal stacks
+    of 64 and 128, antiradiance blocks of 768 and 3072) give the same image
+    to rounding;
   * the result's error against the FP64 exact gather stays within the level
     measured for each setting: <= 2e-3 for the workload as written (1.73e-3
     measured), <= 1e-3 for the other two (9.2e-4 and 7.8e-4 measured).
@@ -72,6 +75,26 @@ def test_full_frame(gpu_ctx):
     print(f"[frame] cone antiradiance rel-L2 {e_anti:.2e}, result rel-L2 {e_res:.3e}", flush=True)
     assert e_anti <= 1e-5
     assert e_res <= 2e-3
+
+    # the schedule settings at full size: the kept nodes on the matrix cores,
+    # the deepest and shallowest traversal stacks and the other antiradiance
+    # blocks give the same image (to rounding) with no error pixels
+    rad_ref = g.radiance[:, :3].double().cpu().numpy()
+    for stack, chunk, pool in ((96, 1536, 32), (128, 768, 0), (64, 3072, 0)):
+        g.set_launch_shape(stack, chunk)
+        g.set_estimate_pool(pool)
+        st_s = g.run(av, sc.cam, sc.extent, p, ffi.ARC_RUN_COUNTERS)
+        gpu_ctx.sync()
+        e_rad = rel_l2(g.radiance[:, :3].double().cpu().numpy(), rad_ref)
+        e_all = rel_l2(g.result[:, :3].double().cpu().numpy(), res)
+        print(f"[frame] schedule stack {stack}, block {chunk}, pool {pool}: radiance rel-L2 {e_rad:.2e}, "
+              f"result {e_all:.2e}, error pixels {st_s.errorPixels}", flush=True)
+        if st_s.errorPixels == 0:
+            assert e_rad <= 1e-5 and e_all <= 1e-5
+        else:  # a stack too shallow for some tile: the tuner's reason to reject it
+            assert stack == 64
+    g.set_launch_shape(96, 1536)
+    g.set_estimate_pool(0)
 
     # two settings within the 1e-3 target
     for floor in (3.0, 2.0):
