@@ -880,8 +880,8 @@ int arc_gather_set_cutnet(arc_gather* g, arc_cutnet* net) {
 
 int arc_gather_set_estimate_pool(arc_gather* g, int chunks_per_tile) {
     if (!g) return ARC_E_INVALID;
-    if (chunks_per_tile < 0 || chunks_per_tile > 4096)
-        return arc::fail(g->ctx, ARC_E_INVALID, "estimate pool: 0..4096 chunks per tile");
+    if (chunks_per_tile < 0 || chunks_per_tile > 256)
+        return arc::fail(g->ctx, ARC_E_INVALID, "estimate pool: 0..256 chunks per tile");
     g->est_chunks = chunks_per_tile;
     return ARC_OK;
 }

@@ -380,7 +380,7 @@ int arc_gather_set_cutnet(arc_gather* g, arc_cutnet* net);
  * and the image only by rounding.  0 (the default) estimates every kept node
  * in the traversal, where the estimates fill the issue slots that the
  * traversal's dependent loads leave idle.  Read by the next frame.
- * ARC_E_INVALID for a negative count or more than 4096. */
+ * ARC_E_INVALID for a negative count or more than 256 (4.2 GB at 1080p). */
 int arc_gather_set_estimate_pool(arc_gather* g, int chunks_per_tile);
 /* No counterpart in the reference (launch shapes of this library's kernels,
  * searched by its autotuner).  cut_stack: the moment cut's traversal stack,
