@@ -143,6 +143,49 @@ def test_one_cut_launch_matches_eager(gpu_ctx):
         b.close()
 
 
+def test_pooled_estimates_replay_matches_eager(gpu_ctx):
+    """Moment-cut views whose kept nodes go to the matrix-core pass (pool of
+    32 chunks per tile; one view with a pool of one chunk per tile, which
+    fills and leaves the rest to the traversal) and other launch shapes,
+    captured and replayed: the pool is cleared by a kernel inside the graph,
+    so every replay gives each view's eager image."""
+    shapes = [(72, 56, 5000, 31, 32, (96, 1536)), (203, 117, 40_000, 32, 1, (128, 768)),
+              (96, 64, 20_000, 33, 32, (64, 3072))]
+    p = default_params(useLightCuts=1, cutMode=ffi.ARC_CUT_ERROR_BOUND, cutPerTile=1,
+                       cutEstimator=ffi.ARC_ESTIMATOR_MOMENTS, cutCriterion=ffi.ARC_CRITERION_BOUND,
+                       cutImageFloor=0.0, antiradianceModel=ffi.ARC_ANTIRAD_CONE,
+                       antiradianceMode=ffi.ARC_ANTI_CLUSTERED)
+    b = Batch(gpu_ctx, lanes=2)
+    views, ref = [], []
+    try:
+        for k, (w, h, n, seed, pool, shape) in enumerate(shapes):
+            sc = S.clutter_1080p(seed=seed, width=w, height=h, num_avpls=n, num_tris=20_000)
+            lane = b.lane(k)
+            g = Gather(lane, w, h, sc.materials)
+            g.set_gbuffer(device_array(sc.pos4), device_array(sc.nrm4), device_array(sc.albedo4))
+            g.set_estimate_pool(pool)
+            g.set_launch_shape(*shape)
+            av = DeviceAvpls.from_scene(sc)
+            g.run(av, sc.cam, sc.extent, p)
+            lane.sync()
+            ref.append(g.result[:, :3].double().cpu().numpy())
+            views.append((g, av, sc.cam, sc.extent, p))
+        b.capture(views)
+        for rep in range(3):
+            for g, *_ in views:
+                g.result.fill_(-1.0)
+            b.launch()
+            gpu_ctx.sync()
+            for k, (g, *_rest) in enumerate(views):
+                e = rel_l2(g.result[:, :3].double().cpu().numpy(), ref[k])
+                print(f"[batch] pooled estimates, replay {rep} view {k}: rel-L2 {e:.2e}", flush=True)
+                assert e <= 1e-5
+    finally:
+        for g, *_ in views:
+            g.close()
+        b.close()
+
+
 def test_replays_match_eager(gpu_ctx, rig):
     import torch
 

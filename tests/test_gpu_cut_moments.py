@@ -150,7 +150,7 @@ def test_tile_order_keeps_the_image(gpu_ctx):
         g.close()
 
 
-@pytest.mark.parametrize("w,h,n", [(200, 120, 60_000), (96, 64, 300_000)])
+@pytest.mark.parametrize("w,h,n", [(200, 120, 60_000), (96, 64, 300_000), (203, 117, 50_000)])
 def test_estimate_pool_matches_traversal_estimates(gpu_ctx, w, h, n):
     """Kept nodes estimated on the matrix cores (the default pool), in the
     traversal (pool 0) and mixed (one chunk per tile: the pool fills and the
