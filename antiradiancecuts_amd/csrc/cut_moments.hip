@@ -412,8 +412,16 @@ __device__ inline int wave_inclusive_sum(int v) {
 // evaluated from there, every lane reading the same record (a broadcast).
 // The next 64 records are loaded while these are evaluated, so a record's
 // memory latency is paid once per 64 leaves, not once per leaf.
+//
+// A leaf whose AVPL faces away from every pixel of the tile adds 0 to every
+// sum (its cosine max(0, n_a . (p - x_a)) is 0), and about half of the
+// leaves do (a surface's AVPLs seen from behind): each lane tests its leaf
+// against the tile's box tb = (centre, half extent): the largest
+// n_a . (p - x_a) over the box is n_a . (c - x_a) + |n_a| . e.  Only the
+// leaves that can face some pixel, with a margin above FP32 rounding, are
+// staged.
 __device__ inline void add_ranges(const float4* __restrict__ crec, const uint32_t* X, int nx, int* pfx,
-                                  float4 (*stage)[3], const PixelF& P, Acc& acc) {
+                                  float4 (*stage)[3], const float* tb, const PixelF& P, Acc& acc) {
     const int lane = threadIdx.x & 63;
     const int c0 = lane < nx ? (int)(X[lane] & 127u) + 1 : 0;
     const int c1 = lane + 64 < nx ? (int)(X[lane + 64] & 127u) + 1 : 0;
@@ -443,12 +451,26 @@ __device__ inline void add_ranges(const float4* __restrict__ crec, const uint32_
     float4 b0 = make_float4(0.f, 0.f, 0.f, 0.f), b1 = b0, b2 = b0;
     fetch(0, b0, b1, b2);
     for (int base = 0; base < total; base += 64) {
-        stage[lane][0] = b0;
-        stage[lane][1] = b1;
-        stage[lane][2] = b2;
+        bool facing = base + lane < total;
+        if (facing) {
+            const float dx = tb[0] - b0.x, dy = tb[1] - b0.y, dz = tb[2] - b0.z;
+            const float reach = b1.x * dx + b1.y * dy + b1.z * dz + fabsf(b1.x) * tb[3] + fabsf(b1.y) * tb[4] +
+                                fabsf(b1.z) * tb[5];
+            const float margin = 1e-5f * (fabsf(b0.x) + fabsf(b0.y) + fabsf(b0.z) + fabsf(tb[0]) + fabsf(tb[1]) +
+                                          fabsf(tb[2]) + tb[3] + tb[4] + tb[5]);
+            facing = reach > -margin;
+        }
+        const uint64_t fm = __ballot(facing);
+        if (facing) {
+            const int slot = (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(fm >> 32),
+                                                            __builtin_amdgcn_mbcnt_lo((uint32_t)fm, 0u));
+            stage[slot][0] = b0;
+            stage[slot][1] = b1;
+            stage[slot][2] = b2;
+        }
         __builtin_amdgcn_wave_barrier();
         if (base + 64 < total) fetch(base + 64, b0, b1, b2);
-        const int cnt = min(64, total - base);
+        const int cnt = __popcll(fm);
         // not unrolled: unrolled, the loop holds several leaves' records in
         // registers and the kernel loses waves per SIMD (measured slower)
 #pragma unroll 1
@@ -517,7 +539,7 @@ __device__ __forceinline__ void cut_tile(const arc_bvh_node* __restrict__ nodes,
                                          float delta, int budget, int criterion, float open2, int exact_count,
                                          float* __restrict__ tile_cost, float4* __restrict__ out_rad,
                                          unsigned long long* __restrict__ counters, int tx, int ty, int tw, int* S,
-                                         int* E, uint32_t* X, int* pfx, float4 (*stage)[3],
+                                         int* E, uint32_t* X, int* pfx, float4 (*stage)[3], float* tb,
                                          const EstPool* __restrict__ pool) {
     const int lane = threadIdx.x & 63;
     const int2 xy = make_int2(tx * 8 + (lane & 7), ty * 8 + (lane >> 3));
@@ -526,6 +548,27 @@ __device__ __forceinline__ void cut_tile(const arc_bvh_node* __restrict__ nodes,
     const PixelF P = load_pixel(gpos, gnrm, galb, mats, q);
     const float thr = live ? th.at(q) : INFINITY;
     const float delta2 = delta * delta;
+    {  // the box of the tile's live pixels, as centre and half extent, for the exact leaves
+        float lx = live ? P.px : INFINITY, ly = live ? P.py : INFINITY, lz = live ? P.pz : INFINITY;
+        float hx = live ? P.px : -INFINITY, hy = live ? P.py : -INFINITY, hz = live ? P.pz : -INFINITY;
+        for (int off = 32; off > 0; off >>= 1) {
+            lx = fminf(lx, __shfl_xor(lx, off));
+            ly = fminf(ly, __shfl_xor(ly, off));
+            lz = fminf(lz, __shfl_xor(lz, off));
+            hx = fmaxf(hx, __shfl_xor(hx, off));
+            hy = fmaxf(hy, __shfl_xor(hy, off));
+            hz = fmaxf(hz, __shfl_xor(hz, off));
+        }
+        if (lane == 0) {
+            tb[0] = 0.5f * (lx + hx);
+            tb[1] = 0.5f * (ly + hy);
+            tb[2] = 0.5f * (lz + hz);
+            tb[3] = 0.5f * (hx - lx);
+            tb[4] = 0.5f * (hy - ly);
+            tb[5] = 0.5f * (hz - lz);
+        }
+        __builtin_amdgcn_wave_barrier();  // lane 0's writes before the wave's reads
+    }
     const bool expansion = criterion == ARC_CRITERION_EXPANSION;
     Acc acc;
     unsigned nvis = 0, nev = 0, nerr = 0, nbud = 0, ntest = 0, nexact = 0;  // wave-uniform
@@ -558,7 +601,7 @@ __device__ __forceinline__ void cut_tile(const arc_bvh_node* __restrict__ nodes,
         ne = 0;
     };
     auto drain_exact = [&]() {
-        if (nx > 0) add_ranges(crec, X, nx, pfx, stage, P, acc);
+        if (nx > 0) add_ranges(crec, X, nx, pfx, stage, tb, P, acc);
         nx = 0;
     };
     int sp = 0;
@@ -670,6 +713,7 @@ __global__ __launch_bounds__(kCutThreads) void k_cut_mom_tile(const arc_bvh_node
     __shared__ uint32_t qex[kCutThreads / 64][kKeep];   // exact ranges: first << 7 | (count - 1)
     __shared__ int qpfx[kCutThreads / 64][kKeep];       // their leaves' prefix counts
     __shared__ float4 lstage[kCutThreads / 64][64][3];  // 64 leaf records
+    __shared__ float tbox[kCutThreads / 64][8];         // the tile's box: centre, half extent
     const int wave = threadIdx.x >> 6;
     // the wave's 8x8 tile: the tile grid is (2 gx) x (2 gy) tiles of the
     // launch's gx x gy blocks of 16x16 pixels
@@ -686,7 +730,7 @@ __global__ __launch_bounds__(kCutThreads) void k_cut_mom_tile(const arc_bvh_node
     }
     cut_tile<false, kPool, kStack>(nodes, crec, mom, gpos, gnrm, galb, mats, W, H, th, delta, budget, criterion, open2,
                            exact_count, tile_cost, out_rad, counters, tx, ty, tw, stk[wave], qest[wave], qex[wave],
-                           qpfx[wave], lstage[wave], pool);
+                           qpfx[wave], lstage[wave], tbox[wave], pool);
 }
 
 // One launch over the tiles of many views: the launch's tiles are the views'
@@ -700,6 +744,7 @@ __global__ __launch_bounds__(kCutThreads) void k_cut_mom_views(const arc::CutVie
     __shared__ uint32_t qex[kCutThreads / 64][kKeep];
     __shared__ int qpfx[kCutThreads / 64][kKeep];
     __shared__ float4 lstage[kCutThreads / 64][64][3];
+    __shared__ float tbox[kCutThreads / 64][8];
     const int wave = threadIdx.x >> 6;
     const int slot = uniform(blockIdx.x * 4 + wave);  // one per wave (the compiler cannot see that)
     if (slot >= nitems) return;  // the whole wave: the cut has no block barrier
@@ -721,7 +766,7 @@ __global__ __launch_bounds__(kCutThreads) void k_cut_mom_views(const arc::CutVie
     th.inv_npix = v.th_inv_npix;
     cut_tile<true, false, kMomStack>(v.nodes, v.crec, v.mom, v.gpos, v.gnrm, v.galb, v.mats, v.W, v.H, th, v.delta, v.budget,
                           v.criterion, v.open2, v.exact_count, v.tile_cost, v.out_rad, v.counters, t % tw, t / tw, tw,
-                          stk[wave], qest[wave], qex[wave], qpfx[wave], lstage[wave], nullptr);
+                          stk[wave], qest[wave], qex[wave], qpfx[wave], lstage[wave], tbox[wave], nullptr);
 }
 
 // ---- kept nodes on the matrix cores ---------------------------------------
