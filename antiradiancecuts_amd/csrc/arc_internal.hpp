@@ -112,6 +112,10 @@ struct AvplsDev {
     int n;
 };
 int bvh_build_points(arc_bvh* b, const f3* pos, const f3* nrm, int n);
+// Per internal node of the last build, the box of its leaves' unit normals
+// (the light cuts' leaf records, crec slots [0, n), record 1): nbox holds
+// (lo, 0), (hi, 0) per node.
+int node_normal_boxes(arc_bvh* b, const float4* crec, float4* nbox);
 int bvh_build_avpls(arc_bvh* b, const AvplsDev& a, const float* rand, uint64_t seed, bool validate);
 
 // The light cuts' node records, kCrec float4 (one 64-byte line) per slot,

@@ -533,7 +533,8 @@ struct NodeRead<true> {
 
 template <bool kUniformAs, bool kPool, int kStack>
 __device__ __forceinline__ void cut_tile(const arc_bvh_node* __restrict__ nodes, const float4* __restrict__ crec,
-                                         const float4* __restrict__ mom, const float4* __restrict__ gpos,
+                                         const float4* __restrict__ mom, const float4* __restrict__ nbox,
+                                         const float4* __restrict__ gpos,
                                          const float4* __restrict__ gnrm, const float4* __restrict__ galb,
                                          const arc_material* __restrict__ mats, int W, int H, const Threshold& th,
                                          float delta, int budget, int criterion, float open2, int exact_count,
@@ -577,6 +578,44 @@ __device__ __forceinline__ void cut_tile(const arc_bvh_node* __restrict__ nodes,
     unsigned npool = 0;                                                     // nodes handed to the pool
     // the queued kept nodes: to the pool when there is room, else estimated here
     auto flush_est = [&]() {
+        if (ne == 0) return;
+        // A kept node whose AVPLs all face away from every pixel of the tile
+        // adds 0 (its estimate would add only the clamped cosine's
+        // second-order residue): the largest n_a . (p - x_a) over the
+        // node's normal box, its box and the tile's box, by interval
+        // products per axis, is below 0 by more than FP32 rounding.  One
+        // lane per queued node; the facing ones are compacted in place.
+        int kept = 0;
+        for (int base = 0; base < ne; base += 64) {
+            const int j = base + lane;
+            bool facing = false;
+            int k = 0;
+            if (j < ne) {
+                k = E[j];
+                const arc_bvh_node& nd = nodes[k];
+                const float4 nl = nbox[2 * (size_t)k], nh = nbox[2 * (size_t)k + 1];
+                float reach = 0.f, scale = 0.f;
+                auto axis = [&](float c, float e, float plo, float phi, float nlo, float nhi) {
+                    const float vlo = (c - e) - phi, vhi = (c + e) - plo;  // p - x over the boxes
+                    reach += fmaxf(fmaxf(nlo * vlo, nlo * vhi), fmaxf(nhi * vlo, nhi * vhi));
+                    scale += fabsf(c) + e + fabsf(plo) + fabsf(phi);
+                };
+                axis(tb[0], tb[3], nd.bbMin.x, nd.bbMax.x, nl.x, nh.x);
+                axis(tb[1], tb[4], nd.bbMin.y, nd.bbMax.y, nl.y, nh.y);
+                axis(tb[2], tb[5], nd.bbMin.z, nd.bbMax.z, nl.z, nh.z);
+                facing = reach > -1e-5f * scale;
+            }
+            const uint64_t fm = __ballot(facing);
+            // every lane has read its entry before any is overwritten (kept <= base)
+            __builtin_amdgcn_wave_barrier();
+            if (facing)
+                E[kept + (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(fm >> 32),
+                                                        __builtin_amdgcn_mbcnt_lo((uint32_t)fm, 0u))] = k;
+            kept += __popcll(fm);
+            __builtin_amdgcn_wave_barrier();
+        }
+        nev -= (unsigned)(ne - kept);
+        ne = kept;
         if (ne == 0) return;
         if constexpr (kPool) {
             int c = 0;
@@ -697,6 +736,7 @@ template <bool kPool, int kStack>
 __global__ __launch_bounds__(kCutThreads) void k_cut_mom_tile(const arc_bvh_node* __restrict__ nodes,
                                                               const float4* __restrict__ crec,
                                                               const float4* __restrict__ mom,
+                                                              const float4* __restrict__ nbox,
                                                               const float4* __restrict__ gpos,
                                                               const float4* __restrict__ gnrm,
                                                               const float4* __restrict__ galb,
@@ -728,7 +768,7 @@ __global__ __launch_bounds__(kCutThreads) void k_cut_mom_tile(const arc_bvh_node
         tx = (b % gx) * 2 + (wave & 1);
         ty = (b / gx) * 2 + (wave >> 1);
     }
-    cut_tile<false, kPool, kStack>(nodes, crec, mom, gpos, gnrm, galb, mats, W, H, th, delta, budget, criterion, open2,
+    cut_tile<false, kPool, kStack>(nodes, crec, mom, nbox, gpos, gnrm, galb, mats, W, H, th, delta, budget, criterion, open2,
                            exact_count, tile_cost, out_rad, counters, tx, ty, tw, stk[wave], qest[wave], qex[wave],
                            qpfx[wave], lstage[wave], tbox[wave], pool);
 }
@@ -764,7 +804,7 @@ __global__ __launch_bounds__(kCutThreads) void k_cut_mom_views(const arc::CutVie
     th.eps = v.th_eps;
     th.floor_frac = v.th_floor;
     th.inv_npix = v.th_inv_npix;
-    cut_tile<true, false, kMomStack>(v.nodes, v.crec, v.mom, v.gpos, v.gnrm, v.galb, v.mats, v.W, v.H, th, v.delta, v.budget,
+    cut_tile<true, false, kMomStack>(v.nodes, v.crec, v.mom, v.nbox, v.gpos, v.gnrm, v.galb, v.mats, v.W, v.H, th, v.delta, v.budget,
                           v.criterion, v.open2, v.exact_count, v.tile_cost, v.out_rad, v.counters, t % tw, t / tw, tw,
                           stk[wave], qest[wave], qex[wave], qpfx[wave], lstage[wave], tbox[wave], nullptr);
 }
@@ -1043,6 +1083,8 @@ int cut_moments_build(arc_gather* g, arc_bvh* b) {
                        b->ranges.as<int>(), b->nodes.as<arc_bvh_node>(), n, g->mom_pre.as<double>(),
                        g->mom_tot.as<double>(), g->mom.as<float4>());
     ARC_LAUNCHED(ctx);
+    ARC_TRY(ensure(ctx, g->mom_nbox, sizeof(float4) * 2 * (size_t)(n - 1)));
+    ARC_TRY(node_normal_boxes(b, g->crec.as<float4>(), g->mom_nbox.as<float4>()));
     g->mom_ready = true;
     return ARC_OK;
 }
@@ -1060,7 +1102,8 @@ int cut_moments(arc_gather* g, arc_bvh* b, const CutEBParams& cp, float4* rad, u
     const int blocks = ((g->width + 15) / 16) * ((g->height + 15) / 16);
     const int ntiles = 4 * blocks;
     if (defer) {
-        *defer = CutView{tm.nodes, tm.crec, tm.mom, g->gpos, g->gnrm, g->galb, g->mats.as<arc_material>(),
+        *defer = CutView{tm.nodes, tm.crec, tm.mom, g->mom_nbox.as<float4>(), g->gpos, g->gnrm, g->galb,
+                         g->mats.as<arc_material>(),
                          th.est, th.sum, th.eps, th.floor_frac, th.inv_npix, g->width, g->height, cp.delta,
                          cp.budget, cp.criterion, cp.opening * cp.opening, cp.exact_count, nullptr, rad, counters,
                          0, ntiles};
@@ -1096,7 +1139,8 @@ int cut_moments(arc_gather* g, arc_bvh* b, const CutEBParams& cp, float4* rad, u
     }
     if (mark) ARC_HIP(ctx, hipEventRecord(mark, ctx->stream));
     auto launch = [&](auto kernel) {
-        hipLaunchKernelGGL(kernel, dim3(blocks), dim3(kCutThreads), 0, ctx->stream, tm.nodes, tm.crec, tm.mom, g->gpos,
+        hipLaunchKernelGGL(kernel, dim3(blocks), dim3(kCutThreads), 0, ctx->stream, tm.nodes, tm.crec, tm.mom,
+                           g->mom_nbox.as<float4>(), g->gpos,
                            g->gnrm, g->galb, g->mats.as<arc_material>(), g->width, g->height, th, cp.delta, cp.budget,
                            cp.criterion, cp.opening * cp.opening, cp.exact_count, order, g->tile_cost.as<float>(), rad,
                            counters, desc);

@@ -545,6 +545,7 @@ void arc_gather_destroy(arc_gather* g) {
     arc::release(g->mom);
     arc::release(g->mom_pre);
     arc::release(g->mom_tot);
+    arc::release(g->mom_nbox);
     arc::release(g->tile_cost);
     arc::release(g->tile_order);
     arc::release(g->est_entries);

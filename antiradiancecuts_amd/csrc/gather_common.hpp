@@ -19,6 +19,7 @@ struct CutView {
     const arc_bvh_node* nodes;
     const float4* crec;
     const float4* mom;
+    const float4* nbox;  // per internal node: the box of its AVPLs' unit normals (lo, hi)
     const float4* gpos;
     const float4* gnrm;
     const float4* galb;
@@ -61,6 +62,7 @@ struct arc_gather {
     // moment estimator: per-internal-node records, and the FP64 prefix sums
     // over the sorted leaves (with per-block offsets) they are made from
     arc::DevBuf mom, mom_pre, mom_tot;
+    arc::DevBuf mom_nbox;  // per internal node: the box of its AVPLs' unit normals
     bool mom_ready = false;  // mom holds the records of the current cut tree
     // moment cut: each 8x8 tile's cost in the last frame (VALU instructions,
     // estimated from its counts) and the launch order made from it, heaviest

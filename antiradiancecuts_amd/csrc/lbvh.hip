@@ -270,7 +270,7 @@ __global__ __launch_bounds__(kBlock) void k_karras(const uint32_t* __restrict__ 
     }
     __syncthreads();
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i == 0) nready[0] = nready[1] = 0;  // the refit's two ready-list lengths
+    if (i == 0) nready[0] = nready[1] = nready[2] = 0;  // the refits' list lengths
     if (i >= num_nodes) return;             // no barrier follows
     const uint32_t mi = win[i - wbase];
     // delta(i, j) (bvh.cu:124-135): -1 outside [0, num_nodes]; on equal codes
@@ -611,10 +611,39 @@ __device__ inline void grow(f3& lo, f3& hi, const f3& a, const f3& b) {
     hi = f3{fmaxf(hi.x, b.x), fmaxf(hi.y, b.y), fmaxf(hi.z, b.z)};
 }
 
-__global__ __launch_bounds__(kRefitLeaves) void k_refit_points_local(arc_bvh_node* __restrict__ nodes,
-                                                                     const int* __restrict__ ranges,
-                                                                     const f3* __restrict__ spos, int n,
-                                                                     f3* __restrict__ chunk_box,
+// Where the range boxes come from and go: the sorted points into the nodes'
+// boxes (the SimpleBvh refit), or the sorted leaves' unit normals (the light
+// cuts' leaf records) into a per-node normal box.
+struct PointsIn {
+    const f3* p;
+    __device__ f3 operator()(int i) const { return p[i]; }
+};
+struct NormalsIn {
+    const float4* crec;
+    __device__ f3 operator()(int i) const {
+        const float4 r = crec[kCrec * (size_t)i + 1];
+        return f3{r.x, r.y, r.z};
+    }
+};
+struct NodesOut {
+    arc_bvh_node* nodes;
+    __device__ void operator()(int k, const f3& lo, const f3& hi) const {
+        nodes[k].bbMin = arc_float3{lo.x, lo.y, lo.z};
+        nodes[k].bbMax = arc_float3{hi.x, hi.y, hi.z};
+        nodes[k].visited = 2;
+    }
+};
+struct BoxOut {
+    float4* box;  // 2 per node: (lo, 0), (hi, 0)
+    __device__ void operator()(int k, const f3& lo, const f3& hi) const {
+        box[2 * (size_t)k] = make_float4(lo.x, lo.y, lo.z, 0.f);
+        box[2 * (size_t)k + 1] = make_float4(hi.x, hi.y, hi.z, 0.f);
+    }
+};
+
+template <class In, class Out>
+__global__ __launch_bounds__(kRefitLeaves) void k_refit_points_local(Out out, const int* __restrict__ ranges,
+                                                                     In spos, int n, f3* __restrict__ chunk_box,
                                                                      f3* __restrict__ block_box,
                                                                      int* __restrict__ crossing,
                                                                      int* __restrict__ ncross) {
@@ -632,7 +661,7 @@ __global__ __launch_bounds__(kRefitLeaves) void k_refit_points_local(arc_bvh_nod
     const int first = inner ? ranges[2 * i] : 0, last = inner ? ranges[2 * i + 1] : 0;
     if (t == 0) xcount = 0;
     if (i < n) {
-        const f3 p = spos[i];
+        const f3 p = spos(i);
         tlo[kRefitLeaves + t] = p;
         thi[kRefitLeaves + t] = p;
     } else {  // beyond the last leaf: neutral
@@ -681,16 +710,14 @@ __global__ __launch_bounds__(kRefitLeaves) void k_refit_points_local(arc_bvh_nod
         l >>= 1;
         r >>= 1;
     }
-    nodes[i].bbMin = arc_float3{blo.x, blo.y, blo.z};
-    nodes[i].bbMax = arc_float3{bhi.x, bhi.y, bhi.z};
-    nodes[i].visited = 2;
+    out(i, blo, bhi);
 }
 
 constexpr int kCrossThreads = 256;
 
-__global__ __launch_bounds__(kCrossThreads) void k_refit_points_crossing(arc_bvh_node* __restrict__ nodes,
-                                                                         const int* __restrict__ ranges,
-                                                                         const f3* __restrict__ spos,
+template <class In, class Out>
+__global__ __launch_bounds__(kCrossThreads) void k_refit_points_crossing(Out out, const int* __restrict__ ranges,
+                                                                         In spos,
                                                                          const f3* __restrict__ chunk_box,
                                                                          const f3* __restrict__ block_box,
                                                                          const int* __restrict__ crossing,
@@ -718,7 +745,7 @@ __global__ __launch_bounds__(kCrossThreads) void k_refit_points_crossing(arc_bvh
         for (int e = lane; e < total; e += 64) {
             f3 a, b;
             if (e < nA) {
-                a = b = spos[first + e];
+                a = b = spos(first + e);
             } else if (e < nA + nB) {
                 const int c = c1 + (e - nA);
                 a = chunk_box[2 * c];
@@ -732,7 +759,7 @@ __global__ __launch_bounds__(kCrossThreads) void k_refit_points_crossing(arc_bvh
                 a = chunk_box[2 * c];
                 b = chunk_box[2 * c + 1];
             } else {
-                a = b = spos[a4 + (e - nA - nB - nC - nD)];
+                a = b = spos(a4 + (e - nA - nB - nC - nD));
             }
             grow(blo, bhi, a, b);
         }
@@ -742,11 +769,7 @@ __global__ __launch_bounds__(kCrossThreads) void k_refit_points_crossing(arc_bvh
             const f3 ohi{__shfl_xor(bhi.x, off), __shfl_xor(bhi.y, off), __shfl_xor(bhi.z, off)};
             grow(blo, bhi, olo, ohi);
         }
-        if (lane == 0) {
-            nodes[k].bbMin = arc_float3{blo.x, blo.y, blo.z};
-            nodes[k].bbMax = arc_float3{bhi.x, bhi.y, bhi.z};
-            nodes[k].visited = 2;
-        }
+        if (lane == 0) out(k, blo, bhi);
     }
 }
 
@@ -757,18 +780,22 @@ inline size_t point_boxes_bytes(int n) {
     return sizeof(f3) * 2 * (chunks + blocks);
 }
 
-void launch_refit_points(arc_bvh* b, int n, int* crossing, int* ncross) {
+template <class In, class Out>
+void launch_range_boxes(arc_bvh* b, int n, In in, Out out, int* crossing, int* ncross) {
     const int rblocks = (n + kRefitLeaves - 1) / kRefitLeaves;
     f3* chunk_box = b->boxes.as<f3>();
     f3* block_box = chunk_box + 2 * (((size_t)n + kChunk - 1) / kChunk);
-    hipLaunchKernelGGL(k_refit_points_local, dim3(rblocks), dim3(kRefitLeaves), 0, b->ctx->stream,
-                       b->nodes.as<arc_bvh_node>(), b->ranges.as<int>(), b->positions.as<f3>(), n, chunk_box,
-                       block_box, crossing, ncross);
+    hipLaunchKernelGGL((k_refit_points_local<In, Out>), dim3(rblocks), dim3(kRefitLeaves), 0, b->ctx->stream, out,
+                       b->ranges.as<int>(), in, n, chunk_box, block_box, crossing, ncross);
     // about ten nodes per block cross blocks; a wave takes one at a time
     const int cblocks = std::max(1, std::min(1024, 2 * rblocks));
-    hipLaunchKernelGGL(k_refit_points_crossing, dim3(cblocks), dim3(kCrossThreads), 0, b->ctx->stream,
-                       b->nodes.as<arc_bvh_node>(), b->ranges.as<int>(), b->positions.as<f3>(), chunk_box,
-                       block_box, crossing, ncross);
+    hipLaunchKernelGGL((k_refit_points_crossing<In, Out>), dim3(cblocks), dim3(kCrossThreads), 0, b->ctx->stream,
+                       out, b->ranges.as<int>(), in, chunk_box, block_box, crossing, ncross);
+}
+
+void launch_refit_points(arc_bvh* b, int n, int* crossing, int* ncross) {
+    launch_range_boxes(b, n, PointsIn{b->positions.as<f3>()}, NodesOut{b->nodes.as<arc_bvh_node>()}, crossing,
+                       ncross);
 }
 
 
@@ -844,6 +871,16 @@ int random_numbers(arc_ctx* ctx, uint64_t seed, int count, float* out) {
     if (count <= 0) return ARC_OK;
     hipLaunchKernelGGL(k_random, dim3(blocks_for(count)), dim3(kBlock), 0, ctx->stream, seed, count, out);
     ARC_LAUNCHED(ctx);
+    return ARC_OK;
+}
+
+int node_normal_boxes(arc_bvh* b, const float4* crec, float4* nbox) {
+    const int n = b->n;
+    if (n < 2) return ARC_OK;
+    // the list and its counter of the range queries: the refit is done with them
+    int* ncross = reinterpret_cast<int*>(b->bounds.as<float>() + 15);  // zeroed by k_karras
+    launch_range_boxes(b, n, NormalsIn{crec}, BoxOut{nbox}, b->ready.as<int>(), ncross);
+    ARC_LAUNCHED(b->ctx);
     return ARC_OK;
 }
 

@@ -222,6 +222,47 @@ def test_launch_shapes_keep_the_image(gpu_ctx):
         g.close()
 
 
+@pytest.mark.parametrize("exact_count", [0, 16])
+def test_nodes_facing_away_add_nothing(gpu_ctx, exact_count):
+    """AVPLs on the plane z = 1 with normals +z, above pixels on z = 0 that
+    face +z: every AVPL faces away from every pixel, so the exact radiance is
+    0.  Kept nodes (one, the root, at a huge bound) and exact leaves are both
+    recognised as facing away from the tile (normal box against the node and
+    tile boxes) and add nothing, where the second-order estimate would leave
+    a positive residue (its clamped cosine is 2/32 at t = -1).  Flipped to
+    face the pixels, the same AVPLs light them."""
+    rng = np.random.default_rng(5)
+    P, N = 64, 4096
+    pos = np.zeros((P, 4), np.float32)
+    pos[:, 0] = rng.uniform(-0.5, 0.5, P)
+    pos[:, 1] = rng.uniform(-0.5, 0.5, P)
+    pos[:, 3] = 1
+    nrm = np.zeros((P, 4), np.float32)
+    nrm[:, 2] = 1
+    nrm[:, 3] = 1
+    alb = np.zeros((P, 4), np.float32)
+    alb[:, :3] = 0.5
+    a = np.stack([rng.uniform(-2, 2, N), rng.uniform(-2, 2, N), np.ones(N)], 1).astype(np.float32)
+    L = rng.uniform(0.5, 2.0, (N, 3)).astype(np.float32)
+    mats = np.array([[0] * 10, [0, 0, 0, 0.5, 0.6, 0.7, 0, 0, 0, 0]], np.float32)
+    out = {}
+    for name, nz in (("away", 1.0), ("facing", -1.0)):
+        an = np.zeros((N, 3), np.float32)
+        an[:, 2] = nz
+        sc = S.Scene(8, 8, pos, nrm, alb, mats, np.array([0, 0, 10], np.float32), 4.0, a, an, -an, L,
+                     np.zeros_like(L), np.ones(N, np.int32), np.ones(N, np.int32))
+        g, av = make_rig(gpu_ctx, sc)
+        try:
+            rad, _, st = run(gpu_ctx, g, av, sc, mom_params(errorBound=1e30 if exact_count == 0 else 0.01,
+                                                            cutExactCount=exact_count))
+        finally:
+            g.close()
+        out[name] = rad
+        print(f"[mom] AVPLs {name}, exact <= {exact_count}: max radiance {rad.max():.3e}", flush=True)
+    assert np.all(out["away"] == 0.0)
+    assert out["facing"].min() > 0.0
+
+
 def test_colocated_node_is_the_sh2_sum(gpu_ctx):
     """Three AVPLs at one point with different normals and colours, seen from
     pixels far away: the root is kept (huge bound) and estimated; its four
