@@ -100,7 +100,7 @@ def parse(argv=None):
     ap.add_argument("--eval-depth", type=int, default=0,
                     help="centroid estimator: a kept node is summed as its descendants this many levels down (0-4)")
     ap.add_argument("--cpu-runs", type=int, default=5, help="CPU baseline: timed runs (median reported)")
-    ap.add_argument("--accurate", default="0.003,3,64",
+    ap.add_argument("--accurate", default="0.004,4,128",
                     help="a second setting of the same frame, timed and checked against FP64 beside the headline: "
                          "errorBound,cutImageFloor,cutExactCount of the moment estimator with the Lightcuts bound "
                          "('' = skip)")
@@ -424,8 +424,9 @@ def profiled_traffic(dom: str, eb: bool, per_tile: bool, eval_depth: int = 0, mo
         names = {next((k for k in table if k.startswith(stem)), stem + ">"): 1}
     elif dom == "radiance":
         name = "k_cut_mom_tile" if mom else ("k_cut_eb_tile" if per_tile else "k_cut_eb_px") if eb else "k_cut_sa"
-        if mom and f"{name}<false>" in table:  # the traversal that estimates its kept nodes itself
-            name = f"{name}<false>"
+        own = next((k for k in table if k.startswith(f"{name}<false")), None) if mom else None
+        if own:  # the traversal that estimates its kept nodes itself (any stack depth)
+            name = own
         elif eb and per_tile and f"{name}<{eval_depth}>" in table:
             name = f"{name}<{eval_depth}>"
         names = {name: 1}
