@@ -706,11 +706,12 @@ __device__ __forceinline__ void cut_tile(const arc_bvh_node* __restrict__ nodes,
     drain_exact();
     acc.flush();
     // the tile's cost in VALU instructions (about 30 per bound test, 130 per
-    // moment estimate made here, 26 per exact pair, and the pixel's own loads
-    // and stores): the next frame's launch order
+    // moment estimate made here, 18 per queued exact leaf: 26 per pair for
+    // the half that face the tile, a few for the rest; and the pixel's own
+    // loads and stores): the next frame's launch order
     if (lane == 0) {
         tile_cost[ty * tw + tx] = 300.f + 30.f * (float)ntest + 130.f * (float)(nev - npool) +
-                                  26.f * (float)nexact + 2.f * (float)npool;
+                                  18.f * (float)nexact + 2.f * (float)npool;
         if constexpr (kPool) {
             pool->head[ty * tw + tx] = head;
             pool->queued[ty * tw + tx] = (float)npool;
