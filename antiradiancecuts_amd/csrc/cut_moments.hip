@@ -23,7 +23,12 @@
 //   them is exact to second order in the node's extent.
 //
 // The estimate of a node is sum_c max(A_c(d^), 0) * K, with d^ the direction
-// from the pixel to c and K the four-point average.  Kept nodes of at most
+// from the pixel to c and K the four-point average.  The records hold A_c / 4
+// (the average's 1/4, which max(., 0) lets through) with its zz term folded
+// into the constant by d^ . d^ = 1:
+//   A_c / 4 = a_c + b_c . d^ + (txx - tzz) x^2 + (tyy - tzz) y^2 + 2 txy xy + ...
+// with a_c = (3/32 I_c + tzz) / 4, b_c = -M_c / 8 and t = 15/128 T_c; the
+// pixel then sums K over the four points without scaling.  Kept nodes of at most
 // cutExactCount AVPLs are summed exactly over their leaves instead (their
 // sorted leaves are contiguous, crec slots [first, first + count)).
 //
@@ -44,8 +49,8 @@ constexpr int kMomTerms = 40;   // FP64 sums per leaf (see leaf_terms)
 constexpr int kScanBlock = 256;  // leaves per block of the prefix scan
 // float4 per internal node record:
 //   0 (c, max_c I_c)             1 (s, first leaf, leaf count, -)
-//   2-4 the four points, 12 floats   5-7 (-1/2 M_c, 3/32 I_c) per channel
-//   8-12 15/32 T_c per channel (xx, yy, zz, 2xy, 2xz, 2yz), 18 floats
+//   2-4 the four points, 12 floats   5-7 (b_c, a_c) per channel
+//   8-12 per channel (txx - tzz, tyy - tzz, 0, 2txy, 2txz, 2tyz), 18 floats
 constexpr int kMom = 13;
 constexpr int kMomStack = 96;    // per-wave stack of the tile traversal (default; 64 and 128 also built)
 
@@ -264,21 +269,22 @@ __global__ __launch_bounds__(kScanBlock) void k_mom_nodes(const int* __restrict_
     o[2] = make_float4(pts[0], pts[1], pts[2], pts[3]);
     o[3] = make_float4(pts[4], pts[5], pts[6], pts[7]);
     o[4] = make_float4(pts[8], pts[9], pts[10], pts[11]);
-    // per channel: A_c(d^) = 3/32 I_c + (-1/2 M_c) . d^ + d^T (15/32 T_c) d^,
+    // per channel: A_c(d^) / 4 = 3/128 I_c + (-1/8 M_c) . d^ + d^T (15/128 T_c) d^,
     // the off-diagonal T terms doubled so that the form is a plain dot
-    // product with (dx^2, dy^2, dz^2, dx dy, dx dz, dy dz)
+    // product with (dx^2, dy^2, dz^2, dx dy, dx dz, dy dz), and the zz term
+    // moved into the constant (dz^2 = 1 - dx^2 - dy^2 for a unit d^)
     float* tf = reinterpret_cast<float*>(o + 8);
     double Ib = 0.0;
 #pragma unroll 1
     for (int c = 0; c < 3; ++c) {
         range_group(pre, tot, first, last, kGroup * (1 + c), S);
         Ib = fmax(Ib, S[0]);
-        o[5 + c] = make_float4((float)(-0.5 * S[1]), (float)(-0.5 * S[2]), (float)(-0.5 * S[3]),
-                               (float)(0.09375 * S[0]));
-        const double f = 0.46875;
-        tf[6 * c + 0] = (float)(f * S[4]);
-        tf[6 * c + 1] = (float)(f * S[5]);
-        tf[6 * c + 2] = (float)(f * S[6]);
+        const double f = 0.25 * 0.46875;
+        o[5 + c] = make_float4((float)(-0.125 * S[1]), (float)(-0.125 * S[2]), (float)(-0.125 * S[3]),
+                               (float)(0.25 * 0.09375 * S[0] + f * S[6]));
+        tf[6 * c + 0] = (float)(f * (S[4] - S[6]));
+        tf[6 * c + 1] = (float)(f * (S[5] - S[6]));
+        tf[6 * c + 2] = 0.f;
         tf[6 * c + 3] = (float)(2.0 * f * S[7]);
         tf[6 * c + 4] = (float)(2.0 * f * S[8]);
         tf[6 * c + 5] = (float)(2.0 * f * S[9]);
@@ -323,12 +329,17 @@ __device__ inline arc_bvh_node uload_node(const arc_bvh_node* n) {
     return b;
 }
 
+// Squared distances are floored at 1e-24 before their reciprocal square root:
+// 1e36 bounds the cube of the result, and at a point that coincides with the
+// pixel the numerators (a cosine times |d|, or |d|^2) are 0.
+constexpr float kDistFloor2 = 1e-24f;
+
 // max(0, n_p . d) / |d|^3 at point x: the pixel-side cosine over the squared
 // distance (cudaGather.cu:45-57, pixel factor).
 __device__ inline float pixel_g(const PixelF& P, const float4& x) {
     const float dx = x.x - P.px, dy = x.y - P.py, dz = x.z - P.pz;
     const float dd = dx * dx + dy * dy + dz * dz;
-    const float inv = dd > 1e-30f ? __builtin_amdgcn_rsqf(dd) : 0.f;
+    const float inv = __builtin_amdgcn_rsqf(fmaxf(dd, kDistFloor2));
     const float u = fmaxf(P.nx * dx + P.ny * dy + P.nz * dz, 0.f);
     return u * (inv * inv * inv);
 }
@@ -343,20 +354,21 @@ __device__ inline void add_moments(const PixelF& P, const float4* __restrict__ r
         else return r[i];
     };
     const float4 m0 = ld(0), p01 = ld(2), p12 = ld(3), p23 = ld(4);
-    const float K = 0.25f * (pixel_g(P, make_float4(p01.x, p01.y, p01.z, 0.f)) +
-                             pixel_g(P, make_float4(p01.w, p12.x, p12.y, 0.f)) +
-                             pixel_g(P, make_float4(p12.z, p12.w, p23.x, 0.f)) +
-                             pixel_g(P, make_float4(p23.y, p23.z, p23.w, 0.f)));
+    const float K = pixel_g(P, make_float4(p01.x, p01.y, p01.z, 0.f)) +
+                    pixel_g(P, make_float4(p01.w, p12.x, p12.y, 0.f)) +
+                    pixel_g(P, make_float4(p12.z, p12.w, p23.x, 0.f)) +
+                    pixel_g(P, make_float4(p23.y, p23.z, p23.w, 0.f));
     const float dx = m0.x - P.px, dy = m0.y - P.py, dz = m0.z - P.pz;
     const float dd = dx * dx + dy * dy + dz * dz;
-    const float inv = dd > 1e-30f ? __builtin_amdgcn_rsqf(dd) : 0.f;
+    const float inv = __builtin_amdgcn_rsqf(fmaxf(dd, kDistFloor2));
     const float i2 = inv * inv;
-    const float xx = dx * dx, yy = dy * dy, zz = dz * dz, xy = dx * dy, xz = dx * dz, yz = dy * dz;
+    const float xx = dx * dx, yy = dy * dy, xy = dx * dy, xz = dx * dz, yz = dy * dz;
     const float4 mr = ld(5), mg = ld(6), mb = ld(7);
     const float4 t0 = ld(8), t1 = ld(9), t2 = ld(10), t3 = ld(11), t4 = ld(12);
-    const float qr = t0.x * xx + t0.y * yy + t0.z * zz + t0.w * xy + t1.x * xz + t1.y * yz;
-    const float qg = t1.z * xx + t1.w * yy + t2.x * zz + t2.y * xy + t2.z * xz + t2.w * yz;
-    const float qb = t3.x * xx + t3.y * yy + t3.z * zz + t3.w * xy + t4.x * xz + t4.y * yz;
+    // (the zz coefficients t0.z, t2.x and t3.z are 0: folded into the constants)
+    const float qr = t0.x * xx + t0.y * yy + t0.w * xy + t1.x * xz + t1.y * yz;
+    const float qg = t1.z * xx + t1.w * yy + t2.y * xy + t2.z * xz + t2.w * yz;
+    const float qb = t3.x * xx + t3.y * yy + t3.w * xy + t4.x * xz + t4.y * yz;
     const float ar = mr.w + inv * (mr.x * dx + mr.y * dy + mr.z * dz) + i2 * qr;
     const float ag = mg.w + inv * (mg.x * dx + mg.y * dy + mg.z * dz) + i2 * qg;
     const float ab = mb.w + inv * (mb.x * dx + mb.y * dy + mb.z * dz) + i2 * qb;
@@ -380,7 +392,9 @@ __device__ inline float leaf_g(const PixelF& P, const float4& r0, const float4& 
 }
 
 // The Lightcuts bound of a node for the pixel (node_bound) with one
-// transcendental: 1 / dmin^2 as the square of 1 / dmin.
+// transcendental: 1 / dmin^2 as the square of 1 / dmin.  The cosine bound
+// min(1, hmax / dmin), 0 where hmax <= 0, is hmax / dmin clamped to [0, 1]
+// (the multiply's output clamp).
 __device__ inline float node_bound1(const PixelF& P, const arc_bvh_node& b, float ilum, float delta2) {
     const float ex = fmaxf(fmaxf(b.bbMin.x - P.px, P.px - b.bbMax.x), 0.f);
     const float ey = fmaxf(fmaxf(b.bbMin.y - P.py, P.py - b.bbMax.y), 0.f);
@@ -390,7 +404,7 @@ __device__ inline float node_bound1(const PixelF& P, const arc_bvh_node& b, floa
                        P.ny * (P.ny > 0.f ? b.bbMax.y - P.py : b.bbMin.y - P.py) +
                        P.nz * (P.nz > 0.f ? b.bbMax.z - P.pz : b.bbMin.z - P.pz);
     const float ri = __builtin_amdgcn_rsqf(dmin2);
-    return hmax > 0.f ? ilum * fminf(1.f, hmax * ri) * (ri * ri) : 0.f;
+    return ilum * __builtin_amdgcn_fmed3f(hmax * ri, 0.f, 1.f) * (ri * ri);
 }
 
 // Inclusive sum over the wave's lanes.
@@ -817,11 +831,12 @@ __global__ __launch_bounds__(kCutThreads) void k_cut_mom_views(const arc::CutVie
 //   |d|^2              = |c|^2 - 2 c . p + |p|^2
 //   n_p . (q_j - p)    = (q_j, -1) . (n_p, n_p . p)          (the four points)
 //   |q_j - p|^2        = |q_j|^2 - 2 q_j . p + |p|^2
-//   (-1/2 M_c) . d     = (mr_c . c, -mr_c) . (1, p)
+//   b_c . d            = (b_c . c, -b_c) . (1, p)
 //   d^T T'_c d         = a quadratic polynomial in p: 10 coefficients,
-// where T'_c is 15/32 T_c plus 3/32 I_c on its diagonal, which folds the
-// constant term in (3/32 I_c = 3/32 I_c |d|^2 / |d|^2).  Per channel
-//   A_c / 4 = ((-1/2 M_c) . d / |d| + d^T T'_c d / |d|^2) / 4,
+// where T'_c is the record's quadratic form plus a_c on its diagonal, which
+// folds the constant term in (a_c = a_c |d|^2 / |d|^2): 15/128 T_c plus
+// 3/128 I_c on the diagonal.  Per channel
+//   A_c / 4 = b_c . d / |d| + d^T T'_c d / |d|^2,
 // and the node adds max(A_c / 4, 0) * sum_j max(n_p . (q_j - p), 0) / |q_j - p|^3.
 // For 16 nodes x 16 pixels each product is one 16x16x4 FP32 MFMA (the
 // quadratic form three, K = 12): 21 per block.  What stays on the VALU is
@@ -870,14 +885,14 @@ __device__ inline void est_rows(const float4* __restrict__ r, float ox, float oy
         const float al = mr.w;
         const float txx = tt[o6] + al, tyy = tt[o6 + 1] + al, tzz = tt[o6 + 2] + al;
         const float txy = tt[o6 + 3], txz = tt[o6 + 4], tyz = tt[o6 + 5];
-        a[9 + ch] = 0.25f * (k == 0 ? mr.x * cx + mr.y * cy + mr.z * cz : -comp(mr.x, mr.y, mr.z));
+        a[9 + ch] = k == 0 ? mr.x * cx + mr.y * cy + mr.z * cz : -comp(mr.x, mr.y, mr.z);
         const float lx = -(2.f * txx * cx + txy * cy + txz * cz);
         const float ly = -(2.f * tyy * cy + txy * cx + tyz * cz);
         const float lz = -(2.f * tzz * cz + txz * cx + tyz * cy);
         const float c0 = txx * cx * cx + tyy * cy * cy + tzz * cz * cz + txy * cx * cy + txz * cx * cz + tyz * cy * cz;
-        a[12 + 3 * ch] = 0.25f * (k == 0 ? c0 : comp(lx, ly, lz));
-        a[13 + 3 * ch] = 0.25f * (k == 0 ? txx : (k == 1 ? tyy : (k == 2 ? tzz : txy)));
-        a[14 + 3 * ch] = 0.25f * (k == 0 ? txz : (k == 1 ? tyz : 0.f));
+        a[12 + 3 * ch] = k == 0 ? c0 : comp(lx, ly, lz);
+        a[13 + 3 * ch] = k == 0 ? txx : (k == 1 ? tyy : (k == 2 ? tzz : txy));
+        a[14 + 3 * ch] = k == 0 ? txz : (k == 1 ? tyz : 0.f);
     }
 }
 

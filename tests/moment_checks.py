@@ -65,7 +65,9 @@ def moment_sums(crec: np.ndarray, ranges: np.ndarray, n: int, ref=None):
 def check_moment_records(mom: np.ndarray, crec: np.ndarray, ranges: np.ndarray, n: int, nodes):
     """The moment estimator's records (13 float4 per internal node: (c, max
     I), (s, first, count, -), the four points' 12 floats, per channel
-    (-M/2, 3I/32), then 15T/32 per channel).  The
+    (-M/8, 3I/128 + t_zz), then per channel t = 15T/128 as (t_xx - t_zz,
+    t_yy - t_zz, 0, 2t_xy, 2t_xz, 2t_yz): the SH2 clamped cosine over 4, its
+    zz term moved into the constant for a unit direction).  The
     positions are taken relative to the centre of the root's box (nodes[0]),
     as the library takes them."""
     ref = 0.5 * (nodes["bbMin"][0].astype(np.float64) + nodes["bbMax"][0].astype(np.float64))
@@ -78,14 +80,16 @@ def check_moment_records(mom: np.ndarray, crec: np.ndarray, ranges: np.ndarray, 
     np.testing.assert_array_equal(count, ranges[:, 1] - ranges[:, 0] + 1)
     scale = Is.max()
     np.testing.assert_allclose(mom[:, 0, 3], Is.max(1), rtol=1e-6, atol=1e-7 * scale)
-    # per channel: 3/32 I_c, -1/2 M_c, 15/32 T_c (off-diagonals doubled)
-    for c in range(3):
-        np.testing.assert_allclose(mom[:, 5 + c, 3], 0.09375 * Is[:, c], rtol=1e-6, atol=1e-7 * scale)
-        np.testing.assert_allclose(mom[:, 5 + c, :3], -0.5 * M[:, c], rtol=1e-6, atol=1e-7 * scale)
+    # per channel, over 4: 3/32 I_c + 15/32 T_zz, -1/2 M_c, 15/32 T_c with the
+    # zz term taken off the diagonal (off-diagonals doubled)
     tf = mom[:, 8:13, :].reshape(k, 20)[:, :18].reshape(k, 3, 6)
     for c in range(3):
-        Tc = 0.46875 * T[:, c]
-        want = np.stack([Tc[:, 0, 0], Tc[:, 1, 1], Tc[:, 2, 2], 2 * Tc[:, 0, 1], 2 * Tc[:, 0, 2], 2 * Tc[:, 1, 2]], 1)
+        Tc = 0.25 * 0.46875 * T[:, c]
+        np.testing.assert_allclose(mom[:, 5 + c, 3], 0.25 * 0.09375 * Is[:, c] + Tc[:, 2, 2], rtol=1e-6,
+                                   atol=1e-7 * scale)
+        np.testing.assert_allclose(mom[:, 5 + c, :3], -0.125 * M[:, c], rtol=1e-6, atol=1e-7 * scale)
+        want = np.stack([Tc[:, 0, 0] - Tc[:, 2, 2], Tc[:, 1, 1] - Tc[:, 2, 2], 0 * Tc[:, 2, 2],
+                         2 * Tc[:, 0, 1], 2 * Tc[:, 0, 2], 2 * Tc[:, 1, 2]], 1)
         np.testing.assert_allclose(tf[:, c], want, rtol=1e-6, atol=1e-7 * scale)
     # position mean and covariance of the four points
     live = W > 1e-6 * W.max()
