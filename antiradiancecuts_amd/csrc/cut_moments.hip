@@ -329,19 +329,25 @@ __device__ inline arc_bvh_node uload_node(const arc_bvh_node* n) {
     return b;
 }
 
-// Squared distances are floored at 1e-24 before their reciprocal square root:
-// 1e36 bounds the cube of the result, and at a point that coincides with the
-// pixel the numerators (a cosine times |d|, or |d|^2) are 0.
+// Squared distances carry 1e-24 (the first addend of their FMA chain) before
+// their reciprocal square root: 1e36 bounds the cube of the result, at a
+// point that coincides with the pixel the numerators (a cosine times |d|, or
+// |d|^2) are 0, and above 1e-17 the addend is below half an ulp.
 constexpr float kDistFloor2 = 1e-24f;
 
+__device__ inline float dist2(float dx, float dy, float dz, float floor2) {
+    return fmaf(dz, dz, fmaf(dy, dy, fmaf(dx, dx, floor2)));
+}
+
 // max(0, n_p . d) / |d|^3 at point x: the pixel-side cosine over the squared
-// distance (cudaGather.cu:45-57, pixel factor).
+// distance (cudaGather.cu:45-57, pixel factor), as the cosine n_p . d / |d|
+// clamped to [0, 1] (the multiply's output clamp; it is at most 1 to
+// rounding) times 1 / |d|^2.
 __device__ inline float pixel_g(const PixelF& P, const float4& x) {
     const float dx = x.x - P.px, dy = x.y - P.py, dz = x.z - P.pz;
-    const float dd = dx * dx + dy * dy + dz * dz;
-    const float inv = __builtin_amdgcn_rsqf(fmaxf(dd, kDistFloor2));
-    const float u = fmaxf(P.nx * dx + P.ny * dy + P.nz * dz, 0.f);
-    return u * (inv * inv * inv);
+    const float inv = __builtin_amdgcn_rsqf(dist2(dx, dy, dz, kDistFloor2));
+    const float c = __builtin_amdgcn_fmed3f((P.nx * dx + P.ny * dy + P.nz * dz) * inv, 0.f, 1.f);
+    return c * (inv * inv);
 }
 
 // The node's estimate for the pixel, per channel, added to the pixel's sums.
@@ -359,8 +365,7 @@ __device__ inline void add_moments(const PixelF& P, const float4* __restrict__ r
                     pixel_g(P, make_float4(p12.z, p12.w, p23.x, 0.f)) +
                     pixel_g(P, make_float4(p23.y, p23.z, p23.w, 0.f));
     const float dx = m0.x - P.px, dy = m0.y - P.py, dz = m0.z - P.pz;
-    const float dd = dx * dx + dy * dy + dz * dz;
-    const float inv = __builtin_amdgcn_rsqf(fmaxf(dd, kDistFloor2));
+    const float inv = __builtin_amdgcn_rsqf(dist2(dx, dy, dz, kDistFloor2));
     const float i2 = inv * inv;
     const float xx = dx * dx, yy = dy * dy, xy = dx * dy, xz = dx * dz, yz = dy * dz;
     const float4 mr = ld(5), mg = ld(6), mb = ld(7);
@@ -381,10 +386,11 @@ __device__ inline void add_moments(const PixelF& P, const float4* __restrict__ r
 // G of one leaf (an AVPL with a unit normal n_a) for the pixel:
 // max(0, n . d) max(0, -n_a . d) / |d|^4 (cudaGather.cu:45-57), from one
 // reciprocal of |d|^2.  With unit normals the cosines need no clamp at 1;
-// coincident points (|d|^2 below 1e-18) give 0, as both cosines vanish there.
+// |d|^2 carries 1e-18 (below half an ulp from 1e-11 up), and coincident
+// points give 0, as both cosines vanish there.
 __device__ inline float leaf_g(const PixelF& P, const float4& r0, const float4& r1) {
     const float dx = r0.x - P.px, dy = r0.y - P.py, dz = r0.z - P.pz;
-    const float dd = fmaxf(dx * dx + dy * dy + dz * dz, 1e-18f);
+    const float dd = dist2(dx, dy, dz, 1e-18f);
     const float u1 = fmaxf(P.nx * dx + P.ny * dy + P.nz * dz, 0.f);
     const float u2 = fmaxf(-(r1.x * dx + r1.y * dy + r1.z * dz), 0.f);
     const float r = __builtin_amdgcn_rcpf(dd);
