@@ -100,7 +100,7 @@ def parse(argv=None):
     ap.add_argument("--eval-depth", type=int, default=0,
                     help="centroid estimator: a kept node is summed as its descendants this many levels down (0-4)")
     ap.add_argument("--cpu-runs", type=int, default=5, help="CPU baseline: timed runs (median reported)")
-    ap.add_argument("--accurate", default="0.004,4,128",
+    ap.add_argument("--accurate", default="0.0045,4,128",
                     help="a second setting of the same frame, timed and checked against FP64 beside the headline: "
                          "errorBound,cutImageFloor,cutExactCount of the moment estimator with the Lightcuts bound "
                          "('' = skip)")
